@@ -1,0 +1,167 @@
+"""Seeded synthetic densities and the trees they live on.
+
+The workloads of the box FGT benchmarks and parity tests: sums of Gaussians
+``a * exp(-|x - c|^2 / w^2)`` periodised over the 3x3 neighbouring cells of
+the unit box ``[-0.5, 0.5]^2``, spot patterns for the multi-density case,
+and the quadtrees they are tabulated on.  Random draws use
+``numpy.random.default_rng(seed)`` in the order centres -> widths ->
+amplitudes.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+from typing import Callable, List, Optional, Sequence
+
+import numpy as np
+
+from .tree import QuadTree, build_adaptive, build_uniform
+
+__all__ = ["GaussianSum", "Workload", "make_workload", "WORKLOADS"]
+
+
+@dataclass
+class GaussianSum:
+    centers: np.ndarray  # (m, 2)
+    widths: np.ndarray  # (m,)
+    amps: np.ndarray  # (m,)
+    nimage: int = 1  # images per side: 1 -> 3x3 cells
+    offset: float = 0.0
+    clip: Optional[tuple] = None
+
+    def __call__(self, p: np.ndarray) -> np.ndarray:
+        x = p[..., 0]
+        y = p[..., 1]
+        out = np.full(x.shape, self.offset, dtype=np.float64)
+        r = range(-self.nimage, self.nimage + 1)
+        for c, w, a in zip(self.centers, self.widths, self.amps):
+            for mx in r:
+                for my in r:
+                    out += a * np.exp(-((x - c[0] - mx) ** 2 + (y - c[1] - my) ** 2) / w**2)
+        if self.clip is not None:
+            out = np.clip(out, *self.clip)
+        return out
+
+    def heat_potential(self, p: np.ndarray, delta: float) -> np.ndarray:
+        """Closed form of ``int exp(-|x-y|^2/delta) f(y) dy`` over R^2 for the image sum
+        (unnormalised kernel): each Gaussian maps to
+        ``a pi delta w^2/(delta+w^2) exp(-|x-c|^2/(delta+w^2))``.  Only exact for
+        ``clip is None``."""
+        x = p[..., 0]
+        y = p[..., 1]
+        out = np.zeros(x.shape)
+        r = range(-self.nimage, self.nimage + 1)
+        for c, w, a in zip(self.centers, self.widths, self.amps):
+            s = delta + w * w
+            for mx in r:
+                for my in r:
+                    out += a * np.pi * delta * w * w / s * np.exp(-((x - c[0] - mx) ** 2 + (y - c[1] - my) ** 2) / s)
+        return out
+
+
+def gaussian_sum(rng: np.random.Generator, m: int, wlo: float, whi: float, alo=-1.0, ahi=1.0, logw=False) -> GaussianSum:
+    c = rng.uniform(-0.5, 0.5, size=(m, 2))
+    if logw:
+        w = np.exp(rng.uniform(np.log(wlo), np.log(whi), size=m))
+    else:
+        w = rng.uniform(wlo, whi, size=m)
+    a = rng.uniform(alo, ahi, size=m)
+    return GaussianSum(c, w, a)
+
+
+class Stack:
+    """nd densities evaluated together: returns (..., nd)."""
+
+    def __init__(self, fields: Sequence[Callable[[np.ndarray], np.ndarray]]):
+        self.fields = list(fields)
+
+    def __call__(self, p):
+        return np.stack([f(p) for f in self.fields], axis=-1)
+
+
+@dataclass
+class Workload:
+    name: str
+    tree: QuadTree
+    fvals: np.ndarray  # (nboxes, npbox, nd)
+    nd: int
+    norder: int
+    ipoly: int
+    delta: float
+    eps: float
+    ifpgh: int
+    iperiod: int
+    density: Callable
+    description: str
+
+    @property
+    def npts(self) -> int:
+        return self.tree.nleaves * self.norder**2
+
+
+def _c1(scale: str):
+    rng = np.random.default_rng(1)
+    g = gaussian_sum(rng, 3, 0.05, 0.1)
+    t = build_uniform(4)
+    return t, g, 1, 8, 1e-3, 1e-6, 1, "uniform level-4 tree (256 leaves), n=8 Legendre, nd=1, delta=1e-3, eps=1e-6, pot"
+
+
+def _c2(scale: str):
+    rng = np.random.default_rng(2)
+    c = rng.uniform(-0.5, 0.5, size=(5, 2))
+    w = np.r_[0.005, rng.uniform(0.05, 0.1, size=4)]
+    a = rng.uniform(-1, 1, size=5)
+    g = GaussianSum(c, w, a)
+    floor = {"full": 7, "small": 3, "tiny": 2}[scale]
+    maxl = {"full": 10, "small": 6, "tiny": 5}[scale]
+    t = build_adaptive(g, 1, 16, 0, 1e-9, maxl, min_level=floor)
+    return t, g, 1, 16, 1e-4, 1e-9, 2, (
+        f"adaptive periodic tree, tail rule tol 1e-9, uniform level-{floor} floor, max level {maxl}, "
+        "n=16 Legendre, nd=1, delta=1e-4, eps=1e-9, pot+grad")
+
+
+def _c3(scale: str, delta=1e-3, eps=1e-9):
+    rng = np.random.default_rng(3)
+    spots = []
+    for _ in range(2):
+        c = rng.uniform(-0.5, 0.5, size=(20, 2))
+        spots.append(GaussianSum(c, np.full(20, 0.02), np.ones(20), clip=(0.0, 1.0)))
+    g3 = gaussian_sum(rng, 5, 0.05, 0.1)
+    # zero-mean: subtract the mean of the 3x3 image sum over the cell
+    g3.offset = -float(np.sum(g3.amps * np.pi * g3.widths**2))
+    dens = Stack([spots[0], spots[1], g3])
+    maxl = {"full": 9, "small": 6, "tiny": 5}[scale]
+    t = build_adaptive(dens, 3, 8, 0, 1e-9, maxl, min_level=2, eta=0.0)
+    return t, dens, 3, 8, delta, eps, 3, (
+        f"nd=3 (two spot fields, one zero-mean Gaussian sum) on an adaptive tree (max level {maxl}), "
+        f"n=8 Legendre, delta={delta:g}, eps={eps:g}, pot+grad+hess")
+
+
+def _c4(scale: str):
+    rng = np.random.default_rng(4)
+    g1 = gaussian_sum(rng, 4, 0.05, 0.1)
+    g2 = gaussian_sum(rng, 4, 0.05, 0.1)
+    lev = {"full": 6, "small": 4, "tiny": 3}[scale]
+    t = build_uniform(lev)
+    return t, Stack([g1, g2]), 2, 8, 1e-3, 1e-9, 1, f"uniform level-{lev} tree, n=8, nd=2, delta=1e-3, eps=1e-9, pot"
+
+
+def _c5(scale: str):
+    rng = np.random.default_rng(5)
+    g = gaussian_sum(rng, 40, 0.002, 0.1, logw=True)
+    floor = {"full": 8, "small": 4, "tiny": 3}[scale]
+    maxl = {"full": 12, "small": 7, "tiny": 5}[scale]
+    t = build_adaptive(g, 1, 16, 0, 1e-10, maxl, min_level=floor)
+    return t, g, 1, 16, 1e-5, 1e-10, 2, (
+        f"adaptive periodic tree, tail rule tol 1e-10, uniform level-{floor} floor, max level {maxl}, "
+        "n=16 Legendre, nd=1, delta=1e-5, eps=1e-10, pot+grad")
+
+
+WORKLOADS = {"c1": _c1, "c2": _c2, "c3": _c3, "c4": _c4, "c5": _c5}
+
+
+def make_workload(name: str, scale: str = "full", **kw) -> Workload:
+    """Build one of the seeded workloads.  ``scale`` in {"full", "small", "tiny"} shrinks
+    the tree depth for CPU-sized parity tests; densities and FGT parameters are kept."""
+    t, dens, nd, n, delta, eps, ifpgh, desc = WORKLOADS[name](scale, **kw)
+    fv = t.sample(dens, n, 0, nd)
+    return Workload(name, t, fv, nd, n, 0, delta, eps, ifpgh, 1, dens, desc)

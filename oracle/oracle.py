@@ -1,0 +1,246 @@
+"""ctypes front end of the checkers.  TEST INFRASTRUCTURE ONLY.
+
+Only tests/, ``__graft_entry__.smoke()`` and bench.py's cpu_baseline leg
+import this module.  It wraps
+
+* ``_build/liboracle.so`` -- the C restatement of the reference boxfgt
+  (boxfgt_oracle.c), and
+* ``_ref/libref_boxfgt.so`` -- the reference Fortran ``boxfgt`` itself,
+  compiled from its sources by ``make -C oracle ref`` (present only where the
+  reference sources were available at build time).
+
+Array conventions follow the Fortran ones with C order reversed:
+fvals/pot ``(nboxes, npbox, nd)``, grad ``(nboxes, npbox, 2, nd)``, hess
+``(nboxes, npbox, 3, nd)``.  Both entry points mutate a copy of ``itree``
+(the colleague lists are recomputed) and return it.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+from typing import Optional
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "_build", "liboracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libref_boxfgt.so")
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+_oracle = None
+_ref = None
+
+
+def build(with_reference: Optional[bool] = None) -> None:
+    """Compile the oracle (and the reference when its sources exist)."""
+    import subprocess
+
+    targets = ["all"]
+    if with_reference is None:
+        with_reference = os.path.isdir(os.environ.get("REF_DIR", "/root/reference"))
+    if with_reference:
+        targets.append("ref")
+    subprocess.run(["make", "-s", "-C", HERE, "-j8"] + targets, check=True)
+
+
+def load_oracle() -> ctypes.CDLL:
+    global _oracle
+    if _oracle is None:
+        if not os.path.exists(ORACLE_SO):
+            raise RuntimeError(f"oracle library missing: {ORACLE_SO} (run `make -C oracle`)")
+        lib = ctypes.CDLL(ORACLE_SO)
+        lib.oracle_boxfgt.restype = ctypes.c_int
+        lib.oracle_boxfgt.argtypes = [
+            ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+            ctypes.c_int, ctypes.c_int, ctypes.c_int, _ip, _ip, _dp, _dp, _dp, ctypes.c_int, _dp, _dp, _dp, _dp, _ip,
+        ]
+        for name in ("oracle_legeexps", "oracle_chebexps"):
+            f = getattr(lib, name)
+            f.restype = None
+            f.argtypes = [ctypes.c_int, _dp, _dp, _dp]
+        lib.oracle_fgtpwterms.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int, _dp, _ip]
+        lib.oracle_pwnodes.argtypes = [ctypes.c_double, ctypes.c_int, _dp, _dp]
+        lib.oracle_periodic_pwnodes.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_int, _dp, _dp]
+        lib.oracle_sph_besselj.argtypes = [ctypes.c_int, ctypes.c_double, _dp]
+        lib.oracle_poly2pw.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp, ctypes.c_double,
+                                       ctypes.c_double, _dp]
+        lib.oracle_pw2pgh.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, ctypes.c_double, ctypes.c_double,
+                                      _dp, _dp, _dp]
+        lib.oracle_loctab_level.argtypes = [ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                            ctypes.c_double, _dp, _dp, _dp, _ip]
+        _oracle = lib
+    return _oracle
+
+
+def reference_available() -> bool:
+    return os.path.exists(REF_SO)
+
+
+def load_reference() -> ctypes.CDLL:
+    global _ref
+    if _ref is None:
+        if not reference_available():
+            raise RuntimeError(f"reference library missing: {REF_SO} (run `make -C oracle ref`)")
+        _ref = ctypes.CDLL(REF_SO, mode=os.RTLD_LAZY | os.RTLD_LOCAL)
+        _ref.boxfgt_.restype = None
+    return _ref
+
+
+def _p(a, t=_dp):
+    return a.ctypes.data_as(t)
+
+
+@dataclass
+class FGTResult:
+    pot: np.ndarray
+    grad: np.ndarray
+    hess: np.ndarray
+    timeinfo: np.ndarray
+    itree: np.ndarray
+    info: Optional[dict] = None
+
+
+def _alloc(nboxes, npbox, nd):
+    return (np.zeros((nboxes, npbox, nd)), np.zeros((nboxes, npbox, 2, nd)), np.zeros((nboxes, npbox, 3, nd)))
+
+
+def oracle_boxfgt(tree, fvals, nd, delta, eps, ipoly, iperiod, norder, ifpgh) -> FGTResult:
+    """Run the C restatement on a copy of the tree arrays."""
+    lib = load_oracle()
+    itree = np.ascontiguousarray(tree.itree, dtype=np.int32).copy()
+    iptr = np.ascontiguousarray(tree.iptr, dtype=np.int32)
+    centers = np.ascontiguousarray(tree.centers, dtype=np.float64)
+    boxsize = np.ascontiguousarray(tree.boxsize, dtype=np.float64)
+    nboxes = centers.shape[0]
+    npbox = norder * norder
+    fv = np.ascontiguousarray(fvals, dtype=np.float64)
+    assert fv.shape == (nboxes, npbox, nd)
+    pot, grad, hess = _alloc(nboxes, npbox, nd)
+    ti = np.zeros(20)
+    info = np.zeros(8, dtype=np.int32)
+    rc = lib.oracle_boxfgt(nd, delta, eps, ipoly, iperiod, norder, npbox, nboxes, tree.nlevels, _p(itree, _ip),
+                           _p(iptr, _ip), _p(centers), _p(boxsize), _p(fv), ifpgh, _p(pot), _p(grad), _p(hess),
+                           _p(ti), _p(info, _ip))
+    if rc != 0:
+        raise RuntimeError(f"oracle_boxfgt failed with status {rc}")
+    keys = ["npwlevel", "npw", "nlevstart", "nlevend", "nleafbox", "npairs"]
+    return FGTResult(pot, grad, hess, ti, itree, {k: int(v) for k, v in zip(keys, info)})
+
+
+def reference_boxfgt(tree, fvals, nd, delta, eps, ipoly, iperiod, norder, ifpgh, nthreads: Optional[int] = None) -> FGTResult:
+    """Run the reference Fortran boxfgt (ifnewtree=0, no extra targets) on a copy of the tree."""
+    lib = load_reference()
+    if nthreads is not None:
+        try:
+            omp = ctypes.CDLL("libomp.so")
+            omp.omp_set_num_threads(int(nthreads))
+        except OSError:
+            pass
+    c_int = ctypes.c_int
+    c_dbl = ctypes.c_double
+    itree = np.ascontiguousarray(tree.itree, dtype=np.int32).copy()
+    iptr = np.ascontiguousarray(tree.iptr, dtype=np.int32).copy()
+    centers = np.ascontiguousarray(tree.centers, dtype=np.float64).copy()
+    boxsize = np.ascontiguousarray(tree.boxsize, dtype=np.float64).copy()
+    nboxes = centers.shape[0]
+    npbox = norder * norder
+    fv = np.ascontiguousarray(fvals, dtype=np.float64)
+    assert fv.shape == (nboxes, npbox, nd)
+    pot, grad, hess = _alloc(nboxes, npbox, nd)
+    ti = np.zeros(20)
+    targs = np.zeros(2)
+    pote = np.zeros(max(nd, 1))
+    grade = np.zeros(2 * max(nd, 1))
+    hesse = np.zeros(3 * max(nd, 1))
+    args = [
+        c_int(nd), c_int(2), c_dbl(delta), c_dbl(eps), c_int(ipoly), c_int(iperiod), c_int(norder), c_int(npbox),
+        c_int(nboxes), c_int(tree.nlevels), c_int(itree.shape[0]),
+    ]
+    refs = [ctypes.byref(a) for a in args]
+    refs += [_p(itree, _ip), _p(iptr, _ip), _p(centers), _p(boxsize), _p(fv)]
+    ifpgh_c, ifnew, ntarg, ifpt = c_int(ifpgh), c_int(0), c_int(0), c_int(0)
+    refs += [ctypes.byref(ifpgh_c), _p(pot), _p(grad), _p(hess), ctypes.byref(ifnew), ctypes.byref(ntarg), _p(targs),
+             ctypes.byref(ifpt), _p(pote), _p(grade), _p(hesse), _p(ti)]
+    lib.boxfgt_(*refs)
+    if args[5].value != iperiod:
+        raise RuntimeError("reference did not restore iperiod")
+    return FGTResult(pot, grad, hess, ti, itree)
+
+
+# -- 1-D pieces, for pinning the oracle against closed forms ----------------
+
+def legeexps(n):
+    lib = load_oracle()
+    x, w, u = np.zeros(n), np.zeros(n), np.zeros((n, n))
+    lib.oracle_legeexps(n, _p(x), _p(w), _p(u))
+    return x, w, u.T.copy()  # u[i, j] = U(i, j): coefficient i from value j
+
+
+def chebexps(n):
+    lib = load_oracle()
+    x, w, u = np.zeros(n), np.zeros(n), np.zeros((n, n))
+    lib.oracle_chebexps(n, _p(x), _p(w), _p(u))
+    return x, w, u.T.copy()
+
+
+def fgtpwterms(bsize, delta, eps, iperiod):
+    lib = load_oracle()
+    pmax = ctypes.c_double(0)
+    npw = ctypes.c_int(0)
+    lib.oracle_fgtpwterms(bsize, delta, eps, iperiod, ctypes.byref(pmax), ctypes.byref(npw))
+    return pmax.value, npw.value
+
+
+def pwnodes(pmax, npw):
+    lib = load_oracle()
+    ws, ts = np.zeros(npw), np.zeros(npw)
+    lib.oracle_pwnodes(pmax, npw, _p(ws), _p(ts))
+    return ws, ts
+
+
+def periodic_pwnodes(bs0, delta, npw):
+    lib = load_oracle()
+    ws, ts = np.zeros(npw), np.zeros(npw)
+    lib.oracle_periodic_pwnodes(bs0, delta, npw, _p(ws), _p(ts))
+    return ws, ts
+
+
+def sph_besselj(n, x):
+    lib = load_oracle()
+    f = np.zeros(n)
+    lib.oracle_sph_besselj(n, x, _p(f))
+    return f
+
+
+def poly2pw(n, ipoly, ws, ts, delta, boxdim):
+    """tab[m, j] (m = node, j = PW index) complex."""
+    lib = load_oracle()
+    npw = len(ts)
+    out = np.zeros((npw, n), dtype=np.complex128)
+    lib.oracle_poly2pw(n, ipoly, npw, _p(np.ascontiguousarray(ws)), _p(np.ascontiguousarray(ts)), delta, boxdim,
+                       out.ctypes.data_as(_dp))
+    return out.T.copy()
+
+
+def pw2pgh(n, ts, xs, delta, boxdim):
+    """E[j, i] tables (npw x n) for pot, d/dx, d2/dx2."""
+    lib = load_oracle()
+    npw = len(ts)
+    outs = [np.zeros((n, npw), dtype=np.complex128) for _ in range(3)]
+    lib.oracle_pw2pgh(n, npw, _p(np.ascontiguousarray(ts)), _p(np.ascontiguousarray(xs)), delta, boxdim,
+                      *[o.ctypes.data_as(_dp) for o in outs])
+    return [o.T.copy() for o in outs]
+
+
+def loctab_level(eps, ipoly, n, delta, boxdim):
+    """Near-field tables of one target level: arrays (13, n_src, n_targ) for k=-6..6 and
+    ind (13, n+1, 2) 1-based sparsity bounds."""
+    lib = load_oracle()
+    t = [np.zeros((13, n, n)) for _ in range(3)]
+    ind = np.zeros((13, n + 1, 2), dtype=np.int32)
+    lib.oracle_loctab_level(eps, ipoly, n, delta, boxdim, *[_p(a) for a in t], _p(ind, _ip))
+    # stored column-major per offset: element (m, j) at m + n*j -> C array [k][j][m]
+    return [a.transpose(0, 2, 1).copy() for a in t] + [ind]
