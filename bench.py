@@ -1,0 +1,240 @@
+"""Benchmark of the MI355X box FGT: leaf points per second of one boxfgt call.
+
+One "step" is one evaluation of the resident plan (leaf values -> PW ->
+translation -> pot/grad -> near field) over the workload's leaf values,
+already in HBM.  The default workload is the adaptive configuration the
+metric is quoted on: a periodic adaptive quadtree of ~20k leaves (levels
+7-10, n=16 Legendre, ~5.1M points), nd=1, delta=1e-4, eps=1e-9, pot+grad.
+
+Prints ONE JSON line.  ``value`` = nleaf*npbox*nd / (time per step).
+``roofline`` describes the kernel with the largest share of the step, its
+algorithmic FLOPs per launch divided by its average duration measured with
+HIP events on the plan's stream over the timed steps.  ``cpu_baseline``
+times the reference Fortran boxfgt (compiled from its sources into
+oracle/_ref) -- or, where that library is absent, the C oracle -- on the
+host cores, on the same tree and density.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FP64_PEAK_TFLOPS = 78.6  # MI355X dense FP64 (vector and matrix)
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_work(plan, w):
+    """Per-launch algorithmic FLOPs (FP64) or bytes of each stage, from the plan's
+    schedule: leaf transforms 4n^2H + 8nPH per PW leaf, evaluation
+    8*ntab*nPH + 4c n^2 H (c = 1, 3, 6 products for pot, pg, pgh), near field
+    {4, 10, 18} n^3 per source/target pair; the merge and translation stages
+    move expansions (16 B per complex coefficient)."""
+    info = plan.info
+    n, nd = w.norder, w.nd
+    P, nexp = info["npw"], info["nexp"]
+    H = (P + 1) // 2
+    L = info["n_pw_leaves"]
+    ntab = w.ifpgh
+    c = {1: 1, 2: 3, 3: 6}[w.ifpgh]
+    up = nd * L * (4 * n * n * H + 8 * n * P * H)
+    down = nd * L * (8 * ntab * n * P * H + 4 * c * n * n * H)
+    near = nd * info["n_near_pairs"] * {1: 4, 2: 10, 3: 18}[w.ifpgh] * n**3
+    merge_bytes = 16 * nd * nexp * (info["n_chunks"] + info["n_ancestors"])
+    return {"leaf_to_pw": ("flop", up), "merge": ("byte", merge_bytes), "translate": ("byte", None),
+            "pw_to_values": ("flop", down), "near_field": ("flop", near)}
+
+
+def cpu_baseline(w, gpu_out, max_threads):
+    """Time the reference Fortran boxfgt (or the C oracle) on the host cores."""
+    from oracle import oracle as O
+
+    nthreads = max(1, min(os.cpu_count() or 1, max_threads))
+    if O.reference_available():
+        kind = "reference"
+        run = lambda: O.reference_boxfgt(w.tree, w.fvals, w.nd, w.delta, w.eps, w.ipoly, w.iperiod, w.norder,  # noqa: E731
+                                         w.ifpgh, nthreads=nthreads)
+    else:
+        kind = "port"
+        nthreads = 1
+        run = lambda: O.oracle_boxfgt(w.tree, w.fvals, w.nd, w.delta, w.eps, w.ipoly, w.iperiod, w.norder,  # noqa: E731
+                                      w.ifpgh)
+    r = run()  # warm-up
+    times = []
+    t_budget = time.time() + 20.0
+    for _ in range(5):
+        t0 = time.perf_counter()
+        r = run()
+        times.append(time.perf_counter() - t0)
+        if time.time() > t_budget:
+            break
+    tmed = float(np.median(times))
+    lv = w.tree.nchild == 0
+    err = {}
+    for name, g, ref in (("pot", gpu_out.get("pot"), r.pot), ("grad", gpu_out.get("grad"), r.grad),
+                         ("hess", gpu_out.get("hess"), r.hess)):
+        if g is None:
+            continue
+        den = np.linalg.norm(ref[lv])
+        err[name] = float(np.linalg.norm((g - ref)[lv]) / den) if den > 0 else 0.0
+    return {
+        "value": w.npts * w.nd / tmed,
+        "unit": "leaf pts/s",
+        "cores": nthreads,
+        "kind": kind,
+        "sample": f"one full boxfgt call on the same workload ({w.tree.nleaves} leaves, {w.npts} pts), "
+                  f"median of {len(times)} after one warm-up; {'amdflang -O3 -fopenmp reference' if kind == 'reference' else 'C oracle, 1 thread'}",
+        "seconds_per_call": tmed,
+        "gpu_vs_cpu_rel_err": err,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5"])
+    ap.add_argument("--scale", default="full", choices=["full", "small", "tiny"])
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "split32"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="run a device copy of this size first (calibrates rocprof byte counters)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the box FGT here has no multi-GPU path)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from heat2d_per_box_amd.boxfgt import BoxFGT
+    from heat2d_per_box_amd.densities import make_workload
+
+    t0 = time.time()
+    w = make_workload(args.config, args.scale)
+    t_workload = time.time() - t0
+    print(f"[bench] workload {args.config}/{args.scale}: {w.tree.nleaves} leaves, {w.npts} pts, built in "
+          f"{t_workload:.1f}s", file=sys.stderr, flush=True)
+    dev = torch.device("cuda:0")
+    if args.copy_probe_mib:
+        a = torch.empty(args.copy_probe_mib * (1 << 20) // 8, dtype=torch.float64, device=dev).fill_(1.0)
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
+    tree = w.tree.copy()
+    t0 = time.time()
+    plan = BoxFGT(tree, w.nd, w.delta, w.eps, w.ipoly, w.norder, w.ifpgh, precision=args.precision)
+    torch.cuda.synchronize()
+    t_plan = time.time() - t0
+    nb, npb, nd = w.fvals.shape
+    fv = torch.from_numpy(w.fvals).to(dev)
+    pot = torch.zeros((nb, npb, nd), dtype=torch.float64, device=dev)
+    grad = torch.zeros((nb, npb, 2, nd), dtype=torch.float64, device=dev) if w.ifpgh >= 2 else None
+    hess = torch.zeros((nb, npb, 3, nd), dtype=torch.float64, device=dev) if w.ifpgh >= 3 else None
+
+    # one checked call for the error columns
+    plan(fv, pot, grad, hess)
+    torch.cuda.synchronize()
+    gpu_out = {"pot": pot.cpu().numpy(), "grad": grad.cpu().numpy() if grad is not None else None,
+               "hess": hess.cpu().numpy() if hess is not None else None}
+    for _ in range(args.warmup):
+        plan(fv, pot, grad, hess)
+    torch.cuda.synchronize()
+    print(f"[bench] plan {t_plan:.2f}s, warm-up done; timing {args.steps} steps", file=sys.stderr, flush=True)
+
+    stream = torch.cuda.current_stream()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    plan.profile_begin(args.steps)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for _ in range(args.steps):
+        plan(fv, pot, grad, hess)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    stage_sum, nprof = plan.profile_end()
+    ms = e0.elapsed_time(e1) / args.steps
+    npts = w.npts * w.nd
+    value = npts / (ms * 1e-3)
+
+    names = ["leaf_to_pw", "merge", "translate", "split", "pw_to_values", "near_field"]
+    stage_ms = {k: 1e3 * v / max(nprof, 1) for k, v in zip(names, stage_sum)}
+    work = algorithmic_work(plan, w)
+    dom = max((k for k in names if k in work and work[k][1]), key=lambda k: stage_ms[k])
+    kind, amount = work[dom]
+    dur = stage_ms[dom] * 1e-3
+    if kind == "flop":
+        roof = {"bound": "mfma", "achieved": amount / dur / 1e12, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s"}
+    else:
+        roof = {"bound": "hbm", "achieved": amount / dur / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s"}
+    roof["frac"] = roof["achieved"] / roof["peak"]
+    roof["traffic"] = None
+    roof["kernel"] = {"leaf_to_pw": "k_up", "merge": "k_reduce", "translate": "k_m2l", "pw_to_values": "k_down",
+                      "near_field": "k_near"}[dom]
+    roof["algorithmic_per_launch"] = amount
+    roof["avg_launch_ms"] = stage_ms[dom]
+
+    # cold call: the host-array drop-in (plan + H2D + kernels + D2H)
+    from heat2d_per_box_amd.boxfgt import boxfgt as boxfgt_host
+
+    t2 = w.tree.copy()
+    hp = np.zeros((nb, npb, nd))
+    hg = np.zeros((nb, npb, 2, nd)) if w.ifpgh >= 2 else None
+    hh = np.zeros((nb, npb, 3, nd)) if w.ifpgh >= 3 else None
+    tc0 = time.perf_counter()
+    boxfgt_host(nd, 2, w.delta, w.eps, w.ipoly, w.iperiod, w.norder, npb, nb, t2.nlevels, t2.ltree, t2.itree,
+                t2.iptr, t2.centers, t2.boxsize, w.fvals, w.ifpgh, hp, hg, hh, precision=args.precision)
+    t_cold = time.perf_counter() - tc0
+
+    out = {
+        "metric": "boxfgt leaf pts/s at eps=1e-9: split-FP32 vs FP64 GPU; adjoint/forward time",
+        "value": value,
+        "unit": "leaf pts/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32x2+f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{args.config}/{args.scale}: {w.description}",
+            "leaves": int(w.tree.nleaves),
+            "points": int(w.npts),
+            "nd": nd,
+            "delta": w.delta,
+            "eps": w.eps,
+            "ifpgh": w.ifpgh,
+            "precision": args.precision,
+            "npwlevel": plan.info["npwlevel"],
+            "npw": plan.info["npw"],
+            "near_pairs": plan.info["n_near_pairs"],
+        },
+        "stage_ms": stage_ms,
+        "t_cold_s": t_cold,
+        "cold_pts_per_s": npts / t_cold,
+        "t_plan_s": t_plan,
+        "roofline": roof,
+    }
+    if not args.no_cpu_baseline:
+        print("[bench] timing the CPU baseline", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(w, gpu_out, args.cpu_threads)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

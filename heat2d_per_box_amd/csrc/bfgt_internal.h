@@ -1,0 +1,127 @@
+// Internal declarations of the MI355X box FGT: the plan object, device
+// buffers, and the kernel launch wrappers shared by the translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/boxfgt_hip.h"
+
+namespace bfgt {
+
+constexpr int kMaxOrder = 32;  // largest leaf order n the kernels accept
+constexpr int kMaxNpw = 200;   // plane-wave length limit (the reference's ws/ts(200))
+constexpr int kNq = 50;        // Gauss-Legendre nodes of the near-field quadrature branch
+
+struct cplx {
+    double x, y;
+};
+
+// One PW leaf: the leaf box, its level, its ancestor at the cutoff level and
+// the scaled offset (c_leaf - c_anc)/sqrt(delta0).
+struct PwLeaf {
+    int box;
+    int level;
+    int anc;
+    int pad;
+    double dx, dy;
+};
+
+// A group of PW leaves sharing one ancestor; partial sums go to slot `slot`.
+struct UpChunk {
+    int anc;
+    int first;  // into the PwLeaf array
+    int count;
+    int slot;
+};
+
+// One near-field interaction: source box and table offsets (k = -6..6).
+struct NearPair {
+    int src;
+    int ix;
+    int iy;
+    int pad;
+};
+
+struct Plan {
+    // parameters
+    int nd = 1, ipoly = 0, iperiod_in = 1, iperiod_eff = 1, n = 8, npbox = 64, ifpgh = 1;
+    int precision = BFGT_PREC_FP64;
+    double delta = 0, eps = 0, delta0 = 0, pmax = 0;
+    int nlevels = 0, nboxes = 0, nboxes_cap = 0;
+    int npwlevel = 0, npw = 0, H = 0, nexp = 0, nlevstart = -1, nlevend = -1;
+    bool has_pw = false, has_near = false;
+    int nleafbox = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    int device = 0;
+
+    // host copies used for scheduling
+    std::vector<double> boxsize, ts, ws;
+
+    // device tables
+    double* d_ts = nullptr;       // npw
+    double* d_ws = nullptr;       // npw
+    double* d_boxsize = nullptr;  // nlevels+1
+    double* d_nodes = nullptr;    // node rules and U matrices (see tables)
+    cplx* d_tpoly = nullptr;      // [level][k][m]
+    cplx* d_tpw = nullptr;        // [level][type][k][i]
+    double* d_tloc = nullptr;     // [level][13][type][j][m], masked
+    int* d_indloc = nullptr;      // [level][13][n+1][2]
+    cplx* d_wshift = nullptr;     // [9][nexp]
+    std::vector<int> h_indloc;    // copy of d_indloc
+
+    // schedules
+    int npwleaf = 0, nchunks = 0, nanc = 0, nm2l = 0;
+    PwLeaf* d_pwleaf = nullptr;
+    UpChunk* d_chunks = nullptr;
+    int* d_anc_box = nullptr;      // ancestor index -> box
+    int* d_anc_chunk = nullptr;    // CSR over chunks [nanc+1]
+    int* d_m2l_ptr = nullptr;      // CSR [nanc+1]
+    int2* d_m2l = nullptr;         // (source ancestor, shift index)
+    int nnear_t = 0, nnear_p = 0;
+    int* d_near_tgt = nullptr;     // target boxes
+    int* d_near_lev = nullptr;     // target level
+    int* d_near_ptr = nullptr;     // CSR [nnear_t+1]
+    NearPair* d_near = nullptr;
+
+    // work buffers
+    cplx* d_partial = nullptr;  // [nchunks][nd][nexp]
+    cplx* d_mp = nullptr;       // [nanc][nd][nexp]
+    cplx* d_loc = nullptr;      // [nanc][nd][nexp]
+
+    // timing
+    hipEvent_t ev[10] = {};
+    std::vector<hipEvent_t> prof_ev;  // 6 events per profiled execution
+    int prof_max = 0, prof_count = 0;
+    bool profiling = false;
+    double t_plan_host = 0;
+    double t_tables = 0;
+
+    std::vector<void*> allocations;
+};
+
+// status helpers
+int set_error(int code, const std::string& msg);
+
+// tables (bfgt_tables.hip)
+int build_tables(Plan& p);
+
+// hot path (bfgt_pw.hip, bfgt_near.hip)
+int launch_upward(Plan& p, const double* fvals);
+int launch_reduce(Plan& p);
+int launch_m2l(Plan& p);
+int launch_downward(Plan& p, double* pot, double* grad, double* hess);
+int launch_near(Plan& p, const double* fvals, double* pot, double* grad, double* hess);
+
+}  // namespace bfgt
+
+#define BFGT_HIP_CHECK(expr)                                                              \
+    do {                                                                                  \
+        hipError_t _e = (expr);                                                           \
+        if (_e != hipSuccess)                                                             \
+            return ::bfgt::set_error(BFGT_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e)); \
+    } while (0)

@@ -1,0 +1,779 @@
+// Host side of the MI355X box FGT: argument checks, the tree lists
+// (colleagues, PW flags, modified list 1, PW list), the work schedules that
+// the kernels consume, and the C ABI declared in include/boxfgt_hip.h.
+//
+// The lists restate the reference's tree routines (common/tree_routs_nd.f):
+// computecoll :106-242, gnd_find_pwexp_boxes :1266-1368,
+// gnd_compute_modified_list1 :864-1055, gnd_compute_listpw :1564-1630; the
+// cutoff-level and PW-length choice restates bfgt/boxfgt.f:133-167 and
+// common/fgtterms.f:72-141.  They are O(nboxes) integer work done once per
+// plan; every floating-point table and every transform runs on the GPU.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "bfgt_internal.h"
+
+struct bfgt_plan {
+    bfgt::Plan p;
+};
+
+namespace bfgt {
+
+thread_local std::string g_err;
+
+int set_error(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+size_t node_buffer_doubles(int n);
+size_t up_smem_bytes(int n, int npw);
+size_t down_smem_bytes(int n, int npw, int ntab);
+size_t near_smem_bytes(int n, int ntab);
+int build_pw_nodes(Plan& p);
+
+namespace {
+
+constexpr int kChunkLeaves = 8;
+
+struct TreeView {
+    int nlevels = 0, nboxes = 0;
+    int *laddr, *ilevel, *iparent, *nchild, *ichild, *ncoll, *coll;
+    const double *centers, *boxsize;
+    int first(int l) const { return laddr[2 * l] - 1; }
+    int last(int l) const { return laddr[2 * l + 1] - 1; }
+};
+
+double per_dist(double d, int iperiod, double bs0) {
+    d = std::fabs(d);
+    if (iperiod == 1) d = std::min(d, bs0 - d);
+    return d;
+}
+
+int check_tree(int nboxes_cap, int nlevels, int ltree, int* itree, const int* iptr, const double* centers,
+               const double* boxsize, TreeView& t) {
+    if (nlevels < 0 || ltree < 2 * (nlevels + 1)) return set_error(BFGT_ERR_TREE, "ltree too small for laddr");
+    for (int k = 0; k < 7; k++)
+        if (iptr[k] < 1 || iptr[k] > ltree) return set_error(BFGT_ERR_TREE, "iptr out of range");
+    t.nlevels = nlevels;
+    t.laddr = itree + iptr[0] - 1;
+    t.ilevel = itree + iptr[1] - 1;
+    t.iparent = itree + iptr[2] - 1;
+    t.nchild = itree + iptr[3] - 1;
+    t.ichild = itree + iptr[4] - 1;
+    t.ncoll = itree + iptr[5] - 1;
+    t.coll = itree + iptr[6] - 1;
+    t.centers = centers;
+    t.boxsize = boxsize;
+    t.nboxes = itree[2 * nlevels + 1];
+    if (t.nboxes < 1 || t.nboxes > nboxes_cap) return set_error(BFGT_ERR_TREE, "box count exceeds nboxes");
+    if ((long)iptr[6] - 1 + 9L * t.nboxes > ltree) return set_error(BFGT_ERR_TREE, "itree shorter than its lists");
+    int prev = 0;
+    for (int l = 0; l <= nlevels; l++) {
+        if (t.laddr[2 * l] != prev + 1 || t.laddr[2 * l + 1] < t.laddr[2 * l] - 1)
+            return set_error(BFGT_ERR_TREE, "laddr is not a level-sorted partition");
+        prev = t.laddr[2 * l + 1];
+    }
+    for (int l = 0; l <= nlevels; l++)
+        for (int b = t.first(l); b <= t.last(l); b++) {
+            if (t.ilevel[b] != l) return set_error(BFGT_ERR_TREE, "ilevel disagrees with laddr");
+            int nc = t.nchild[b];
+            if (nc != 0 && nc != 4) return set_error(BFGT_ERR_TREE, "nchild must be 0 or 4");
+            for (int c = 0; c < nc; c++) {
+                int k = t.ichild[4 * b + c];
+                if (k < 1 || k > t.nboxes || t.ilevel[k - 1] != l + 1)
+                    return set_error(BFGT_ERR_TREE, "child id out of range");
+            }
+            if (l > 0) {
+                int dad = t.iparent[b];
+                if (dad < 1 || dad > t.nboxes) return set_error(BFGT_ERR_TREE, "parent id out of range");
+            }
+        }
+    for (int l = 1; l <= nlevels; l++)
+        if (!(boxsize[l] > 0)) return set_error(BFGT_ERR_TREE, "boxsize must be positive");
+    return BFGT_OK;
+}
+
+// computecoll (common/tree_routs_nd.f:106-242)
+void computecoll(TreeView& t, int iperiod) {
+    double bs0 = t.boxsize[0];
+    for (int i = 0; i < t.nboxes; i++) {
+        t.ncoll[i] = 0;
+        for (int j = 0; j < 9; j++) t.coll[9 * i + j] = -1;
+    }
+    t.ncoll[0] = 1;
+    t.coll[0] = 1;
+    for (int l = 1; l <= t.nlevels; l++)
+        for (int b = t.first(l); b <= t.last(l); b++) {
+            int dad = t.iparent[b] - 1;
+            for (int i = 0; i < t.ncoll[dad]; i++) {
+                int jb = t.coll[9 * dad + i] - 1;
+                for (int c = 0; c < 4; c++) {
+                    int kb = t.ichild[4 * jb + c];
+                    if (kb <= 0) continue;
+                    kb -= 1;
+                    bool nb = true;
+                    for (int d = 0; d < 2 && nb; d++)
+                        nb = per_dist(t.centers[2 * kb + d] - t.centers[2 * b + d], iperiod, bs0) <=
+                             1.05 * t.boxsize[l];
+                    if (nb && t.ncoll[b] < 9) t.coll[9 * b + t.ncoll[b]++] = kb + 1;
+                }
+            }
+        }
+}
+
+// gnd_find_pwexp_boxes (common/tree_routs_nd.f:1266-1368)
+std::vector<char> pwexp_flags(const TreeView& t, int npwlevel) {
+    std::vector<char> f(t.nboxes, 0);
+    if (npwlevel <= 0) f[0] = 1;
+    if (npwlevel > t.nlevels) return f;
+    int nc = std::max(npwlevel, 0);
+    for (int b = t.first(nc); b <= t.last(nc); b++) {
+        if (t.nchild[b] == 0) {
+            for (int j = 0; j < t.ncoll[b]; j++)
+                if (t.nchild[t.coll[9 * b + j] - 1] > 0) {
+                    f[b] = 1;
+                    break;
+                }
+        } else {
+            f[b] = 1;
+        }
+    }
+    for (int l = nc + 1; l <= t.nlevels; l++)
+        for (int b = t.first(l); b <= t.last(l); b++) f[b] = 1;
+    return f;
+}
+
+// gnd_compute_modified_list1 (common/tree_routs_nd.f:864-1055), 0-based ids
+std::vector<std::vector<int>> modified_list1(const TreeView& t, int npwlevel, const std::vector<char>& pwexp,
+                                             int iperiod) {
+    std::vector<std::vector<int>> L(t.nboxes);
+    double bs0 = t.boxsize[0];
+    if (t.nchild[0] == 0) L[0].push_back(0);
+    int nlevend = std::min(t.nlevels, npwlevel);
+    auto close = [&](int a, int b, double dt) {
+        for (int d = 0; d < 2; d++)
+            if (per_dist(t.centers[2 * a + d] - t.centers[2 * b + d], iperiod, bs0) >= dt) return false;
+        return true;
+    };
+    for (int l = 1; l <= nlevend; l++)
+        for (int b = t.first(l); b <= t.last(l); b++) {
+            if (t.nchild[b] != 0) continue;
+            int dad = t.iparent[b] - 1;
+            for (int i = 0; i < t.ncoll[b]; i++) {
+                int jb = t.coll[9 * b + i] - 1;
+                if (t.nchild[jb] == 0) {
+                    if (pwexp[b] != 1 || jb != b) L[b].push_back(jb);
+                } else if (l < npwlevel) {
+                    double dt = 1.05 * (t.boxsize[l] + t.boxsize[l + 1]) / 2.0;
+                    for (int c = 0; c < 4; c++) {
+                        int kb = t.ichild[4 * jb + c] - 1;
+                        if (t.nchild[kb] == 0 && close(kb, b, dt)) L[b].push_back(kb);
+                    }
+                }
+            }
+            double dt = 1.05 * (t.boxsize[l] + t.boxsize[l - 1]) / 2.0;
+            for (int i = 0; i < t.ncoll[dad]; i++) {
+                int jb = t.coll[9 * dad + i] - 1;
+                if (t.nchild[jb] == 0 && close(jb, b, dt)) L[b].push_back(jb);
+            }
+        }
+    return L;
+}
+
+// index of the mp->loc shift (gnd_find_pwshift_ind, bfgt/bfgt_pwrouts.f:1848-1892)
+int pwshift_index(int iperiod, const double* tc, const double* sc, double bs0, double xmin) {
+    int jx[2];
+    for (int i = 0; i < 2; i++) {
+        long j = std::lround((tc[i] - sc[i]) / xmin);
+        if (iperiod == 1) {
+            long jp = std::lround((tc[i] - sc[i] - bs0) / xmin);
+            long jm = std::lround((tc[i] - sc[i] + bs0) / xmin);
+            if (std::labs(j) > std::labs(jp)) j = jp;
+            if (std::labs(j) > std::labs(jm)) j = jm;
+        }
+        jx[i] = (int)j;
+    }
+    if (std::abs(jx[0]) > 1 || std::abs(jx[1]) > 1) return -1;
+    return (jx[0] + 1) + 3 * (jx[1] + 1);
+}
+
+// near-field table offsets (gnd_find_loctab_ind, bfgt/bfgt_local.f:1729-1765)
+void loctab_index(int iperiod, const double* tc, const double* sc, double sbs, double bs0, int* ixy) {
+    double bs = sbs / 4;
+    for (int i = 0; i < 2; i++) {
+        double dx = (tc[i] - sc[i]) / bs;
+        if (iperiod == 1) {
+            double dp = dx - bs0 / bs, dm = dx + bs0 / bs;
+            if (std::fabs(dx) > std::fabs(dp)) dx = dp;
+            if (std::fabs(dx) > std::fabs(dm)) dx = dm;
+        }
+        ixy[i] = (int)dx;
+    }
+}
+
+template <class T>
+int upload(Plan& p, T*& dst, const std::vector<T>& src) {
+    dst = nullptr;
+    if (src.empty()) return BFGT_OK;
+    BFGT_HIP_CHECK(hipMalloc(&dst, sizeof(T) * src.size()));
+    p.allocations.push_back(dst);
+    BFGT_HIP_CHECK(hipMemcpyAsync(dst, src.data(), sizeof(T) * src.size(), hipMemcpyHostToDevice, p.stream));
+    return BFGT_OK;
+}
+
+template <class T>
+int dalloc(Plan& p, T*& dst, size_t count) {
+    dst = nullptr;
+    if (count == 0) return BFGT_OK;
+    BFGT_HIP_CHECK(hipMalloc(&dst, sizeof(T) * count));
+    p.allocations.push_back(dst);
+    return BFGT_OK;
+}
+
+void release(Plan& p) {
+    if (p.stream) hipStreamSynchronize(p.stream);
+    for (void* a : p.allocations) hipFree(a);
+    p.allocations.clear();
+    for (auto& e : p.ev)
+        if (e) {
+            hipEventDestroy(e);
+            e = nullptr;
+        }
+    for (auto& e : p.prof_ev) hipEventDestroy(e);
+    p.prof_ev.clear();
+    if (p.own_stream && p.stream) hipStreamDestroy(p.stream);
+    p.stream = nullptr;
+}
+
+}  // namespace
+
+// fgtpwterms (common/fgtterms.f:72-141)
+static void pw_terms(double bsize, double delta, double eps, int iperiod, double* pmax, int* npw) {
+    if (iperiod == 0) {
+        double rmax = bsize / std::sqrt(delta);
+        double d = std::sqrt(std::log(1.0 / eps));
+        *pmax = 2.0 * d;
+        double h = 2 * M_PI / (rmax + d);
+        *npw = 2 * ((int)(*pmax / h) + 1);
+    } else {
+        *pmax = 0.0;
+        *npw = 2 * ((int)(bsize * std::sqrt(std::log(1.0 / eps) / delta) / M_PI) + 1) + 1;
+    }
+}
+
+int plan_create(Plan& p, int nd, int ndim, double delta, double eps, int ipoly, int iperiod, int norder, int nboxes,
+                int nlevels, int ltree, int* itree, const int* iptr, const double* centers, const double* boxsize,
+                int ifpgh, int precision, void* stream) {
+    auto t0 = std::chrono::steady_clock::now();
+    if (ndim != 2) return set_error(BFGT_ERR_NDIM, "only ndim=2 is supported");
+    if (ipoly != 0 && ipoly != 1) return set_error(BFGT_ERR_IPOLY, "ipoly must be 0 or 1");
+    if (ifpgh < 1 || ifpgh > 3) return set_error(BFGT_ERR_IFPGH, "ifpgh must be 1, 2 or 3");
+    if (norder < 1 || norder > kMaxOrder) return set_error(BFGT_ERR_NORDER, "norder must be in 1..32");
+    if (nd < 1 || !(delta > 0) || !(eps > 0) || eps >= 1 || !itree || !iptr || !centers || !boxsize)
+        return set_error(BFGT_ERR_ARG, "invalid scalar or null array argument");
+    if (iperiod != 0 && iperiod != 1) return set_error(BFGT_ERR_ARG, "iperiod must be 0 or 1");
+    if (precision != BFGT_PREC_FP64 && precision != BFGT_PREC_SPLIT32)
+        return set_error(BFGT_ERR_ARG, "unknown precision");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return set_error(BFGT_ERR_NODEVICE, "no GPU visible");
+    TreeView t;
+    int rc = check_tree(nboxes, nlevels, ltree, itree, iptr, centers, boxsize, t);
+    if (rc) return rc;
+
+    p.nd = nd;
+    p.ipoly = ipoly;
+    p.iperiod_in = iperiod;
+    p.n = norder;
+    p.npbox = norder * norder;
+    p.ifpgh = ifpgh;
+    p.precision = precision;
+    p.delta = delta;
+    p.eps = eps;
+    p.nlevels = nlevels;
+    p.nboxes = t.nboxes;
+    p.nboxes_cap = nboxes;
+    p.boxsize.assign(boxsize, boxsize + nlevels + 1);
+    BFGT_HIP_CHECK(hipGetDevice(&p.device));
+    if (stream) {
+        p.stream = (hipStream_t)stream;
+    } else {
+        BFGT_HIP_CHECK(hipStreamCreateWithFlags(&p.stream, hipStreamNonBlocking));
+        p.own_stream = true;
+    }
+    for (auto& e : p.ev) BFGT_HIP_CHECK(hipEventCreate(&e));
+
+    // cutoff level and PW length (bfgt/boxfgt.f:133-167)
+    double dcut = std::sqrt(delta * std::log(1.0 / eps));
+    int npwlevel = nlevels + 1;
+    for (int i = nlevels; i >= 0; i--)
+        if (boxsize[i] >= dcut) {
+            npwlevel = i;
+            break;
+        }
+    if (boxsize[nlevels] > dcut) npwlevel = nlevels + 1;
+    if (boxsize[0] <= dcut) npwlevel = 0;
+    if (npwlevel <= 1 && iperiod == 1) npwlevel = 0;
+    int npw = 0;
+    double pmax = 0;
+    if (npwlevel <= nlevels) {
+        if (npwlevel <= 0) pw_terms(boxsize[0], delta, eps, iperiod, &pmax, &npw);
+        else pw_terms(2 * boxsize[npwlevel], delta, eps, 0, &pmax, &npw);
+    }
+    if (npw > kMaxNpw) return set_error(BFGT_ERR_NPW, "plane-wave length exceeds 200");
+    p.pmax = pmax;
+    p.npw = npw;
+    p.H = (npw + 1) / 2;
+    p.nexp = npw * p.H;
+    // Fourier-series regime (bfgt/boxfgt.f:492-523)
+    bool fourier = (npwlevel <= 1 && iperiod == 1);
+    p.iperiod_eff = fourier ? 0 : iperiod;
+    if (fourier) npwlevel = 0;
+    p.delta0 = fourier ? 1.0 : delta;
+    p.npwlevel = npwlevel;
+    computecoll(t, p.iperiod_eff);
+    p.nlevend = nlevels;
+    if (npwlevel < nlevels) p.nlevend = npwlevel;
+    if (npwlevel == 0 && iperiod == 1) p.nlevend = -1;
+    p.nleafbox = 0;
+    for (int l = 1; l <= nlevels; l++)
+        for (int b = t.first(l); b <= t.last(l); b++)
+            if (t.nchild[b] == 0) p.nleafbox++;
+    p.has_pw = !(npwlevel >= nlevels && npwlevel > 0) && npw > 0;
+    p.nlevstart = p.has_pw ? std::max(npwlevel, 0) : -1;
+    std::vector<char> pwexp = pwexp_flags(t, npwlevel);
+    auto list1 = modified_list1(t, npwlevel, pwexp, p.iperiod_eff);
+
+    // resource checks for the kernels
+    hipDeviceProp_t prop;
+    BFGT_HIP_CHECK(hipGetDeviceProperties(&prop, p.device));
+    size_t lds_max = prop.sharedMemPerBlock;
+    if (p.has_pw && (up_smem_bytes(p.n, npw) > lds_max || down_smem_bytes(p.n, npw, ifpgh) > lds_max ||
+                     p.nexp > 2048))
+        return set_error(BFGT_ERR_RESOURCE, "plane-wave expansion too large for the leaf kernels");
+    if (near_smem_bytes(p.n, ifpgh) > lds_max || 9 * p.n * p.n * sizeof(double) > lds_max)
+        return set_error(BFGT_ERR_RESOURCE, "leaf order too large for the near-field kernels");
+
+    // device tables
+    BFGT_HIP_CHECK(hipSetDevice(p.device));
+    rc = upload(p, p.d_boxsize, p.boxsize);
+    if (rc) return rc;
+    if ((rc = dalloc(p, p.d_nodes, node_buffer_doubles(p.n)))) return rc;
+    size_t nn = (size_t)p.n * p.n;
+    if ((rc = dalloc(p, p.d_tloc, (size_t)(nlevels + 1) * 13 * 3 * nn))) return rc;
+    if ((rc = dalloc(p, p.d_indloc, (size_t)(nlevels + 1) * 13 * 2 * (p.n + 1)))) return rc;
+    BFGT_HIP_CHECK(hipMemsetAsync(p.d_tloc, 0, sizeof(double) * (nlevels + 1) * 13 * 3 * nn, p.stream));
+    BFGT_HIP_CHECK(hipMemsetAsync(p.d_indloc, 0, sizeof(int) * (nlevels + 1) * 13 * 2 * (p.n + 1), p.stream));
+    if (p.has_pw) {
+        if ((rc = dalloc(p, p.d_ts, npw))) return rc;
+        if ((rc = dalloc(p, p.d_ws, npw))) return rc;
+        if ((rc = dalloc(p, p.d_tpoly, (size_t)(nlevels + 1) * npw * p.n))) return rc;
+        if ((rc = dalloc(p, p.d_tpw, (size_t)(nlevels + 1) * 3 * npw * p.n))) return rc;
+        if ((rc = dalloc(p, p.d_wshift, (size_t)9 * p.nexp))) return rc;
+        if ((rc = build_pw_nodes(p))) return rc;
+    }
+    if ((rc = build_tables(p))) return rc;
+
+    // PW schedule: leaves grouped by their cutoff-level ancestor
+    std::vector<PwLeaf> leaves;
+    std::vector<UpChunk> chunks;
+    std::vector<int> anc_box, anc_chunk;
+    std::vector<int> m2l_ptr;
+    std::vector<int2> m2l;
+    if (p.has_pw) {
+        int L0 = p.nlevstart;
+        int a0 = t.first(L0);
+        p.nanc = t.last(L0) - a0 + 1;
+        std::vector<std::vector<PwLeaf>> by_anc(p.nanc);
+        double sq = std::sqrt(p.delta0);
+        for (int l = L0; l <= nlevels; l++)
+            for (int b = t.first(l); b <= t.last(l); b++) {
+                if (!(pwexp[b] == 1 && t.nchild[b] == 0)) continue;
+                int a = b;
+                while (t.ilevel[a] > L0) a = t.iparent[a] - 1;
+                PwLeaf lf;
+                lf.box = b;
+                lf.level = l;
+                lf.anc = a - a0;
+                lf.pad = 0;
+                lf.dx = (centers[2 * b] - centers[2 * a]) / sq;
+                lf.dy = (centers[2 * b + 1] - centers[2 * a + 1]) / sq;
+                by_anc[a - a0].push_back(lf);
+            }
+        anc_chunk.push_back(0);
+        for (int a = 0; a < p.nanc; a++) {
+            auto& v = by_anc[a];
+            std::stable_sort(v.begin(), v.end(), [](const PwLeaf& x, const PwLeaf& y) { return x.level < y.level; });
+            for (size_t s = 0; s < v.size(); s += kChunkLeaves) {
+                UpChunk c;
+                c.anc = a;
+                c.first = (int)(leaves.size() + s);
+                c.count = (int)std::min<size_t>(kChunkLeaves, v.size() - s);
+                c.slot = (int)chunks.size();
+                chunks.push_back(c);
+            }
+            leaves.insert(leaves.end(), v.begin(), v.end());
+            anc_chunk.push_back((int)chunks.size());
+            anc_box.push_back(a0 + a);
+        }
+        // PW translation lists at the cutoff level (gnd_compute_listpw)
+        double xmin = boxsize[L0];
+        m2l_ptr.push_back(0);
+        for (int a = 0; a < p.nanc; a++) {
+            int b = a0 + a;
+            for (int i = 0; i < t.ncoll[b]; i++) {
+                int jb = t.coll[9 * b + i] - 1;
+                if (jb == b || !(t.nchild[jb] > 0 || t.nchild[b] > 0)) continue;
+                int k = pwshift_index(p.iperiod_eff, centers + 2 * b, centers + 2 * jb, boxsize[0], xmin);
+                if (k < 0) return set_error(BFGT_ERR_TREE, "PW-list partner is not adjacent");
+                m2l.push_back(make_int2(jb - a0, k));
+            }
+            m2l_ptr.push_back((int)m2l.size());
+        }
+    }
+    p.npwleaf = (int)leaves.size();
+    p.nchunks = (int)chunks.size();
+    p.nm2l = (int)m2l.size();
+
+    // near-field schedule (bfgt/boxfgt.f:923-1071), pairs with empty tables dropped
+    std::vector<int> ntgt, nlev, nptr;
+    std::vector<NearPair> pairs;
+    const int ni = 2 * (p.n + 1);
+    auto rows_empty = [&](int lev, int off) {
+        const int* ind = p.h_indloc.data() + ((size_t)lev * 13 + (off + 6)) * ni;
+        return ind[2 * p.n + 1] - ind[2 * p.n] + 1 == 0;
+    };
+    bool tree_ok = true;
+    auto add_pair = [&](int lev, int src, const int* ixy) {
+        if (std::abs(ixy[0]) > 6 || std::abs(ixy[1]) > 6) {
+            tree_ok = false;
+            return;
+        }
+        if (rows_empty(lev, ixy[0]) || rows_empty(lev, ixy[1])) return;
+        pairs.push_back(NearPair{src, ixy[0], ixy[1], 0});
+    };
+    if (!(npwlevel == 0 && p.iperiod_eff == 0)) {
+        nptr.push_back(0);
+        double bs0 = boxsize[0];
+        for (int l = 0; l <= p.nlevend; l++)
+            for (int b = t.first(l); b <= t.last(l); b++) {
+                size_t before = pairs.size();
+                for (int jb : list1[b]) {
+                    int jl = t.ilevel[jb];
+                    double bs = boxsize[jl];
+                    int ixy[2];
+                    if (p.iperiod_eff == 1 && jl == 1) {
+                        loctab_index(0, centers + 2 * b, centers + 2 * jb, bs, bs0, ixy);
+                        add_pair(l, jb, ixy);
+                        // periodic images of a level-1 source (bfgt/boxfgt.f:951-1053)
+                        int im[2], nim = 0;
+                        double dxy[2], ct[3][2], bsh = bs * 0.25;
+                        for (int i = 0; i < 2; i++) {
+                            dxy[i] = centers[2 * jb + i] - centers[2 * b + i];
+                            im[i] = std::fabs(dxy[i]) > bsh;
+                            nim += im[i];
+                        }
+                        if (nim > 0) {
+                            for (int i = 0; i < 2; i++) {
+                                if (dxy[i] < -bsh) ct[0][i] = centers[2 * jb + i] + bs0;
+                                else if (dxy[i] > bsh) ct[0][i] = centers[2 * jb + i] - bs0;
+                                else ct[0][i] = centers[2 * jb + i];
+                            }
+                            if (nim == 2) {
+                                ct[1][0] = centers[2 * jb];
+                                ct[1][1] = ct[0][1];
+                                ct[2][0] = ct[0][0];
+                                ct[2][1] = centers[2 * jb + 1];
+                            }
+                            for (int k = 0; k < (1 << nim) - 1; k++) {
+                                loctab_index(0, centers + 2 * b, ct[k], bs, bs0, ixy);
+                                add_pair(l, jb, ixy);
+                            }
+                        }
+                    } else {
+                        loctab_index(p.iperiod_eff, centers + 2 * b, centers + 2 * jb, bs, bs0, ixy);
+                        add_pair(l, jb, ixy);
+                    }
+                }
+                if (pairs.size() > before) {
+                    ntgt.push_back(b);
+                    nlev.push_back(l);
+                    nptr.push_back((int)pairs.size());
+                }
+            }
+    }
+    if (!tree_ok) return set_error(BFGT_ERR_TREE, "near-field offset outside the 2:1 table set (tree not level-restricted)");
+    p.nnear_t = (int)ntgt.size();
+    p.nnear_p = (int)pairs.size();
+    p.has_near = p.nnear_t > 0;
+
+    if ((rc = upload(p, p.d_pwleaf, leaves))) return rc;
+    if ((rc = upload(p, p.d_chunks, chunks))) return rc;
+    if ((rc = upload(p, p.d_anc_box, anc_box))) return rc;
+    if ((rc = upload(p, p.d_anc_chunk, anc_chunk))) return rc;
+    if ((rc = upload(p, p.d_m2l_ptr, m2l_ptr))) return rc;
+    if ((rc = upload(p, p.d_m2l, m2l))) return rc;
+    if ((rc = upload(p, p.d_near_tgt, ntgt))) return rc;
+    if ((rc = upload(p, p.d_near_lev, nlev))) return rc;
+    if ((rc = upload(p, p.d_near_ptr, nptr))) return rc;
+    if ((rc = upload(p, p.d_near, pairs))) return rc;
+    if (p.has_pw) {
+        size_t len = (size_t)nd * p.nexp;
+        if ((rc = dalloc(p, p.d_partial, (size_t)p.nchunks * len))) return rc;
+        if ((rc = dalloc(p, p.d_mp, (size_t)p.nanc * len))) return rc;
+        if ((rc = dalloc(p, p.d_loc, (size_t)p.nanc * len))) return rc;
+    }
+    BFGT_HIP_CHECK(hipStreamSynchronize(p.stream));
+    auto t1 = std::chrono::steady_clock::now();
+    p.t_plan_host = std::chrono::duration<double>(t1 - t0).count() - p.t_tables;
+    return BFGT_OK;
+}
+
+int plan_execute(Plan& p, const double* fvals, double* pot, double* grad, double* hess, double* timeinfo) {
+    if (!fvals || !pot || (p.ifpgh >= 2 && !grad) || (p.ifpgh >= 3 && !hess))
+        return set_error(BFGT_ERR_ARG, "null device array");
+    BFGT_HIP_CHECK(hipSetDevice(p.device));
+    int rc;
+    hipEvent_t* ev = p.ev;
+    if (p.profiling && p.prof_count < p.prof_max) ev = p.prof_ev.data() + 6 * p.prof_count++;
+    BFGT_HIP_CHECK(hipEventRecord(ev[0], p.stream));
+    if (p.has_pw) {
+        if ((rc = launch_upward(p, fvals))) return rc;
+        BFGT_HIP_CHECK(hipEventRecord(ev[1], p.stream));
+        if ((rc = launch_reduce(p))) return rc;
+        BFGT_HIP_CHECK(hipEventRecord(ev[2], p.stream));
+        if ((rc = launch_m2l(p))) return rc;
+        BFGT_HIP_CHECK(hipEventRecord(ev[3], p.stream));
+        if ((rc = launch_downward(p, pot, grad, hess))) return rc;
+        BFGT_HIP_CHECK(hipEventRecord(ev[4], p.stream));
+    } else {
+        for (int i = 1; i <= 4; i++) BFGT_HIP_CHECK(hipEventRecord(ev[i], p.stream));
+    }
+    if ((rc = launch_near(p, fvals, pot, grad, hess))) return rc;
+    BFGT_HIP_CHECK(hipEventRecord(ev[5], p.stream));
+    if (timeinfo) {
+        BFGT_HIP_CHECK(hipEventSynchronize(ev[5]));
+        float ms[5];
+        for (int i = 0; i < 5; i++) BFGT_HIP_CHECK(hipEventElapsedTime(&ms[i], ev[i], ev[i + 1]));
+        for (int i = 0; i < 20; i++) timeinfo[i] = 0;
+        timeinfo[1] = ms[0] * 1e-3;  // leaf values -> PW (with the merge folded in)
+        timeinfo[2] = ms[1] * 1e-3;  // merge: sum of partial expansions per cutoff box
+        timeinfo[3] = ms[2] * 1e-3;  // mp -> loc at the cutoff level
+        timeinfo[4] = 0.0;           // split: folded into the evaluation
+        timeinfo[5] = ms[3] * 1e-3;  // loc -> pot/grad/hess
+        timeinfo[6] = ms[4] * 1e-3;  // near field
+    }
+    return BFGT_OK;
+}
+
+}  // namespace bfgt
+
+using bfgt::set_error;
+
+extern "C" {
+
+const char* bfgt_last_error(void) { return bfgt::g_err.c_str(); }
+
+int bfgt_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int bfgt_plan_create(bfgt_plan_t* out, int nd, int ndim, double delta, double eps, int ipoly, int iperiod,
+                     int norder, int nboxes, int nlevels, int ltree, int* itree, const int* iptr,
+                     const double* centers, const double* boxsize, int ifpgh, int precision, void* stream) {
+    if (!out) return set_error(BFGT_ERR_ARG, "null plan pointer");
+    *out = nullptr;
+    std::unique_ptr<bfgt_plan> h(new bfgt_plan());
+    int rc = bfgt::plan_create(h->p, nd, ndim, delta, eps, ipoly, iperiod, norder, nboxes, nlevels, ltree, itree,
+                               iptr, centers, boxsize, ifpgh, precision, stream);
+    if (rc) {
+        bfgt::release(h->p);
+        return rc;
+    }
+    *out = h.release();
+    return BFGT_OK;
+}
+
+int bfgt_plan_execute(bfgt_plan_t plan, const double* d_fvals, double* d_pot, double* d_grad, double* d_hess,
+                      double* timeinfo) {
+    if (!plan) return set_error(BFGT_ERR_ARG, "null plan");
+    return bfgt::plan_execute(plan->p, d_fvals, d_pot, d_grad, d_hess, timeinfo);
+}
+
+int bfgt_plan_destroy(bfgt_plan_t plan) {
+    if (!plan) return BFGT_OK;
+    bfgt::release(plan->p);
+    delete plan;
+    return BFGT_OK;
+}
+
+int bfgt_plan_profile_begin(bfgt_plan_t plan, int max_steps) {
+    if (!plan || max_steps < 1) return set_error(BFGT_ERR_ARG, "invalid profiling request");
+    bfgt::Plan& p = plan->p;
+    BFGT_HIP_CHECK(hipSetDevice(p.device));
+    while ((int)p.prof_ev.size() < 6 * max_steps) {
+        hipEvent_t e;
+        BFGT_HIP_CHECK(hipEventCreate(&e));
+        p.prof_ev.push_back(e);
+    }
+    p.prof_max = max_steps;
+    p.prof_count = 0;
+    p.profiling = true;
+    return BFGT_OK;
+}
+
+int bfgt_plan_profile_end(bfgt_plan_t plan, double* stage_seconds, int* nsteps) {
+    if (!plan) return set_error(BFGT_ERR_ARG, "null plan");
+    bfgt::Plan& p = plan->p;
+    p.profiling = false;
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    for (int s = 0; s < p.prof_count; s++) {
+        hipEvent_t* ev = p.prof_ev.data() + 6 * s;
+        BFGT_HIP_CHECK(hipEventSynchronize(ev[5]));
+        float ms;
+        BFGT_HIP_CHECK(hipEventElapsedTime(&ms, ev[0], ev[1]));
+        acc[0] += ms * 1e-3;
+        BFGT_HIP_CHECK(hipEventElapsedTime(&ms, ev[1], ev[2]));
+        acc[1] += ms * 1e-3;
+        BFGT_HIP_CHECK(hipEventElapsedTime(&ms, ev[2], ev[3]));
+        acc[2] += ms * 1e-3;
+        BFGT_HIP_CHECK(hipEventElapsedTime(&ms, ev[3], ev[4]));
+        acc[4] += ms * 1e-3;
+        BFGT_HIP_CHECK(hipEventElapsedTime(&ms, ev[4], ev[5]));
+        acc[5] += ms * 1e-3;
+    }
+    if (stage_seconds) std::memcpy(stage_seconds, acc, sizeof(acc));
+    if (nsteps) *nsteps = p.prof_count;
+    return BFGT_OK;
+}
+
+int bfgt_plan_query(bfgt_plan_t plan, int* iv, double* dv) {
+    if (!plan) return set_error(BFGT_ERR_ARG, "null plan");
+    const bfgt::Plan& p = plan->p;
+    if (iv) {
+        int v[16] = {p.npwlevel, p.npw, p.nexp, p.nlevstart, p.nlevend, p.nleafbox, p.npwleaf, p.nchunks,
+                     p.nanc,     p.nnear_t, p.nnear_p, p.has_pw, p.has_near, p.iperiod_eff, p.nboxes, p.precision};
+        std::memcpy(iv, v, sizeof(v));
+    }
+    if (dv) {
+        double v[8] = {p.delta0, p.pmax, p.t_plan_host, p.t_tables, 0, 0, 0, 0};
+        std::memcpy(dv, v, sizeof(v));
+    }
+    return BFGT_OK;
+}
+
+int bfgt_plan_get_table(bfgt_plan_t plan, int which, int level, void* host_out, size_t nbytes) {
+    if (!plan || !host_out) return set_error(BFGT_ERR_ARG, "null argument");
+    const bfgt::Plan& p = plan->p;
+    if (level < 0 || level > p.nlevels) return set_error(BFGT_ERR_ARG, "level out of range");
+    size_t nn = (size_t)p.n * p.n;
+    const void* src = nullptr;
+    size_t sz = 0;
+    switch (which) {
+        case 0:
+            sz = sizeof(bfgt::cplx) * p.npw * p.n;
+            src = p.d_tpoly ? (const void*)(p.d_tpoly + (size_t)level * p.npw * p.n) : nullptr;
+            break;
+        case 1:
+            sz = sizeof(bfgt::cplx) * 3 * p.npw * p.n;
+            src = p.d_tpw ? (const void*)(p.d_tpw + (size_t)level * 3 * p.npw * p.n) : nullptr;
+            break;
+        case 2:
+            sz = sizeof(double) * 13 * 3 * nn;
+            src = p.d_tloc + (size_t)level * 13 * 3 * nn;
+            break;
+        case 3:
+            sz = sizeof(int) * 13 * 2 * (p.n + 1);
+            src = p.d_indloc + (size_t)level * 13 * 2 * (p.n + 1);
+            break;
+        case 4:
+            sz = sizeof(bfgt::cplx) * 9 * p.nexp;
+            src = p.d_wshift;
+            break;
+        default:
+            return set_error(BFGT_ERR_ARG, "unknown table");
+    }
+    if (!src) return set_error(BFGT_ERR_ARG, "table not built for this plan");
+    if (nbytes < sz) return set_error(BFGT_ERR_ARG, "output buffer too small");
+    BFGT_HIP_CHECK(hipMemcpy(host_out, src, sz, hipMemcpyDeviceToHost));
+    return BFGT_OK;
+}
+
+int bfgt_hip_eval(int nd, int ndim, double delta, double eps, int ipoly, int iperiod, int norder, int npbox,
+                  int nboxes, int nlevels, int ltree, int* itree, const int* iptr, const double* centers,
+                  const double* boxsize, const double* fvals, int ifpgh, double* pot, double* grad, double* hess,
+                  int ifnewtree, int ntarg, const double* targs, int ifpghtarg, double* pote, double* grade,
+                  double* hesse, double* timeinfo, int precision) {
+    (void)ntarg;
+    (void)targs;
+    (void)pote;
+    (void)grade;
+    (void)hesse;
+    auto t0 = std::chrono::steady_clock::now();
+    if (ifnewtree != 0) return set_error(BFGT_ERR_IFNEWTREE, "ifnewtree=1 is not supported");
+    if (ifpghtarg > 0) return set_error(BFGT_ERR_TARGETS, "extra targets (ifpghtarg>0) are not supported");
+    if (npbox != norder * norder) return set_error(BFGT_ERR_NORDER, "npbox must equal norder**2");
+    if (ifpgh < 1 || ifpgh > 3) return set_error(BFGT_ERR_IFPGH, "ifpgh must be 1, 2 or 3");
+    if (!fvals || !pot || (ifpgh >= 2 && !grad) || (ifpgh >= 3 && !hess))
+        return set_error(BFGT_ERR_ARG, "null output array");
+    bfgt_plan_t plan = nullptr;
+    int rc = bfgt_plan_create(&plan, nd, ndim, delta, eps, ipoly, iperiod, norder, nboxes, nlevels, ltree, itree,
+                              iptr, centers, boxsize, ifpgh, precision, nullptr);
+    if (rc) return rc;
+    bfgt::Plan& p = plan->p;
+    size_t nv = (size_t)nboxes * npbox * nd;
+    double *dfv = nullptr, *dpot = nullptr, *dgrad = nullptr, *dhess = nullptr;
+    auto fail = [&](int code) {
+        hipFree(dfv);
+        hipFree(dpot);
+        hipFree(dgrad);
+        hipFree(dhess);
+        bfgt_plan_destroy(plan);
+        return code;
+    };
+#define BFGT_EVAL_CHECK(expr)                                                                        \
+    do {                                                                                             \
+        hipError_t _e = (expr);                                                                      \
+        if (_e != hipSuccess) return fail(set_error(BFGT_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e))); \
+    } while (0)
+    BFGT_EVAL_CHECK(hipMalloc(&dfv, nv * sizeof(double)));
+    BFGT_EVAL_CHECK(hipMalloc(&dpot, nv * sizeof(double)));
+    BFGT_EVAL_CHECK(hipMemcpyAsync(dfv, fvals, nv * sizeof(double), hipMemcpyHostToDevice, p.stream));
+    BFGT_EVAL_CHECK(hipMemcpyAsync(dpot, pot, nv * sizeof(double), hipMemcpyHostToDevice, p.stream));
+    if (ifpgh >= 2) {
+        BFGT_EVAL_CHECK(hipMalloc(&dgrad, 2 * nv * sizeof(double)));
+        BFGT_EVAL_CHECK(hipMemcpyAsync(dgrad, grad, 2 * nv * sizeof(double), hipMemcpyHostToDevice, p.stream));
+    }
+    if (ifpgh >= 3) {
+        BFGT_EVAL_CHECK(hipMalloc(&dhess, 3 * nv * sizeof(double)));
+        BFGT_EVAL_CHECK(hipMemcpyAsync(dhess, hess, 3 * nv * sizeof(double), hipMemcpyHostToDevice, p.stream));
+    }
+    auto t1 = std::chrono::steady_clock::now();
+    double ti[20];
+    rc = bfgt::plan_execute(p, dfv, dpot, dgrad, dhess, ti);
+    if (rc) return fail(rc);
+    BFGT_EVAL_CHECK(hipMemcpyAsync(pot, dpot, nv * sizeof(double), hipMemcpyDeviceToHost, p.stream));
+    if (ifpgh >= 2)
+        BFGT_EVAL_CHECK(hipMemcpyAsync(grad, dgrad, 2 * nv * sizeof(double), hipMemcpyDeviceToHost, p.stream));
+    if (ifpgh >= 3)
+        BFGT_EVAL_CHECK(hipMemcpyAsync(hess, dhess, 3 * nv * sizeof(double), hipMemcpyDeviceToHost, p.stream));
+    BFGT_EVAL_CHECK(hipStreamSynchronize(p.stream));
+#undef BFGT_EVAL_CHECK
+    if (timeinfo) {
+        for (int i = 0; i < 20; i++) timeinfo[i] = ti[i];
+        timeinfo[0] = std::chrono::duration<double>(t1 - t0).count();
+    }
+    return fail(BFGT_OK);
+}
+
+}  // extern "C"

@@ -41,6 +41,32 @@ class GaussianSum:
             out = np.clip(out, *self.clip)
         return out
 
+    def sample_boxes(self, centers: np.ndarray, size, norder: int, ipoly: int) -> np.ndarray:
+        """Values on the grids of boxes (m, norder**2), evaluating each Gaussian image only on
+        boxes within 6.5 widths of it (the neglected terms are below 1e-18 of its amplitude)."""
+        from .tree import box_grid
+
+        pts = box_grid(centers, size, norder, ipoly)
+        m = centers.shape[0]
+        half = 0.5 * np.broadcast_to(np.asarray(size, dtype=np.float64), (m,))
+        out = np.full(pts.shape[:2], self.offset, dtype=np.float64)
+        r = range(-self.nimage, self.nimage + 1)
+        for c, w, a in zip(self.centers, self.widths, self.amps):
+            reach = 6.5 * w
+            for mx in r:
+                for my in r:
+                    gx, gy = c[0] + mx, c[1] + my
+                    dx = np.maximum(np.abs(centers[:, 0] - gx) - half, 0.0)
+                    dy = np.maximum(np.abs(centers[:, 1] - gy) - half, 0.0)
+                    sel = np.nonzero(dx * dx + dy * dy < reach * reach)[0]
+                    if sel.size == 0:
+                        continue
+                    q = pts[sel]
+                    out[sel] += a * np.exp(-((q[..., 0] - gx) ** 2 + (q[..., 1] - gy) ** 2) / w**2)
+        if self.clip is not None:
+            out = np.clip(out, *self.clip)
+        return out
+
     def heat_potential(self, p: np.ndarray, delta: float) -> np.ndarray:
         """Closed form of ``int exp(-|x-y|^2/delta) f(y) dy`` over R^2 for the image sum
         (unnormalised kernel): each Gaussian maps to
@@ -77,6 +103,11 @@ class Stack:
     def __call__(self, p):
         return np.stack([f(p) for f in self.fields], axis=-1)
 
+    def sample_boxes(self, centers, size, norder, ipoly):
+        from .tree import eval_on_boxes
+
+        return np.stack([eval_on_boxes(f, centers, size, norder, ipoly, 1)[..., 0] for f in self.fields], axis=-1)
+
 
 @dataclass
 class Workload:
@@ -111,12 +142,14 @@ def _c2(scale: str):
     w = np.r_[0.005, rng.uniform(0.05, 0.1, size=4)]
     a = rng.uniform(-1, 1, size=5)
     g = GaussianSum(c, w, a)
-    floor = {"full": 7, "small": 3, "tiny": 2}[scale]
-    maxl = {"full": 10, "small": 6, "tiny": 5}[scale]
-    t = build_adaptive(g, 1, 16, 0, 1e-9, maxl, min_level=floor)
+    floor = {"full": 7, "small": 4, "tiny": 3}[scale]
+    maxl = {"full": 10, "small": 7, "tiny": 5}[scale]
+    rad = {"full": 0.03, "small": 0.03, "tiny": 0.02}[scale]
+    # the sharp Gaussian is refined to the deepest level within 6 widths of its centre
+    t = build_adaptive(g, 1, 16, 0, 1e-9, maxl, min_level=floor, discs=[(c[0, 0], c[0, 1], rad, maxl)])
     return t, g, 1, 16, 1e-4, 1e-9, 2, (
-        f"adaptive periodic tree, tail rule tol 1e-9, uniform level-{floor} floor, max level {maxl}, "
-        "n=16 Legendre, nd=1, delta=1e-4, eps=1e-9, pot+grad")
+        f"adaptive periodic tree: tail rule (tol 1e-9), uniform level-{floor} floor, levels up to {maxl} "
+        f"within {rad} of the sharp Gaussian; n=16 Legendre, nd=1, delta=1e-4, eps=1e-9, pot+grad")
 
 
 def _c3(scale: str, delta=1e-3, eps=1e-9):
@@ -130,7 +163,7 @@ def _c3(scale: str, delta=1e-3, eps=1e-9):
     g3.offset = -float(np.sum(g3.amps * np.pi * g3.widths**2))
     dens = Stack([spots[0], spots[1], g3])
     maxl = {"full": 9, "small": 6, "tiny": 5}[scale]
-    t = build_adaptive(dens, 3, 8, 0, 1e-9, maxl, min_level=2, eta=0.0)
+    t = build_adaptive(dens, 3, 8, 0, 1e-9, maxl, min_level=2, eta=1.0)
     return t, dens, 3, 8, delta, eps, 3, (
         f"nd=3 (two spot fields, one zero-mean Gaussian sum) on an adaptive tree (max level {maxl}), "
         f"n=8 Legendre, delta={delta:g}, eps={eps:g}, pot+grad+hess")
@@ -150,10 +183,12 @@ def _c5(scale: str):
     g = gaussian_sum(rng, 40, 0.002, 0.1, logw=True)
     floor = {"full": 8, "small": 4, "tiny": 3}[scale]
     maxl = {"full": 12, "small": 7, "tiny": 5}[scale]
-    t = build_adaptive(g, 1, 16, 0, 1e-10, maxl, min_level=floor)
+    # Gaussians narrower than 0.01 are refined to the deepest level within one width
+    discs = [(cx, cy, 1.0 * wi, maxl) for (cx, cy), wi in zip(g.centers, g.widths) if wi < 0.01]
+    t = build_adaptive(g, 1, 16, 0, 1e-10, maxl, min_level=floor, discs=discs)
     return t, g, 1, 16, 1e-5, 1e-10, 2, (
-        f"adaptive periodic tree, tail rule tol 1e-10, uniform level-{floor} floor, max level {maxl}, "
-        "n=16 Legendre, nd=1, delta=1e-5, eps=1e-10, pot+grad")
+        f"adaptive periodic tree: tail rule (tol 1e-10), uniform level-{floor} floor, levels up to {maxl} "
+        "within one width of the Gaussians narrower than 0.01; n=16 Legendre, nd=1, delta=1e-5, eps=1e-10, pot+grad")
 
 
 WORKLOADS = {"c1": _c1, "c2": _c2, "c3": _c3, "c4": _c4, "c5": _c5}

@@ -1,0 +1,114 @@
+/*
+ * boxfgt_hip.h -- C ABI of the MI355X box fast Gauss transform.
+ *
+ * Plain pointers, sizes and status codes; no device or framework types in
+ * the signatures.  Arrays use the reference's Fortran layouts (first index
+ * fastest):
+ *   itree(ltree) int32, iptr(8) int32 (1-based offsets into itree),
+ *   centers(2,nboxes), boxsize(0:nlevels), fvals(nd,npbox,nboxes),
+ *   pot(nd,npbox,nboxes), grad(nd,2,npbox,nboxes), hess(nd,3,npbox,nboxes)
+ *   with hess ordered xx, xy, yy.  Outputs are accumulated (+=) on leaf boxes
+ *   only, exactly like the reference.  All functions return BFGT_OK (0) or
+ *   one of the BFGT_ERR_* codes; bfgt_last_error() describes the last failure.
+ */
+#ifndef BOXFGT_HIP_H
+#define BOXFGT_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define BFGT_OK 0
+#define BFGT_ERR_NDIM 1      /* ndim != 2 */
+#define BFGT_ERR_IPOLY 2     /* ipoly not 0 (Legendre) or 1 (Chebyshev) */
+#define BFGT_ERR_IFPGH 3     /* ifpgh not in 1..3 */
+#define BFGT_ERR_IFNEWTREE 4 /* ifnewtree=1 (tree refinement inside the FGT) is not supported */
+#define BFGT_ERR_TARGETS 5   /* ifpghtarg > 0 (extra targets) is not supported */
+#define BFGT_ERR_NORDER 6    /* norder < 1, norder > 32, or npbox != norder**2 */
+#define BFGT_ERR_NPW 7       /* plane-wave length above 200 (the reference's ws/ts(200)) */
+#define BFGT_ERR_TREE 8      /* inconsistent tree arrays */
+#define BFGT_ERR_ARG 9       /* null pointer or invalid scalar argument */
+#define BFGT_ERR_HIP 10      /* HIP runtime failure */
+#define BFGT_ERR_NODEVICE 11 /* no GPU visible */
+#define BFGT_ERR_RESOURCE 12 /* kernel resource limits (LDS) exceeded for this size */
+
+/* arithmetic of the leaf transforms and near-field contractions */
+#define BFGT_PREC_FP64 0    /* all-FP64 (the drop-in) */
+#define BFGT_PREC_SPLIT32 1 /* split-FP32 hi/lo products, FP64 weights/phases/accumulation */
+
+/*
+ * One complete box FGT call on host arrays, synchronous.
+ * Replaces: subroutine boxfgt(nd,ndim,delta,eps,ipoly,iperiod,norder,npbox,
+ *   nboxes,nlevels,ltree,itree,iptr,centers,boxsize,fvals,ifpgh,pot,grad,
+ *   hess,ifnewtree,ntarg,targs,ifpghtarg,pote,grade,hesse,timeinfo)
+ *   -- bfgt/boxfgt.f:5-8 (arguments :91-109).
+ * Side effects as in the reference: the colleague lists in itree are
+ * recomputed (bfgt/boxfgt.f:521-528); timeinfo(1..20) is zeroed and filled
+ * (slot 1 set-up, 2 leaf->PW, 3 merge, 4 translation, 5 split (fused into 6),
+ * 6 PW->pot/grad/hess, 7 near field).  ifnewtree must be 0 and ifpghtarg
+ * 0; pote/grade/hesse/targs are not touched.  `precision` selects
+ * BFGT_PREC_FP64 or BFGT_PREC_SPLIT32.
+ */
+int bfgt_hip_eval(int nd, int ndim, double delta, double eps, int ipoly, int iperiod, int norder, int npbox,
+                  int nboxes, int nlevels, int ltree, int *itree, const int *iptr, const double *centers,
+                  const double *boxsize, const double *fvals, int ifpgh, double *pot, double *grad, double *hess,
+                  int ifnewtree, int ntarg, const double *targs, int ifpghtarg, double *pote, double *grade,
+                  double *hesse, double *timeinfo, int precision);
+
+/* Resident plan: the tree, lists and FP64 tables stay in device memory for
+ * repeated calls with the same tree, delta and eps (the solvers call 2-3x per
+ * time step on one tree).  Splits the reference call into its set-up
+ * (bfgtmain STEP 0, bfgt/boxfgt.f:485-698) and its evaluation (steps 1-6,
+ * bfgt/boxfgt.f:720-1071). */
+typedef struct bfgt_plan *bfgt_plan_t;
+
+/* Set-up.  Host tree arrays (itree's colleague lists are recomputed, as in
+ * the reference).  `stream` is a hipStream_t or NULL for an internal stream. */
+int bfgt_plan_create(bfgt_plan_t *plan, int nd, int ndim, double delta, double eps, int ipoly, int iperiod,
+                     int norder, int nboxes, int nlevels, int ltree, int *itree, const int *iptr,
+                     const double *centers, const double *boxsize, int ifpgh, int precision, void *stream);
+
+/* Evaluation on device arrays (fvals, and pot/grad/hess which are
+ * accumulated), asynchronous on the plan's stream; allocates nothing.
+ * grad/hess may be NULL when ifpgh does not ask for them.  timeinfo, when
+ * non-NULL, is filled after a stream synchronisation. */
+int bfgt_plan_execute(bfgt_plan_t plan, const double *d_fvals, double *d_pot, double *d_grad, double *d_hess,
+                      double *timeinfo);
+
+int bfgt_plan_destroy(bfgt_plan_t plan);
+
+/* Per-stage device timing over a window of executions.  After
+ * bfgt_plan_profile_begin(plan, max_steps) the next max_steps executions record
+ * HIP events around each stage on the plan's stream (the same stream the
+ * kernels run on); bfgt_plan_profile_end synchronises and returns the summed
+ * seconds per stage: [0] leaf values -> PW (merge folded in), [1] merge of
+ * partial expansions, [2] mp -> loc translation, [3] split (0: folded into
+ * [4]), [4] loc -> pot/grad/hess, [5] near field -- the timeinfo(2..7) slots of
+ * the reference (bfgt/boxfgt.f:748-1076) -- and the number of executions seen. */
+int bfgt_plan_profile_begin(bfgt_plan_t plan, int max_steps);
+int bfgt_plan_profile_end(bfgt_plan_t plan, double *stage_seconds, int *nsteps);
+
+/* Plan facts: ivals[0..15] = {npwlevel, npw, nexp, nlevstart, nlevend, nleafbox,
+ * n_pw_leaves, n_chunks, n_ancestors, n_near_targets, n_near_pairs, has_pw,
+ * has_near, iperiod_eff, nboxes, precision}; dvals[0..7] = {delta0, pmax,
+ * t_plan_host_s, t_tables_s, 0,0,0,0}. */
+int bfgt_plan_query(bfgt_plan_t plan, int *ivals, double *dvals);
+
+/* Copy one device table of one level to host memory (parity tests of the FP64
+ * table generation).  which: 0 values->PW (npw x n complex, [k][m]),
+ * 1 PW->pot/d/dd (3 x npw x n complex, [t][k][i]), 2 near-field masked
+ * tables (13 x 3 x n x n double, [k+6][t][j][m]), 3 near-field sparsity
+ * bounds (13 x (n+1) x 2 int, 1-based), 4 M2L shifts (9 x nexp complex). */
+int bfgt_plan_get_table(bfgt_plan_t plan, int which, int level, void *host_out, size_t nbytes);
+
+const char *bfgt_last_error(void);
+int bfgt_device_count(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BOXFGT_HIP_H */

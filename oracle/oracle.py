@@ -236,11 +236,12 @@ def pw2pgh(n, ts, xs, delta, boxdim):
 
 
 def loctab_level(eps, ipoly, n, delta, boxdim):
-    """Near-field tables of one target level: arrays (13, n_src, n_targ) for k=-6..6 and
-    ind (13, n+1, 2) 1-based sparsity bounds."""
+    """Near-field tables of one target level: arrays (13, n_targ, n_src) for k=-6..6
+    (entry [k+6][j, m] couples source node m to target node j) and ind (13, n+1, 2)
+    1-based sparsity bounds (row j: source range of target j; row n: target range)."""
     lib = load_oracle()
     t = [np.zeros((13, n, n)) for _ in range(3)]
     ind = np.zeros((13, n + 1, 2), dtype=np.int32)
     lib.oracle_loctab_level(eps, ipoly, n, delta, boxdim, *[_p(a) for a in t], _p(ind, _ip))
-    # stored column-major per offset: element (m, j) at m + n*j -> C array [k][j][m]
-    return [a.transpose(0, 2, 1).copy() for a in t] + [ind]
+    # stored column-major per offset: element (m, j) at m + n*j, i.e. C array [k][j][m]
+    return t + [ind]

@@ -1,0 +1,227 @@
+"""GPU parity of the HIP box FGT against the oracle (and the compiled reference).
+
+Every case runs the HIP library through its C ABI (ctypes) on device arrays
+and compares with the CPU oracle on the same tree and density.  Both compute
+in FP64 and follow the same algorithm, so the bar is a relative L2 difference
+over leaf points of 1e-12 (1e-11 for gradients/Hessians), far below eps.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from heat2d_per_box_amd.densities import make_workload, GaussianSum
+from heat2d_per_box_amd.tree import build_uniform, build_adaptive
+from tests.fields import FourierMode, leaf_rel_err
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"pot": 1e-12, "grad": 1e-11, "hess": 1e-11}
+
+
+def run_gpu(tree, fvals, nd, delta, eps, ipoly, iperiod, norder, ifpgh, precision="fp64", repeat=1):
+    import torch
+    from heat2d_per_box_amd.boxfgt import BoxFGT
+
+    t = tree.copy()
+    plan = BoxFGT(t, nd, delta, eps, ipoly, norder, ifpgh, precision=precision, iperiod=iperiod)
+    fv = torch.from_numpy(np.ascontiguousarray(fvals)).cuda()
+    nb, npb = fvals.shape[0], fvals.shape[1]
+    pot = torch.zeros((nb, npb, nd), dtype=torch.float64, device="cuda")
+    grad = torch.zeros((nb, npb, 2, nd), dtype=torch.float64, device="cuda") if ifpgh >= 2 else None
+    hess = torch.zeros((nb, npb, 3, nd), dtype=torch.float64, device="cuda") if ifpgh >= 3 else None
+    for _ in range(repeat):
+        plan(fv, pot, grad, hess)
+    torch.cuda.synchronize()
+    out = {
+        "pot": pot.cpu().numpy(),
+        "grad": grad.cpu().numpy() if grad is not None else None,
+        "hess": hess.cpu().numpy() if hess is not None else None,
+        "itree": t.itree,
+        "plan": plan,
+    }
+    return out
+
+
+def compare(tree, got, ref, ifpgh, scale=1.0):
+    errs = {"pot": leaf_rel_err(tree, got["pot"], scale * ref.pot)}
+    if ifpgh >= 2:
+        errs["grad"] = leaf_rel_err(tree, got["grad"], scale * ref.grad)
+    if ifpgh >= 3:
+        errs["hess"] = leaf_rel_err(tree, got["hess"], scale * ref.hess)
+    return errs
+
+
+def assert_close(errs, tol=TOL, mult=1.0):
+    for k, v in errs.items():
+        assert v < tol[k] * mult, errs
+
+
+WORKLOADS = [
+    ("c1", "full", {}),
+    ("c2", "small", {}),
+    ("c3", "tiny", {"delta": 1e-5, "eps": 1e-6}),
+    ("c3", "tiny", {"delta": 1e-5, "eps": 1e-9}),
+    ("c3", "tiny", {"delta": 1e-3, "eps": 1e-6}),
+    ("c3", "tiny", {"delta": 1e-3, "eps": 1e-9}),
+    ("c3", "tiny", {"delta": 1e-1, "eps": 1e-6}),
+    ("c3", "tiny", {"delta": 1e-1, "eps": 1e-9}),
+    ("c4", "small", {}),
+    ("c5", "small", {}),
+]
+
+
+@pytest.mark.parametrize("name,scale,kw", WORKLOADS, ids=lambda v: str(v))
+def test_workload_matches_oracle(device, name, scale, kw):
+    w = make_workload(name, scale, **kw)
+    print(f"case {name}/{scale}/{kw}: leaves={w.tree.nleaves}", flush=True)
+    ref = O.oracle_boxfgt(w.tree, w.fvals, w.nd, w.delta, w.eps, w.ipoly, w.iperiod, w.norder, w.ifpgh)
+    got = run_gpu(w.tree, w.fvals, w.nd, w.delta, w.eps, w.ipoly, w.iperiod, w.norder, w.ifpgh)
+    info = got["plan"].info
+    assert info["npwlevel"] == ref.info["npwlevel"] and info["npw"] == ref.info["npw"]
+    assert np.array_equal(got["itree"], ref.itree)
+    assert_close(compare(w.tree, got, ref, w.ifpgh))
+
+
+@pytest.mark.skipif(not O.reference_available(), reason="reference library not built")
+@pytest.mark.parametrize("name,kw", [("c1", {}), ("c3", {"delta": 1e-5, "eps": 1e-9}), ("c3", {"delta": 1e-1, "eps": 1e-9})])
+def test_workload_matches_reference(device, name, kw):
+    w = make_workload(name, "full" if name == "c1" else "tiny", **kw)
+    ref = O.reference_boxfgt(w.tree, w.fvals, w.nd, w.delta, w.eps, w.ipoly, w.iperiod, w.norder, w.ifpgh)
+    got = run_gpu(w.tree, w.fvals, w.nd, w.delta, w.eps, w.ipoly, w.iperiod, w.norder, w.ifpgh)
+    assert np.array_equal(got["itree"], ref.itree)
+    assert_close(compare(w.tree, got, ref, w.ifpgh))
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_random_trees_match_oracle(device, seed):
+    """Random regimes: periodic/free space, Legendre/Chebyshev, n, nd, ifpgh, delta, eps."""
+    rng = np.random.default_rng(200 + seed)
+    iperiod = int(rng.integers(0, 2))
+    ipoly = int(rng.integers(0, 2))
+    n = int(rng.choice([4, 6, 8, 12, 16]))
+    nd = int(rng.integers(1, 4))
+    ifpgh = int(rng.integers(1, 4))
+    delta = float(10 ** rng.uniform(-5.5, -0.5))
+    eps = float(rng.choice([1e-6, 1e-9, 1e-12]))
+    g = GaussianSum(rng.uniform(-0.5, 0.5, (3, 2)), rng.uniform(0.02, 0.1, 3), rng.uniform(-1, 1, 3))
+    fun = lambda p: np.stack([g(p), np.sin(3 * p[..., 0]) * g(p), g(p) ** 2][:nd], axis=-1)  # noqa: E731
+    t = build_adaptive(fun, nd, n, ipoly, 1e-6, 6, iperiod=iperiod, min_level=1, eta=0.0)
+    fv = t.sample(fun, n, ipoly, nd)
+    print(f"seed {seed}: iperiod={iperiod} ipoly={ipoly} n={n} nd={nd} ifpgh={ifpgh} delta={delta:.3g} eps={eps:g} leaves={t.nleaves}", flush=True)
+    ref = O.oracle_boxfgt(t, fv, nd, delta, eps, ipoly, iperiod, n, ifpgh)
+    got = run_gpu(t, fv, nd, delta, eps, ipoly, iperiod, n, ifpgh)
+    assert np.array_equal(got["itree"], ref.itree)
+    assert_close(compare(t, got, ref, ifpgh))
+
+
+def test_fourier_mode_closed_form(device):
+    """Heat semigroup on a Fourier mode: pot = pi delta exp(-delta|k|^2/4) f."""
+    t = build_uniform(4)
+    f = FourierMode(2, -1, phi=0.7)
+    n = 16
+    fv = t.sample(f, n, 0, 1)
+    for delta in (1e-1, 1e-3, 1e-4):
+        got = run_gpu(t, fv, 1, delta, 1e-9, 0, 1, n, 3)
+        pot, grad, hess = f.exact(t.grid_points(n, 0), delta)
+        assert leaf_rel_err(t, got["pot"][..., 0], pot) < 2e-8
+        assert leaf_rel_err(t, got["grad"][..., 0], grad) < 2e-8
+        assert leaf_rel_err(t, got["hess"][..., 0], hess) < 5e-8
+
+
+def test_plan_reuse_accumulates(device):
+    w = make_workload("c1", "full")
+    ref = O.oracle_boxfgt(w.tree, w.fvals, w.nd, w.delta, w.eps, w.ipoly, w.iperiod, w.norder, w.ifpgh)
+    got = run_gpu(w.tree, w.fvals, w.nd, w.delta, w.eps, w.ipoly, w.iperiod, w.norder, w.ifpgh, repeat=3)
+    assert_close(compare(w.tree, got, ref, w.ifpgh, scale=3.0))
+
+
+def test_gpu_tables_match_oracle(device):
+    """FP64 tables generated on the GPU against the oracle's tables."""
+    w = make_workload("c2", "tiny")
+    from heat2d_per_box_amd.boxfgt import BoxFGT
+
+    t = w.tree.copy()
+    plan = BoxFGT(t, 1, w.delta, w.eps, 0, w.norder, 2)
+    n, npw = w.norder, plan.info["npw"]
+    L0 = plan.info["nlevstart"]
+    pmax, npw_o = O.fgtpwterms(2 * t.boxsize[L0], w.delta, w.eps, 0)
+    assert npw_o == npw
+    ws, ts = O.pwnodes(pmax, npw)
+    from heat2d_per_box_amd.tree import poly_nodes
+
+    for lev in range(L0, t.nlevels + 1):
+        tp = plan.table(0, lev)  # [k][m]
+        ref = O.poly2pw(n, 0, ws, ts, w.delta, t.boxsize[lev]).T
+        assert np.max(np.abs(tp - ref)) < 1e-13 * np.max(np.abs(ref))
+        e = plan.table(1, lev)
+        r = O.pw2pgh(n, ts, poly_nodes(n, 0), w.delta, t.boxsize[lev])
+        for k in range(3):
+            assert np.max(np.abs(e[k] - r[k])) < 1e-12 * np.max(np.abs(r[k]))
+    for lev in range(0, t.nlevels + 1):
+        tab, tabx, tabxx, ind = O.loctab_level(w.eps, 0, n, w.delta, t.boxsize[lev])
+        gi = plan.table(3, lev)
+        assert np.array_equal(gi, ind)
+        g = plan.table(2, lev)  # [q][t][j][m], masked
+        for q in range(13):
+            mask = np.zeros((n, n), dtype=bool)
+            j0, j1 = ind[q][n]
+            for j in range(n):
+                if j0 <= j + 1 <= j1:
+                    a, b = ind[q][j]
+                    mask[j, max(a - 1, 0):max(b, 0)] = True
+            for tt, T in enumerate((tab, tabx, tabxx)):
+                want = np.where(mask, T[q], 0.0)
+                big = max(np.max(np.abs(T[q])), 1e-300)
+                assert np.max(np.abs(g[q, tt] - want)) < 1e-12 * big, (lev, q, tt)
+
+
+def test_host_entry_point_matches_oracle(device):
+    """bfgt_hip_eval on host arrays: accumulation into existing pot, itree side effects."""
+    from heat2d_per_box_amd.boxfgt import boxfgt
+
+    w = make_workload("c3", "tiny", delta=1e-5, eps=1e-9)
+    t = w.tree.copy()
+    nb, npb, nd = w.fvals.shape
+    pot = np.ones((nb, npb, nd))
+    grad = np.full((nb, npb, 2, nd), 2.0)
+    hess = np.full((nb, npb, 3, nd), -1.0)
+    ti = np.zeros(20)
+    boxfgt(nd, 2, w.delta, w.eps, 0, 1, w.norder, npb, nb, t.nlevels, t.ltree, t.itree, t.iptr, t.centers, t.boxsize,
+           w.fvals, 3, pot, grad, hess, timeinfo=ti)
+    ref = O.oracle_boxfgt(w.tree, w.fvals, nd, w.delta, w.eps, 0, 1, w.norder, 3)
+    lv = t.nchild == 0
+    assert np.array_equal(t.itree, ref.itree)
+    assert leaf_rel_err(t, pot - 1.0, ref.pot) < 1e-12
+    assert leaf_rel_err(t, grad - 2.0, ref.grad) < 1e-11
+    assert leaf_rel_err(t, hess + 1.0, ref.hess) < 1e-11
+    assert np.all(pot[~lv] == 1.0)  # non-leaf boxes untouched
+    assert ti[0] > 0 and np.all(ti >= 0)
+
+
+def test_empty_and_edge_trees(device):
+    """Root-only tree (no near field, no PW leaves) and a single-level tree."""
+    from heat2d_per_box_amd.tree import build_tree
+
+    for refined in (set(), {(0, 0, 0)}):
+        t = build_tree(refined)
+        n = 8
+        fv = t.sample(lambda p: np.cos(2 * np.pi * p[..., 0]), n, 0, 1)
+        for delta in (1e-1, 1e-2):
+            ref = O.oracle_boxfgt(t, fv, 1, delta, 1e-9, 0, 1, n, 2)
+            got = run_gpu(t, fv, 1, delta, 1e-9, 0, 1, n, 2)
+            if np.linalg.norm(ref.pot) == 0:
+                assert np.linalg.norm(got["pot"]) == 0
+            else:
+                assert_close(compare(t, got, ref, 2))
+
+
+def test_npw_limit_is_an_error(device):
+    """A shallow periodic tree with small delta needs more than 200 plane waves: the
+    reference overflows its ws/ts(200) arrays; here the plan refuses with BFGT_ERR_NPW."""
+    from heat2d_per_box_amd.boxfgt import BoxFGT, BoxFGTError
+    from heat2d_per_box_amd.tree import build_tree
+
+    t = build_tree(set())
+    with pytest.raises(BoxFGTError) as ei:
+        BoxFGT(t, 1, 1e-4, 1e-9, 0, 8, 1)
+    assert ei.value.code == 7
