@@ -52,13 +52,29 @@ def build(verbose: bool = False) -> str:
     with ThreadPoolExecutor(max_workers=4) as ex:
         objs = list(ex.map(lambda s: _compile(s, verbose), SOURCES))
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs
+    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-Wl,-soname,libboxfgt_hip.so", "-o", tmp] + objs
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
     build_fortran(verbose)
+    build_fortran_driver(verbose)
     return LIB
+
+
+FDRIVER = os.path.join(BUILD, "test_boxfgt_dropin")
+
+
+def build_fortran_driver(verbose: bool = False) -> str | None:
+    """Link the Fortran known-answer driver (tests/fortran) against the drop-in."""
+    src = os.path.join(os.path.dirname(HERE), "tests", "fortran", "test_boxfgt_dropin.f90")
+    if not (os.path.exists(src) and os.path.exists(FLIB) and os.path.exists(FLANG)):
+        return None
+    cmd = [FLANG, "-O2", src, FLIB, f"-L{HERE}", "-lboxfgt_hip", "-Wl,-rpath,$ORIGIN/..", "-o", FDRIVER]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    return FDRIVER
 
 
 def build_fortran(verbose: bool = False) -> str | None:

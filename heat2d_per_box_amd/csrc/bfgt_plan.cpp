@@ -38,6 +38,7 @@ size_t node_buffer_doubles(int n);
 size_t up_smem_bytes(int n, int npw);
 size_t down_smem_bytes(int n, int npw, int ntab);
 size_t near_smem_bytes(int n, int ntab);
+int up_max_out();
 int build_pw_nodes(Plan& p);
 
 namespace {
@@ -303,12 +304,8 @@ int plan_create(Plan& p, int nd, int ndim, double delta, double eps, int ipoly, 
     p.nboxes_cap = nboxes;
     p.boxsize.assign(boxsize, boxsize + nlevels + 1);
     BFGT_HIP_CHECK(hipGetDevice(&p.device));
-    if (stream) {
-        p.stream = (hipStream_t)stream;
-    } else {
-        BFGT_HIP_CHECK(hipStreamCreateWithFlags(&p.stream, hipStreamNonBlocking));
-        p.own_stream = true;
-    }
+    p.stream = (hipStream_t)stream;  // NULL: the device's default stream
+    p.own_stream = false;
     for (auto& e : p.ev) BFGT_HIP_CHECK(hipEventCreate(&e));
 
     // cutoff level and PW length (bfgt/boxfgt.f:133-167)
@@ -357,7 +354,7 @@ int plan_create(Plan& p, int nd, int ndim, double delta, double eps, int ipoly, 
     BFGT_HIP_CHECK(hipGetDeviceProperties(&prop, p.device));
     size_t lds_max = prop.sharedMemPerBlock;
     if (p.has_pw && (up_smem_bytes(p.n, npw) > lds_max || down_smem_bytes(p.n, npw, ifpgh) > lds_max ||
-                     p.nexp > 2048))
+                     p.nexp > up_max_out()))
         return set_error(BFGT_ERR_RESOURCE, "plane-wave expansion too large for the leaf kernels");
     if (near_smem_bytes(p.n, ifpgh) > lds_max || 9 * p.n * p.n * sizeof(double) > lds_max)
         return set_error(BFGT_ERR_RESOURCE, "leaf order too large for the near-field kernels");
@@ -726,10 +723,22 @@ int bfgt_hip_eval(int nd, int ndim, double delta, double eps, int ipoly, int ipe
     if (ifpgh < 1 || ifpgh > 3) return set_error(BFGT_ERR_IFPGH, "ifpgh must be 1, 2 or 3");
     if (!fvals || !pot || (ifpgh >= 2 && !grad) || (ifpgh >= 3 && !hess))
         return set_error(BFGT_ERR_ARG, "null output array");
+    if (ndim != 2) return set_error(BFGT_ERR_NDIM, "only ndim=2 is supported");
+    if (ipoly != 0 && ipoly != 1) return set_error(BFGT_ERR_IPOLY, "ipoly must be 0 or 1");
+    if (norder < 1 || norder > bfgt::kMaxOrder) return set_error(BFGT_ERR_NORDER, "norder must be in 1..32");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return set_error(BFGT_ERR_NODEVICE, "no GPU visible");
+    hipStream_t stream = nullptr;
+    if (hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess)
+        return set_error(BFGT_ERR_HIP, "hipStreamCreateWithFlags failed");
     bfgt_plan_t plan = nullptr;
     int rc = bfgt_plan_create(&plan, nd, ndim, delta, eps, ipoly, iperiod, norder, nboxes, nlevels, ltree, itree,
-                              iptr, centers, boxsize, ifpgh, precision, nullptr);
-    if (rc) return rc;
+                              iptr, centers, boxsize, ifpgh, precision, stream);
+    if (rc) {
+        hipStreamDestroy(stream);
+        return rc;
+    }
+    plan->p.own_stream = true;  // released with the plan
     bfgt::Plan& p = plan->p;
     size_t nv = (size_t)nboxes * npbox * nd;
     double *dfv = nullptr, *dpot = nullptr, *dgrad = nullptr, *dhess = nullptr;

