@@ -158,12 +158,17 @@ class BoxFGT:
     def table(self, which: int, level: int) -> np.ndarray:
         n, npw, nexp = self.norder, self.info["npw"], self.info["nexp"]
         shapes = {
-            0: ((npw, n), np.complex128),
+            0: ((n, npw), np.complex128),
             1: ((3, npw, n), np.complex128),
             2: ((13, 3, n, n), np.float64),
             3: ((13, n + 1, 2), np.int32),
-            4: ((9, nexp), np.complex128),
+            4: ((9, npw, (npw + 1) // 2), np.complex128),
         }
+        H = (npw + 1) // 2
+        if self.info["has_pw"]:
+            shapes[5] = ((1 << max(level - self.info["nlevstart"], 0), npw), np.complex128)
+            shapes[6] = ((self.info["n_ancestors"], self.nd, npw, H), np.complex128)
+            shapes[7] = shapes[6]
         shape, dt = shapes[which]
         out = np.zeros(shape, dtype=dt)
         check(lib().bfgt_plan_get_table(self._h, which, level, out.ctypes.data_as(ctypes.c_void_p), out.nbytes))

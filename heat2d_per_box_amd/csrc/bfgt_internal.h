@@ -20,12 +20,15 @@ struct cplx {
     double x, y;
 };
 
-// One PW leaf: the leaf box, its level, its ancestor at the cutoff level and
-// the scaled offset (c_leaf - c_anc)/sqrt(delta0).
+// One PW leaf: the leaf box, its level, its ancestor at the cutoff level,
+// its cell (jx, jy) inside that ancestor at the leaf's level, and the scaled
+// offset (c_leaf - c_anc)/sqrt(delta0).
 struct PwLeaf {
     int box;
     int level;
     int anc;
+    int jx;
+    int jy;
     int pad;
     double dx, dy;
 };
@@ -72,6 +75,10 @@ struct Plan {
     double* d_tloc = nullptr;     // [level][13][type][j][m], masked
     int* d_indloc = nullptr;      // [level][13][n+1][2]
     cplx* d_wshift = nullptr;     // [9][nexp]
+    cplx* d_phase = nullptr;      // leaf->ancestor phases, [levbase[l] + j*npw + k]
+    int* d_levbase = nullptr;     // [nlevels+1]
+    std::vector<int> levbase;     // host copy (kept alive for the asynchronous upload)
+    size_t nphase = 0;
     std::vector<int> h_indloc;    // copy of d_indloc
 
     // schedules

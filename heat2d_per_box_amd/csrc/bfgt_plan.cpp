@@ -375,6 +375,19 @@ int plan_create(Plan& p, int nd, int ndim, double delta, double eps, int ipoly, 
         if ((rc = dalloc(p, p.d_tpoly, (size_t)(nlevels + 1) * npw * p.n))) return rc;
         if ((rc = dalloc(p, p.d_tpw, (size_t)(nlevels + 1) * 3 * npw * p.n))) return rc;
         if ((rc = dalloc(p, p.d_wshift, (size_t)9 * p.nexp))) return rc;
+        // leaf->ancestor phase tables: 2^(l-L0) cells per axis at level l
+        std::vector<int>& levbase = p.levbase;
+        levbase.assign(nlevels + 1, 0);
+        size_t tot = 0;
+        for (int l = p.nlevstart; l <= nlevels; l++) {
+            levbase[l] = (int)tot;
+            tot += ((size_t)1 << (l - p.nlevstart)) * npw;
+            if (tot > ((size_t)1 << 24))
+                return set_error(BFGT_ERR_RESOURCE, "tree too deep below the cutoff level for the phase tables");
+        }
+        p.nphase = tot;
+        if ((rc = dalloc(p, p.d_phase, tot))) return rc;
+        if ((rc = upload(p, p.d_levbase, levbase))) return rc;
         if ((rc = build_pw_nodes(p))) return rc;
     }
     if ((rc = build_tables(p))) return rc;
@@ -400,7 +413,12 @@ int plan_create(Plan& p, int nd, int ndim, double delta, double eps, int ipoly, 
                 lf.box = b;
                 lf.level = l;
                 lf.anc = a - a0;
+                double half = 0.5 * ((1 << (l - L0)) - 1);
+                lf.jx = (int)std::lround((centers[2 * b] - centers[2 * a]) / boxsize[l] + half);
+                lf.jy = (int)std::lround((centers[2 * b + 1] - centers[2 * a + 1]) / boxsize[l] + half);
                 lf.pad = 0;
+                if (lf.jx < 0 || lf.jy < 0 || lf.jx >= (1 << (l - L0)) || lf.jy >= (1 << (l - L0)))
+                    return set_error(BFGT_ERR_TREE, "leaf centre outside its cutoff-level ancestor");
                 lf.dx = (centers[2 * b] - centers[2 * a]) / sq;
                 lf.dy = (centers[2 * b + 1] - centers[2 * a + 1]) / sq;
                 by_anc[a - a0].push_back(lf);
@@ -696,6 +714,19 @@ int bfgt_plan_get_table(bfgt_plan_t plan, int which, int level, void* host_out, 
         case 4:
             sz = sizeof(bfgt::cplx) * 9 * p.nexp;
             src = p.d_wshift;
+            break;
+        case 5:
+            if (!p.has_pw || level < p.nlevstart) return set_error(BFGT_ERR_ARG, "no phase table for this level");
+            sz = sizeof(bfgt::cplx) * ((size_t)1 << (level - p.nlevstart)) * p.npw;
+            src = p.d_phase + p.levbase[level];
+            break;
+        case 6:
+            sz = sizeof(bfgt::cplx) * (size_t)p.nanc * p.nd * p.nexp;
+            src = p.d_mp;
+            break;
+        case 7:
+            sz = sizeof(bfgt::cplx) * (size_t)p.nanc * p.nd * p.nexp;
+            src = p.d_loc;
             break;
         default:
             return set_error(BFGT_ERR_ARG, "unknown table");

@@ -11,9 +11,14 @@
 // 1-D contractions.  Only leaf values, the cutoff-level expansions and the
 // outputs touch HBM.
 //
-// Expansion layout: [slot][density][k2][k1] complex, k1 < npw the x
-// frequency, k2 < H = (npw+1)/2 the retained half of the y frequencies
-// (pwexp(npw, H, nd), bfgt/bfgt_pwrouts.f:98-147).
+// Expansion layout on the device: [slot][density][k1][k2] complex, k1 < npw
+// the x frequency, k2 < H = (npw+1)/2 the retained half of the y frequencies
+// (the reference's pwexp(npw, H, nd), bfgt/bfgt_pwrouts.f:98-147, with the
+// two frequency axes swapped so that tiles of y frequencies are contiguous).
+//
+// Two implementations: MFMA kernels (v_mfma_f64_16x16x4f64) for leaf orders
+// n <= 16 and expansions of at most 128 plane waves, and vector-FMA kernels
+// for the remaining sizes.
 #include <hip/hip_runtime.h>
 
 #include "bfgt_internal.h"
@@ -21,6 +26,8 @@
 namespace bfgt {
 
 namespace {
+
+typedef double v4d __attribute__((ext_vector_type(4)));
 
 __device__ inline cplx cmul(cplx a, cplx b) { return cplx{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
 __device__ inline cplx cfma(cplx a, cplx b, cplx c) {
@@ -31,28 +38,251 @@ __device__ inline cplx cexpi(double t) {
     sincos(t, &s, &c);
     return cplx{c, s};
 }
+__device__ inline v4d mfma(double a, double b, v4d c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
 
-constexpr int kUpThreads = 256;
-constexpr int kUpMaxOut = 32 * kUpThreads;  // largest nexp the upward kernel accepts
+constexpr int kWave = 64;
+constexpr int kWavesPerBlock = 4;
 
-// Leaf values -> PW expansions about the cutoff-level ancestor, summed over
-// the leaves of one chunk.  One block per chunk; each thread accumulates
-// MAXE expansion coefficients in registers.
+// ---------------------------------------------------------------------------
+// MFMA kernels
+// ---------------------------------------------------------------------------
+//
+// Lane layout of v_mfma_f64_16x16x4f64 on gfx950 (lane l, q = l/16,
+// c = l%16), measured on the device: A (16x4) holds A[c][q], B (4x16) holds
+// B[q][c], and element r of the 16x16 accumulator holds D[4r + q][c].
+__device__ inline int drow(int q, int r) { return 4 * r + q; }
+
+constexpr int kUpMaxTM = 4;  // x-frequency tiles of 16 (ceil(npw/2) <= 64)
+
+// Upward pass.  One wave per (chunk, y-frequency tile tN): for every leaf of
+// the chunk
+//   G  = F * Ty              (n x 16 tile, Ty = T diag(py), real x complex)
+//   out += Tx^T G            (Tx = T diag(px)), using T(m, npw-1-k) = conj T(m, k)
+// so that only ceil(npw/2) x frequencies are multiplied; the accumulators
+// AC, BD, AD, BC give both halves at the end.
+__global__ __launch_bounds__(kWave * kWavesPerBlock) void k_up_mfma(
+    int n, int P, int nd, int nTM, int nTN, int nitems, const double* __restrict__ fvals,
+    const PwLeaf* __restrict__ leaves, const UpChunk* __restrict__ chunks, const cplx* __restrict__ tpoly,
+    const cplx* __restrict__ phase, const int* __restrict__ levbase, cplx* __restrict__ partial) {
+    __shared__ double sG[kWavesPerBlock][2][256];
+    const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    const int item = blockIdx.x * kWavesPerBlock + wave;
+    if (item >= nitems) return;  // no block-wide barriers in this kernel
+    const int ci = item / nTN, tN = item % nTN;
+    const int q = lane >> 4, cc = lane & 15;
+    const int H = (P + 1) / 2, Mlo = (P + 1) / 2, nexp = P * H, npbox = n * n;
+    const int nks = (n + 3) / 4;
+    const int k2 = 16 * tN + cc;
+    double* gre = sG[wave][0];
+    double* gim = sG[wave][1];
+    const UpChunk ch = chunks[ci];
+    for (int d = 0; d < nd; d++) {
+        v4d acc[4][kUpMaxTM];
+#pragma unroll
+        for (int p = 0; p < 4; p++)
+#pragma unroll
+            for (int t = 0; t < kUpMaxTM; t++) acc[p][t] = v4d{0, 0, 0, 0};
+        for (int li = 0; li < ch.count; li++) {
+            const PwLeaf lf = leaves[ch.first + li];
+            const cplx* T = tpoly + (size_t)lf.level * n * P;
+            const cplx* ph = phase + levbase[lf.level];
+            const double* f = fvals + (size_t)lf.box * npbox * nd + d;
+            // y phase of this lane's column (upward shift: conjugate of the table)
+            cplx py = (k2 < H) ? ph[(size_t)lf.jy * P + k2] : cplx{0, 0};
+            py.y = -py.y;
+            v4d Gr = {0, 0, 0, 0}, Gi = {0, 0, 0, 0};
+            for (int s = 0; s < nks; s++) {
+                int m2 = 4 * s + q;
+                double a = (cc < n && m2 < n) ? f[(size_t)(cc + n * m2) * nd] : 0.0;
+                cplx ty = (m2 < n && k2 < H) ? cmul(T[(size_t)m2 * P + k2], py) : cplx{0, 0};
+                Gr = mfma(a, ty.x, Gr);
+                Gi = mfma(a, ty.y, Gi);
+            }
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                gre[drow(q, r) * 16 + cc] = Gr[r];
+                gim[drow(q, r) * 16 + cc] = Gi[r];
+            }
+            __builtin_amdgcn_wave_barrier();
+#pragma unroll
+            for (int t = 0; t < kUpMaxTM; t++) {
+                if (t < nTM) {
+                    int k1 = 16 * t + cc;
+                    cplx px = (k1 < Mlo) ? ph[(size_t)lf.jx * P + k1] : cplx{0, 0};
+                    px.y = -px.y;
+                    for (int s = 0; s < nks; s++) {
+                        int m1 = 4 * s + q;
+                        cplx tx = (m1 < n && k1 < Mlo) ? cmul(T[(size_t)m1 * P + k1], px) : cplx{0, 0};
+                        double br = gre[m1 * 16 + cc], bi = gim[m1 * 16 + cc];
+                        acc[0][t] = mfma(tx.x, br, acc[0][t]);
+                        acc[1][t] = mfma(tx.y, bi, acc[1][t]);
+                        acc[2][t] = mfma(tx.x, bi, acc[2][t]);
+                        acc[3][t] = mfma(tx.y, br, acc[3][t]);
+                    }
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
+        }
+        cplx* out = partial + ((size_t)ch.slot * nd + d) * nexp;
+#pragma unroll
+        for (int t = 0; t < kUpMaxTM; t++) {
+            if (t < nTM) {
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    int k1 = 16 * t + drow(q, r);
+                    if (k1 < Mlo && k2 < H) {
+                        double AC = acc[0][t][r], BD = acc[1][t][r], AD = acc[2][t][r], BC = acc[3][t][r];
+                        out[(size_t)k1 * H + k2] = cplx{AC - BD, AD + BC};
+                        int km = P - 1 - k1;
+                        if (km > k1) out[(size_t)km * H + k2] = cplx{AC + BD, AD - BC};
+                    }
+                }
+            }
+        }
+    }
+}
+
+// Downward pass.  One wave per PW leaf; y frequencies are processed in tiles
+// of 16:
+//   FF_t = (E_t diag(px))^T L   over x frequencies, folded with
+//          E(npw-1-m) = conj E(m): FF = a (L_m + L_mirror) + i b (L_m - L_mirror)
+//   out^T += E_u^T (FF_v diag(py w))   real parts, w = 1/2 on the middle
+//          frequency of an odd expansion (bfgt/bfgt_pwrouts.f:591-595)
+// for the (v, u) pairs pot (0,0), gx (1,0), gy (0,1), hxx (2,0), hxy (1,1),
+// hyy (0,2).
+template <int NTAB>
+__global__ __launch_bounds__(kWave * kWavesPerBlock) void k_down_mfma(
+    int n, int P, int nd, int nleaf, const PwLeaf* __restrict__ leaves, const cplx* __restrict__ tpw,
+    const cplx* __restrict__ phase, const int* __restrict__ levbase, const cplx* __restrict__ loc,
+    double* __restrict__ pot, double* __restrict__ grad, double* __restrict__ hess) {
+    constexpr int NOUT = NTAB == 1 ? 1 : (NTAB == 2 ? 3 : 6);
+    __shared__ double sFF[kWavesPerBlock][NTAB][2][256];
+    const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    const int item = blockIdx.x * kWavesPerBlock + wave;
+    if (item >= nleaf) return;  // no block-wide barriers in this kernel
+    const int q = lane >> 4, cc = lane & 15;
+    const int H = (P + 1) / 2, Mh = (P + 1) / 2, P2 = P / 2, nexp = P * H, npbox = n * n;
+    const int nks1 = (Mh + 3) / 4, nTN = (H + 15) / 16;
+    const PwLeaf lf = leaves[item];
+    const cplx* E = tpw + (size_t)lf.level * 3 * P * n;  // [t][m][i]
+    const cplx* ph = phase + levbase[lf.level];
+    const int vv[6] = {0, 1, 0, 2, 1, 0};
+    const int uu[6] = {0, 0, 1, 0, 1, 2};
+    for (int d = 0; d < nd; d++) {
+        const cplx* L = loc + ((size_t)lf.anc * nd + d) * nexp;
+        v4d out[NOUT];
+#pragma unroll
+        for (int o = 0; o < NOUT; o++) out[o] = v4d{0, 0, 0, 0};
+        for (int tN = 0; tN < nTN; tN++) {
+            const int m2 = 16 * tN + cc;
+            v4d FFr[NTAB], FFi[NTAB];
+#pragma unroll
+            for (int t = 0; t < NTAB; t++) FFr[t] = FFi[t] = v4d{0, 0, 0, 0};
+            for (int s = 0; s < nks1; s++) {
+                const int m = 4 * s + q;
+                cplx S{0, 0}, Dd{0, 0}, px{0, 0};
+                if (m < Mh) {
+                    px = ph[(size_t)lf.jx * P + m];
+                    if (m2 < H) {
+                        cplx a = L[(size_t)m * H + m2];
+                        int mm = P - 1 - m;
+                        if (mm != m) {
+                            cplx b = L[(size_t)mm * H + m2];
+                            S = cplx{a.x + b.x, a.y + b.y};
+                            Dd = cplx{a.x - b.x, a.y - b.y};
+                        } else {
+                            S = a;
+                        }
+                    }
+                }
+#pragma unroll
+                for (int t = 0; t < NTAB; t++) {
+                    cplx e = (m < Mh && cc < n) ? cmul(E[((size_t)t * P + m) * n + cc], px) : cplx{0, 0};
+                    FFr[t] = mfma(e.x, S.x, FFr[t]);
+                    FFr[t] = mfma(-e.y, Dd.y, FFr[t]);
+                    FFi[t] = mfma(e.x, S.y, FFi[t]);
+                    FFi[t] = mfma(e.y, Dd.x, FFi[t]);
+                }
+            }
+            // fold the y phase and weight of this lane's column, stage as [m2][k1]
+            cplx py{0, 0};
+            if (m2 < H) {
+                py = ph[(size_t)lf.jy * P + m2];
+                double w = (m2 < P2) ? 1.0 : 0.5;
+                py.x *= w;
+                py.y *= w;
+            }
+#pragma unroll
+            for (int t = 0; t < NTAB; t++)
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    cplx v = cmul(cplx{FFr[t][r], FFi[t][r]}, py);
+                    sFF[wave][t][0][cc * 16 + drow(q, r)] = v.x;
+                    sFF[wave][t][1][cc * 16 + drow(q, r)] = v.y;
+                }
+            __builtin_amdgcn_wave_barrier();
+#pragma unroll
+            for (int s = 0; s < 4; s++) {
+                const int m2s = 16 * tN + 4 * s + q;
+                cplx eu[NTAB];
+#pragma unroll
+                for (int u = 0; u < NTAB; u++)
+                    eu[u] = (m2s < H && cc < n) ? E[((size_t)u * P + m2s) * n + cc] : cplx{0, 0};
+#pragma unroll
+                for (int o = 0; o < NOUT; o++) {
+                    const int v = vv[o], u = uu[o];
+                    double fr = sFF[wave][v][0][(4 * s + q) * 16 + cc];
+                    double fi = sFF[wave][v][1][(4 * s + q) * 16 + cc];
+                    out[o] = mfma(eu[u].x, fr, out[o]);
+                    out[o] = mfma(-eu[u].y, fi, out[o]);
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
+        }
+        // out^T holds [k2 = drow(q, r)][k1 = cc]
+        if (cc < n) {
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                int k2 = drow(q, r);
+                if (k2 < n) {
+                    size_t pb = (size_t)lf.box * npbox + cc + (size_t)n * k2;
+                    pot[pb * nd + d] += 2.0 * out[0][r];
+                    if (NTAB >= 2) {
+                        grad[(pb * 2 + 0) * nd + d] += 2.0 * out[1][r];
+                        grad[(pb * 2 + 1) * nd + d] += 2.0 * out[2][r];
+                    }
+                    if (NTAB >= 3) {
+                        hess[(pb * 3 + 0) * nd + d] += 2.0 * out[3][r];
+                        hess[(pb * 3 + 1) * nd + d] += 2.0 * out[4][r];
+                        hess[(pb * 3 + 2) * nd + d] += 2.0 * out[5][r];
+                    }
+                }
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// vector-FMA kernels (leaf orders above 16 or expansions above 128 waves)
+// ---------------------------------------------------------------------------
+
+constexpr int kVecThreads = 256;
+
 template <int kUpMaxE>
-__global__ __launch_bounds__(kUpThreads) void k_up(int n, int npw, int nd, const double* __restrict__ fvals,
-                                                   const PwLeaf* __restrict__ leaves,
-                                                   const UpChunk* __restrict__ chunks,
-                                                   const cplx* __restrict__ tpoly, const double* __restrict__ ts,
-                                                   cplx* __restrict__ partial) {
+__global__ __launch_bounds__(kVecThreads) void k_up_vec(int n, int npw, int nd, const double* __restrict__ fvals,
+                                                        const PwLeaf* __restrict__ leaves,
+                                                        const UpChunk* __restrict__ chunks,
+                                                        const cplx* __restrict__ tpoly, const double* __restrict__ ts,
+                                                        cplx* __restrict__ partial) {
     extern __shared__ double smem[];
     const int H = (npw + 1) / 2;
     const int nexp = npw * H;
     const int npbox = n * n;
-    double* sf = smem;                           // n*n
-    cplx* sT = reinterpret_cast<cplx*>(sf + npbox);  // npw*n : sT[k*n + m]
-    cplx* sff = sT + npw * n;                    // n*H : sff[m1 + n*k2]
-    cplx* spx = sff + n * H;                     // npw
-    cplx* spy = spx + npw;                       // H
+    double* sf = smem;                               // n*n
+    cplx* sT = reinterpret_cast<cplx*>(sf + npbox);  // n*npw : sT[m*npw + k]
+    cplx* sff = sT + npw * n;                        // n*H : sff[m1 + n*k2]
+    cplx* spx = sff + n * H;                         // npw
+    cplx* spy = spx + npw;                           // H
     const UpChunk ch = chunks[blockIdx.x];
     const int tid = threadIdx.x;
     for (int d = 0; d < nd; d++) {
@@ -65,37 +295,34 @@ __global__ __launch_bounds__(kUpThreads) void k_up(int n, int npw, int nd, const
             __syncthreads();
             if (lf.level != cur_level) {
                 const cplx* src = tpoly + (size_t)lf.level * npw * n;
-                for (int e = tid; e < npw * n; e += kUpThreads) sT[e] = src[e];
+                for (int e = tid; e < npw * n; e += kVecThreads) sT[e] = src[e];
                 cur_level = lf.level;
             }
             const double* fb = fvals + (size_t)lf.box * npbox * nd;
-            for (int e = tid; e < npbox; e += kUpThreads) sf[e] = fb[(size_t)e * nd + d];
-            for (int k = tid; k < npw; k += kUpThreads) spx[k] = cexpi(-ts[k] * lf.dx);
-            for (int k = tid; k < H; k += kUpThreads) spy[k] = cexpi(-ts[k] * lf.dy);
+            for (int e = tid; e < npbox; e += kVecThreads) sf[e] = fb[(size_t)e * nd + d];
+            for (int k = tid; k < npw; k += kVecThreads) spx[k] = cexpi(-ts[k] * lf.dx);
+            for (int k = tid; k < H; k += kVecThreads) spy[k] = cexpi(-ts[k] * lf.dy);
             __syncthreads();
-            // y-transform to the half spectrum, y phase folded in
-            for (int e = tid; e < n * H; e += kUpThreads) {
+            for (int e = tid; e < n * H; e += kVecThreads) {
                 int m1 = e % n, k2 = e / n;
                 double sr = 0.0, si = 0.0;
-                const cplx* trow = sT + k2 * n;
                 for (int m2 = 0; m2 < n; m2++) {
                     double f = sf[m1 + n * m2];
-                    sr = fma(trow[m2].x, f, sr);
-                    si = fma(trow[m2].y, f, si);
+                    cplx t = sT[m2 * npw + k2];
+                    sr = fma(t.x, f, sr);
+                    si = fma(t.y, f, si);
                 }
                 sff[e] = cmul(cplx{sr, si}, spy[k2]);
             }
             __syncthreads();
-            // x-transform to the full spectrum, x phase folded in
 #pragma unroll
             for (int r = 0; r < kUpMaxE; r++) {
-                int e = tid + r * kUpThreads;
+                int e = tid + r * kVecThreads;
                 if (e < nexp) {
                     int k1 = e % npw, k2 = e / npw;
-                    const cplx* trow = sT + k1 * n;
                     const cplx* fcol = sff + n * k2;
                     cplx s{0.0, 0.0};
-                    for (int m1 = 0; m1 < n; m1++) s = cfma(trow[m1], fcol[m1], s);
+                    for (int m1 = 0; m1 < n; m1++) s = cfma(sT[m1 * npw + k1], fcol[m1], s);
                     acc[r] = cfma(s, spx[k1], acc[r]);
                 }
             }
@@ -103,11 +330,112 @@ __global__ __launch_bounds__(kUpThreads) void k_up(int n, int npw, int nd, const
         cplx* out = partial + ((size_t)ch.slot * nd + d) * nexp;
 #pragma unroll
         for (int r = 0; r < kUpMaxE; r++) {
-            int e = tid + r * kUpThreads;
-            if (e < nexp) out[e] = acc[r];
+            int e = tid + r * kVecThreads;
+            if (e < nexp) out[(size_t)(e % npw) * H + e / npw] = acc[r];
         }
     }
 }
+
+constexpr int kDownChunk = 8;  // y-frequency columns staged in LDS at a time
+
+template <int NQ>
+__global__ __launch_bounds__(kVecThreads) void k_down_vec(int n, int npw, int nd, int ntab,
+                                                          const PwLeaf* __restrict__ leaves,
+                                                          const cplx* __restrict__ tpw, const double* __restrict__ ts,
+                                                          const cplx* __restrict__ loc, double* __restrict__ pot,
+                                                          double* __restrict__ grad, double* __restrict__ hess) {
+    extern __shared__ double smem[];
+    const int H = (npw + 1) / 2;
+    const int npw2 = npw / 2;
+    const int nexp = npw * H;
+    const int npbox = n * n;
+    cplx* sE = reinterpret_cast<cplx*>(smem);  // ntab*npw*n : [t][k][i]
+    cplx* sL = sE + ntab * npw * n;            // npw*kDownChunk : [mm][m1]
+    cplx* sff = sL + npw * kDownChunk;         // ntab*kDownChunk*n : [t][mm][k1]
+    cplx* spx = sff + ntab * kDownChunk * n;   // npw
+    cplx* spy = spx + npw;                     // H
+    const PwLeaf lf = leaves[blockIdx.x];
+    const int tid = threadIdx.x;
+    const cplx* E = tpw + (size_t)lf.level * 3 * npw * n;
+    for (int e = tid; e < ntab * npw * n; e += kVecThreads) sE[e] = E[e];
+    for (int k = tid; k < npw; k += kVecThreads) spx[k] = cexpi(ts[k] * lf.dx);
+    for (int k = tid; k < H; k += kVecThreads) spy[k] = cexpi(ts[k] * lf.dy);
+    for (int d = 0; d < nd; d++) {
+        double acc[NQ][6];
+#pragma unroll
+        for (int r = 0; r < NQ; r++)
+#pragma unroll
+            for (int c = 0; c < 6; c++) acc[r][c] = 0.0;
+        const cplx* L = loc + ((size_t)lf.anc * nd + d) * nexp;
+        for (int c0 = 0; c0 < H; c0 += kDownChunk) {
+            const int mc = min(kDownChunk, H - c0);
+            __syncthreads();
+            for (int e = tid; e < npw * mc; e += kVecThreads) {
+                int m1 = e % npw, mm = e / npw;
+                sL[mm * npw + m1] = cmul(L[(size_t)m1 * H + c0 + mm], spx[m1]);
+            }
+            __syncthreads();
+            for (int e = tid; e < ntab * mc * n; e += kVecThreads) {
+                int t = e / (mc * n), r = e % (mc * n);
+                int mm = r / n, k1 = r % n;
+                const cplx* Et = sE + t * npw * n;
+                const cplx* Lc = sL + mm * npw;
+                cplx s{0.0, 0.0};
+                for (int m1 = 0; m1 < npw; m1++) s = cfma(Et[m1 * n + k1], Lc[m1], s);
+                sff[(t * kDownChunk + mm) * n + k1] = cmul(s, spy[c0 + mm]);
+            }
+            __syncthreads();
+#pragma unroll
+            for (int r = 0; r < NQ; r++) {
+                int q = tid + r * kVecThreads;
+                if (q < npbox) {
+                    int k1 = q % n, k2 = q / n;
+                    for (int mm = 0; mm < mc; mm++) {
+                        int m2 = c0 + mm;
+                        double w = (m2 < npw2) ? 1.0 : 0.5;
+                        cplx e0 = sE[(0 * npw + m2) * n + k2];
+                        cplx f0 = sff[(0 * kDownChunk + mm) * n + k1];
+                        acc[r][0] += w * (e0.x * f0.x - e0.y * f0.y);
+                        if (ntab >= 2) {
+                            cplx e1 = sE[(1 * npw + m2) * n + k2];
+                            cplx f1 = sff[(1 * kDownChunk + mm) * n + k1];
+                            acc[r][1] += w * (e0.x * f1.x - e0.y * f1.y);
+                            acc[r][2] += w * (e1.x * f0.x - e1.y * f0.y);
+                            if (ntab >= 3) {
+                                cplx e2 = sE[(2 * npw + m2) * n + k2];
+                                cplx f2 = sff[(2 * kDownChunk + mm) * n + k1];
+                                acc[r][3] += w * (e0.x * f2.x - e0.y * f2.y);
+                                acc[r][4] += w * (e1.x * f1.x - e1.y * f1.y);
+                                acc[r][5] += w * (e2.x * f0.x - e2.y * f0.y);
+                            }
+                        }
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < NQ; r++) {
+            int q = tid + r * kVecThreads;
+            if (q < npbox) {
+                size_t pb = (size_t)lf.box * npbox + q;
+                pot[pb * nd + d] += 2.0 * acc[r][0];
+                if (ntab >= 2) {
+                    grad[(pb * 2 + 0) * nd + d] += 2.0 * acc[r][1];
+                    grad[(pb * 2 + 1) * nd + d] += 2.0 * acc[r][2];
+                }
+                if (ntab >= 3) {
+                    hess[(pb * 3 + 0) * nd + d] += 2.0 * acc[r][3];
+                    hess[(pb * 3 + 1) * nd + d] += 2.0 * acc[r][4];
+                    hess[(pb * 3 + 2) * nd + d] += 2.0 * acc[r][5];
+                }
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// expansion-level stages
+// ---------------------------------------------------------------------------
 
 // mp of each cutoff-level box = sum of its chunks' partial sums (in chunk order).
 __global__ void k_reduce(int nd, int nexp, const int* __restrict__ anc_chunk, const cplx* __restrict__ partial,
@@ -144,112 +472,9 @@ __global__ void k_m2l(int nd, int nexp, const int* __restrict__ m2l_ptr, const i
     }
 }
 
-constexpr int kDownThreads = 256;
-constexpr int kDownChunk = 8;  // y-frequency columns staged in LDS at a time
-
-// Local expansion of the cutoff-level ancestor -> pot/grad/hess on one leaf
-// (split + evaluation, bfgt/boxfgt.f:845-900, bfgt/bfgt_pwrouts.f:549-784).
-// The expansion is streamed through LDS kDownChunk y-frequency columns at a
-// time; each thread keeps the running sums of its NQ output points.
-template <int NQ>
-__global__ __launch_bounds__(kDownThreads) void k_down(int n, int npw, int nd, int ntab,
-                                                       const PwLeaf* __restrict__ leaves,
-                                                       const cplx* __restrict__ tpw, const double* __restrict__ ts,
-                                                       const cplx* __restrict__ loc, double* __restrict__ pot,
-                                                       double* __restrict__ grad, double* __restrict__ hess) {
-    extern __shared__ double smem[];
-    const int H = (npw + 1) / 2;
-    const int npw2 = npw / 2;
-    const int nexp = npw * H;
-    const int npbox = n * n;
-    cplx* sE = reinterpret_cast<cplx*>(smem);  // ntab*npw*n : [t][k][i]
-    cplx* sL = sE + ntab * npw * n;            // npw*kDownChunk : [mm][m1]
-    cplx* sff = sL + npw * kDownChunk;         // ntab*kDownChunk*n : [t][mm][k1]
-    cplx* spx = sff + ntab * kDownChunk * n;   // npw
-    cplx* spy = spx + npw;                     // H
-    const PwLeaf lf = leaves[blockIdx.x];
-    const int tid = threadIdx.x;
-    const cplx* E = tpw + (size_t)lf.level * 3 * npw * n;
-    for (int e = tid; e < ntab * npw * n; e += kDownThreads) sE[e] = E[e];
-    for (int k = tid; k < npw; k += kDownThreads) spx[k] = cexpi(ts[k] * lf.dx);
-    for (int k = tid; k < H; k += kDownThreads) spy[k] = cexpi(ts[k] * lf.dy);
-    for (int d = 0; d < nd; d++) {
-        double acc[NQ][6];
-#pragma unroll
-        for (int r = 0; r < NQ; r++)
-#pragma unroll
-            for (int c = 0; c < 6; c++) acc[r][c] = 0.0;
-        const cplx* L = loc + ((size_t)lf.anc * nd + d) * nexp;
-        for (int c0 = 0; c0 < H; c0 += kDownChunk) {
-            const int mc = min(kDownChunk, H - c0);
-            __syncthreads();
-            for (int e = tid; e < npw * mc; e += kDownThreads) {
-                int m1 = e % npw, mm = e / npw;
-                sL[mm * npw + m1] = cmul(L[(size_t)(c0 + mm) * npw + m1], spx[m1]);
-            }
-            __syncthreads();
-            // x contraction of the staged columns, y phase folded in
-            for (int e = tid; e < ntab * mc * n; e += kDownThreads) {
-                int t = e / (mc * n), r = e % (mc * n);
-                int mm = r / n, k1 = r % n;
-                const cplx* Et = sE + t * npw * n;
-                const cplx* Lc = sL + mm * npw;
-                cplx s{0.0, 0.0};
-                for (int m1 = 0; m1 < npw; m1++) s = cfma(Et[m1 * n + k1], Lc[m1], s);
-                sff[(t * kDownChunk + mm) * n + k1] = cmul(s, spy[c0 + mm]);
-            }
-            __syncthreads();
-            // y contraction into the per-point sums (real parts; the middle
-            // frequency of an odd expansion counts half)
-#pragma unroll
-            for (int r = 0; r < NQ; r++) {
-                int q = tid + r * kDownThreads;
-                if (q < npbox) {
-                    int k1 = q % n, k2 = q / n;
-                    for (int mm = 0; mm < mc; mm++) {
-                        int m2 = c0 + mm;
-                        double w = (m2 < npw2) ? 1.0 : 0.5;
-                        cplx e0 = sE[(0 * npw + m2) * n + k2];
-                        cplx f0 = sff[(0 * kDownChunk + mm) * n + k1];
-                        acc[r][0] += w * (e0.x * f0.x - e0.y * f0.y);
-                        if (ntab >= 2) {
-                            cplx e1 = sE[(1 * npw + m2) * n + k2];
-                            cplx f1 = sff[(1 * kDownChunk + mm) * n + k1];
-                            acc[r][1] += w * (e0.x * f1.x - e0.y * f1.y);
-                            acc[r][2] += w * (e1.x * f0.x - e1.y * f0.y);
-                            if (ntab >= 3) {
-                                cplx e2 = sE[(2 * npw + m2) * n + k2];
-                                cplx f2 = sff[(2 * kDownChunk + mm) * n + k1];
-                                acc[r][3] += w * (e0.x * f2.x - e0.y * f2.y);
-                                acc[r][4] += w * (e1.x * f1.x - e1.y * f1.y);
-                                acc[r][5] += w * (e2.x * f0.x - e2.y * f0.y);
-                            }
-                        }
-                    }
-                }
-            }
-        }
-#pragma unroll
-        for (int r = 0; r < NQ; r++) {
-            int q = tid + r * kDownThreads;
-            if (q < npbox) {
-                size_t pb = (size_t)lf.box * npbox + q;
-                pot[pb * nd + d] += 2.0 * acc[r][0];
-                if (ntab >= 2) {
-                    grad[(pb * 2 + 0) * nd + d] += 2.0 * acc[r][1];
-                    grad[(pb * 2 + 1) * nd + d] += 2.0 * acc[r][2];
-                }
-                if (ntab >= 3) {
-                    hess[(pb * 3 + 0) * nd + d] += 2.0 * acc[r][3];
-                    hess[(pb * 3 + 1) * nd + d] += 2.0 * acc[r][4];
-                    hess[(pb * 3 + 2) * nd + d] += 2.0 * acc[r][5];
-                }
-            }
-        }
-    }
-}
-
 }  // namespace
+
+bool use_mfma(const Plan& p) { return p.n <= 16 && (p.npw + 1) / 2 <= 16 * kUpMaxTM; }
 
 size_t up_smem_bytes(int n, int npw) {
     int H = (npw + 1) / 2;
@@ -261,27 +486,38 @@ size_t down_smem_bytes(int n, int npw, int ntab) {
     return sizeof(cplx) * ((size_t)ntab * npw * n + (size_t)npw * kDownChunk + (size_t)ntab * kDownChunk * n + npw + H);
 }
 
-int up_max_out() { return kUpMaxOut; }
+int up_max_out() { return 32 * kVecThreads; }
 
 template <int E>
-static int launch_up_t(Plan& p, const double* fvals, size_t sh) {
+static int launch_up_vec(Plan& p, const double* fvals, size_t sh) {
     if (sh > 65536)
-        BFGT_HIP_CHECK(hipFuncSetAttribute((const void*)k_up<E>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
-    k_up<E><<<p.nchunks, kUpThreads, sh, p.stream>>>(p.n, p.npw, p.nd, fvals, p.d_pwleaf, p.d_chunks, p.d_tpoly,
-                                                     p.d_ts, p.d_partial);
+        BFGT_HIP_CHECK(hipFuncSetAttribute((const void*)k_up_vec<E>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
+    k_up_vec<E><<<p.nchunks, kVecThreads, sh, p.stream>>>(p.n, p.npw, p.nd, fvals, p.d_pwleaf, p.d_chunks, p.d_tpoly,
+                                                          p.d_ts, p.d_partial);
     BFGT_HIP_CHECK(hipGetLastError());
     return BFGT_OK;
 }
 
 int launch_upward(Plan& p, const double* fvals) {
     if (p.nchunks == 0) return BFGT_OK;
+    if (use_mfma(p)) {
+        int H = (p.npw + 1) / 2;
+        int nTM = (H + 15) / 16, nTN = (H + 15) / 16;
+        int nitems = p.nchunks * nTN;
+        int blocks = (nitems + kWavesPerBlock - 1) / kWavesPerBlock;
+        k_up_mfma<<<blocks, kWave * kWavesPerBlock, 0, p.stream>>>(p.n, p.npw, p.nd, nTM, nTN, nitems, fvals,
+                                                                  p.d_pwleaf, p.d_chunks, p.d_tpoly, p.d_phase,
+                                                                  p.d_levbase, p.d_partial);
+        BFGT_HIP_CHECK(hipGetLastError());
+        return BFGT_OK;
+    }
     size_t sh = up_smem_bytes(p.n, p.npw);
-    int e = (p.nexp + kUpThreads - 1) / kUpThreads;
-    if (e <= 2) return launch_up_t<2>(p, fvals, sh);
-    if (e <= 4) return launch_up_t<4>(p, fvals, sh);
-    if (e <= 8) return launch_up_t<8>(p, fvals, sh);
-    if (e <= 16) return launch_up_t<16>(p, fvals, sh);
-    if (e <= 32) return launch_up_t<32>(p, fvals, sh);
+    int e = (p.nexp + kVecThreads - 1) / kVecThreads;
+    if (e <= 2) return launch_up_vec<2>(p, fvals, sh);
+    if (e <= 4) return launch_up_vec<4>(p, fvals, sh);
+    if (e <= 8) return launch_up_vec<8>(p, fvals, sh);
+    if (e <= 16) return launch_up_vec<16>(p, fvals, sh);
+    if (e <= 32) return launch_up_vec<32>(p, fvals, sh);
     return set_error(BFGT_ERR_RESOURCE, "plane-wave expansion too long for the leaf kernels");
 }
 
@@ -302,22 +538,37 @@ int launch_m2l(Plan& p) {
 }
 
 template <int NQ>
-static int launch_down_t(Plan& p, double* pot, double* grad, double* hess, size_t sh) {
+static int launch_down_vec(Plan& p, double* pot, double* grad, double* hess, size_t sh) {
     if (sh > 65536)
-        BFGT_HIP_CHECK(hipFuncSetAttribute((const void*)k_down<NQ>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
-    k_down<NQ><<<p.npwleaf, kDownThreads, sh, p.stream>>>(p.n, p.npw, p.nd, p.ifpgh, p.d_pwleaf, p.d_tpw, p.d_ts,
-                                                          p.d_loc, pot, grad, hess);
+        BFGT_HIP_CHECK(hipFuncSetAttribute((const void*)k_down_vec<NQ>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
+    k_down_vec<NQ><<<p.npwleaf, kVecThreads, sh, p.stream>>>(p.n, p.npw, p.nd, p.ifpgh, p.d_pwleaf, p.d_tpw, p.d_ts,
+                                                              p.d_loc, pot, grad, hess);
+    BFGT_HIP_CHECK(hipGetLastError());
+    return BFGT_OK;
+}
+
+template <int NTAB>
+static int launch_down_mfma(Plan& p, double* pot, double* grad, double* hess) {
+    int blocks = (p.npwleaf + kWavesPerBlock - 1) / kWavesPerBlock;
+    k_down_mfma<NTAB><<<blocks, kWave * kWavesPerBlock, 0, p.stream>>>(p.n, p.npw, p.nd, p.npwleaf, p.d_pwleaf,
+                                                                      p.d_tpw, p.d_phase, p.d_levbase, p.d_loc, pot,
+                                                                      grad, hess);
     BFGT_HIP_CHECK(hipGetLastError());
     return BFGT_OK;
 }
 
 int launch_downward(Plan& p, double* pot, double* grad, double* hess) {
     if (p.npwleaf == 0) return BFGT_OK;
+    if (use_mfma(p)) {
+        if (p.ifpgh == 1) return launch_down_mfma<1>(p, pot, grad, hess);
+        if (p.ifpgh == 2) return launch_down_mfma<2>(p, pot, grad, hess);
+        return launch_down_mfma<3>(p, pot, grad, hess);
+    }
     size_t sh = down_smem_bytes(p.n, p.npw, p.ifpgh);
-    int q = (p.n * p.n + kDownThreads - 1) / kDownThreads;
-    if (q <= 1) return launch_down_t<1>(p, pot, grad, hess, sh);
-    if (q <= 2) return launch_down_t<2>(p, pot, grad, hess, sh);
-    return launch_down_t<4>(p, pot, grad, hess, sh);
+    int q = (p.n * p.n + kVecThreads - 1) / kVecThreads;
+    if (q <= 1) return launch_down_vec<1>(p, pot, grad, hess, sh);
+    if (q <= 2) return launch_down_vec<2>(p, pot, grad, hess, sh);
+    return launch_down_vec<4>(p, pot, grad, hess, sh);
 }
 
 }  // namespace bfgt

@@ -169,7 +169,8 @@ __device__ void sph_besselj(int n, double x, double* fj) {
     for (int k = 0; k < n; k++) fj[k] *= sc;
 }
 
-// values -> PW tables.  grid (npw - npw/2, nlev_pw), block 64.
+// values -> PW tables, layout [level][m][k] (node m, PW index k fastest).
+// grid (npw - npw/2, nlev_pw), block 64.
 __global__ void k_poly2pw(int n, int ipoly, int npw, int lev0, const double* ws, const double* ts, double delta0,
                           const double* boxsize, const double* nodes, cplx* tab) {
     NodeLayout L{n};
@@ -240,10 +241,11 @@ __global__ void k_poly2pw(int n, int ipoly, int npw, int lev0, const double* ws,
             sr += tre[k] * u;
             si += tim[k] * u;
         }
+        // layout [level][m][k]: the PW index is the fastest
         size_t base = (size_t)lev * npw * n;
-        tab[base + (size_t)j * n + t] = cplx{sr, si};
+        tab[base + (size_t)t * npw + j] = cplx{sr, si};
         int jm = npw - 1 - j;
-        if (jm != j) tab[base + (size_t)jm * n + t] = cplx{sr, -si};
+        if (jm != j) tab[base + (size_t)t * npw + jm] = cplx{sr, -si};
     }
 }
 
@@ -555,14 +557,15 @@ __global__ void k_loctab(double eps, int ipoly, int n, double delta, const doubl
 }
 
 // mp -> loc diagonal shifts for offsets (kx, ky) in {-1,0,1}^2 at the cutoff
-// level, index (ky+1)*3 + (kx+1); element j2 + npw*j1 (x frequency fastest).
+// level, index (ky+1)*3 + (kx+1); element j2*H + j1 for x frequency j2 < npw
+// and retained y frequency j1 < H (the device expansion layout).
 __global__ void k_m2l_shifts(int npw, double xmin, const double* ts, cplx* w) {
     int H = (npw + 1) / 2;
     int nexp = npw * H;
     int k = blockIdx.x;
     int kx = k % 3 - 1, ky = k / 3 - 1;
     for (int e = threadIdx.x; e < nexp; e += blockDim.x) {
-        int j2 = e % npw, j1 = e / npw;
+        int j2 = e / H, j1 = e % H;
         double sx, cx, sy, cy;
         sincos(ts[j2] * xmin, &sx, &cx);
         sincos(ts[j1] * xmin, &sy, &cy);
@@ -573,6 +576,8 @@ __global__ void k_m2l_shifts(int npw, double xmin, const double* ts, cplx* w) {
 }
 
 }  // namespace
+
+int build_phase_tables(Plan& p);
 
 int build_tables(Plan& p) {
     hipEvent_t e0, e1;
@@ -599,6 +604,8 @@ int build_tables(Plan& p) {
         double xmin = p.boxsize[p.nlevstart] / std::sqrt(p.delta0);
         k_m2l_shifts<<<9, 256, 0, p.stream>>>(p.npw, xmin, p.d_ts, p.d_wshift);
         BFGT_HIP_CHECK(hipGetLastError());
+        int rc = build_phase_tables(p);
+        if (rc) return rc;
     }
     size_t nind = (size_t)(p.nlevels + 1) * 13 * 2 * (n + 1);
     p.h_indloc.resize(nind);
@@ -650,6 +657,41 @@ int build_pw_nodes(Plan& p) {
     bool periodic = (p.iperiod_in == 1 && p.npwlevel == 0);
     k_pwnodes<<<(p.npw + 255) / 256, 256, 0, p.stream>>>(p.npw, periodic ? 1 : 0, p.pmax, p.boxsize[0], p.delta,
                                                          p.d_ws, p.d_ts);
+    BFGT_HIP_CHECK(hipGetLastError());
+    return BFGT_OK;
+}
+
+}  // namespace bfgt
+
+namespace bfgt {
+namespace {
+
+// Leaf-to-ancestor phases: for every level l >= L0 and every cell j of a
+// level-L0 box at level l, phase[levbase[l] + j*npw + k] =
+// exp(i t_k ((j + 1/2) b_l - b_L0/2) / sqrt(delta0)) -- the product of the
+// per-level merge/split diagonals (bfgt/bfgt_pwrouts.f:1645-1703) between a
+// leaf and its cutoff-level ancestor, evaluated directly.
+__global__ void k_phase_tables(int npw, int lev0, int nlev, const double* ts, const double* boxsize,
+                               double delta0, const int* levbase, cplx* phase) {
+    int l = lev0 + blockIdx.y;
+    if (l > nlev) return;
+    int nj = 1 << (l - lev0);
+    double sq = sqrt(delta0);
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < nj * npw; e += gridDim.x * blockDim.x) {
+        int j = e / npw, k = e % npw;
+        double off = ((j + 0.5) * boxsize[l] - 0.5 * boxsize[lev0]) / sq;
+        double s, c;
+        sincos(ts[k] * off, &s, &c);
+        phase[levbase[l] + e] = cplx{c, s};
+    }
+}
+
+}  // namespace
+
+int build_phase_tables(Plan& p) {
+    int nl = p.nlevels - p.nlevstart + 1;
+    k_phase_tables<<<dim3(64, nl), 256, 0, p.stream>>>(p.npw, p.nlevstart, p.nlevels, p.d_ts, p.d_boxsize,
+                                                      p.delta0, p.d_levbase, p.d_phase);
     BFGT_HIP_CHECK(hipGetLastError());
     return BFGT_OK;
 }

@@ -99,10 +99,14 @@ int bfgt_plan_profile_end(bfgt_plan_t plan, double *stage_seconds, int *nsteps);
 int bfgt_plan_query(bfgt_plan_t plan, int *ivals, double *dvals);
 
 /* Copy one device table of one level to host memory (parity tests of the FP64
- * table generation).  which: 0 values->PW (npw x n complex, [k][m]),
+ * table generation).  which: 0 values->PW (n x npw complex, [m][k]),
  * 1 PW->pot/d/dd (3 x npw x n complex, [t][k][i]), 2 near-field masked
  * tables (13 x 3 x n x n double, [k+6][t][j][m]), 3 near-field sparsity
- * bounds (13 x (n+1) x 2 int, 1-based), 4 M2L shifts (9 x nexp complex). */
+ * bounds (13 x (n+1) x 2 int, 1-based), 4 M2L shifts (9 x npw x H complex,
+ * [offset][x frequency][y frequency], H = (npw+1)/2), 5 leaf-to-ancestor
+ * phases of one level (2^(level-nlevstart) x npw complex), 6 / 7 the
+ * cutoff-level mp / loc expansions of the last execution (n_ancestors x nd x
+ * npw x H complex; `level` ignored). */
 int bfgt_plan_get_table(bfgt_plan_t plan, int which, int level, void *host_out, size_t nbytes);
 
 const char *bfgt_last_error(void);

@@ -150,8 +150,8 @@ def test_gpu_tables_match_oracle(device):
     from heat2d_per_box_amd.tree import poly_nodes
 
     for lev in range(L0, t.nlevels + 1):
-        tp = plan.table(0, lev)  # [k][m]
-        ref = O.poly2pw(n, 0, ws, ts, w.delta, t.boxsize[lev]).T
+        tp = plan.table(0, lev)  # [m][k]
+        ref = O.poly2pw(n, 0, ws, ts, w.delta, t.boxsize[lev])
         assert np.max(np.abs(tp - ref)) < 1e-13 * np.max(np.abs(ref))
         e = plan.table(1, lev)
         r = O.pw2pgh(n, ts, poly_nodes(n, 0), w.delta, t.boxsize[lev])
