@@ -104,6 +104,8 @@ def main():
     ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5"])
     ap.add_argument("--scale", default="full", choices=["full", "small", "tiny"])
     ap.add_argument("--precision", default="fp64", choices=["fp64", "split32"])
+    ap.add_argument("--delta", type=float, default=None, help="override delta (config c3 only)")
+    ap.add_argument("--eps", type=float, default=None, help="override eps (config c3 only)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -120,7 +122,13 @@ def main():
     from heat2d_per_box_amd.densities import make_workload
 
     t0 = time.time()
-    w = make_workload(args.config, args.scale)
+    kw = {}
+    if args.config == "c3":
+        if args.delta is not None:
+            kw["delta"] = args.delta
+        if args.eps is not None:
+            kw["eps"] = args.eps
+    w = make_workload(args.config, args.scale, **kw)
     t_workload = time.time() - t0
     print(f"[bench] workload {args.config}/{args.scale}: {w.tree.nleaves} leaves, {w.npts} pts, built in "
           f"{t_workload:.1f}s", file=sys.stderr, flush=True)

@@ -120,7 +120,8 @@ class BoxFGT:
                 "n_ancestors", "n_near_targets", "n_near_pairs", "has_pw", "has_near", "iperiod_eff", "nboxes",
                 "precision"]
         self.info = PlanInfo({k: int(v) for k, v in zip(keys, iv)})
-        self.info.update({"delta0": dv[0], "pmax": dv[1], "t_plan_host": dv[2], "t_tables": dv[3]})
+        self.info.update({"delta0": dv[0], "pmax": dv[1], "t_plan_host": dv[2], "t_tables": dv[3],
+                          "mfma": bool(dv[4])})
 
     def _check_tensor(self, t, shape, name):
         import torch
@@ -168,7 +169,9 @@ class BoxFGT:
         if self.info["has_pw"]:
             shapes[5] = ((1 << max(level - self.info["nlevstart"], 0), npw), np.complex128)
             shapes[6] = ((self.info["n_ancestors"], self.nd, npw, H), np.complex128)
-            shapes[7] = shapes[6]
+            # local expansions as stored: mirror-folded (S, D) on the MFMA path
+            shapes[7] = (((self.info["n_ancestors"], self.nd, H, H, 2), np.complex128) if self.info["mfma"]
+                         else shapes[6])
         shape, dt = shapes[which]
         out = np.zeros(shape, dtype=dt)
         check(lib().bfgt_plan_get_table(self._h, which, level, out.ctypes.data_as(ctypes.c_void_p), out.nbytes))

@@ -87,6 +87,13 @@ struct Plan {
     UpChunk* d_chunks = nullptr;
     int* d_anc_box = nullptr;      // ancestor index -> box
     int* d_anc_chunk = nullptr;    // CSR over chunks [nanc+1]
+    // merge of chunk partials: level 1 sums groups of chunks (into mp
+    // directly when an ancestor has one group), level 2 sums the groups of
+    // ancestors with several
+    int nred1 = 0, nred2 = 0, ngslot = 0;
+    int4* d_red1 = nullptr;        // (anc, first chunk, count, dest slot or -1 for mp)
+    int4* d_red2 = nullptr;        // (anc, first group slot, count, 0)
+    cplx* d_gpart = nullptr;       // [ngslot][nd][nexp]
     int* d_m2l_ptr = nullptr;      // CSR [nanc+1]
     int2* d_m2l = nullptr;         // (source ancestor, shift index)
     int nnear_t = 0, nnear_p = 0;

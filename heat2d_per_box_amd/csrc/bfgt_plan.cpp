@@ -39,6 +39,8 @@ size_t up_smem_bytes(int n, int npw);
 size_t down_smem_bytes(int n, int npw, int ntab);
 size_t near_smem_bytes(int n, int ntab);
 int up_max_out();
+size_t loc_buffer_cplx(const Plan& p);
+bool use_mfma(const Plan& p);
 int build_pw_nodes(Plan& p);
 
 namespace {
@@ -398,6 +400,8 @@ int plan_create(Plan& p, int nd, int ndim, double delta, double eps, int ipoly, 
     std::vector<int> anc_box, anc_chunk;
     std::vector<int> m2l_ptr;
     std::vector<int2> m2l;
+    std::vector<int4> red1, red2;
+    int ngslot = 0;
     if (p.has_pw) {
         int L0 = p.nlevstart;
         int a0 = t.first(L0);
@@ -439,6 +443,18 @@ int plan_create(Plan& p, int nd, int ndim, double delta, double eps, int ipoly, 
             anc_chunk.push_back((int)chunks.size());
             anc_box.push_back(a0 + a);
         }
+        // two-level merge schedule of the chunk partial sums
+        for (int a = 0; a < p.nanc; a++) {
+            int c0 = anc_chunk[a], c1 = anc_chunk[a + 1], nc = c1 - c0;
+            if (nc <= 8) {
+                red1.push_back(make_int4(a, c0, nc, -1));
+                continue;
+            }
+            int G = std::max(8, (int)std::ceil(std::sqrt((double)nc)));
+            int g0 = ngslot;
+            for (int c = c0; c < c1; c += G) red1.push_back(make_int4(a, c, std::min(G, c1 - c), ngslot++));
+            red2.push_back(make_int4(a, g0, ngslot - g0, 0));
+        }
         // PW translation lists at the cutoff level (gnd_compute_listpw)
         double xmin = boxsize[L0];
         m2l_ptr.push_back(0);
@@ -457,6 +473,9 @@ int plan_create(Plan& p, int nd, int ndim, double delta, double eps, int ipoly, 
     p.npwleaf = (int)leaves.size();
     p.nchunks = (int)chunks.size();
     p.nm2l = (int)m2l.size();
+    p.nred1 = (int)red1.size();
+    p.nred2 = (int)red2.size();
+    p.ngslot = ngslot;
 
     // near-field schedule (bfgt/boxfgt.f:923-1071), pairs with empty tables dropped
     std::vector<int> ntgt, nlev, nptr;
@@ -534,6 +553,8 @@ int plan_create(Plan& p, int nd, int ndim, double delta, double eps, int ipoly, 
     if ((rc = upload(p, p.d_chunks, chunks))) return rc;
     if ((rc = upload(p, p.d_anc_box, anc_box))) return rc;
     if ((rc = upload(p, p.d_anc_chunk, anc_chunk))) return rc;
+    if ((rc = upload(p, p.d_red1, red1))) return rc;
+    if ((rc = upload(p, p.d_red2, red2))) return rc;
     if ((rc = upload(p, p.d_m2l_ptr, m2l_ptr))) return rc;
     if ((rc = upload(p, p.d_m2l, m2l))) return rc;
     if ((rc = upload(p, p.d_near_tgt, ntgt))) return rc;
@@ -543,8 +564,9 @@ int plan_create(Plan& p, int nd, int ndim, double delta, double eps, int ipoly, 
     if (p.has_pw) {
         size_t len = (size_t)nd * p.nexp;
         if ((rc = dalloc(p, p.d_partial, (size_t)p.nchunks * len))) return rc;
+        if ((rc = dalloc(p, p.d_gpart, (size_t)p.ngslot * len))) return rc;
         if ((rc = dalloc(p, p.d_mp, (size_t)p.nanc * len))) return rc;
-        if ((rc = dalloc(p, p.d_loc, (size_t)p.nanc * len))) return rc;
+        if ((rc = dalloc(p, p.d_loc, (size_t)p.nanc * loc_buffer_cplx(p)))) return rc;
     }
     BFGT_HIP_CHECK(hipStreamSynchronize(p.stream));
     auto t1 = std::chrono::steady_clock::now();
@@ -681,7 +703,7 @@ int bfgt_plan_query(bfgt_plan_t plan, int* iv, double* dv) {
         std::memcpy(iv, v, sizeof(v));
     }
     if (dv) {
-        double v[8] = {p.delta0, p.pmax, p.t_plan_host, p.t_tables, 0, 0, 0, 0};
+        double v[8] = {p.delta0, p.pmax, p.t_plan_host, p.t_tables, bfgt::use_mfma(p) ? 1.0 : 0.0, 0, 0, 0};
         std::memcpy(dv, v, sizeof(v));
     }
     return BFGT_OK;
@@ -725,7 +747,9 @@ int bfgt_plan_get_table(bfgt_plan_t plan, int which, int level, void* host_out, 
             src = p.d_mp;
             break;
         case 7:
-            sz = sizeof(bfgt::cplx) * (size_t)p.nanc * p.nd * p.nexp;
+            // MFMA path: mirror-folded (S, D) pairs [anc][d][ceil(npw/2)][H]; else [anc][d][npw][H]
+            sz = sizeof(bfgt::cplx) * (size_t)p.nanc * p.nd *
+                 (bfgt::use_mfma(p) ? 2 * (size_t)((p.npw + 1) / 2) * ((p.npw + 1) / 2) : (size_t)p.nexp);
             src = p.d_loc;
             break;
         default:

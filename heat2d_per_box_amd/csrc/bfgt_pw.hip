@@ -21,6 +21,8 @@
 // for the remaining sizes.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "bfgt_internal.h"
 
 namespace bfgt {
@@ -54,193 +56,171 @@ __device__ inline int drow(int q, int r) { return 4 * r + q; }
 
 constexpr int kUpMaxTM = 4;  // x-frequency tiles of 16 (ceil(npw/2) <= 64)
 
+// A leaf's y-frequency MFMA accumulator holds, in element r of lane (q, c),
+// row 4r + q -- exactly the B operand of K-step r of a following MFMA whose
+// K index is that row.  Both passes below chain two GEMMs through registers
+// this way, with no LDS staging.
+
 // Upward pass.  One wave per (chunk, y-frequency tile tN): for every leaf of
 // the chunk
-//   G  = F * Ty              (n x 16 tile, Ty = T diag(py), real x complex)
-//   out += Tx^T G            (Tx = T diag(px)), using T(m, npw-1-k) = conj T(m, k)
-// so that only ceil(npw/2) x frequencies are multiplied; the accumulators
-// AC, BD, AD, BC give both halves at the end.
+//   G  = F (T diag py)        (rows m1 < 16, columns k2 of the tile)
+//   out += (T diag px)^T G    with T(m, npw-1-k) = conj T(m, k), so that only
+// ceil(npw/2) x frequencies are multiplied; the accumulators AC, BD, AD, BC
+// give both halves of the x spectrum at the end.  NKS = ceil(n/4),
+// NTM = ceil(ceil(npw/2)/16) x-frequency tiles.
+template <int NKS, int NTM>
 __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_up_mfma(
-    int n, int P, int nd, int nTM, int nTN, int nitems, const double* __restrict__ fvals,
+    int n, int P, int nd, int nTN, int nitems, const double* __restrict__ fvals,
     const PwLeaf* __restrict__ leaves, const UpChunk* __restrict__ chunks, const cplx* __restrict__ tpoly,
     const cplx* __restrict__ phase, const int* __restrict__ levbase, cplx* __restrict__ partial) {
-    __shared__ double sG[kWavesPerBlock][2][256];
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
     const int item = blockIdx.x * kWavesPerBlock + wave;
     if (item >= nitems) return;  // no block-wide barriers in this kernel
     const int ci = item / nTN, tN = item % nTN;
     const int q = lane >> 4, cc = lane & 15;
     const int H = (P + 1) / 2, Mlo = (P + 1) / 2, nexp = P * H, npbox = n * n;
-    const int nks = (n + 3) / 4;
     const int k2 = 16 * tN + cc;
-    double* gre = sG[wave][0];
-    double* gim = sG[wave][1];
+    const bool k2ok = k2 < H;
     const UpChunk ch = chunks[ci];
     for (int d = 0; d < nd; d++) {
-        v4d acc[4][kUpMaxTM];
+        v4d acc[4][NTM];
 #pragma unroll
         for (int p = 0; p < 4; p++)
 #pragma unroll
-            for (int t = 0; t < kUpMaxTM; t++) acc[p][t] = v4d{0, 0, 0, 0};
+            for (int t = 0; t < NTM; t++) acc[p][t] = v4d{0, 0, 0, 0};
         for (int li = 0; li < ch.count; li++) {
             const PwLeaf lf = leaves[ch.first + li];
             const cplx* T = tpoly + (size_t)lf.level * n * P;
             const cplx* ph = phase + levbase[lf.level];
             const double* f = fvals + (size_t)lf.box * npbox * nd + d;
             // y phase of this lane's column (upward shift: conjugate of the table)
-            cplx py = (k2 < H) ? ph[(size_t)lf.jy * P + k2] : cplx{0, 0};
+            cplx py = k2ok ? ph[lf.jy * P + k2] : cplx{0, 0};
             py.y = -py.y;
             v4d Gr = {0, 0, 0, 0}, Gi = {0, 0, 0, 0};
-            for (int s = 0; s < nks; s++) {
-                int m2 = 4 * s + q;
-                double a = (cc < n && m2 < n) ? f[(size_t)(cc + n * m2) * nd] : 0.0;
-                cplx ty = (m2 < n && k2 < H) ? cmul(T[(size_t)m2 * P + k2], py) : cplx{0, 0};
+#pragma unroll
+            for (int s = 0; s < NKS; s++) {
+                const int m2 = 4 * s + q;
+                const bool mok = m2 < n;
+                double a = (cc < n && mok) ? f[(cc + n * m2) * nd] : 0.0;
+                cplx ty = (mok && k2ok) ? cmul(T[m2 * P + k2], py) : cplx{0, 0};
                 Gr = mfma(a, ty.x, Gr);
                 Gi = mfma(a, ty.y, Gi);
             }
 #pragma unroll
-            for (int r = 0; r < 4; r++) {
-                gre[drow(q, r) * 16 + cc] = Gr[r];
-                gim[drow(q, r) * 16 + cc] = Gi[r];
-            }
-            __builtin_amdgcn_wave_barrier();
+            for (int t = 0; t < NTM; t++) {
+                const int k1 = 16 * t + cc;
+                const bool k1ok = k1 < Mlo;
+                cplx px = k1ok ? ph[lf.jx * P + k1] : cplx{0, 0};
+                px.y = -px.y;
 #pragma unroll
-            for (int t = 0; t < kUpMaxTM; t++) {
-                if (t < nTM) {
-                    int k1 = 16 * t + cc;
-                    cplx px = (k1 < Mlo) ? ph[(size_t)lf.jx * P + k1] : cplx{0, 0};
-                    px.y = -px.y;
-                    for (int s = 0; s < nks; s++) {
-                        int m1 = 4 * s + q;
-                        cplx tx = (m1 < n && k1 < Mlo) ? cmul(T[(size_t)m1 * P + k1], px) : cplx{0, 0};
-                        double br = gre[m1 * 16 + cc], bi = gim[m1 * 16 + cc];
-                        acc[0][t] = mfma(tx.x, br, acc[0][t]);
-                        acc[1][t] = mfma(tx.y, bi, acc[1][t]);
-                        acc[2][t] = mfma(tx.x, bi, acc[2][t]);
-                        acc[3][t] = mfma(tx.y, br, acc[3][t]);
-                    }
+                for (int s = 0; s < NKS; s++) {
+                    const int m1 = 4 * s + q;
+                    cplx tx = (m1 < n && k1ok) ? cmul(T[m1 * P + k1], px) : cplx{0, 0};
+                    acc[0][t] = mfma(tx.x, Gr[s], acc[0][t]);
+                    acc[1][t] = mfma(tx.y, Gi[s], acc[1][t]);
+                    acc[2][t] = mfma(tx.x, Gi[s], acc[2][t]);
+                    acc[3][t] = mfma(tx.y, Gr[s], acc[3][t]);
                 }
             }
-            __builtin_amdgcn_wave_barrier();
         }
         cplx* out = partial + ((size_t)ch.slot * nd + d) * nexp;
 #pragma unroll
-        for (int t = 0; t < kUpMaxTM; t++) {
-            if (t < nTM) {
+        for (int t = 0; t < NTM; t++) {
 #pragma unroll
-                for (int r = 0; r < 4; r++) {
-                    int k1 = 16 * t + drow(q, r);
-                    if (k1 < Mlo && k2 < H) {
-                        double AC = acc[0][t][r], BD = acc[1][t][r], AD = acc[2][t][r], BC = acc[3][t][r];
-                        out[(size_t)k1 * H + k2] = cplx{AC - BD, AD + BC};
-                        int km = P - 1 - k1;
-                        if (km > k1) out[(size_t)km * H + k2] = cplx{AC + BD, AD - BC};
-                    }
+            for (int r = 0; r < 4; r++) {
+                int k1 = 16 * t + drow(q, r);
+                if (k1 < Mlo && k2ok) {
+                    double AC = acc[0][t][r], BD = acc[1][t][r], AD = acc[2][t][r], BC = acc[3][t][r];
+                    out[k1 * H + k2] = cplx{AC - BD, AD + BC};
+                    int km = P - 1 - k1;
+                    if (km > k1) out[km * H + k2] = cplx{AC + BD, AD - BC};
                 }
             }
         }
     }
 }
 
+// Mirror-folded local expansion of a cutoff box: for x frequency m < Mh =
+// ceil(npw/2) and retained y frequency k2 < H, S = L(m, k2) + L(npw-1-m, k2)
+// and D = L(m, k2) - L(npw-1-m, k2) (S = L, D = 0 on the middle frequency of
+// an odd expansion).
+struct Fold {
+    cplx S, D;
+};
+
 // Downward pass.  One wave per PW leaf; y frequencies are processed in tiles
 // of 16:
-//   FF_t = (E_t diag(px))^T L   over x frequencies, folded with
-//          E(npw-1-m) = conj E(m): FF = a (L_m + L_mirror) + i b (L_m - L_mirror)
-//   out^T += E_u^T (FF_v diag(py w))   real parts, w = 1/2 on the middle
-//          frequency of an odd expansion (bfgt/bfgt_pwrouts.f:591-595)
+//   FF^T = S^T a + i D^T b        a + i b = E_t diag(px), the x contraction
+//                                 folded with E(npw-1-m) = conj E(m)
+//   out^T += Re[(E_u diag(py w))^T FF_v^T]   w = 1/2 on the middle frequency
+//                                 of an odd expansion (bfgt/bfgt_pwrouts.f:591-595)
 // for the (v, u) pairs pot (0,0), gx (1,0), gy (0,1), hxx (2,0), hxy (1,1),
-// hyy (0,2).
+// hyy (0,2).  The second product reads the first one's accumulators directly.
 template <int NTAB>
 __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_down_mfma(
     int n, int P, int nd, int nleaf, const PwLeaf* __restrict__ leaves, const cplx* __restrict__ tpw,
-    const cplx* __restrict__ phase, const int* __restrict__ levbase, const cplx* __restrict__ loc,
+    const cplx* __restrict__ phase, const int* __restrict__ levbase, const Fold* __restrict__ fold,
     double* __restrict__ pot, double* __restrict__ grad, double* __restrict__ hess) {
     constexpr int NOUT = NTAB == 1 ? 1 : (NTAB == 2 ? 3 : 6);
-    __shared__ double sFF[kWavesPerBlock][NTAB][2][256];
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
     const int item = blockIdx.x * kWavesPerBlock + wave;
     if (item >= nleaf) return;  // no block-wide barriers in this kernel
     const int q = lane >> 4, cc = lane & 15;
-    const int H = (P + 1) / 2, Mh = (P + 1) / 2, P2 = P / 2, nexp = P * H, npbox = n * n;
-    const int nks1 = (Mh + 3) / 4, nTN = (H + 15) / 16;
+    const int H = (P + 1) / 2, Mh = (P + 1) / 2, P2 = P / 2, npbox = n * n;
+    const int nks1 = (Mh + 3) / 4, nT = (H + 15) / 16;
+    const bool ccok = cc < n;
     const PwLeaf lf = leaves[item];
     const cplx* E = tpw + (size_t)lf.level * 3 * P * n;  // [t][m][i]
     const cplx* ph = phase + levbase[lf.level];
-    const int vv[6] = {0, 1, 0, 2, 1, 0};
-    const int uu[6] = {0, 0, 1, 0, 1, 2};
+    constexpr int vv[6] = {0, 1, 0, 2, 1, 0};
+    constexpr int uu[6] = {0, 0, 1, 0, 1, 2};
     for (int d = 0; d < nd; d++) {
-        const cplx* L = loc + ((size_t)lf.anc * nd + d) * nexp;
+        const Fold* F = fold + ((size_t)lf.anc * nd + d) * Mh * H;
         v4d out[NOUT];
 #pragma unroll
         for (int o = 0; o < NOUT; o++) out[o] = v4d{0, 0, 0, 0};
-        for (int tN = 0; tN < nTN; tN++) {
-            const int m2 = 16 * tN + cc;
+        for (int tM = 0; tM < nT; tM++) {
+            const int m2 = 16 * tM + cc;
+            const bool m2ok = m2 < H;
             v4d FFr[NTAB], FFi[NTAB];
 #pragma unroll
             for (int t = 0; t < NTAB; t++) FFr[t] = FFi[t] = v4d{0, 0, 0, 0};
             for (int s = 0; s < nks1; s++) {
                 const int m = 4 * s + q;
-                cplx S{0, 0}, Dd{0, 0}, px{0, 0};
-                if (m < Mh) {
-                    px = ph[(size_t)lf.jx * P + m];
-                    if (m2 < H) {
-                        cplx a = L[(size_t)m * H + m2];
-                        int mm = P - 1 - m;
-                        if (mm != m) {
-                            cplx b = L[(size_t)mm * H + m2];
-                            S = cplx{a.x + b.x, a.y + b.y};
-                            Dd = cplx{a.x - b.x, a.y - b.y};
-                        } else {
-                            S = a;
-                        }
-                    }
-                }
+                const bool mok = m < Mh;
+                Fold sd{{0, 0}, {0, 0}};
+                if (mok && m2ok) sd = F[m * H + m2];
+                cplx px = mok ? ph[lf.jx * P + m] : cplx{0, 0};
 #pragma unroll
                 for (int t = 0; t < NTAB; t++) {
-                    cplx e = (m < Mh && cc < n) ? cmul(E[((size_t)t * P + m) * n + cc], px) : cplx{0, 0};
-                    FFr[t] = mfma(e.x, S.x, FFr[t]);
-                    FFr[t] = mfma(-e.y, Dd.y, FFr[t]);
-                    FFi[t] = mfma(e.x, S.y, FFi[t]);
-                    FFi[t] = mfma(e.y, Dd.x, FFi[t]);
+                    cplx e = (mok && ccok) ? cmul(E[(t * P + m) * n + cc], px) : cplx{0, 0};
+                    FFr[t] = mfma(sd.S.x, e.x, FFr[t]);
+                    FFr[t] = mfma(-sd.D.y, e.y, FFr[t]);
+                    FFi[t] = mfma(sd.S.y, e.x, FFi[t]);
+                    FFi[t] = mfma(sd.D.x, e.y, FFi[t]);
                 }
             }
-            // fold the y phase and weight of this lane's column, stage as [m2][k1]
-            cplx py{0, 0};
-            if (m2 < H) {
-                py = ph[(size_t)lf.jy * P + m2];
-                double w = (m2 < P2) ? 1.0 : 0.5;
-                py.x *= w;
-                py.y *= w;
-            }
-#pragma unroll
-            for (int t = 0; t < NTAB; t++)
-#pragma unroll
-                for (int r = 0; r < 4; r++) {
-                    cplx v = cmul(cplx{FFr[t][r], FFi[t][r]}, py);
-                    sFF[wave][t][0][cc * 16 + drow(q, r)] = v.x;
-                    sFF[wave][t][1][cc * 16 + drow(q, r)] = v.y;
-                }
-            __builtin_amdgcn_wave_barrier();
+            // element s of FF holds FF^T[16 tM + 4s + q][cc]: the B operand of K-step s
 #pragma unroll
             for (int s = 0; s < 4; s++) {
-                const int m2s = 16 * tN + 4 * s + q;
+                const int m2s = 16 * tM + 4 * s + q;
+                const bool ok = m2s < H;
+                cplx pyw = ok ? ph[lf.jy * P + m2s] : cplx{0, 0};
+                const double w = (m2s < P2) ? 1.0 : 0.5;
+                pyw.x *= w;
+                pyw.y *= w;
                 cplx eu[NTAB];
 #pragma unroll
-                for (int u = 0; u < NTAB; u++)
-                    eu[u] = (m2s < H && cc < n) ? E[((size_t)u * P + m2s) * n + cc] : cplx{0, 0};
+                for (int u = 0; u < NTAB; u++) eu[u] = (ok && ccok) ? cmul(E[(u * P + m2s) * n + cc], pyw) : cplx{0, 0};
 #pragma unroll
                 for (int o = 0; o < NOUT; o++) {
-                    const int v = vv[o], u = uu[o];
-                    double fr = sFF[wave][v][0][(4 * s + q) * 16 + cc];
-                    double fi = sFF[wave][v][1][(4 * s + q) * 16 + cc];
-                    out[o] = mfma(eu[u].x, fr, out[o]);
-                    out[o] = mfma(-eu[u].y, fi, out[o]);
+                    out[o] = mfma(eu[uu[o]].x, FFr[vv[o]][s], out[o]);
+                    out[o] = mfma(-eu[uu[o]].y, FFi[vv[o]][s], out[o]);
                 }
             }
-            __builtin_amdgcn_wave_barrier();
         }
         // out^T holds [k2 = drow(q, r)][k1 = cc]
-        if (cc < n) {
+        if (ccok) {
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 int k2 = drow(q, r);
@@ -437,44 +417,105 @@ __global__ __launch_bounds__(kVecThreads) void k_down_vec(int n, int npw, int nd
 // expansion-level stages
 // ---------------------------------------------------------------------------
 
-// mp of each cutoff-level box = sum of its chunks' partial sums (in chunk order).
-__global__ void k_reduce(int nd, int nexp, const int* __restrict__ anc_chunk, const cplx* __restrict__ partial,
-                         cplx* __restrict__ mp) {
-    const int a = blockIdx.x;
-    const int len = nd * nexp;
-    const int c0 = anc_chunk[a], c1 = anc_chunk[a + 1];
-    for (int e = blockIdx.y * blockDim.x + threadIdx.x; e < len; e += gridDim.y * blockDim.x) {
-        double sr = 0.0, si = 0.0;
-        for (int c = c0; c < c1; c++) {
-            cplx v = partial[(size_t)c * len + e];
-            sr += v.x;
-            si += v.y;
-        }
-        mp[(size_t)a * len + e] = cplx{sr, si};
+// Sum of `count` consecutive expansion blocks of length len starting at
+// block `first`, element e; four independent accumulators keep several loads
+// in flight.  Summation order is fixed, so results are deterministic.
+__device__ inline cplx sum_blocks(const cplx* __restrict__ src, size_t len, int first, int count, int e) {
+    double r0 = 0, i0 = 0, r1 = 0, i1 = 0, r2 = 0, i2 = 0, r3 = 0, i3 = 0;
+    int c = 0;
+    for (; c + 4 <= count; c += 4) {
+        cplx a = src[(size_t)(first + c) * len + e];
+        cplx b = src[(size_t)(first + c + 1) * len + e];
+        cplx d = src[(size_t)(first + c + 2) * len + e];
+        cplx f = src[(size_t)(first + c + 3) * len + e];
+        r0 += a.x; i0 += a.y; r1 += b.x; i1 += b.y; r2 += d.x; i2 += d.y; r3 += f.x; i3 += f.y;
     }
+    for (; c < count; c++) {
+        cplx a = src[(size_t)(first + c) * len + e];
+        r0 += a.x;
+        i0 += a.y;
+    }
+    return cplx{(r0 + r1) + (r2 + r3), (i0 + i1) + (i2 + i3)};
+}
+
+// Merge, level 1: each group of chunk partial sums of one cutoff box is summed
+// into the box's mp (dest < 0) or into a group slot.
+__global__ void k_reduce1(int len, const int4* __restrict__ groups, const cplx* __restrict__ partial,
+                          cplx* __restrict__ gpart, cplx* __restrict__ mp) {
+    const int4 g = groups[blockIdx.x];
+    for (int e = blockIdx.y * blockDim.x + threadIdx.x; e < len; e += gridDim.y * blockDim.x) {
+        cplx s = sum_blocks(partial, len, g.y, g.z, e);
+        if (g.w < 0) mp[(size_t)g.x * len + e] = s;
+        else gpart[(size_t)g.w * len + e] = s;
+    }
+}
+
+// Merge, level 2: cutoff boxes with several groups sum their group slots.
+__global__ void k_reduce2(int len, const int4* __restrict__ boxes, const cplx* __restrict__ gpart,
+                          cplx* __restrict__ mp) {
+    const int4 g = boxes[blockIdx.x];
+    for (int e = blockIdx.y * blockDim.x + threadIdx.x; e < len; e += gridDim.y * blockDim.x)
+        mp[(size_t)g.x * len + e] = sum_blocks(gpart, len, g.y, g.z, e);
 }
 
 // loc = mp(self) + sum over the PW list of shift(offset) * mp(source)
 // (bfgt/boxfgt.f:805-822).
+__device__ inline cplx m2l_at(const cplx* __restrict__ wshift, const cplx* __restrict__ mp, int nexp, size_t len,
+                              int a, int e, int j0, int j1, const int2* __restrict__ m2l, int q) {
+    cplx s = mp[(size_t)a * len + e];
+    for (int j = j0; j < j1; j++) {
+        int2 sj = m2l[j];
+        s = cfma(wshift[(size_t)sj.y * nexp + q], mp[(size_t)sj.x * len + e], s);
+    }
+    return s;
+}
+
 __global__ void k_m2l(int nd, int nexp, const int* __restrict__ m2l_ptr, const int2* __restrict__ m2l,
                       const cplx* __restrict__ wshift, const cplx* __restrict__ mp, cplx* __restrict__ loc) {
     const int a = blockIdx.x;
     const int len = nd * nexp;
     const int j0 = m2l_ptr[a], j1 = m2l_ptr[a + 1];
-    for (int e = blockIdx.y * blockDim.x + threadIdx.x; e < len; e += gridDim.y * blockDim.x) {
-        int q = e % nexp;
-        cplx s = mp[(size_t)a * len + e];
-        for (int j = j0; j < j1; j++) {
-            int2 sj = m2l[j];
-            s = cfma(wshift[(size_t)sj.y * nexp + q], mp[(size_t)sj.x * len + e], s);
+    for (int e = blockIdx.y * blockDim.x + threadIdx.x; e < len; e += gridDim.y * blockDim.x)
+        loc[(size_t)a * len + e] = m2l_at(wshift, mp, nexp, len, a, e, j0, j1, m2l, e % nexp);
+}
+
+// Same translation, written mirror-folded (struct Fold) for the MFMA
+// downward pass.
+__global__ void k_m2l_fold(int nd, int P, const int* __restrict__ m2l_ptr, const int2* __restrict__ m2l,
+                           const cplx* __restrict__ wshift, const cplx* __restrict__ mp, Fold* __restrict__ fold) {
+    const int a = blockIdx.x;
+    const int H = (P + 1) / 2, Mh = (P + 1) / 2, nexp = P * H;
+    const int len = nd * nexp;
+    const int flen = nd * Mh * H;
+    const int j0 = m2l_ptr[a], j1 = m2l_ptr[a + 1];
+    for (int e = blockIdx.y * blockDim.x + threadIdx.x; e < flen; e += gridDim.y * blockDim.x) {
+        int d = e / (Mh * H), r = e % (Mh * H);
+        int m = r / H, k2 = r % H;
+        int q1 = m * H + k2, mm = P - 1 - m;
+        cplx l1 = m2l_at(wshift, mp, nexp, len, a, d * nexp + q1, j0, j1, m2l, q1);
+        Fold f;
+        if (mm != m) {
+            int q2 = mm * H + k2;
+            cplx l2 = m2l_at(wshift, mp, nexp, len, a, d * nexp + q2, j0, j1, m2l, q2);
+            f.S = cplx{l1.x + l2.x, l1.y + l2.y};
+            f.D = cplx{l1.x - l2.x, l1.y - l2.y};
+        } else {
+            f.S = l1;
+            f.D = cplx{0, 0};
         }
-        loc[(size_t)a * len + e] = s;
+        fold[(size_t)a * flen + e] = f;
     }
 }
 
 }  // namespace
 
 bool use_mfma(const Plan& p) { return p.n <= 16 && (p.npw + 1) / 2 <= 16 * kUpMaxTM; }
+
+size_t loc_buffer_cplx(const Plan& p) {
+    // the MFMA path stores the mirror-folded local expansion: 2 * Mh * H >= npw * H
+    size_t H = (p.npw + 1) / 2;
+    return std::max((size_t)p.nexp, 2 * H * H) * p.nd;
+}
 
 size_t up_smem_bytes(int n, int npw) {
     int H = (npw + 1) / 2;
@@ -498,18 +539,38 @@ static int launch_up_vec(Plan& p, const double* fvals, size_t sh) {
     return BFGT_OK;
 }
 
+template <int NKS, int NTM>
+static int launch_up_mfma(Plan& p, const double* fvals, int nTN) {
+    int nitems = p.nchunks * nTN;
+    int blocks = (nitems + kWavesPerBlock - 1) / kWavesPerBlock;
+    k_up_mfma<NKS, NTM><<<blocks, kWave * kWavesPerBlock, 0, p.stream>>>(p.n, p.npw, p.nd, nTN, nitems, fvals,
+                                                                        p.d_pwleaf, p.d_chunks, p.d_tpoly,
+                                                                        p.d_phase, p.d_levbase, p.d_partial);
+    BFGT_HIP_CHECK(hipGetLastError());
+    return BFGT_OK;
+}
+
+template <int NKS>
+static int launch_up_mfma_tm(Plan& p, const double* fvals, int nTM, int nTN) {
+    switch (nTM) {
+        case 1: return launch_up_mfma<NKS, 1>(p, fvals, nTN);
+        case 2: return launch_up_mfma<NKS, 2>(p, fvals, nTN);
+        case 3: return launch_up_mfma<NKS, 3>(p, fvals, nTN);
+        default: return launch_up_mfma<NKS, 4>(p, fvals, nTN);
+    }
+}
+
 int launch_upward(Plan& p, const double* fvals) {
     if (p.nchunks == 0) return BFGT_OK;
     if (use_mfma(p)) {
         int H = (p.npw + 1) / 2;
         int nTM = (H + 15) / 16, nTN = (H + 15) / 16;
-        int nitems = p.nchunks * nTN;
-        int blocks = (nitems + kWavesPerBlock - 1) / kWavesPerBlock;
-        k_up_mfma<<<blocks, kWave * kWavesPerBlock, 0, p.stream>>>(p.n, p.npw, p.nd, nTM, nTN, nitems, fvals,
-                                                                  p.d_pwleaf, p.d_chunks, p.d_tpoly, p.d_phase,
-                                                                  p.d_levbase, p.d_partial);
-        BFGT_HIP_CHECK(hipGetLastError());
-        return BFGT_OK;
+        switch ((p.n + 3) / 4) {
+            case 1: return launch_up_mfma_tm<1>(p, fvals, nTM, nTN);
+            case 2: return launch_up_mfma_tm<2>(p, fvals, nTM, nTN);
+            case 3: return launch_up_mfma_tm<3>(p, fvals, nTM, nTN);
+            default: return launch_up_mfma_tm<4>(p, fvals, nTM, nTN);
+        }
     }
     size_t sh = up_smem_bytes(p.n, p.npw);
     int e = (p.nexp + kVecThreads - 1) / kVecThreads;
@@ -524,12 +585,27 @@ int launch_upward(Plan& p, const double* fvals) {
 int launch_reduce(Plan& p) {
     int len = p.nd * p.nexp;
     int gy = (len + 255) / 256;
-    k_reduce<<<dim3(p.nanc, gy), 256, 0, p.stream>>>(p.nd, p.nexp, p.d_anc_chunk, p.d_partial, p.d_mp);
-    BFGT_HIP_CHECK(hipGetLastError());
+    if (p.nred1 > 0) {
+        k_reduce1<<<dim3(p.nred1, gy), 256, 0, p.stream>>>(len, p.d_red1, p.d_partial, p.d_gpart, p.d_mp);
+        BFGT_HIP_CHECK(hipGetLastError());
+    }
+    if (p.nred2 > 0) {
+        k_reduce2<<<dim3(p.nred2, gy), 256, 0, p.stream>>>(len, p.d_red2, p.d_gpart, p.d_mp);
+        BFGT_HIP_CHECK(hipGetLastError());
+    }
     return BFGT_OK;
 }
 
 int launch_m2l(Plan& p) {
+    if (use_mfma(p)) {
+        int Mh = (p.npw + 1) / 2, H = Mh;
+        int flen = p.nd * Mh * H;
+        int gy = (flen + 255) / 256;
+        k_m2l_fold<<<dim3(p.nanc, gy), 256, 0, p.stream>>>(p.nd, p.npw, p.d_m2l_ptr, p.d_m2l, p.d_wshift, p.d_mp,
+                                                           reinterpret_cast<Fold*>(p.d_loc));
+        BFGT_HIP_CHECK(hipGetLastError());
+        return BFGT_OK;
+    }
     int len = p.nd * p.nexp;
     int gy = (len + 255) / 256;
     k_m2l<<<dim3(p.nanc, gy), 256, 0, p.stream>>>(p.nd, p.nexp, p.d_m2l_ptr, p.d_m2l, p.d_wshift, p.d_mp, p.d_loc);
@@ -550,9 +626,9 @@ static int launch_down_vec(Plan& p, double* pot, double* grad, double* hess, siz
 template <int NTAB>
 static int launch_down_mfma(Plan& p, double* pot, double* grad, double* hess) {
     int blocks = (p.npwleaf + kWavesPerBlock - 1) / kWavesPerBlock;
-    k_down_mfma<NTAB><<<blocks, kWave * kWavesPerBlock, 0, p.stream>>>(p.n, p.npw, p.nd, p.npwleaf, p.d_pwleaf,
-                                                                      p.d_tpw, p.d_phase, p.d_levbase, p.d_loc, pot,
-                                                                      grad, hess);
+    k_down_mfma<NTAB><<<blocks, kWave * kWavesPerBlock, 0, p.stream>>>(
+        p.n, p.npw, p.nd, p.npwleaf, p.d_pwleaf, p.d_tpw, p.d_phase, p.d_levbase,
+        reinterpret_cast<const Fold*>(p.d_loc), pot, grad, hess);
     BFGT_HIP_CHECK(hipGetLastError());
     return BFGT_OK;
 }

@@ -95,7 +95,7 @@ int bfgt_plan_profile_end(bfgt_plan_t plan, double *stage_seconds, int *nsteps);
 /* Plan facts: ivals[0..15] = {npwlevel, npw, nexp, nlevstart, nlevend, nleafbox,
  * n_pw_leaves, n_chunks, n_ancestors, n_near_targets, n_near_pairs, has_pw,
  * has_near, iperiod_eff, nboxes, precision}; dvals[0..7] = {delta0, pmax,
- * t_plan_host_s, t_tables_s, 0,0,0,0}. */
+ * t_plan_host_s, t_tables_s, matrix_core_path (1.0 / 0.0), 0,0,0}. */
 int bfgt_plan_query(bfgt_plan_t plan, int *ivals, double *dvals);
 
 /* Copy one device table of one level to host memory (parity tests of the FP64
@@ -104,9 +104,12 @@ int bfgt_plan_query(bfgt_plan_t plan, int *ivals, double *dvals);
  * tables (13 x 3 x n x n double, [k+6][t][j][m]), 3 near-field sparsity
  * bounds (13 x (n+1) x 2 int, 1-based), 4 M2L shifts (9 x npw x H complex,
  * [offset][x frequency][y frequency], H = (npw+1)/2), 5 leaf-to-ancestor
- * phases of one level (2^(level-nlevstart) x npw complex), 6 / 7 the
- * cutoff-level mp / loc expansions of the last execution (n_ancestors x nd x
- * npw x H complex; `level` ignored). */
+ * phases of one level (2^(level-nlevstart) x npw complex), 6 the cutoff-level
+ * mp expansions of the last execution (n_ancestors x nd x npw x H complex;
+ * `level` ignored), 7 its loc expansions as stored: on the matrix-core path
+ * (dvals[4] = 1) mirror-folded pairs S = L(m,.) + L(npw-1-m,.),
+ * D = L(m,.) - L(npw-1-m,.) (n_ancestors x nd x ceil(npw/2) x H x 2 complex),
+ * otherwise n_ancestors x nd x npw x H complex. */
 int bfgt_plan_get_table(bfgt_plan_t plan, int which, int level, void *host_out, size_t nbytes);
 
 const char *bfgt_last_error(void);
