@@ -192,6 +192,26 @@ def main():
     roof["algorithmic_per_launch"] = amount
     roof["avg_launch_ms"] = stage_ms[dom]
 
+    # adjoint (transposed operator) on the same plan: cotangents of every output
+    adj = None
+    if plan.info["mfma"] or not plan.info["has_pw"]:
+        gen = torch.Generator(device=dev).manual_seed(7)
+        gp = torch.randn((nb, npb, nd), dtype=torch.float64, device=dev, generator=gen)
+        gg = torch.randn((nb, npb, 2, nd), dtype=torch.float64, device=dev, generator=gen) if w.ifpgh >= 2 else None
+        gh = torch.randn((nb, npb, 3, nd), dtype=torch.float64, device=dev, generator=gen) if w.ifpgh >= 3 else None
+        fbar = torch.zeros((nb, npb, nd), dtype=torch.float64, device=dev)
+        for _ in range(max(args.warmup, 1)):
+            plan.adjoint(gp, fbar, gg, gh)
+        torch.cuda.synchronize()
+        e0.record(stream)
+        for _ in range(args.steps):
+            plan.adjoint(gp, fbar, gg, gh)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        adj_ms = e0.elapsed_time(e1) / args.steps
+        adj = {"ms_per_call": adj_ms, "over_forward": adj_ms / ms,
+               "note": "fbar += A^T (gpot, ggrad, ghess) on the same resident plan, FP64"}
+
     # cold call: the host-array drop-in (plan + H2D + kernels + D2H)
     from heat2d_per_box_amd.boxfgt import boxfgt as boxfgt_host
 
@@ -235,6 +255,7 @@ def main():
         "cold_pts_per_s": npts / t_cold,
         "t_plan_s": t_plan,
         "roofline": roof,
+        "adjoint": adj,
     }
     if not args.no_cpu_baseline:
         print("[bench] timing the CPU baseline", file=sys.stderr, flush=True)

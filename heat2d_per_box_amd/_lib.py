@@ -32,6 +32,7 @@ EXPORTS = [
     "bfgt_hip_eval",
     "bfgt_plan_create",
     "bfgt_plan_execute",
+    "bfgt_plan_adjoint",
     "bfgt_plan_destroy",
     "bfgt_plan_query",
     "bfgt_plan_get_table",
@@ -73,6 +74,8 @@ def lib() -> ctypes.CDLL:
                                    _i, _i, _vp]
     L.bfgt_plan_execute.restype = _i
     L.bfgt_plan_execute.argtypes = [_vp, _vp, _vp, _vp, _vp, _dp]
+    L.bfgt_plan_adjoint.restype = _i
+    L.bfgt_plan_adjoint.argtypes = [_vp, _vp, _vp, _vp, _vp, _dp]
     L.bfgt_plan_destroy.restype = _i
     L.bfgt_plan_destroy.argtypes = [_vp]
     L.bfgt_plan_query.restype = _i

@@ -144,6 +144,18 @@ class BoxFGT:
         check(lib().bfgt_plan_execute(self._h, f, p, g, h, _ptr(ti) if ti is not None else None))
         return ti
 
+    def adjoint(self, gpot, fbar, ggrad=None, ghess=None, timeinfo: bool = False):
+        """Transposed operator: ``fbar += A^T (gpot, ggrad, ghess)`` on device
+        tensors, A the map fvals -> (pot, grad, hess) of this plan's ifpgh."""
+        nb, npb, nd = self.nboxes, self.npbox, self.nd
+        gp = self._check_tensor(gpot, (nb, npb, nd), "gpot")
+        fb = self._check_tensor(fbar, (nb, npb, nd), "fbar")
+        gg = self._check_tensor(ggrad, (nb, npb, 2, nd), "ggrad")
+        gh = self._check_tensor(ghess, (nb, npb, 3, nd), "ghess")
+        ti = np.zeros(20) if timeinfo else None
+        check(lib().bfgt_plan_adjoint(self._h, gp, gg, gh, fb, _ptr(ti) if ti is not None else None))
+        return ti
+
     def profile_begin(self, max_steps: int) -> None:
         """Record per-stage HIP events for the next ``max_steps`` calls."""
         check(lib().bfgt_plan_profile_begin(self._h, int(max_steps)))

@@ -49,6 +49,15 @@ struct NearPair {
     int pad;
 };
 
+// The same interaction seen from the source: target box, target level and
+// the table offsets (the adjoint of the near field runs over these).
+struct NearPairT {
+    int tgt;
+    int lev;
+    int ix;
+    int iy;
+};
+
 struct Plan {
     // parameters
     int nd = 1, ipoly = 0, iperiod_in = 1, iperiod_eff = 1, n = 8, npbox = 64, ifpgh = 1;
@@ -102,6 +111,18 @@ struct Plan {
     int* d_near_ptr = nullptr;     // CSR [nnear_t+1]
     NearPair* d_near = nullptr;
 
+    // adjoint (transposed operator): conjugate-transposed PW tables, the
+    // transposed PW translation lists and the near pairs grouped by source
+    bool has_adjoint = false;
+    cplx* d_tadj_e = nullptr;      // [level][t][i][k] = conj(tpw[level][t][k][i])
+    cplx* d_tadj_t = nullptr;      // [level][k][i] = conj(tpoly[level][i][k])
+    int* d_m2lT_ptr = nullptr;     // CSR [nanc+1]
+    int2* d_m2lT = nullptr;        // (target ancestor, shift index)
+    int nnearT_s = 0;
+    int* d_nearT_src = nullptr;    // source boxes
+    int* d_nearT_ptr = nullptr;    // CSR [nnearT_s+1]
+    NearPairT* d_nearT = nullptr;
+
     // work buffers
     cplx* d_partial = nullptr;  // [nchunks][nd][nexp]
     cplx* d_mp = nullptr;       // [nanc][nd][nexp]
@@ -130,6 +151,14 @@ int launch_reduce(Plan& p);
 int launch_m2l(Plan& p);
 int launch_downward(Plan& p, double* pot, double* grad, double* hess);
 int launch_near(Plan& p, const double* fvals, double* pot, double* grad, double* hess);
+
+// adjoint (bfgt_pw.hip, bfgt_near.hip)
+bool use_mfma(const Plan& p);
+int build_adjoint_tables(Plan& p);
+int launch_adj_eval(Plan& p, const double* gpot, const double* ggrad, const double* ghess);
+int launch_adj_m2l(Plan& p);
+int launch_adj_transform(Plan& p, double* fbar);
+int launch_near_adj(Plan& p, const double* gpot, const double* ggrad, const double* ghess, double* fbar);
 
 }  // namespace bfgt
 

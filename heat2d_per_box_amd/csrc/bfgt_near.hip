@@ -115,6 +115,95 @@ __global__ __launch_bounds__(kNearThreads) void k_near(int n, int nd, int ntab, 
     }
 }
 
+// Adjoint of the near field: one block per source leaf accumulates, over
+// every (target, offset) pair that reads it,
+//   fbar(j1, j2) += sum_o sum_{k1,k2} Tx_{x(o)}(k1, j1) Ty_{y(o)}(k2, j2) g_o(k1, k2)
+// with the target's tables read transposed and o over the channels pot (0,0),
+// gx (1,0), gy (0,1), hxx (2,0), hxy (1,1), hyy (0,2) (x table, y table).
+__global__ __launch_bounds__(kNearThreads) void k_near_adj(int n, int nd, int ntab, const int* __restrict__ src,
+                                                           const int* __restrict__ ptr,
+                                                           const NearPairT* __restrict__ pairs,
+                                                           const double* __restrict__ tloc,
+                                                           const double* __restrict__ gpot,
+                                                           const double* __restrict__ ggrad,
+                                                           const double* __restrict__ ghess,
+                                                           double* __restrict__ fbar) {
+    extern __shared__ double smem[];
+    const int nn = n * n;
+    const int nout = ntab == 1 ? 1 : (ntab == 2 ? 3 : 6);
+    double* sg = smem;            // nout * nn : [o][k1 + n*k2]
+    double* sz = sg + nout * nn;  // ntab * nn : [u][j1 + n*k2]
+    const int s = src[blockIdx.x];
+    const int p0 = ptr[blockIdx.x], p1 = ptr[blockIdx.x + 1];
+    const int tid = threadIdx.x;
+    for (int d = 0; d < nd; d++) {
+        double acc[kNearMaxQ];
+#pragma unroll
+        for (int r = 0; r < kNearMaxQ; r++) acc[r] = 0.0;
+        for (int p = p0; p < p1; p++) {
+            const NearPairT pr = pairs[p];
+            const double* T = tloc + (size_t)pr.lev * 13 * 3 * nn;
+            const double* Tx = T + (size_t)(pr.ix + 6) * 3 * nn;
+            const double* Ty = T + (size_t)(pr.iy + 6) * 3 * nn;
+            const size_t tb = (size_t)pr.tgt * nn;
+            __syncthreads();
+            for (int e = tid; e < nn; e += kNearThreads) {
+                size_t pb = tb + e;
+                sg[e] = gpot[pb * nd + d];
+                if (ntab >= 2) {
+                    sg[nn + e] = ggrad[(pb * 2 + 0) * nd + d];
+                    sg[2 * nn + e] = ggrad[(pb * 2 + 1) * nd + d];
+                }
+                if (ntab >= 3) {
+                    sg[3 * nn + e] = ghess[(pb * 3 + 0) * nd + d];
+                    sg[4 * nn + e] = ghess[(pb * 3 + 1) * nd + d];
+                    sg[5 * nn + e] = ghess[(pb * 3 + 2) * nd + d];
+                }
+            }
+            __syncthreads();
+            // x pass: Z_u(j1, k2) = sum over channels with y table u of Tx_x(o)^T g_o
+            for (int e = tid; e < ntab * nn; e += kNearThreads) {
+                int u = e / nn, r = e % nn;
+                int j1 = r % n, k2 = r / n;
+                double z = 0.0;
+                for (int k1 = 0; k1 < n; k1++) {
+                    int g = k1 + n * k2;
+                    int ti = k1 * n + j1;
+                    if (u == 0) {
+                        z = fma(Tx[ti], sg[g], z);
+                        if (ntab >= 2) z = fma(Tx[nn + ti], sg[nn + g], z);
+                        if (ntab >= 3) z = fma(Tx[2 * nn + ti], sg[3 * nn + g], z);
+                    } else if (u == 1) {
+                        z = fma(Tx[ti], sg[2 * nn + g], z);
+                        if (ntab >= 3) z = fma(Tx[nn + ti], sg[4 * nn + g], z);
+                    } else {
+                        z = fma(Tx[ti], sg[5 * nn + g], z);
+                    }
+                }
+                sz[e] = z;
+            }
+            __syncthreads();
+            // y pass: fbar(j1, j2) += sum_u Ty_u^T Z_u
+#pragma unroll
+            for (int r = 0; r < kNearMaxQ; r++) {
+                int q = tid + r * kNearThreads;
+                if (q < nn) {
+                    int j1 = q % n, j2 = q / n;
+                    double a = 0.0;
+                    for (int u = 0; u < ntab; u++)
+                        for (int k2 = 0; k2 < n; k2++) a = fma(Ty[u * nn + k2 * n + j2], sz[u * nn + j1 + n * k2], a);
+                    acc[r] += a;
+                }
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < kNearMaxQ; r++) {
+            int q = tid + r * kNearThreads;
+            if (q < nn && p1 > p0) fbar[((size_t)s * nn + q) * nd + d] += acc[r];
+        }
+    }
+}
+
 }  // namespace
 
 size_t near_smem_bytes(int n, int ntab) { return sizeof(double) * (size_t)n * n * (1 + ntab); }
@@ -125,6 +214,19 @@ int launch_near(Plan& p, const double* fvals, double* pot, double* grad, double*
     size_t sh = near_smem_bytes(p.n, ntab);
     k_near<<<p.nnear_t, kNearThreads, sh, p.stream>>>(p.n, p.nd, ntab, p.d_near_tgt, p.d_near_lev, p.d_near_ptr,
                                                       p.d_near, p.d_tloc, fvals, pot, grad, hess);
+    BFGT_HIP_CHECK(hipGetLastError());
+    return BFGT_OK;
+}
+
+int launch_near_adj(Plan& p, const double* gpot, const double* ggrad, const double* ghess, double* fbar) {
+    if (p.nnearT_s == 0) return BFGT_OK;
+    int ntab = p.ifpgh;
+    int nout = ntab == 1 ? 1 : (ntab == 2 ? 3 : 6);
+    size_t sh = sizeof(double) * (size_t)p.n * p.n * (nout + ntab);
+    if (sh > 65536)
+        BFGT_HIP_CHECK(hipFuncSetAttribute((const void*)k_near_adj, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
+    k_near_adj<<<p.nnearT_s, kNearThreads, sh, p.stream>>>(p.n, p.nd, ntab, p.d_nearT_src, p.d_nearT_ptr, p.d_nearT,
+                                                           p.d_tloc, gpot, ggrad, ghess, fbar);
     BFGT_HIP_CHECK(hipGetLastError());
     return BFGT_OK;
 }

@@ -40,7 +40,6 @@ size_t down_smem_bytes(int n, int npw, int ntab);
 size_t near_smem_bytes(int n, int ntab);
 int up_max_out();
 size_t loc_buffer_cplx(const Plan& p);
-bool use_mfma(const Plan& p);
 int build_pw_nodes(Plan& p);
 
 namespace {
@@ -549,6 +548,43 @@ int plan_create(Plan& p, int nd, int ndim, double delta, double eps, int ipoly, 
     p.nnear_p = (int)pairs.size();
     p.has_near = p.nnear_t > 0;
 
+    // transposed schedules for the adjoint: PW translation lists of the
+    // sources, near pairs grouped by source (ascending box, then target order)
+    std::vector<int> m2lT_ptr;
+    std::vector<int2> m2lT;
+    if (p.has_pw) {
+        std::vector<std::vector<int2>> byc(p.nanc);
+        for (int a = 0; a < p.nanc; a++)
+            for (int j = m2l_ptr[a]; j < m2l_ptr[a + 1]; j++) byc[m2l[j].x].push_back(make_int2(a, m2l[j].y));
+        m2lT_ptr.push_back(0);
+        for (int c = 0; c < p.nanc; c++) {
+            m2lT.insert(m2lT.end(), byc[c].begin(), byc[c].end());
+            m2lT_ptr.push_back((int)m2lT.size());
+        }
+    }
+    std::vector<int> nsrc, nsptr;
+    std::vector<NearPairT> pairsT;
+    {
+        std::vector<std::pair<int, NearPairT>> tmp;
+        tmp.reserve(pairs.size());
+        for (int it = 0; it < p.nnear_t; it++)
+            for (int j = nptr[it]; j < nptr[it + 1]; j++)
+                tmp.push_back({pairs[j].src, NearPairT{ntgt[it], nlev[it], pairs[j].ix, pairs[j].iy}});
+        std::stable_sort(tmp.begin(), tmp.end(),
+                         [](const std::pair<int, NearPairT>& x, const std::pair<int, NearPairT>& y) {
+                             return x.first < y.first;
+                         });
+        for (size_t j = 0; j < tmp.size(); j++) {
+            if (j == 0 || tmp[j].first != tmp[j - 1].first) {
+                nsrc.push_back(tmp[j].first);
+                nsptr.push_back((int)j);
+            }
+            pairsT.push_back(tmp[j].second);
+        }
+        nsptr.push_back((int)pairsT.size());
+    }
+    p.nnearT_s = (int)nsrc.size();
+
     if ((rc = upload(p, p.d_pwleaf, leaves))) return rc;
     if ((rc = upload(p, p.d_chunks, chunks))) return rc;
     if ((rc = upload(p, p.d_anc_box, anc_box))) return rc;
@@ -561,6 +597,18 @@ int plan_create(Plan& p, int nd, int ndim, double delta, double eps, int ipoly, 
     if ((rc = upload(p, p.d_near_lev, nlev))) return rc;
     if ((rc = upload(p, p.d_near_ptr, nptr))) return rc;
     if ((rc = upload(p, p.d_near, pairs))) return rc;
+    if ((rc = upload(p, p.d_m2lT_ptr, m2lT_ptr))) return rc;
+    if ((rc = upload(p, p.d_m2lT, m2lT))) return rc;
+    if ((rc = upload(p, p.d_nearT_src, nsrc))) return rc;
+    if ((rc = upload(p, p.d_nearT_ptr, nsptr))) return rc;
+    if ((rc = upload(p, p.d_nearT, pairsT))) return rc;
+    p.has_adjoint = !p.has_pw || use_mfma(p);
+    if (p.has_pw && p.has_adjoint) {
+        size_t per = (size_t)(nlevels + 1) * p.n * p.npw;
+        if ((rc = dalloc(p, p.d_tadj_e, 3 * per))) return rc;
+        if ((rc = dalloc(p, p.d_tadj_t, per))) return rc;
+        if ((rc = build_adjoint_tables(p))) return rc;
+    }
     if (p.has_pw) {
         size_t len = (size_t)nd * p.nexp;
         if ((rc = dalloc(p, p.d_partial, (size_t)p.nchunks * len))) return rc;
@@ -611,6 +659,44 @@ int plan_execute(Plan& p, const double* fvals, double* pot, double* grad, double
     return BFGT_OK;
 }
 
+int plan_adjoint(Plan& p, const double* gpot, const double* ggrad, const double* ghess, double* fbar,
+                 double* timeinfo) {
+    if (!gpot || !fbar || (p.ifpgh >= 2 && !ggrad) || (p.ifpgh >= 3 && !ghess))
+        return set_error(BFGT_ERR_ARG, "null device array");
+    if (!p.has_adjoint)
+        return set_error(BFGT_ERR_RESOURCE, "the adjoint needs leaf order <= 16 and at most 128 plane waves");
+    BFGT_HIP_CHECK(hipSetDevice(p.device));
+    int rc;
+    hipEvent_t* ev = p.ev;
+    BFGT_HIP_CHECK(hipEventRecord(ev[0], p.stream));
+    if (p.has_pw) {
+        if ((rc = launch_adj_eval(p, gpot, ggrad, ghess))) return rc;
+        BFGT_HIP_CHECK(hipEventRecord(ev[1], p.stream));
+        if ((rc = launch_reduce(p))) return rc;
+        BFGT_HIP_CHECK(hipEventRecord(ev[2], p.stream));
+        if ((rc = launch_adj_m2l(p))) return rc;
+        BFGT_HIP_CHECK(hipEventRecord(ev[3], p.stream));
+        if ((rc = launch_adj_transform(p, fbar))) return rc;
+        BFGT_HIP_CHECK(hipEventRecord(ev[4], p.stream));
+    } else {
+        for (int i = 1; i <= 4; i++) BFGT_HIP_CHECK(hipEventRecord(ev[i], p.stream));
+    }
+    if ((rc = launch_near_adj(p, gpot, ggrad, ghess, fbar))) return rc;
+    BFGT_HIP_CHECK(hipEventRecord(ev[5], p.stream));
+    if (timeinfo) {
+        BFGT_HIP_CHECK(hipEventSynchronize(ev[5]));
+        float ms[5];
+        for (int i = 0; i < 5; i++) BFGT_HIP_CHECK(hipEventElapsedTime(&ms[i], ev[i], ev[i + 1]));
+        for (int i = 0; i < 20; i++) timeinfo[i] = 0;
+        timeinfo[1] = ms[3] * 1e-3;  // transposed leaf transform (PW -> leaf values)
+        timeinfo[2] = ms[1] * 1e-3;  // merge of partial cotangent expansions
+        timeinfo[3] = ms[2] * 1e-3;  // transposed translation
+        timeinfo[5] = ms[0] * 1e-3;  // transposed evaluation (leaf cotangents -> PW)
+        timeinfo[6] = ms[4] * 1e-3;  // transposed near field
+    }
+    return BFGT_OK;
+}
+
 }  // namespace bfgt
 
 using bfgt::set_error;
@@ -645,6 +731,12 @@ int bfgt_plan_execute(bfgt_plan_t plan, const double* d_fvals, double* d_pot, do
                       double* timeinfo) {
     if (!plan) return set_error(BFGT_ERR_ARG, "null plan");
     return bfgt::plan_execute(plan->p, d_fvals, d_pot, d_grad, d_hess, timeinfo);
+}
+
+int bfgt_plan_adjoint(bfgt_plan_t plan, const double* d_gpot, const double* d_ggrad, const double* d_ghess,
+                      double* d_fbar, double* timeinfo) {
+    if (!plan) return set_error(BFGT_ERR_ARG, "null plan");
+    return bfgt::plan_adjoint(plan->p, d_gpot, d_ggrad, d_ghess, d_fbar, timeinfo);
 }
 
 int bfgt_plan_destroy(bfgt_plan_t plan) {

@@ -61,18 +61,50 @@ constexpr int kUpMaxTM = 4;  // x-frequency tiles of 16 (ceil(npw/2) <= 64)
 // K index is that row.  Both passes below chain two GEMMs through registers
 // this way, with no LDS staging.
 
+// Output channels of the evaluation, as (x table, y table) pairs: pot (0,0),
+// d/dx (1,0), d/dy (0,1), d2/dx2 (2,0), d2/dxdy (1,1), d2/dy2 (0,2), where
+// table 0, 1, 2 is the value, first and second derivative table.
+constexpr int nchan(int ntab) { return ntab == 1 ? 1 : (ntab == 2 ? 3 : 6); }
+constexpr int chan_x(int o) { return o == 1 || o == 4 ? 1 : (o == 3 ? 2 : 0); }
+constexpr int chan_y(int o) { return o == 2 || o == 4 ? 1 : (o == 5 ? 2 : 0); }
+
+// Leaf arrays read by the upward pass: the leaf values (forward), or the
+// cotangents of pot, grad and hess (adjoint of the evaluation).
+struct LeafIn {
+    const double* pot;   // [box][npbox][nd]
+    const double* grad;  // [box][npbox][2][nd]
+    const double* hess;  // [box][npbox][3][nd]
+};
+
+__device__ inline double leaf_in(const LeafIn& in, int o, size_t pb, int nd, int d) {
+    switch (o) {
+        case 0: return in.pot[pb * nd + d];
+        case 1: return in.grad[(pb * 2 + 0) * nd + d];
+        case 2: return in.grad[(pb * 2 + 1) * nd + d];
+        case 3: return in.hess[(pb * 3 + 0) * nd + d];
+        case 4: return in.hess[(pb * 3 + 1) * nd + d];
+        default: return in.hess[(pb * 3 + 2) * nd + d];
+    }
+}
+
 // Upward pass.  One wave per (chunk, y-frequency tile tN): for every leaf of
 // the chunk
-//   G  = F (T diag py)        (rows m1 < 16, columns k2 of the tile)
-//   out += (T diag px)^T G    with T(m, npw-1-k) = conj T(m, k), so that only
-// ceil(npw/2) x frequencies are multiplied; the accumulators AC, BD, AD, BC
-// give both halves of the x spectrum at the end.  NKS = ceil(n/4),
-// NTM = ceil(ceil(npw/2)/16) x-frequency tiles.
-template <int NKS, int NTM>
+//   G_v  = sum_o F_o (T_y(o) diag py w)    (rows m1 < 16, columns k2 of the tile)
+//   out += sum_v (T_v diag px)^T G_v       with T(m, npw-1-k) = conj T(m, k), so
+// that only ceil(npw/2) x frequencies are multiplied; the accumulators AC,
+// BD, AD, BC give both halves of the x spectrum at the end.  T_t(m, k) is
+// table t at tab[level*lstride + t*tstride + m*npw + k]; o runs over the
+// channels of NTAB tables (chan_x, chan_y) and v over the x tables.  The
+// forward transform (bfgt/bfgt_pwrouts.f:98-147) is NTAB = 1 with the
+// values-to-PW table and w = 1; the adjoint of the evaluation reads the
+// cotangents with the conjugated evaluation tables and w = 1/2 on the middle
+// y frequency of an odd expansion.  NKS = ceil(n/4), NTM = ceil(ceil(npw/2)/16).
+template <int NKS, int NTM, int NTAB>
 __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_up_mfma(
-    int n, int P, int nd, int nTN, int nitems, const double* __restrict__ fvals,
-    const PwLeaf* __restrict__ leaves, const UpChunk* __restrict__ chunks, const cplx* __restrict__ tpoly,
+    int n, int P, int nd, int nTN, int nitems, LeafIn in, const PwLeaf* __restrict__ leaves,
+    const UpChunk* __restrict__ chunks, const cplx* __restrict__ tab, int lstride, int tstride, int ymid_half,
     const cplx* __restrict__ phase, const int* __restrict__ levbase, cplx* __restrict__ partial) {
+    constexpr int NOUT = nchan(NTAB);
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
     const int item = blockIdx.x * kWavesPerBlock + wave;
     if (item >= nitems) return;  // no block-wide barriers in this kernel
@@ -81,6 +113,7 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_up_mfma(
     const int H = (P + 1) / 2, Mlo = (P + 1) / 2, nexp = P * H, npbox = n * n;
     const int k2 = 16 * tN + cc;
     const bool k2ok = k2 < H;
+    const double wy = (ymid_half && k2 >= P / 2) ? 0.5 : 1.0;
     const UpChunk ch = chunks[ci];
     for (int d = 0; d < nd; d++) {
         v4d acc[4][NTM];
@@ -90,21 +123,30 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_up_mfma(
             for (int t = 0; t < NTM; t++) acc[p][t] = v4d{0, 0, 0, 0};
         for (int li = 0; li < ch.count; li++) {
             const PwLeaf lf = leaves[ch.first + li];
-            const cplx* T = tpoly + (size_t)lf.level * n * P;
+            const cplx* T = tab + (size_t)lf.level * lstride;
             const cplx* ph = phase + levbase[lf.level];
-            const double* f = fvals + (size_t)lf.box * npbox * nd + d;
+            const size_t pb0 = (size_t)lf.box * npbox;
             // y phase of this lane's column (upward shift: conjugate of the table)
             cplx py = k2ok ? ph[lf.jy * P + k2] : cplx{0, 0};
-            py.y = -py.y;
-            v4d Gr = {0, 0, 0, 0}, Gi = {0, 0, 0, 0};
+            py.x *= wy;
+            py.y *= -wy;
+            v4d Gr[NTAB], Gi[NTAB];
+#pragma unroll
+            for (int v = 0; v < NTAB; v++) Gr[v] = Gi[v] = v4d{0, 0, 0, 0};
 #pragma unroll
             for (int s = 0; s < NKS; s++) {
                 const int m2 = 4 * s + q;
                 const bool mok = m2 < n;
-                double a = (cc < n && mok) ? f[(cc + n * m2) * nd] : 0.0;
-                cplx ty = (mok && k2ok) ? cmul(T[m2 * P + k2], py) : cplx{0, 0};
-                Gr = mfma(a, ty.x, Gr);
-                Gi = mfma(a, ty.y, Gi);
+                cplx ty[NTAB];
+#pragma unroll
+                for (int u = 0; u < NTAB; u++)
+                    ty[u] = (mok && k2ok) ? cmul(T[u * tstride + m2 * P + k2], py) : cplx{0, 0};
+#pragma unroll
+                for (int o = 0; o < NOUT; o++) {
+                    double a = (cc < n && mok) ? leaf_in(in, o, pb0 + cc + n * m2, nd, d) : 0.0;
+                    Gr[chan_x(o)] = mfma(a, ty[chan_y(o)].x, Gr[chan_x(o)]);
+                    Gi[chan_x(o)] = mfma(a, ty[chan_y(o)].y, Gi[chan_x(o)]);
+                }
             }
 #pragma unroll
             for (int t = 0; t < NTM; t++) {
@@ -113,13 +155,16 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_up_mfma(
                 cplx px = k1ok ? ph[lf.jx * P + k1] : cplx{0, 0};
                 px.y = -px.y;
 #pragma unroll
-                for (int s = 0; s < NKS; s++) {
-                    const int m1 = 4 * s + q;
-                    cplx tx = (m1 < n && k1ok) ? cmul(T[m1 * P + k1], px) : cplx{0, 0};
-                    acc[0][t] = mfma(tx.x, Gr[s], acc[0][t]);
-                    acc[1][t] = mfma(tx.y, Gi[s], acc[1][t]);
-                    acc[2][t] = mfma(tx.x, Gi[s], acc[2][t]);
-                    acc[3][t] = mfma(tx.y, Gr[s], acc[3][t]);
+                for (int v = 0; v < NTAB; v++) {
+#pragma unroll
+                    for (int s = 0; s < NKS; s++) {
+                        const int m1 = 4 * s + q;
+                        cplx tx = (m1 < n && k1ok) ? cmul(T[v * tstride + m1 * P + k1], px) : cplx{0, 0};
+                        acc[0][t] = mfma(tx.x, Gr[v][s], acc[0][t]);
+                        acc[1][t] = mfma(tx.y, Gi[v][s], acc[1][t]);
+                        acc[2][t] = mfma(tx.x, Gi[v][s], acc[2][t]);
+                        acc[3][t] = mfma(tx.y, Gr[v][s], acc[3][t]);
+                    }
                 }
             }
         }
@@ -150,18 +195,23 @@ struct Fold {
 
 // Downward pass.  One wave per PW leaf; y frequencies are processed in tiles
 // of 16:
-//   FF^T = S^T a + i D^T b        a + i b = E_t diag(px), the x contraction
-//                                 folded with E(npw-1-m) = conj E(m)
-//   out^T += Re[(E_u diag(py w))^T FF_v^T]   w = 1/2 on the middle frequency
-//                                 of an odd expansion (bfgt/bfgt_pwrouts.f:591-595)
-// for the (v, u) pairs pot (0,0), gx (1,0), gy (0,1), hxx (2,0), hxy (1,1),
-// hyy (0,2).  The second product reads the first one's accumulators directly.
+//   FF_t^T = S^T a_t + i D^T b_t      a_t + i b_t = E_t diag(px), the x
+//                                     contraction folded with
+//                                     E_t(npw-1-m) = conj E_t(m)
+//   out_o^T += Re[(E_y(o) diag(py w))^T FF_x(o)^T]
+// for the NTAB-table channels o (chan_x, chan_y).  The second product reads
+// the first one's accumulators directly.  E_t(m, i) is table t at
+// tab[level*lstride + (t*npw + m)*n + i].  The forward evaluation
+// (bfgt/bfgt_pwrouts.f:549-784) uses the PW-to-values tables and w = 1/2 on
+// the middle y frequency of an odd expansion (:591-595); the adjoint of the
+// forward transform is NTAB = 1 with the conjugated values-to-PW table and
+// w = 1, writing the leaf-value cotangent.
 template <int NTAB>
 __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_down_mfma(
-    int n, int P, int nd, int nleaf, const PwLeaf* __restrict__ leaves, const cplx* __restrict__ tpw,
-    const cplx* __restrict__ phase, const int* __restrict__ levbase, const Fold* __restrict__ fold,
+    int n, int P, int nd, int nleaf, const PwLeaf* __restrict__ leaves, const cplx* __restrict__ tab, int lstride,
+    int ymid_half, const cplx* __restrict__ phase, const int* __restrict__ levbase, const Fold* __restrict__ fold,
     double* __restrict__ pot, double* __restrict__ grad, double* __restrict__ hess) {
-    constexpr int NOUT = NTAB == 1 ? 1 : (NTAB == 2 ? 3 : 6);
+    constexpr int NOUT = nchan(NTAB);
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
     const int item = blockIdx.x * kWavesPerBlock + wave;
     if (item >= nleaf) return;  // no block-wide barriers in this kernel
@@ -170,10 +220,8 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_down_mfma(
     const int nks1 = (Mh + 3) / 4, nT = (H + 15) / 16;
     const bool ccok = cc < n;
     const PwLeaf lf = leaves[item];
-    const cplx* E = tpw + (size_t)lf.level * 3 * P * n;  // [t][m][i]
+    const cplx* E = tab + (size_t)lf.level * lstride;  // [t][m][i]
     const cplx* ph = phase + levbase[lf.level];
-    constexpr int vv[6] = {0, 1, 0, 2, 1, 0};
-    constexpr int uu[6] = {0, 0, 1, 0, 1, 2};
     for (int d = 0; d < nd; d++) {
         const Fold* F = fold + ((size_t)lf.anc * nd + d) * Mh * H;
         v4d out[NOUT];
@@ -206,7 +254,7 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_down_mfma(
                 const int m2s = 16 * tM + 4 * s + q;
                 const bool ok = m2s < H;
                 cplx pyw = ok ? ph[lf.jy * P + m2s] : cplx{0, 0};
-                const double w = (m2s < P2) ? 1.0 : 0.5;
+                const double w = (ymid_half && m2s >= P2) ? 0.5 : 1.0;
                 pyw.x *= w;
                 pyw.y *= w;
                 cplx eu[NTAB];
@@ -214,8 +262,8 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_down_mfma(
                 for (int u = 0; u < NTAB; u++) eu[u] = (ok && ccok) ? cmul(E[(u * P + m2s) * n + cc], pyw) : cplx{0, 0};
 #pragma unroll
                 for (int o = 0; o < NOUT; o++) {
-                    out[o] = mfma(eu[uu[o]].x, FFr[vv[o]][s], out[o]);
-                    out[o] = mfma(-eu[uu[o]].y, FFi[vv[o]][s], out[o]);
+                    out[o] = mfma(eu[chan_y(o)].x, FFr[chan_x(o)][s], out[o]);
+                    out[o] = mfma(-eu[chan_y(o)].y, FFi[chan_x(o)][s], out[o]);
                 }
             }
         }
@@ -238,6 +286,30 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_down_mfma(
                     }
                 }
             }
+        }
+    }
+}
+
+// Adjoint tables: tadj_e[level][t][i][k] = conj(E_t[level][k][i]) (the
+// adjoint of the evaluation, read by the upward kernel) and
+// tadj_t[level][k][i] = conj(T[level][i][k]) (the adjoint of the forward
+// transform, read by the downward kernel).
+__global__ void k_adjoint_tables(int n, int P, int nlev, const cplx* __restrict__ tpoly, const cplx* __restrict__ tpw,
+                                 cplx* __restrict__ tadj_e, cplx* __restrict__ tadj_t) {
+    const int per = n * P;
+    const size_t tot = (size_t)nlev * 4 * per;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < tot; e += (size_t)gridDim.x * blockDim.x) {
+        int l = (int)(e / (4 * per));
+        int r = (int)(e % (4 * per));
+        int t = r / per, w = r % per;
+        if (t < 3) {
+            int i = w / P, k = w % P;  // destination [t][i][k]
+            cplx v = tpw[((size_t)l * 3 + t) * per + (size_t)k * n + i];
+            tadj_e[((size_t)l * 3 + t) * per + w] = cplx{v.x, -v.y};
+        } else {
+            int k = w / n, i = w % n;  // destination [k][i]
+            cplx v = tpoly[(size_t)l * per + (size_t)i * P + k];
+            tadj_t[(size_t)l * per + w] = cplx{v.x, -v.y};
         }
     }
 }
@@ -460,12 +532,16 @@ __global__ void k_reduce2(int len, const int4* __restrict__ boxes, const cplx* _
 
 // loc = mp(self) + sum over the PW list of shift(offset) * mp(source)
 // (bfgt/boxfgt.f:805-822).
+// The adjoint translation runs over the transposed lists with conjugated
+// shifts (csign = -1): a translation by a pure phase is unitary.
 __device__ inline cplx m2l_at(const cplx* __restrict__ wshift, const cplx* __restrict__ mp, int nexp, size_t len,
-                              int a, int e, int j0, int j1, const int2* __restrict__ m2l, int q) {
+                              int a, int e, int j0, int j1, const int2* __restrict__ m2l, int q, double csign) {
     cplx s = mp[(size_t)a * len + e];
     for (int j = j0; j < j1; j++) {
         int2 sj = m2l[j];
-        s = cfma(wshift[(size_t)sj.y * nexp + q], mp[(size_t)sj.x * len + e], s);
+        cplx w = wshift[(size_t)sj.y * nexp + q];
+        w.y *= csign;
+        s = cfma(w, mp[(size_t)sj.x * len + e], s);
     }
     return s;
 }
@@ -476,13 +552,14 @@ __global__ void k_m2l(int nd, int nexp, const int* __restrict__ m2l_ptr, const i
     const int len = nd * nexp;
     const int j0 = m2l_ptr[a], j1 = m2l_ptr[a + 1];
     for (int e = blockIdx.y * blockDim.x + threadIdx.x; e < len; e += gridDim.y * blockDim.x)
-        loc[(size_t)a * len + e] = m2l_at(wshift, mp, nexp, len, a, e, j0, j1, m2l, e % nexp);
+        loc[(size_t)a * len + e] = m2l_at(wshift, mp, nexp, len, a, e, j0, j1, m2l, e % nexp, 1.0);
 }
 
 // Same translation, written mirror-folded (struct Fold) for the MFMA
 // downward pass.
 __global__ void k_m2l_fold(int nd, int P, const int* __restrict__ m2l_ptr, const int2* __restrict__ m2l,
-                           const cplx* __restrict__ wshift, const cplx* __restrict__ mp, Fold* __restrict__ fold) {
+                           const cplx* __restrict__ wshift, double csign, const cplx* __restrict__ mp,
+                           Fold* __restrict__ fold) {
     const int a = blockIdx.x;
     const int H = (P + 1) / 2, Mh = (P + 1) / 2, nexp = P * H;
     const int len = nd * nexp;
@@ -492,11 +569,11 @@ __global__ void k_m2l_fold(int nd, int P, const int* __restrict__ m2l_ptr, const
         int d = e / (Mh * H), r = e % (Mh * H);
         int m = r / H, k2 = r % H;
         int q1 = m * H + k2, mm = P - 1 - m;
-        cplx l1 = m2l_at(wshift, mp, nexp, len, a, d * nexp + q1, j0, j1, m2l, q1);
+        cplx l1 = m2l_at(wshift, mp, nexp, len, a, d * nexp + q1, j0, j1, m2l, q1, csign);
         Fold f;
         if (mm != m) {
             int q2 = mm * H + k2;
-            cplx l2 = m2l_at(wshift, mp, nexp, len, a, d * nexp + q2, j0, j1, m2l, q2);
+            cplx l2 = m2l_at(wshift, mp, nexp, len, a, d * nexp + q2, j0, j1, m2l, q2, csign);
             f.S = cplx{l1.x + l2.x, l1.y + l2.y};
             f.D = cplx{l1.x - l2.x, l1.y - l2.y};
         } else {
@@ -539,38 +616,59 @@ static int launch_up_vec(Plan& p, const double* fvals, size_t sh) {
     return BFGT_OK;
 }
 
-template <int NKS, int NTM>
-static int launch_up_mfma(Plan& p, const double* fvals, int nTN) {
+// Upward MFMA launch, dispatched on NKS = ceil(n/4), NTM and NTAB.
+struct UpArgs {
+    LeafIn in;
+    const cplx* tab;
+    int lstride, tstride, ymid_half, ntab;
+};
+
+template <int NKS, int NTM, int NTAB>
+static int launch_up_mfma3(Plan& p, const UpArgs& u, int nTN) {
     int nitems = p.nchunks * nTN;
     int blocks = (nitems + kWavesPerBlock - 1) / kWavesPerBlock;
-    k_up_mfma<NKS, NTM><<<blocks, kWave * kWavesPerBlock, 0, p.stream>>>(p.n, p.npw, p.nd, nTN, nitems, fvals,
-                                                                        p.d_pwleaf, p.d_chunks, p.d_tpoly,
-                                                                        p.d_phase, p.d_levbase, p.d_partial);
+    k_up_mfma<NKS, NTM, NTAB><<<blocks, kWave * kWavesPerBlock, 0, p.stream>>>(
+        p.n, p.npw, p.nd, nTN, nitems, u.in, p.d_pwleaf, p.d_chunks, u.tab, u.lstride, u.tstride, u.ymid_half,
+        p.d_phase, p.d_levbase, p.d_partial);
     BFGT_HIP_CHECK(hipGetLastError());
     return BFGT_OK;
 }
 
+template <int NKS, int NTM>
+static int launch_up_mfma2(Plan& p, const UpArgs& u, int nTN) {
+    switch (u.ntab) {
+        case 1: return launch_up_mfma3<NKS, NTM, 1>(p, u, nTN);
+        case 2: return launch_up_mfma3<NKS, NTM, 2>(p, u, nTN);
+        default: return launch_up_mfma3<NKS, NTM, 3>(p, u, nTN);
+    }
+}
+
 template <int NKS>
-static int launch_up_mfma_tm(Plan& p, const double* fvals, int nTM, int nTN) {
+static int launch_up_mfma1(Plan& p, const UpArgs& u, int nTM, int nTN) {
     switch (nTM) {
-        case 1: return launch_up_mfma<NKS, 1>(p, fvals, nTN);
-        case 2: return launch_up_mfma<NKS, 2>(p, fvals, nTN);
-        case 3: return launch_up_mfma<NKS, 3>(p, fvals, nTN);
-        default: return launch_up_mfma<NKS, 4>(p, fvals, nTN);
+        case 1: return launch_up_mfma2<NKS, 1>(p, u, nTN);
+        case 2: return launch_up_mfma2<NKS, 2>(p, u, nTN);
+        case 3: return launch_up_mfma2<NKS, 3>(p, u, nTN);
+        default: return launch_up_mfma2<NKS, 4>(p, u, nTN);
+    }
+}
+
+static int launch_up_mfma(Plan& p, const UpArgs& u) {
+    int H = (p.npw + 1) / 2;
+    int nTM = (H + 15) / 16, nTN = (H + 15) / 16;
+    switch ((p.n + 3) / 4) {
+        case 1: return launch_up_mfma1<1>(p, u, nTM, nTN);
+        case 2: return launch_up_mfma1<2>(p, u, nTM, nTN);
+        case 3: return launch_up_mfma1<3>(p, u, nTM, nTN);
+        default: return launch_up_mfma1<4>(p, u, nTM, nTN);
     }
 }
 
 int launch_upward(Plan& p, const double* fvals) {
     if (p.nchunks == 0) return BFGT_OK;
     if (use_mfma(p)) {
-        int H = (p.npw + 1) / 2;
-        int nTM = (H + 15) / 16, nTN = (H + 15) / 16;
-        switch ((p.n + 3) / 4) {
-            case 1: return launch_up_mfma_tm<1>(p, fvals, nTM, nTN);
-            case 2: return launch_up_mfma_tm<2>(p, fvals, nTM, nTN);
-            case 3: return launch_up_mfma_tm<3>(p, fvals, nTM, nTN);
-            default: return launch_up_mfma_tm<4>(p, fvals, nTM, nTN);
-        }
+        UpArgs u{LeafIn{fvals, nullptr, nullptr}, p.d_tpoly, p.n * p.npw, 0, 0, 1};
+        return launch_up_mfma(p, u);
     }
     size_t sh = up_smem_bytes(p.n, p.npw);
     int e = (p.nexp + kVecThreads - 1) / kVecThreads;
@@ -596,16 +694,18 @@ int launch_reduce(Plan& p) {
     return BFGT_OK;
 }
 
+static int launch_m2l_fold(Plan& p, const int* ptr, const int2* list, double csign) {
+    int Mh = (p.npw + 1) / 2, H = Mh;
+    int flen = p.nd * Mh * H;
+    int gy = (flen + 255) / 256;
+    k_m2l_fold<<<dim3(p.nanc, gy), 256, 0, p.stream>>>(p.nd, p.npw, ptr, list, p.d_wshift, csign, p.d_mp,
+                                                       reinterpret_cast<Fold*>(p.d_loc));
+    BFGT_HIP_CHECK(hipGetLastError());
+    return BFGT_OK;
+}
+
 int launch_m2l(Plan& p) {
-    if (use_mfma(p)) {
-        int Mh = (p.npw + 1) / 2, H = Mh;
-        int flen = p.nd * Mh * H;
-        int gy = (flen + 255) / 256;
-        k_m2l_fold<<<dim3(p.nanc, gy), 256, 0, p.stream>>>(p.nd, p.npw, p.d_m2l_ptr, p.d_m2l, p.d_wshift, p.d_mp,
-                                                           reinterpret_cast<Fold*>(p.d_loc));
-        BFGT_HIP_CHECK(hipGetLastError());
-        return BFGT_OK;
-    }
+    if (use_mfma(p)) return launch_m2l_fold(p, p.d_m2l_ptr, p.d_m2l, 1.0);
     int len = p.nd * p.nexp;
     int gy = (len + 255) / 256;
     k_m2l<<<dim3(p.nanc, gy), 256, 0, p.stream>>>(p.nd, p.nexp, p.d_m2l_ptr, p.d_m2l, p.d_wshift, p.d_mp, p.d_loc);
@@ -624,10 +724,11 @@ static int launch_down_vec(Plan& p, double* pot, double* grad, double* hess, siz
 }
 
 template <int NTAB>
-static int launch_down_mfma(Plan& p, double* pot, double* grad, double* hess) {
+static int launch_down_mfma(Plan& p, const cplx* tab, int lstride, int ymid_half, double* pot, double* grad,
+                            double* hess) {
     int blocks = (p.npwleaf + kWavesPerBlock - 1) / kWavesPerBlock;
     k_down_mfma<NTAB><<<blocks, kWave * kWavesPerBlock, 0, p.stream>>>(
-        p.n, p.npw, p.nd, p.npwleaf, p.d_pwleaf, p.d_tpw, p.d_phase, p.d_levbase,
+        p.n, p.npw, p.nd, p.npwleaf, p.d_pwleaf, tab, lstride, ymid_half, p.d_phase, p.d_levbase,
         reinterpret_cast<const Fold*>(p.d_loc), pot, grad, hess);
     BFGT_HIP_CHECK(hipGetLastError());
     return BFGT_OK;
@@ -636,15 +737,46 @@ static int launch_down_mfma(Plan& p, double* pot, double* grad, double* hess) {
 int launch_downward(Plan& p, double* pot, double* grad, double* hess) {
     if (p.npwleaf == 0) return BFGT_OK;
     if (use_mfma(p)) {
-        if (p.ifpgh == 1) return launch_down_mfma<1>(p, pot, grad, hess);
-        if (p.ifpgh == 2) return launch_down_mfma<2>(p, pot, grad, hess);
-        return launch_down_mfma<3>(p, pot, grad, hess);
+        const int ls = 3 * p.npw * p.n;
+        if (p.ifpgh == 1) return launch_down_mfma<1>(p, p.d_tpw, ls, 1, pot, grad, hess);
+        if (p.ifpgh == 2) return launch_down_mfma<2>(p, p.d_tpw, ls, 1, pot, grad, hess);
+        return launch_down_mfma<3>(p, p.d_tpw, ls, 1, pot, grad, hess);
     }
     size_t sh = down_smem_bytes(p.n, p.npw, p.ifpgh);
     int q = (p.n * p.n + kVecThreads - 1) / kVecThreads;
     if (q <= 1) return launch_down_vec<1>(p, pot, grad, hess, sh);
     if (q <= 2) return launch_down_vec<2>(p, pot, grad, hess, sh);
     return launch_down_vec<4>(p, pot, grad, hess, sh);
+}
+
+// ---------------------------------------------------------------------------
+// adjoint (transposed operator) of the PW stages, MFMA path only
+// ---------------------------------------------------------------------------
+
+int build_adjoint_tables(Plan& p) {
+    int nlev = p.nlevels + 1;
+    size_t tot = (size_t)nlev * 4 * p.n * p.npw;
+    int blocks = (int)std::min<size_t>((tot + 255) / 256, 4096);
+    k_adjoint_tables<<<blocks, 256, 0, p.stream>>>(p.n, p.npw, nlev, p.d_tpoly, p.d_tpw, p.d_tadj_e, p.d_tadj_t);
+    BFGT_HIP_CHECK(hipGetLastError());
+    return BFGT_OK;
+}
+
+// cotangents of pot/grad/hess at PW leaves -> per-chunk partial sums of the
+// cutoff-level local-expansion cotangent
+int launch_adj_eval(Plan& p, const double* gpot, const double* ggrad, const double* ghess) {
+    if (p.nchunks == 0) return BFGT_OK;
+    UpArgs u{LeafIn{gpot, ggrad, ghess}, p.d_tadj_e, 3 * p.n * p.npw, p.n * p.npw, 1, p.ifpgh};
+    return launch_up_mfma(p, u);
+}
+
+// local-expansion cotangent (in d_mp) -> multipole cotangent, mirror-folded in d_loc
+int launch_adj_m2l(Plan& p) { return launch_m2l_fold(p, p.d_m2lT_ptr, p.d_m2lT, -1.0); }
+
+// multipole cotangent -> leaf-value cotangent at PW leaves (accumulated)
+int launch_adj_transform(Plan& p, double* fbar) {
+    if (p.npwleaf == 0) return BFGT_OK;
+    return launch_down_mfma<1>(p, p.d_tadj_t, p.n * p.npw, 0, fbar, nullptr, nullptr);
 }
 
 }  // namespace bfgt

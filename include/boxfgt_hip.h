@@ -79,6 +79,24 @@ int bfgt_plan_create(bfgt_plan_t *plan, int nd, int ndim, double delta, double e
 int bfgt_plan_execute(bfgt_plan_t plan, const double *d_fvals, double *d_pot, double *d_grad, double *d_hess,
                       double *timeinfo);
 
+/* Adjoint (transposed operator) of bfgt_plan_execute, on device arrays,
+ * asynchronous on the plan's stream; allocates nothing.  No counterpart in
+ * the reference: it exists for callers that differentiate through the
+ * transform (gradient checks, adjoint solves).  With A the real-linear map
+ * fvals -> (pot, grad, hess) of the plan's ifpgh on leaf boxes,
+ *   fbar += A^T (gpot, ggrad, ghess)
+ * in the plain Euclidean inner product of the arrays as laid out above
+ * (entries of non-leaf boxes are neither read nor written).  ggrad/ghess may
+ * be NULL when ifpgh does not ask for them.  Every stage is transposed:
+ * conjugated, transposed PW tables; the translation over the transposed
+ * lists with conjugated shifts; near-field tables read transposed per
+ * (target, source) pair.  Arithmetic is FP64 for every plan precision.
+ * Needs norder <= 16 and npw <= 128 (BFGT_ERR_RESOURCE otherwise).
+ * timeinfo, when non-NULL, is filled like bfgt_plan_execute's, per
+ * transposed stage. */
+int bfgt_plan_adjoint(bfgt_plan_t plan, const double *d_gpot, const double *d_ggrad, const double *d_ghess,
+                      double *d_fbar, double *timeinfo);
+
 int bfgt_plan_destroy(bfgt_plan_t plan);
 
 /* Per-stage device timing over a window of executions.  After
