@@ -31,6 +31,10 @@ HIP_FLAGS = [
     "-Wno-unused-result",
     # no packed FP32 instructions anywhere in the device code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # keep MFMA accumulators in ordinary VGPRs: the PW kernels feed one
+    # product's accumulator to the next as an operand, and the AGPR form
+    # copies every accumulator out and back on each loop iteration
+    "-mllvm", "-amdgpu-mfma-vgpr-form",
 ]
 
 SOURCES = ["bfgt_tables.hip", "bfgt_pw.hip", "bfgt_near.hip", "bfgt_plan.cpp"]

@@ -84,7 +84,8 @@ struct Plan {
     double* d_tloc = nullptr;     // [level][13][type][j][m], masked
     int* d_indloc = nullptr;      // [level][13][n+1][2]
     cplx* d_wshift = nullptr;     // [9][nexp]
-    cplx* d_phase = nullptr;      // leaf->ancestor phases, [levbase[l] + j*npw + k]
+    cplx* d_phase = nullptr;      // leaf->ancestor phases, [levbase[l] + j*pstride + k], zero for k >= npw
+    int pstride = 0;              // frequency stride of the phase and padded tables (>= npw)
     int* d_levbase = nullptr;     // [nlevels+1]
     std::vector<int> levbase;     // host copy (kept alive for the asynchronous upload)
     size_t nphase = 0;
@@ -111,11 +112,21 @@ struct Plan {
     int* d_near_ptr = nullptr;     // CSR [nnear_t+1]
     NearPair* d_near = nullptr;
 
-    // adjoint (transposed operator): conjugate-transposed PW tables, the
-    // transposed PW translation lists and the near pairs grouped by source
+    // zero-padded PW tables of the matrix-core kernels (16 point rows or
+    // columns, pstride frequencies), so that every load in their inner loops
+    // is unconditional and in bounds:
+    //   tup_f[l][m][k] = T(m, k)                 values -> PW (upward, forward)
+    //   tup_a[l][t][m][k] = conj E_t(k, m)       adjoint of the evaluation (upward)
+    //   tdn_f[l][t][k][i] = E_t(k, i)            PW -> values (downward, forward)
+    //   tdn_a[l][k][i] = conj T(i, k)            adjoint of the transform (downward)
+    cplx* d_tup_f = nullptr;
+    cplx* d_tup_a = nullptr;
+    cplx* d_tdn_f = nullptr;
+    cplx* d_tdn_a = nullptr;
+
+    // adjoint (transposed operator): the transposed PW translation lists and
+    // the near pairs grouped by source
     bool has_adjoint = false;
-    cplx* d_tadj_e = nullptr;      // [level][t][i][k] = conj(tpw[level][t][k][i])
-    cplx* d_tadj_t = nullptr;      // [level][k][i] = conj(tpoly[level][i][k])
     int* d_m2lT_ptr = nullptr;     // CSR [nanc+1]
     int2* d_m2lT = nullptr;        // (target ancestor, shift index)
     int nnearT_s = 0;
@@ -154,7 +165,8 @@ int launch_near(Plan& p, const double* fvals, double* pot, double* grad, double*
 
 // adjoint (bfgt_pw.hip, bfgt_near.hip)
 bool use_mfma(const Plan& p);
-int build_adjoint_tables(Plan& p);
+int build_padded_tables(Plan& p);
+size_t fold_len(const Plan& p);
 int launch_adj_eval(Plan& p, const double* gpot, const double* ggrad, const double* ghess);
 int launch_adj_m2l(Plan& p);
 int launch_adj_transform(Plan& p, double* fbar);

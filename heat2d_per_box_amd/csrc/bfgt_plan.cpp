@@ -377,12 +377,14 @@ int plan_create(Plan& p, int nd, int ndim, double delta, double eps, int ipoly, 
         if ((rc = dalloc(p, p.d_tpw, (size_t)(nlevels + 1) * 3 * npw * p.n))) return rc;
         if ((rc = dalloc(p, p.d_wshift, (size_t)9 * p.nexp))) return rc;
         // leaf->ancestor phase tables: 2^(l-L0) cells per axis at level l
+        const int H = (npw + 1) / 2;
+        p.pstride = use_mfma(p) ? std::max(npw, 16 * ((H + 15) / 16)) : npw;
         std::vector<int>& levbase = p.levbase;
         levbase.assign(nlevels + 1, 0);
         size_t tot = 0;
         for (int l = p.nlevstart; l <= nlevels; l++) {
             levbase[l] = (int)tot;
-            tot += ((size_t)1 << (l - p.nlevstart)) * npw;
+            tot += ((size_t)1 << (l - p.nlevstart)) * p.pstride;
             if (tot > ((size_t)1 << 24))
                 return set_error(BFGT_ERR_RESOURCE, "tree too deep below the cutoff level for the phase tables");
         }
@@ -392,6 +394,14 @@ int plan_create(Plan& p, int nd, int ndim, double delta, double eps, int ipoly, 
         if ((rc = build_pw_nodes(p))) return rc;
     }
     if ((rc = build_tables(p))) return rc;
+    if (p.has_pw && use_mfma(p)) {
+        size_t per = (size_t)(nlevels + 1) * 16 * p.pstride;
+        if ((rc = dalloc(p, p.d_tup_f, per))) return rc;
+        if ((rc = dalloc(p, p.d_tup_a, 3 * per))) return rc;
+        if ((rc = dalloc(p, p.d_tdn_f, 3 * per))) return rc;
+        if ((rc = dalloc(p, p.d_tdn_a, per))) return rc;
+        if ((rc = build_padded_tables(p))) return rc;
+    }
 
     // PW schedule: leaves grouped by their cutoff-level ancestor
     std::vector<PwLeaf> leaves;
@@ -603,18 +613,12 @@ int plan_create(Plan& p, int nd, int ndim, double delta, double eps, int ipoly, 
     if ((rc = upload(p, p.d_nearT_ptr, nsptr))) return rc;
     if ((rc = upload(p, p.d_nearT, pairsT))) return rc;
     p.has_adjoint = !p.has_pw || use_mfma(p);
-    if (p.has_pw && p.has_adjoint) {
-        size_t per = (size_t)(nlevels + 1) * p.n * p.npw;
-        if ((rc = dalloc(p, p.d_tadj_e, 3 * per))) return rc;
-        if ((rc = dalloc(p, p.d_tadj_t, per))) return rc;
-        if ((rc = build_adjoint_tables(p))) return rc;
-    }
     if (p.has_pw) {
         size_t len = (size_t)nd * p.nexp;
         if ((rc = dalloc(p, p.d_partial, (size_t)p.nchunks * len))) return rc;
         if ((rc = dalloc(p, p.d_gpart, (size_t)p.ngslot * len))) return rc;
         if ((rc = dalloc(p, p.d_mp, (size_t)p.nanc * len))) return rc;
-        if ((rc = dalloc(p, p.d_loc, (size_t)p.nanc * loc_buffer_cplx(p)))) return rc;
+        if ((rc = dalloc(p, p.d_loc, (size_t)p.nanc * std::max(loc_buffer_cplx(p), 2 * fold_len(p))))) return rc;
     }
     BFGT_HIP_CHECK(hipStreamSynchronize(p.stream));
     auto t1 = std::chrono::steady_clock::now();
@@ -832,16 +836,19 @@ int bfgt_plan_get_table(bfgt_plan_t plan, int which, int level, void* host_out, 
         case 5:
             if (!p.has_pw || level < p.nlevstart) return set_error(BFGT_ERR_ARG, "no phase table for this level");
             sz = sizeof(bfgt::cplx) * ((size_t)1 << (level - p.nlevstart)) * p.npw;
-            src = p.d_phase + p.levbase[level];
-            break;
+            if (nbytes < sz) return set_error(BFGT_ERR_ARG, "output buffer too small");
+            BFGT_HIP_CHECK(hipMemcpy2D(host_out, sizeof(bfgt::cplx) * p.npw, p.d_phase + p.levbase[level],
+                                       sizeof(bfgt::cplx) * p.pstride, sizeof(bfgt::cplx) * p.npw,
+                                       (size_t)1 << (level - p.nlevstart), hipMemcpyDeviceToHost));
+            return BFGT_OK;
         case 6:
             sz = sizeof(bfgt::cplx) * (size_t)p.nanc * p.nd * p.nexp;
             src = p.d_mp;
             break;
         case 7:
-            // MFMA path: mirror-folded (S, D) pairs [anc][d][ceil(npw/2)][H]; else [anc][d][npw][H]
-            sz = sizeof(bfgt::cplx) * (size_t)p.nanc * p.nd *
-                 (bfgt::use_mfma(p) ? 2 * (size_t)((p.npw + 1) / 2) * ((p.npw + 1) / 2) : (size_t)p.nexp);
+            // MFMA path: mirror-folded (S, D) pairs [anc][d][Mh4][H16] (zero padded); else [anc][d][npw][H]
+            sz = sizeof(bfgt::cplx) * (size_t)p.nanc *
+                 (bfgt::use_mfma(p) ? 2 * bfgt::fold_len(p) : (size_t)p.nd * p.nexp);
             src = p.d_loc;
             break;
         default:

@@ -87,22 +87,30 @@ __device__ inline double leaf_in(const LeafIn& in, int o, size_t pb, int nd, int
     }
 }
 
+// The matrix-core kernels read zero-padded tables (16 point rows/columns,
+// ps >= npw frequencies, zero beyond npw), zero-padded phases and a
+// zero-padded folded expansion, so their inner loops have no bounds checks:
+// every load is unconditional and in bounds, and padding rows or columns
+// either contribute exact zeros or land in outputs that are not stored.
+
 // Upward pass.  One wave per (chunk, y-frequency tile tN): for every leaf of
 // the chunk
 //   G_v  = sum_o F_o (T_y(o) diag py w)    (rows m1 < 16, columns k2 of the tile)
 //   out += sum_v (T_v diag px)^T G_v       with T(m, npw-1-k) = conj T(m, k), so
 // that only ceil(npw/2) x frequencies are multiplied; the accumulators AC,
 // BD, AD, BC give both halves of the x spectrum at the end.  T_t(m, k) is
-// table t at tab[level*lstride + t*tstride + m*npw + k]; o runs over the
-// channels of NTAB tables (chan_x, chan_y) and v over the x tables.  The
-// forward transform (bfgt/bfgt_pwrouts.f:98-147) is NTAB = 1 with the
-// values-to-PW table and w = 1; the adjoint of the evaluation reads the
-// cotangents with the conjugated evaluation tables and w = 1/2 on the middle
-// y frequency of an odd expansion.  NKS = ceil(n/4), NTM = ceil(ceil(npw/2)/16).
+// tab[level*lstride + t*16*ps + m*ps + k]; o runs over the channels of NTAB
+// tables (chan_x, chan_y) and v over the x tables.  The forward transform
+// (bfgt/bfgt_pwrouts.f:98-147) is NTAB = 1 with the values-to-PW table and
+// w = 1; the adjoint of the evaluation reads the cotangents with the
+// conjugated evaluation tables and w = 1/2 on the middle y frequency of an
+// odd expansion.  Leaf values are read at clamped point indices (finite
+// values) and meet zero table rows beyond n.  NKS = ceil(n/4),
+// NTM = ceil(ceil(npw/2)/16).
 template <int NKS, int NTM, int NTAB>
 __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_up_mfma(
-    int n, int P, int nd, int nTN, int nitems, LeafIn in, const PwLeaf* __restrict__ leaves,
-    const UpChunk* __restrict__ chunks, const cplx* __restrict__ tab, int lstride, int tstride, int ymid_half,
+    int n, int P, int ps, int nd, int nTN, int nitems, LeafIn in, const PwLeaf* __restrict__ leaves,
+    const UpChunk* __restrict__ chunks, const cplx* __restrict__ tab, int lstride, int ymid_half,
     const cplx* __restrict__ phase, const int* __restrict__ levbase, cplx* __restrict__ partial) {
     constexpr int NOUT = nchan(NTAB);
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
@@ -111,9 +119,13 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_up_mfma(
     const int ci = item / nTN, tN = item % nTN;
     const int q = lane >> 4, cc = lane & 15;
     const int H = (P + 1) / 2, Mlo = (P + 1) / 2, nexp = P * H, npbox = n * n;
+    const int tstride = 16 * ps;
     const int k2 = 16 * tN + cc;
-    const bool k2ok = k2 < H;
     const double wy = (ymid_half && k2 >= P / 2) ? 0.5 : 1.0;
+    const int ccl = min(cc, n - 1);
+    int pidx[NKS];  // clamped point index of (cc, m2 = 4s + q)
+#pragma unroll
+    for (int s = 0; s < NKS; s++) pidx[s] = ccl + n * min(4 * s + q, n - 1);
     const UpChunk ch = chunks[ci];
     for (int d = 0; d < nd; d++) {
         v4d acc[4][NTM];
@@ -127,7 +139,7 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_up_mfma(
             const cplx* ph = phase + levbase[lf.level];
             const size_t pb0 = (size_t)lf.box * npbox;
             // y phase of this lane's column (upward shift: conjugate of the table)
-            cplx py = k2ok ? ph[lf.jy * P + k2] : cplx{0, 0};
+            cplx py = ph[lf.jy * ps + k2];
             py.x *= wy;
             py.y *= -wy;
             v4d Gr[NTAB], Gi[NTAB];
@@ -136,14 +148,12 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_up_mfma(
 #pragma unroll
             for (int s = 0; s < NKS; s++) {
                 const int m2 = 4 * s + q;
-                const bool mok = m2 < n;
                 cplx ty[NTAB];
 #pragma unroll
-                for (int u = 0; u < NTAB; u++)
-                    ty[u] = (mok && k2ok) ? cmul(T[u * tstride + m2 * P + k2], py) : cplx{0, 0};
+                for (int u = 0; u < NTAB; u++) ty[u] = cmul(T[u * tstride + m2 * ps + k2], py);
 #pragma unroll
                 for (int o = 0; o < NOUT; o++) {
-                    double a = (cc < n && mok) ? leaf_in(in, o, pb0 + cc + n * m2, nd, d) : 0.0;
+                    double a = leaf_in(in, o, pb0 + pidx[s], nd, d);
                     Gr[chan_x(o)] = mfma(a, ty[chan_y(o)].x, Gr[chan_x(o)]);
                     Gi[chan_x(o)] = mfma(a, ty[chan_y(o)].y, Gi[chan_x(o)]);
                 }
@@ -151,15 +161,14 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_up_mfma(
 #pragma unroll
             for (int t = 0; t < NTM; t++) {
                 const int k1 = 16 * t + cc;
-                const bool k1ok = k1 < Mlo;
-                cplx px = k1ok ? ph[lf.jx * P + k1] : cplx{0, 0};
+                cplx px = ph[lf.jx * ps + k1];
                 px.y = -px.y;
 #pragma unroll
                 for (int v = 0; v < NTAB; v++) {
 #pragma unroll
                     for (int s = 0; s < NKS; s++) {
                         const int m1 = 4 * s + q;
-                        cplx tx = (m1 < n && k1ok) ? cmul(T[v * tstride + m1 * P + k1], px) : cplx{0, 0};
+                        cplx tx = cmul(T[v * tstride + m1 * ps + k1], px);
                         acc[0][t] = mfma(tx.x, Gr[v][s], acc[0][t]);
                         acc[1][t] = mfma(tx.y, Gi[v][s], acc[1][t]);
                         acc[2][t] = mfma(tx.x, Gi[v][s], acc[2][t]);
@@ -174,7 +183,7 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_up_mfma(
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 int k1 = 16 * t + drow(q, r);
-                if (k1 < Mlo && k2ok) {
+                if (k1 < Mlo && k2 < H) {
                     double AC = acc[0][t][r], BD = acc[1][t][r], AD = acc[2][t][r], BC = acc[3][t][r];
                     out[k1 * H + k2] = cplx{AC - BD, AD + BC};
                     int km = P - 1 - k1;
@@ -188,10 +197,13 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_up_mfma(
 // Mirror-folded local expansion of a cutoff box: for x frequency m < Mh =
 // ceil(npw/2) and retained y frequency k2 < H, S = L(m, k2) + L(npw-1-m, k2)
 // and D = L(m, k2) - L(npw-1-m, k2) (S = L, D = 0 on the middle frequency of
-// an odd expansion).
+// an odd expansion).  Stored [box][density][Mh4][H16] with
+// Mh4 = 4 ceil(Mh/4), H16 = 16 ceil(H/16), zero in the padding.
 struct Fold {
     cplx S, D;
 };
+
+__device__ __host__ inline int round_up(int a, int b) { return (a + b - 1) / b * b; }
 
 // Downward pass.  One wave per PW leaf; y frequencies are processed in tiles
 // of 16:
@@ -200,48 +212,48 @@ struct Fold {
 //                                     E_t(npw-1-m) = conj E_t(m)
 //   out_o^T += Re[(E_y(o) diag(py w))^T FF_x(o)^T]
 // for the NTAB-table channels o (chan_x, chan_y).  The second product reads
-// the first one's accumulators directly.  E_t(m, i) is table t at
-// tab[level*lstride + (t*npw + m)*n + i].  The forward evaluation
+// the first one's accumulators directly.  E_t(m, i) is
+// tab[level*lstride + (t*ps + m)*16 + i].  The forward evaluation
 // (bfgt/bfgt_pwrouts.f:549-784) uses the PW-to-values tables and w = 1/2 on
 // the middle y frequency of an odd expansion (:591-595); the adjoint of the
 // forward transform is NTAB = 1 with the conjugated values-to-PW table and
 // w = 1, writing the leaf-value cotangent.
 template <int NTAB>
 __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_down_mfma(
-    int n, int P, int nd, int nleaf, const PwLeaf* __restrict__ leaves, const cplx* __restrict__ tab, int lstride,
-    int ymid_half, const cplx* __restrict__ phase, const int* __restrict__ levbase, const Fold* __restrict__ fold,
-    double* __restrict__ pot, double* __restrict__ grad, double* __restrict__ hess) {
+    int n, int P, int ps, int nd, int nleaf, const PwLeaf* __restrict__ leaves, const cplx* __restrict__ tab,
+    int lstride, int ymid_half, const cplx* __restrict__ phase, const int* __restrict__ levbase,
+    const Fold* __restrict__ fold, double* __restrict__ pot, double* __restrict__ grad, double* __restrict__ hess) {
     constexpr int NOUT = nchan(NTAB);
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
     const int item = blockIdx.x * kWavesPerBlock + wave;
     if (item >= nleaf) return;  // no block-wide barriers in this kernel
     const int q = lane >> 4, cc = lane & 15;
-    const int H = (P + 1) / 2, Mh = (P + 1) / 2, P2 = P / 2, npbox = n * n;
-    const int nks1 = (Mh + 3) / 4, nT = (H + 15) / 16;
-    const bool ccok = cc < n;
+    const int H = (P + 1) / 2, P2 = P / 2, npbox = n * n;
+    const int Mh4 = round_up(H, 4), H16 = round_up(H, 16);
+    const int nks1 = Mh4 / 4, nT = H16 / 16;
+    const int tstride = 16 * ps;
     const PwLeaf lf = leaves[item];
     const cplx* E = tab + (size_t)lf.level * lstride;  // [t][m][i]
     const cplx* ph = phase + levbase[lf.level];
+    const cplx* phx = ph + lf.jx * ps;
+    const cplx* phy = ph + lf.jy * ps;
     for (int d = 0; d < nd; d++) {
-        const Fold* F = fold + ((size_t)lf.anc * nd + d) * Mh * H;
+        const Fold* F = fold + ((size_t)lf.anc * nd + d) * Mh4 * H16;
         v4d out[NOUT];
 #pragma unroll
         for (int o = 0; o < NOUT; o++) out[o] = v4d{0, 0, 0, 0};
         for (int tM = 0; tM < nT; tM++) {
             const int m2 = 16 * tM + cc;
-            const bool m2ok = m2 < H;
             v4d FFr[NTAB], FFi[NTAB];
 #pragma unroll
             for (int t = 0; t < NTAB; t++) FFr[t] = FFi[t] = v4d{0, 0, 0, 0};
             for (int s = 0; s < nks1; s++) {
                 const int m = 4 * s + q;
-                const bool mok = m < Mh;
-                Fold sd{{0, 0}, {0, 0}};
-                if (mok && m2ok) sd = F[m * H + m2];
-                cplx px = mok ? ph[lf.jx * P + m] : cplx{0, 0};
+                const Fold sd = F[m * H16 + m2];
+                const cplx px = phx[m];
 #pragma unroll
                 for (int t = 0; t < NTAB; t++) {
-                    cplx e = (mok && ccok) ? cmul(E[(t * P + m) * n + cc], px) : cplx{0, 0};
+                    cplx e = cmul(E[t * tstride + m * 16 + cc], px);
                     FFr[t] = mfma(sd.S.x, e.x, FFr[t]);
                     FFr[t] = mfma(-sd.D.y, e.y, FFr[t]);
                     FFi[t] = mfma(sd.S.y, e.x, FFi[t]);
@@ -252,14 +264,13 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_down_mfma(
 #pragma unroll
             for (int s = 0; s < 4; s++) {
                 const int m2s = 16 * tM + 4 * s + q;
-                const bool ok = m2s < H;
-                cplx pyw = ok ? ph[lf.jy * P + m2s] : cplx{0, 0};
+                cplx pyw = phy[m2s];
                 const double w = (ymid_half && m2s >= P2) ? 0.5 : 1.0;
                 pyw.x *= w;
                 pyw.y *= w;
                 cplx eu[NTAB];
 #pragma unroll
-                for (int u = 0; u < NTAB; u++) eu[u] = (ok && ccok) ? cmul(E[(u * P + m2s) * n + cc], pyw) : cplx{0, 0};
+                for (int u = 0; u < NTAB; u++) eu[u] = cmul(E[u * tstride + m2s * 16 + cc], pyw);
 #pragma unroll
                 for (int o = 0; o < NOUT; o++) {
                     out[o] = mfma(eu[chan_y(o)].x, FFr[chan_x(o)][s], out[o]);
@@ -268,48 +279,24 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_down_mfma(
             }
         }
         // out^T holds [k2 = drow(q, r)][k1 = cc]
-        if (ccok) {
+        if (cc < n) {
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 int k2 = drow(q, r);
                 if (k2 < n) {
                     size_t pb = (size_t)lf.box * npbox + cc + (size_t)n * k2;
                     pot[pb * nd + d] += 2.0 * out[0][r];
-                    if (NTAB >= 2) {
+                    if constexpr (NTAB >= 2) {
                         grad[(pb * 2 + 0) * nd + d] += 2.0 * out[1][r];
                         grad[(pb * 2 + 1) * nd + d] += 2.0 * out[2][r];
                     }
-                    if (NTAB >= 3) {
+                    if constexpr (NTAB >= 3) {
                         hess[(pb * 3 + 0) * nd + d] += 2.0 * out[3][r];
                         hess[(pb * 3 + 1) * nd + d] += 2.0 * out[4][r];
                         hess[(pb * 3 + 2) * nd + d] += 2.0 * out[5][r];
                     }
                 }
             }
-        }
-    }
-}
-
-// Adjoint tables: tadj_e[level][t][i][k] = conj(E_t[level][k][i]) (the
-// adjoint of the evaluation, read by the upward kernel) and
-// tadj_t[level][k][i] = conj(T[level][i][k]) (the adjoint of the forward
-// transform, read by the downward kernel).
-__global__ void k_adjoint_tables(int n, int P, int nlev, const cplx* __restrict__ tpoly, const cplx* __restrict__ tpw,
-                                 cplx* __restrict__ tadj_e, cplx* __restrict__ tadj_t) {
-    const int per = n * P;
-    const size_t tot = (size_t)nlev * 4 * per;
-    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < tot; e += (size_t)gridDim.x * blockDim.x) {
-        int l = (int)(e / (4 * per));
-        int r = (int)(e % (4 * per));
-        int t = r / per, w = r % per;
-        if (t < 3) {
-            int i = w / P, k = w % P;  // destination [t][i][k]
-            cplx v = tpw[((size_t)l * 3 + t) * per + (size_t)k * n + i];
-            tadj_e[((size_t)l * 3 + t) * per + w] = cplx{v.x, -v.y};
-        } else {
-            int k = w / n, i = w % n;  // destination [k][i]
-            cplx v = tpoly[(size_t)l * per + (size_t)i * P + k];
-            tadj_t[(size_t)l * per + w] = cplx{v.x, -v.y};
         }
     }
 }
@@ -555,30 +542,32 @@ __global__ void k_m2l(int nd, int nexp, const int* __restrict__ m2l_ptr, const i
         loc[(size_t)a * len + e] = m2l_at(wshift, mp, nexp, len, a, e, j0, j1, m2l, e % nexp, 1.0);
 }
 
-// Same translation, written mirror-folded (struct Fold) for the MFMA
-// downward pass.
+// Same translation, written mirror-folded and zero-padded (struct Fold) for
+// the MFMA downward pass.
 __global__ void k_m2l_fold(int nd, int P, const int* __restrict__ m2l_ptr, const int2* __restrict__ m2l,
                            const cplx* __restrict__ wshift, double csign, const cplx* __restrict__ mp,
                            Fold* __restrict__ fold) {
     const int a = blockIdx.x;
-    const int H = (P + 1) / 2, Mh = (P + 1) / 2, nexp = P * H;
+    const int H = (P + 1) / 2, nexp = P * H;
+    const int Mh4 = round_up(H, 4), H16 = round_up(H, 16);
     const int len = nd * nexp;
-    const int flen = nd * Mh * H;
+    const int flen = nd * Mh4 * H16;
     const int j0 = m2l_ptr[a], j1 = m2l_ptr[a + 1];
     for (int e = blockIdx.y * blockDim.x + threadIdx.x; e < flen; e += gridDim.y * blockDim.x) {
-        int d = e / (Mh * H), r = e % (Mh * H);
-        int m = r / H, k2 = r % H;
-        int q1 = m * H + k2, mm = P - 1 - m;
-        cplx l1 = m2l_at(wshift, mp, nexp, len, a, d * nexp + q1, j0, j1, m2l, q1, csign);
-        Fold f;
-        if (mm != m) {
-            int q2 = mm * H + k2;
-            cplx l2 = m2l_at(wshift, mp, nexp, len, a, d * nexp + q2, j0, j1, m2l, q2, csign);
-            f.S = cplx{l1.x + l2.x, l1.y + l2.y};
-            f.D = cplx{l1.x - l2.x, l1.y - l2.y};
-        } else {
-            f.S = l1;
-            f.D = cplx{0, 0};
+        int d = e / (Mh4 * H16), r = e % (Mh4 * H16);
+        int m = r / H16, k2 = r % H16;
+        Fold f{{0, 0}, {0, 0}};
+        if (m < H && k2 < H) {
+            int q1 = m * H + k2, mm = P - 1 - m;
+            cplx l1 = m2l_at(wshift, mp, nexp, len, a, d * nexp + q1, j0, j1, m2l, q1, csign);
+            if (mm != m) {
+                int q2 = mm * H + k2;
+                cplx l2 = m2l_at(wshift, mp, nexp, len, a, d * nexp + q2, j0, j1, m2l, q2, csign);
+                f.S = cplx{l1.x + l2.x, l1.y + l2.y};
+                f.D = cplx{l1.x - l2.x, l1.y - l2.y};
+            } else {
+                f.S = l1;
+            }
         }
         fold[(size_t)a * flen + e] = f;
     }
@@ -587,6 +576,11 @@ __global__ void k_m2l_fold(int nd, int P, const int* __restrict__ m2l_ptr, const
 }  // namespace
 
 bool use_mfma(const Plan& p) { return p.n <= 16 && (p.npw + 1) / 2 <= 16 * kUpMaxTM; }
+
+size_t fold_len(const Plan& p) {
+    int H = (p.npw + 1) / 2;
+    return (size_t)p.nd * round_up(H, 4) * round_up(H, 16);
+}
 
 size_t loc_buffer_cplx(const Plan& p) {
     // the MFMA path stores the mirror-folded local expansion: 2 * Mh * H >= npw * H
@@ -620,7 +614,7 @@ static int launch_up_vec(Plan& p, const double* fvals, size_t sh) {
 struct UpArgs {
     LeafIn in;
     const cplx* tab;
-    int lstride, tstride, ymid_half, ntab;
+    int lstride, ymid_half, ntab;
 };
 
 template <int NKS, int NTM, int NTAB>
@@ -628,7 +622,7 @@ static int launch_up_mfma3(Plan& p, const UpArgs& u, int nTN) {
     int nitems = p.nchunks * nTN;
     int blocks = (nitems + kWavesPerBlock - 1) / kWavesPerBlock;
     k_up_mfma<NKS, NTM, NTAB><<<blocks, kWave * kWavesPerBlock, 0, p.stream>>>(
-        p.n, p.npw, p.nd, nTN, nitems, u.in, p.d_pwleaf, p.d_chunks, u.tab, u.lstride, u.tstride, u.ymid_half,
+        p.n, p.npw, p.pstride, p.nd, nTN, nitems, u.in, p.d_pwleaf, p.d_chunks, u.tab, u.lstride, u.ymid_half,
         p.d_phase, p.d_levbase, p.d_partial);
     BFGT_HIP_CHECK(hipGetLastError());
     return BFGT_OK;
@@ -667,7 +661,7 @@ static int launch_up_mfma(Plan& p, const UpArgs& u) {
 int launch_upward(Plan& p, const double* fvals) {
     if (p.nchunks == 0) return BFGT_OK;
     if (use_mfma(p)) {
-        UpArgs u{LeafIn{fvals, nullptr, nullptr}, p.d_tpoly, p.n * p.npw, 0, 0, 1};
+        UpArgs u{LeafIn{fvals, nullptr, nullptr}, p.d_tup_f, 16 * p.pstride, 0, 1};
         return launch_up_mfma(p, u);
     }
     size_t sh = up_smem_bytes(p.n, p.npw);
@@ -695,8 +689,7 @@ int launch_reduce(Plan& p) {
 }
 
 static int launch_m2l_fold(Plan& p, const int* ptr, const int2* list, double csign) {
-    int Mh = (p.npw + 1) / 2, H = Mh;
-    int flen = p.nd * Mh * H;
+    int flen = (int)fold_len(p);
     int gy = (flen + 255) / 256;
     k_m2l_fold<<<dim3(p.nanc, gy), 256, 0, p.stream>>>(p.nd, p.npw, ptr, list, p.d_wshift, csign, p.d_mp,
                                                        reinterpret_cast<Fold*>(p.d_loc));
@@ -728,7 +721,7 @@ static int launch_down_mfma(Plan& p, const cplx* tab, int lstride, int ymid_half
                             double* hess) {
     int blocks = (p.npwleaf + kWavesPerBlock - 1) / kWavesPerBlock;
     k_down_mfma<NTAB><<<blocks, kWave * kWavesPerBlock, 0, p.stream>>>(
-        p.n, p.npw, p.nd, p.npwleaf, p.d_pwleaf, tab, lstride, ymid_half, p.d_phase, p.d_levbase,
+        p.n, p.npw, p.pstride, p.nd, p.npwleaf, p.d_pwleaf, tab, lstride, ymid_half, p.d_phase, p.d_levbase,
         reinterpret_cast<const Fold*>(p.d_loc), pot, grad, hess);
     BFGT_HIP_CHECK(hipGetLastError());
     return BFGT_OK;
@@ -737,10 +730,10 @@ static int launch_down_mfma(Plan& p, const cplx* tab, int lstride, int ymid_half
 int launch_downward(Plan& p, double* pot, double* grad, double* hess) {
     if (p.npwleaf == 0) return BFGT_OK;
     if (use_mfma(p)) {
-        const int ls = 3 * p.npw * p.n;
-        if (p.ifpgh == 1) return launch_down_mfma<1>(p, p.d_tpw, ls, 1, pot, grad, hess);
-        if (p.ifpgh == 2) return launch_down_mfma<2>(p, p.d_tpw, ls, 1, pot, grad, hess);
-        return launch_down_mfma<3>(p, p.d_tpw, ls, 1, pot, grad, hess);
+        const int ls = 3 * 16 * p.pstride;
+        if (p.ifpgh == 1) return launch_down_mfma<1>(p, p.d_tdn_f, ls, 1, pot, grad, hess);
+        if (p.ifpgh == 2) return launch_down_mfma<2>(p, p.d_tdn_f, ls, 1, pot, grad, hess);
+        return launch_down_mfma<3>(p, p.d_tdn_f, ls, 1, pot, grad, hess);
     }
     size_t sh = down_smem_bytes(p.n, p.npw, p.ifpgh);
     int q = (p.n * p.n + kVecThreads - 1) / kVecThreads;
@@ -753,20 +746,11 @@ int launch_downward(Plan& p, double* pot, double* grad, double* hess) {
 // adjoint (transposed operator) of the PW stages, MFMA path only
 // ---------------------------------------------------------------------------
 
-int build_adjoint_tables(Plan& p) {
-    int nlev = p.nlevels + 1;
-    size_t tot = (size_t)nlev * 4 * p.n * p.npw;
-    int blocks = (int)std::min<size_t>((tot + 255) / 256, 4096);
-    k_adjoint_tables<<<blocks, 256, 0, p.stream>>>(p.n, p.npw, nlev, p.d_tpoly, p.d_tpw, p.d_tadj_e, p.d_tadj_t);
-    BFGT_HIP_CHECK(hipGetLastError());
-    return BFGT_OK;
-}
-
 // cotangents of pot/grad/hess at PW leaves -> per-chunk partial sums of the
 // cutoff-level local-expansion cotangent
 int launch_adj_eval(Plan& p, const double* gpot, const double* ggrad, const double* ghess) {
     if (p.nchunks == 0) return BFGT_OK;
-    UpArgs u{LeafIn{gpot, ggrad, ghess}, p.d_tadj_e, 3 * p.n * p.npw, p.n * p.npw, 1, p.ifpgh};
+    UpArgs u{LeafIn{gpot, ggrad, ghess}, p.d_tup_a, 3 * 16 * p.pstride, 1, p.ifpgh};
     return launch_up_mfma(p, u);
 }
 
@@ -776,7 +760,7 @@ int launch_adj_m2l(Plan& p) { return launch_m2l_fold(p, p.d_m2lT_ptr, p.d_m2lT, 
 // multipole cotangent -> leaf-value cotangent at PW leaves (accumulated)
 int launch_adj_transform(Plan& p, double* fbar) {
     if (p.npwleaf == 0) return BFGT_OK;
-    return launch_down_mfma<1>(p, p.d_tadj_t, p.n * p.npw, 0, fbar, nullptr, nullptr);
+    return launch_down_mfma<1>(p, p.d_tdn_a, 16 * p.pstride, 0, fbar, nullptr, nullptr);
 }
 
 }  // namespace bfgt

@@ -18,6 +18,8 @@
 // the reference's sparse loops sum.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include <cmath>
 
 #include "bfgt_internal.h"
@@ -667,22 +669,68 @@ namespace bfgt {
 namespace {
 
 // Leaf-to-ancestor phases: for every level l >= L0 and every cell j of a
-// level-L0 box at level l, phase[levbase[l] + j*npw + k] =
+// level-L0 box at level l, phase[levbase[l] + j*pstride + k] =
 // exp(i t_k ((j + 1/2) b_l - b_L0/2) / sqrt(delta0)) -- the product of the
 // per-level merge/split diagonals (bfgt/bfgt_pwrouts.f:1645-1703) between a
-// leaf and its cutoff-level ancestor, evaluated directly.
-__global__ void k_phase_tables(int npw, int lev0, int nlev, const double* ts, const double* boxsize,
+// leaf and its cutoff-level ancestor, evaluated directly; zero for the
+// padding frequencies k >= npw.
+__global__ void k_phase_tables(int npw, int pstride, int lev0, int nlev, const double* ts, const double* boxsize,
                                double delta0, const int* levbase, cplx* phase) {
     int l = lev0 + blockIdx.y;
     if (l > nlev) return;
     int nj = 1 << (l - lev0);
     double sq = sqrt(delta0);
-    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < nj * npw; e += gridDim.x * blockDim.x) {
-        int j = e / npw, k = e % npw;
-        double off = ((j + 0.5) * boxsize[l] - 0.5 * boxsize[lev0]) / sq;
-        double s, c;
-        sincos(ts[k] * off, &s, &c);
-        phase[levbase[l] + e] = cplx{c, s};
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < nj * pstride; e += gridDim.x * blockDim.x) {
+        int j = e / pstride, k = e % pstride;
+        cplx v{0.0, 0.0};
+        if (k < npw) {
+            double off = ((j + 0.5) * boxsize[l] - 0.5 * boxsize[lev0]) / sq;
+            double s, c;
+            sincos(ts[k] * off, &s, &c);
+            v = cplx{c, s};
+        }
+        phase[levbase[l] + e] = v;
+    }
+}
+
+// Zero-padded copies of the PW tables for the matrix-core kernels (layouts in
+// the Plan declaration); levels above the cutoff are zero.
+__global__ void k_pad_tables(int n, int npw, int ps, int lev0, int nlev, const cplx* __restrict__ tpoly,
+                             const cplx* __restrict__ tpw, cplx* __restrict__ tup_f, cplx* __restrict__ tup_a,
+                             cplx* __restrict__ tdn_f, cplx* __restrict__ tdn_a) {
+    const int per = 16 * ps;  // one padded table of one level
+    const size_t tot = (size_t)nlev * 8 * per;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < tot; e += (size_t)gridDim.x * blockDim.x) {
+        const int l = (int)(e / (8 * per));
+        const int r = (int)(e % (8 * per));
+        const int slot = r / per, w = r % per;
+        const bool lev_ok = l >= lev0;
+        const cplx* T = tpoly + (size_t)l * npw * n;      // [m][k]
+        const cplx* E = tpw + (size_t)l * 3 * npw * n;    // [t][k][i]
+        cplx v{0.0, 0.0};
+        if (slot == 0) {  // tup_f [m][k]
+            int m = w / ps, k = w % ps;
+            if (lev_ok && m < n && k < npw) v = T[m * npw + k];
+            tup_f[(size_t)l * per + w] = v;
+        } else if (slot <= 3) {  // tup_a [t][m][k]
+            int t = slot - 1, m = w / ps, k = w % ps;
+            if (lev_ok && m < n && k < npw) {
+                v = E[(t * npw + k) * n + m];
+                v.y = -v.y;
+            }
+            tup_a[((size_t)l * 3 + t) * per + w] = v;
+        } else if (slot <= 6) {  // tdn_f [t][k][i]
+            int t = slot - 4, k = w / 16, i = w % 16;
+            if (lev_ok && k < npw && i < n) v = E[(t * npw + k) * n + i];
+            tdn_f[((size_t)l * 3 + t) * per + w] = v;
+        } else {  // tdn_a [k][i]
+            int k = w / 16, i = w % 16;
+            if (lev_ok && k < npw && i < n) {
+                v = T[i * npw + k];
+                v.y = -v.y;
+            }
+            tdn_a[(size_t)l * per + w] = v;
+        }
     }
 }
 
@@ -690,8 +738,18 @@ __global__ void k_phase_tables(int npw, int lev0, int nlev, const double* ts, co
 
 int build_phase_tables(Plan& p) {
     int nl = p.nlevels - p.nlevstart + 1;
-    k_phase_tables<<<dim3(64, nl), 256, 0, p.stream>>>(p.npw, p.nlevstart, p.nlevels, p.d_ts, p.d_boxsize,
-                                                      p.delta0, p.d_levbase, p.d_phase);
+    k_phase_tables<<<dim3(64, nl), 256, 0, p.stream>>>(p.npw, p.pstride, p.nlevstart, p.nlevels, p.d_ts,
+                                                      p.d_boxsize, p.delta0, p.d_levbase, p.d_phase);
+    BFGT_HIP_CHECK(hipGetLastError());
+    return BFGT_OK;
+}
+
+int build_padded_tables(Plan& p) {
+    int nlev = p.nlevels + 1;
+    size_t tot = (size_t)nlev * 8 * 16 * p.pstride;
+    int blocks = (int)std::min<size_t>((tot + 255) / 256, 4096);
+    k_pad_tables<<<blocks, 256, 0, p.stream>>>(p.n, p.npw, p.pstride, p.nlevstart, nlev, p.d_tpoly, p.d_tpw,
+                                               p.d_tup_f, p.d_tup_a, p.d_tdn_f, p.d_tdn_a);
     BFGT_HIP_CHECK(hipGetLastError());
     return BFGT_OK;
 }
