@@ -52,8 +52,9 @@ def algorithmic_work(plan, w):
             "pw_to_values": ("flop", down), "near_field": ("flop", near)}
 
 
-def cpu_baseline(w, gpu_out, max_threads):
-    """Time the reference Fortran boxfgt (or the C oracle) on the host cores."""
+def cpu_baseline(w, gpu_out, max_threads, extra=None):
+    """Time the reference Fortran boxfgt (or the C oracle) on the host cores, and
+    report the relative L2 difference (leaf points) of the GPU outputs from it."""
     from oracle import oracle as O
 
     nthreads = max(1, min(os.cpu_count() or 1, max_threads))
@@ -77,13 +78,19 @@ def cpu_baseline(w, gpu_out, max_threads):
             break
     tmed = float(np.median(times))
     lv = w.tree.nchild == 0
-    err = {}
-    for name, g, ref in (("pot", gpu_out.get("pot"), r.pot), ("grad", gpu_out.get("grad"), r.grad),
-                         ("hess", gpu_out.get("hess"), r.hess)):
-        if g is None:
-            continue
-        den = np.linalg.norm(ref[lv])
-        err[name] = float(np.linalg.norm((g - ref)[lv]) / den) if den > 0 else 0.0
+
+    def rel(outs):
+        e = {}
+        for name, ref in (("pot", r.pot), ("grad", r.grad), ("hess", r.hess)):
+            g = outs.get(name)
+            if g is None or ref is None:
+                continue
+            den = np.linalg.norm(ref[lv])
+            e[name] = float(np.linalg.norm((g - ref)[lv]) / den) if den > 0 else 0.0
+        return e
+
+    err = rel(gpu_out)
+    err_extra = {k: rel(v) for k, v in (extra or {}).items()}
     return {
         "value": w.npts * w.nd / tmed,
         "unit": "leaf pts/s",
@@ -93,6 +100,7 @@ def cpu_baseline(w, gpu_out, max_threads):
                   f"median of {len(times)} after one warm-up; {'amdflang -O3 -fopenmp reference' if kind == 'reference' else 'C oracle, 1 thread'}",
         "seconds_per_call": tmed,
         "gpu_vs_cpu_rel_err": err,
+        **{f"gpu_{k}_vs_cpu_rel_err": v for k, v in err_extra.items()},
     }
 
 
@@ -107,6 +115,7 @@ def main():
     ap.add_argument("--delta", type=float, default=None, help="override delta (config c3 only)")
     ap.add_argument("--eps", type=float, default=None, help="override eps (config c3 only)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-split", action="store_true", help="skip timing the other precision")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="run a device copy of this size first (calibrates rocprof byte counters)")
@@ -192,6 +201,41 @@ def main():
     roof["algorithmic_per_launch"] = amount
     roof["avg_launch_ms"] = stage_ms[dom]
 
+    # the other arithmetic on the same tree (split-FP32 leaf transforms vs all-FP64)
+    other = None
+    if not args.no_split:
+        oprec = "split32" if args.precision == "fp64" else "fp64"
+        plan2 = BoxFGT(w.tree.copy(), w.nd, w.delta, w.eps, w.ipoly, w.norder, w.ifpgh, precision=oprec)
+        pot2 = torch.zeros_like(pot)
+        grad2 = torch.zeros_like(grad) if grad is not None else None
+        hess2 = torch.zeros_like(hess) if hess is not None else None
+        plan2(fv, pot2, grad2, hess2)
+        torch.cuda.synchronize()
+        out2 = {"pot": pot2.cpu().numpy(), "grad": grad2.cpu().numpy() if grad2 is not None else None,
+                "hess": hess2.cpu().numpy() if hess2 is not None else None}
+        for _ in range(args.warmup):
+            plan2(fv, pot2, grad2, hess2)
+        torch.cuda.synchronize()
+        plan2.profile_begin(args.steps)
+        e0.record(stream)
+        for _ in range(args.steps):
+            plan2(fv, pot2, grad2, hess2)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ssum2, np2 = plan2.profile_end()
+        ms2 = e0.elapsed_time(e1) / args.steps
+        lv = w.tree.nchild == 0
+        diff = {}
+        for k in ("pot", "grad", "hess"):
+            if gpu_out[k] is not None:
+                den = np.linalg.norm(gpu_out[k][lv])
+                diff[k] = float(np.linalg.norm((out2[k] - gpu_out[k])[lv]) / den) if den > 0 else 0.0
+        other = {"precision": oprec, "ms_per_step": ms2, "value": npts / (ms2 * 1e-3),
+                 "time_ratio_vs_" + args.precision: ms2 / ms,
+                 "stage_ms": {k: 1e3 * v / max(np2, 1) for k, v in zip(names, ssum2)},
+                 "rel_diff_vs_" + args.precision + "_gpu": diff, "_out": out2}
+        plan2.close()
+
     # adjoint (transposed operator) on the same plan: cotangents of every output
     adj = None
     if plan.info["mfma"] or not plan.info["has_pw"]:
@@ -257,9 +301,13 @@ def main():
         "roofline": roof,
         "adjoint": adj,
     }
+    if other is not None:
+        out2 = other.pop("_out")
+        out[other["precision"]] = other
     if not args.no_cpu_baseline:
         print("[bench] timing the CPU baseline", file=sys.stderr, flush=True)
-        out["cpu_baseline"] = cpu_baseline(w, gpu_out, args.cpu_threads)
+        out["cpu_baseline"] = cpu_baseline(w, gpu_out, args.cpu_threads,
+                                           {other["precision"]: out2} if other is not None else None)
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out))

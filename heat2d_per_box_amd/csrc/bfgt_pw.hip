@@ -302,6 +302,214 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_down_mfma(
 }
 
 // ---------------------------------------------------------------------------
+// split-FP32 variants of the leaf transforms (v_mfma_f32_16x16x4f32)
+// ---------------------------------------------------------------------------
+//
+// Every FP64 operand x is split into x_hi = fl32(x), x_lo = fl32(x - x_hi); a
+// product a*b becomes a_hi*b_hi + a_hi*b_lo + a_lo*b_hi in FP32 MFMAs that
+// accumulate in FP32.  The table entries, phases (exp(i t x) in FP64) and
+// their products are formed in FP64 before the split; a product whose second
+// factor is an FP32 accumulator (the chained GEMM) uses two MFMAs.  Results
+// are converted to FP64 where they leave the kernel: the PW expansions (the
+// merge and translation stay FP64) and the pot/grad/hess accumulation.
+//
+// Lane layout of v_mfma_f32_16x16x4f32 on gfx950 (measured): A holds A[c][q],
+// B holds B[q][c], element r of the accumulator holds D[4q + r][c] -- the
+// transpose of the FP64 instruction's row order.  Register r of lane (q, c)
+// therefore serves as the B operand of a K-step whose K index is 4q + r, and
+// the A operand of that step is loaded at K index 4q + r.
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+__device__ inline v4f mfma32(float a, float b, v4f c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+
+struct Split {
+    float h, l;
+};
+__device__ inline Split split(double x) {
+    float h = (float)x;
+    return Split{h, (float)(x - (double)h)};
+}
+__device__ inline Split neg(Split s) { return Split{-s.h, -s.l}; }
+// c += a*b, both factors split
+__device__ inline v4f mfma3(Split a, Split b, v4f c) {
+    c = mfma32(a.l, b.h, c);
+    c = mfma32(a.h, b.l, c);
+    return mfma32(a.h, b.h, c);
+}
+// c += a*b, a split, b an FP32 accumulator value
+__device__ inline v4f mfma2(Split a, float b, v4f c) {
+    c = mfma32(a.l, b, c);
+    return mfma32(a.h, b, c);
+}
+__device__ inline int drow32(int q, int r) { return 4 * q + r; }
+
+// Forward upward pass, split-FP32 (same tiling as k_up_mfma with NTAB = 1).
+template <int NKS, int NTM>
+__global__ __launch_bounds__(kWave * kWavesPerBlock) void k_up_split(
+    int n, int P, int ps, int nd, int nTN, int nitems, const double* __restrict__ fvals,
+    const PwLeaf* __restrict__ leaves, const UpChunk* __restrict__ chunks, const cplx* __restrict__ tab, int lstride,
+    const cplx* __restrict__ phase, const int* __restrict__ levbase, cplx* __restrict__ partial) {
+    const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    const int item = blockIdx.x * kWavesPerBlock + wave;
+    if (item >= nitems) return;  // no block-wide barriers in this kernel
+    const int ci = item / nTN, tN = item % nTN;
+    const int q = lane >> 4, cc = lane & 15;
+    const int H = (P + 1) / 2, Mlo = (P + 1) / 2, nexp = P * H, npbox = n * n;
+    const int k2 = 16 * tN + cc;
+    const int ccl = min(cc, n - 1);
+    int pidx[NKS];
+#pragma unroll
+    for (int s = 0; s < NKS; s++) pidx[s] = ccl + n * min(4 * s + q, n - 1);
+    const UpChunk ch = chunks[ci];
+    for (int d = 0; d < nd; d++) {
+        v4f acc[4][NTM];
+#pragma unroll
+        for (int p = 0; p < 4; p++)
+#pragma unroll
+            for (int t = 0; t < NTM; t++) acc[p][t] = v4f{0, 0, 0, 0};
+        for (int li = 0; li < ch.count; li++) {
+            const PwLeaf lf = leaves[ch.first + li];
+            const cplx* T = tab + (size_t)lf.level * lstride;
+            const cplx* ph = phase + levbase[lf.level];
+            const size_t pb0 = (size_t)lf.box * npbox;
+            cplx py = ph[lf.jy * ps + k2];
+            py.y = -py.y;
+            // G = F (T diag py): K index m2 = 4s + q
+            v4f Gr = {0, 0, 0, 0}, Gi = {0, 0, 0, 0};
+#pragma unroll
+            for (int s = 0; s < NKS; s++) {
+                const int m2 = 4 * s + q;
+                cplx ty = cmul(T[m2 * ps + k2], py);
+                Split a = split(fvals[(pb0 + pidx[s]) * nd + d]);
+                Gr = mfma3(a, split(ty.x), Gr);
+                Gi = mfma3(a, split(ty.y), Gi);
+            }
+            // out += (T diag px)^T G: K-step r has K index m1 = 4q + r
+#pragma unroll
+            for (int t = 0; t < NTM; t++) {
+                const int k1 = 16 * t + cc;
+                cplx px = ph[lf.jx * ps + k1];
+                px.y = -px.y;
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const int m1 = drow32(q, r);
+                    cplx tx = cmul(T[m1 * ps + k1], px);
+                    Split xr = split(tx.x), xi = split(tx.y);
+                    acc[0][t] = mfma2(xr, Gr[r], acc[0][t]);
+                    acc[1][t] = mfma2(xi, Gi[r], acc[1][t]);
+                    acc[2][t] = mfma2(xr, Gi[r], acc[2][t]);
+                    acc[3][t] = mfma2(xi, Gr[r], acc[3][t]);
+                }
+            }
+        }
+        cplx* out = partial + ((size_t)ch.slot * nd + d) * nexp;
+#pragma unroll
+        for (int t = 0; t < NTM; t++) {
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                int k1 = 16 * t + drow32(q, r);
+                if (k1 < Mlo && k2 < H) {
+                    double AC = acc[0][t][r], BD = acc[1][t][r], AD = acc[2][t][r], BC = acc[3][t][r];
+                    out[k1 * H + k2] = cplx{AC - BD, AD + BC};
+                    int km = P - 1 - k1;
+                    if (km > k1) out[km * H + k2] = cplx{AC + BD, AD - BC};
+                }
+            }
+        }
+    }
+}
+
+// Forward downward pass, split-FP32 (same tiling as k_down_mfma).
+template <int NTAB>
+__global__ __launch_bounds__(kWave * kWavesPerBlock) void k_down_split(
+    int n, int P, int ps, int nd, int nleaf, const PwLeaf* __restrict__ leaves, const cplx* __restrict__ tab,
+    int lstride, const cplx* __restrict__ phase, const int* __restrict__ levbase, const Fold* __restrict__ fold,
+    double* __restrict__ pot, double* __restrict__ grad, double* __restrict__ hess) {
+    constexpr int NOUT = nchan(NTAB);
+    const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    const int item = blockIdx.x * kWavesPerBlock + wave;
+    if (item >= nleaf) return;  // no block-wide barriers in this kernel
+    const int q = lane >> 4, cc = lane & 15;
+    const int H = (P + 1) / 2, P2 = P / 2, npbox = n * n;
+    const int Mh4 = round_up(H, 4), H16 = round_up(H, 16);
+    const int nks1 = Mh4 / 4, nT = H16 / 16;
+    const int tstride = 16 * ps;
+    const PwLeaf lf = leaves[item];
+    const cplx* E = tab + (size_t)lf.level * lstride;
+    const cplx* ph = phase + levbase[lf.level];
+    const cplx* phx = ph + lf.jx * ps;
+    const cplx* phy = ph + lf.jy * ps;
+    for (int d = 0; d < nd; d++) {
+        const Fold* F = fold + ((size_t)lf.anc * nd + d) * Mh4 * H16;
+        v4f out[NOUT];
+#pragma unroll
+        for (int o = 0; o < NOUT; o++) out[o] = v4f{0, 0, 0, 0};
+        for (int tM = 0; tM < nT; tM++) {
+            const int m2 = 16 * tM + cc;
+            v4f FFr[NTAB], FFi[NTAB];
+#pragma unroll
+            for (int t = 0; t < NTAB; t++) FFr[t] = FFi[t] = v4f{0, 0, 0, 0};
+            for (int s = 0; s < nks1; s++) {
+                const int m = 4 * s + q;
+                const Fold sd = F[m * H16 + m2];
+                const cplx px = phx[m];
+                Split sr = split(sd.S.x), si = split(sd.S.y), dr = split(sd.D.x), di = neg(split(sd.D.y));
+#pragma unroll
+                for (int t = 0; t < NTAB; t++) {
+                    cplx e = cmul(E[t * tstride + m * 16 + cc], px);
+                    Split er = split(e.x), ei = split(e.y);
+                    FFr[t] = mfma3(sr, er, FFr[t]);
+                    FFr[t] = mfma3(di, ei, FFr[t]);
+                    FFi[t] = mfma3(si, er, FFi[t]);
+                    FFi[t] = mfma3(dr, ei, FFi[t]);
+                }
+            }
+            // register r of FF holds FF^T[16 tM + 4q + r][cc]: K-step r, K index 4q + r
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int m2s = 16 * tM + drow32(q, r);
+                cplx pyw = phy[m2s];
+                const double w = (m2s >= P2) ? 0.5 : 1.0;
+                pyw.x *= w;
+                pyw.y *= w;
+                Split eur[NTAB], eui[NTAB];
+#pragma unroll
+                for (int u = 0; u < NTAB; u++) {
+                    cplx eu = cmul(E[u * tstride + m2s * 16 + cc], pyw);
+                    eur[u] = split(eu.x);
+                    eui[u] = split(-eu.y);
+                }
+#pragma unroll
+                for (int o = 0; o < NOUT; o++) {
+                    out[o] = mfma2(eur[chan_y(o)], FFr[chan_x(o)][r], out[o]);
+                    out[o] = mfma2(eui[chan_y(o)], FFi[chan_x(o)][r], out[o]);
+                }
+            }
+        }
+        // register r of out^T holds [k2 = 4q + r][k1 = cc]
+        if (cc < n) {
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                int k2 = drow32(q, r);
+                if (k2 < n) {
+                    size_t pb = (size_t)lf.box * npbox + cc + (size_t)n * k2;
+                    pot[pb * nd + d] += 2.0 * (double)out[0][r];
+                    if constexpr (NTAB >= 2) {
+                        grad[(pb * 2 + 0) * nd + d] += 2.0 * (double)out[1][r];
+                        grad[(pb * 2 + 1) * nd + d] += 2.0 * (double)out[2][r];
+                    }
+                    if constexpr (NTAB >= 3) {
+                        hess[(pb * 3 + 0) * nd + d] += 2.0 * (double)out[3][r];
+                        hess[(pb * 3 + 1) * nd + d] += 2.0 * (double)out[4][r];
+                        hess[(pb * 3 + 2) * nd + d] += 2.0 * (double)out[5][r];
+                    }
+                }
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // vector-FMA kernels (leaf orders above 16 or expansions above 128 waves)
 // ---------------------------------------------------------------------------
 
@@ -658,8 +866,41 @@ static int launch_up_mfma(Plan& p, const UpArgs& u) {
     }
 }
 
+template <int NKS, int NTM>
+static int launch_up_split2(Plan& p, const double* fvals, int nTN) {
+    int nitems = p.nchunks * nTN;
+    int blocks = (nitems + kWavesPerBlock - 1) / kWavesPerBlock;
+    k_up_split<NKS, NTM><<<blocks, kWave * kWavesPerBlock, 0, p.stream>>>(
+        p.n, p.npw, p.pstride, p.nd, nTN, nitems, fvals, p.d_pwleaf, p.d_chunks, p.d_tup_f, 16 * p.pstride,
+        p.d_phase, p.d_levbase, p.d_partial);
+    BFGT_HIP_CHECK(hipGetLastError());
+    return BFGT_OK;
+}
+
+template <int NKS>
+static int launch_up_split1(Plan& p, const double* fvals, int nTM, int nTN) {
+    switch (nTM) {
+        case 1: return launch_up_split2<NKS, 1>(p, fvals, nTN);
+        case 2: return launch_up_split2<NKS, 2>(p, fvals, nTN);
+        case 3: return launch_up_split2<NKS, 3>(p, fvals, nTN);
+        default: return launch_up_split2<NKS, 4>(p, fvals, nTN);
+    }
+}
+
+static int launch_up_split(Plan& p, const double* fvals) {
+    int H = (p.npw + 1) / 2;
+    int nTM = (H + 15) / 16, nTN = (H + 15) / 16;
+    switch ((p.n + 3) / 4) {
+        case 1: return launch_up_split1<1>(p, fvals, nTM, nTN);
+        case 2: return launch_up_split1<2>(p, fvals, nTM, nTN);
+        case 3: return launch_up_split1<3>(p, fvals, nTM, nTN);
+        default: return launch_up_split1<4>(p, fvals, nTM, nTN);
+    }
+}
+
 int launch_upward(Plan& p, const double* fvals) {
     if (p.nchunks == 0) return BFGT_OK;
+    if (use_mfma(p) && p.precision == BFGT_PREC_SPLIT32) return launch_up_split(p, fvals);
     if (use_mfma(p)) {
         UpArgs u{LeafIn{fvals, nullptr, nullptr}, p.d_tup_f, 16 * p.pstride, 0, 1};
         return launch_up_mfma(p, u);
@@ -727,8 +968,23 @@ static int launch_down_mfma(Plan& p, const cplx* tab, int lstride, int ymid_half
     return BFGT_OK;
 }
 
+template <int NTAB>
+static int launch_down_split(Plan& p, double* pot, double* grad, double* hess) {
+    int blocks = (p.npwleaf + kWavesPerBlock - 1) / kWavesPerBlock;
+    k_down_split<NTAB><<<blocks, kWave * kWavesPerBlock, 0, p.stream>>>(
+        p.n, p.npw, p.pstride, p.nd, p.npwleaf, p.d_pwleaf, p.d_tdn_f, 3 * 16 * p.pstride, p.d_phase, p.d_levbase,
+        reinterpret_cast<const Fold*>(p.d_loc), pot, grad, hess);
+    BFGT_HIP_CHECK(hipGetLastError());
+    return BFGT_OK;
+}
+
 int launch_downward(Plan& p, double* pot, double* grad, double* hess) {
     if (p.npwleaf == 0) return BFGT_OK;
+    if (use_mfma(p) && p.precision == BFGT_PREC_SPLIT32) {
+        if (p.ifpgh == 1) return launch_down_split<1>(p, pot, grad, hess);
+        if (p.ifpgh == 2) return launch_down_split<2>(p, pot, grad, hess);
+        return launch_down_split<3>(p, pot, grad, hess);
+    }
     if (use_mfma(p)) {
         const int ls = 3 * 16 * p.pstride;
         if (p.ifpgh == 1) return launch_down_mfma<1>(p, p.d_tdn_f, ls, 1, pot, grad, hess);
