@@ -48,8 +48,25 @@ def algorithmic_work(plan, w):
     down = nd * L * (8 * ntab * n * P * H + 4 * c * n * n * H)
     near = nd * info["n_near_pairs"] * {1: 4, 2: 10, 3: 18}[w.ifpgh] * n**3
     merge_bytes = 16 * nd * nexp * (info["n_chunks"] + info["n_ancestors"])
-    return {"leaf_to_pw": ("flop", up), "merge": ("byte", merge_bytes), "translate": ("byte", None),
+    # translation: every cutoff box reads its own and its PW-list partners' mp once (list
+    # lengths from the plan) and writes its local expansion
+    translate_bytes = 16 * nd * nexp * (info["n_ancestors"] * 2 + info.get("n_m2l", 0))
+    return {"leaf_to_pw": ("flop", up), "merge": ("byte", merge_bytes), "translate": ("byte", translate_bytes),
             "pw_to_values": ("flop", down), "near_field": ("flop", near)}
+
+
+def measured_traffic(workload, precision, kernel):
+    """HBM bytes per launch of `kernel` recorded from rocprofv3 counters in
+    profiles/boxfgt_traffic.txt for this workload and precision, else None."""
+    path = os.path.join(ROOT, "profiles", "boxfgt_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        f = line.split()
+        if len(f) == 5 and not line.startswith("#") and f[:3] == [workload, precision, kernel]:
+            return {"bytes": int(f[3]) + int(f[4]), "fetch_bytes": int(f[3]), "write_bytes": int(f[4]),
+                    "source": "profiles/boxfgt_traffic.txt (rocprofv3 FETCH_SIZE/WRITE_SIZE, calibrated)"}
+    return None
 
 
 def cpu_baseline(w, gpu_out, max_threads, extra=None):
@@ -143,9 +160,11 @@ def main():
           f"{t_workload:.1f}s", file=sys.stderr, flush=True)
     dev = torch.device("cuda:0")
     if args.copy_probe_mib:
+        # a plain streaming kernel of known traffic (reads and writes copy_probe_mib MiB each):
+        # calibrates the FETCH_SIZE / WRITE_SIZE counters of a profiled run
         a = torch.empty(args.copy_probe_mib * (1 << 20) // 8, dtype=torch.float64, device=dev).fill_(1.0)
         b = torch.empty_like(a)
-        b.copy_(a)
+        torch.mul(a, 2.0, out=b)
         torch.cuda.synchronize()
         del a, b
     tree = w.tree.copy()
@@ -195,11 +214,27 @@ def main():
     else:
         roof = {"bound": "hbm", "achieved": amount / dur / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s"}
     roof["frac"] = roof["achieved"] / roof["peak"]
-    roof["traffic"] = None
-    roof["kernel"] = {"leaf_to_pw": "k_up", "merge": "k_reduce", "translate": "k_m2l", "pw_to_values": "k_down",
-                      "near_field": "k_near"}[dom]
+    kern = {"leaf_to_pw": "k_up", "merge": "k_reduce", "translate": "k_m2l", "pw_to_values": "k_down",
+            "near_field": "k_near"}
+    roof["kernel"] = kern[dom]
+    tr = measured_traffic(f"{args.config}/{args.scale}", args.precision, kern[dom])
+    roof["traffic"] = tr["bytes"] if tr else None
+    if tr:
+        roof["traffic_detail"] = tr
     roof["algorithmic_per_launch"] = amount
     roof["avg_launch_ms"] = stage_ms[dom]
+    # every stage against its own bound
+    stages_roof = {}
+    for k in names:
+        if k not in work or not work[k][1] or stage_ms[k] <= 0:
+            continue
+        knd, amt = work[k]
+        t = stage_ms[k] * 1e-3
+        if knd == "flop":
+            stages_roof[k] = {"bound": "mfma", "achieved_tflops": amt / t / 1e12,
+                              "frac": amt / t / 1e12 / FP64_PEAK_TFLOPS}
+        else:
+            stages_roof[k] = {"bound": "hbm", "achieved_gbs": amt / t / 1e9, "frac": amt / t / 1e9 / HBM_PEAK_GBS}
 
     # the other arithmetic on the same tree (split-FP32 leaf transforms vs all-FP64)
     other = None
@@ -299,6 +334,7 @@ def main():
         "cold_pts_per_s": npts / t_cold,
         "t_plan_s": t_plan,
         "roofline": roof,
+        "stage_roofline": stages_roof,
         "adjoint": adj,
     }
     if other is not None:

@@ -799,7 +799,8 @@ int bfgt_plan_query(bfgt_plan_t plan, int* iv, double* dv) {
         std::memcpy(iv, v, sizeof(v));
     }
     if (dv) {
-        double v[8] = {p.delta0, p.pmax, p.t_plan_host, p.t_tables, bfgt::use_mfma(p) ? 1.0 : 0.0, 0, 0, 0};
+        double v[8] = {p.delta0, p.pmax, p.t_plan_host, p.t_tables, bfgt::use_mfma(p) ? 1.0 : 0.0, (double)p.nm2l,
+                       (double)p.nnearT_s, 0};
         std::memcpy(dv, v, sizeof(v));
     }
     return BFGT_OK;
