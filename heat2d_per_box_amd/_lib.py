@@ -33,6 +33,7 @@ EXPORTS = [
     "bfgt_plan_create",
     "bfgt_plan_execute",
     "bfgt_plan_adjoint",
+    "bfgt_plan_eval_targets",
     "bfgt_plan_destroy",
     "bfgt_plan_query",
     "bfgt_plan_get_table",
@@ -74,6 +75,8 @@ def lib() -> ctypes.CDLL:
                                    _i, _i, _vp]
     L.bfgt_plan_execute.restype = _i
     L.bfgt_plan_execute.argtypes = [_vp, _vp, _vp, _vp, _vp, _dp]
+    L.bfgt_plan_eval_targets.restype = _i
+    L.bfgt_plan_eval_targets.argtypes = [_vp, _i, _vp, _i, _vp, _vp, _vp, _vp, _vp, _vp]
     L.bfgt_plan_adjoint.restype = _i
     L.bfgt_plan_adjoint.argtypes = [_vp, _vp, _vp, _vp, _vp, _dp]
     L.bfgt_plan_destroy.restype = _i

@@ -71,6 +71,17 @@ def boxfgt(nd, ndim, delta, eps, ipoly, iperiod, norder, npbox, nboxes, nlevels,
         raise ValueError("grad must hold (nd, 2, npbox, nboxes)")
     if ifpgh >= 3 and (hess is None or hess.size < 3 * nv):
         raise ValueError("hess must hold (nd, 3, npbox, nboxes)")
+    if ifpghtarg > 0 and ntarg > 0:
+        targs = _f64(targs, "targs")
+        pote = _f64(pote, "pote")
+        grade = _f64(grade, "grade")
+        hesse = _f64(hesse, "hesse")
+        if targs is None or targs.size < 2 * ntarg or pote is None or pote.size < ntarg * nd:
+            raise ValueError("targs must hold (2, ntarg) and pote (nd, ntarg)")
+        if ifpghtarg >= 2 and (grade is None or grade.size < 2 * ntarg * nd):
+            raise ValueError("grade must hold (nd, 2, ntarg)")
+        if ifpghtarg >= 3 and (hesse is None or hesse.size < 3 * ntarg * nd):
+            raise ValueError("hesse must hold (nd, 3, ntarg)")
     ti = np.zeros(20) if timeinfo is None else _f64(timeinfo, "timeinfo")
     rc = lib().bfgt_hip_eval(
         nd, ndim, delta, eps, ipoly, iperiod, norder, npbox, nboxes, nlevels, ltree, _ptr(itree, _ip),
@@ -143,6 +154,20 @@ class BoxFGT:
         ti = np.zeros(20) if timeinfo else None
         check(lib().bfgt_plan_execute(self._h, f, p, g, h, _ptr(ti) if ti is not None else None))
         return ti
+
+    def eval_targets(self, targs, ifpghtarg: int, pot, pote, grad=None, hess=None, grade=None, hesse=None):
+        """Interpolate the leaf arrays at targets (ntarg, 2) into pote (ntarg, nd),
+        grade (ntarg, 2, nd), hesse (ntarg, 3, nd): device tensors, overwritten."""
+        nb, npb, nd = self.nboxes, self.npbox, self.nd
+        nt = int(targs.shape[0])
+        tg = self._check_tensor(targs, (nt, 2), "targs")
+        p = self._check_tensor(pot, (nb, npb, nd), "pot")
+        g = self._check_tensor(grad, (nb, npb, 2, nd), "grad")
+        h = self._check_tensor(hess, (nb, npb, 3, nd), "hess")
+        pe = self._check_tensor(pote, (nt, nd), "pote")
+        ge = self._check_tensor(grade, (nt, 2, nd), "grade")
+        he = self._check_tensor(hesse, (nt, 3, nd), "hesse")
+        check(lib().bfgt_plan_eval_targets(self._h, nt, tg, int(ifpghtarg), p, g, h, pe, ge, he))
 
     def adjoint(self, gpot, fbar, ggrad=None, ghess=None, timeinfo: bool = False):
         """Transposed operator: ``fbar += A^T (gpot, ggrad, ghess)`` on device

@@ -58,6 +58,15 @@ struct NearPairT {
     int iy;
 };
 
+// Tree node for locating targets: centre, 0-based children in the
+// reference's order (mm, pm, mp, pp; -1 for a leaf) and level.
+struct TreeNode {
+    double cx, cy;
+    int child[4];
+    int level;
+    int pad;
+};
+
 struct Plan {
     // parameters
     int nd = 1, ipoly = 0, iperiod_in = 1, iperiod_eff = 1, n = 8, npbox = 64, ifpgh = 1;
@@ -111,6 +120,7 @@ struct Plan {
     int* d_near_lev = nullptr;     // target level
     int* d_near_ptr = nullptr;     // CSR [nnear_t+1]
     NearPair* d_near = nullptr;
+    TreeNode* d_tnodes = nullptr;  // [nboxes], for extra targets
 
     // zero-padded PW tables of the matrix-core kernels (16 point rows or
     // columns, pstride frequencies), so that every load in their inner loops
@@ -162,6 +172,10 @@ int launch_reduce(Plan& p);
 int launch_m2l(Plan& p);
 int launch_downward(Plan& p, double* pot, double* grad, double* hess);
 int launch_near(Plan& p, const double* fvals, double* pot, double* grad, double* hess);
+
+// extra targets (bfgt_targets.hip)
+int launch_targets(Plan& p, int ntarg, const double* targs, int ifpghtarg, const double* pot, const double* grad,
+                   const double* hess, double* pote, double* grade, double* hesse);
 
 // adjoint (bfgt_pw.hip, bfgt_near.hip)
 bool use_mfma(const Plan& p);

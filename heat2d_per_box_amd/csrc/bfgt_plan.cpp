@@ -607,6 +607,18 @@ int plan_create(Plan& p, int nd, int ndim, double delta, double eps, int ipoly, 
     if ((rc = upload(p, p.d_near_lev, nlev))) return rc;
     if ((rc = upload(p, p.d_near_ptr, nptr))) return rc;
     if ((rc = upload(p, p.d_near, pairs))) return rc;
+    {
+        std::vector<TreeNode> tn(t.nboxes);
+        for (int b = 0; b < t.nboxes; b++) {
+            TreeNode& q = tn[b];
+            q.cx = centers[2 * b];
+            q.cy = centers[2 * b + 1];
+            for (int c = 0; c < 4; c++) q.child[c] = t.nchild[b] > 0 ? t.ichild[4 * b + c] - 1 : -1;
+            q.level = t.ilevel[b];
+            q.pad = 0;
+        }
+        if ((rc = upload(p, p.d_tnodes, tn))) return rc;
+    }
     if ((rc = upload(p, p.d_m2lT_ptr, m2lT_ptr))) return rc;
     if ((rc = upload(p, p.d_m2lT, m2lT))) return rc;
     if ((rc = upload(p, p.d_nearT_src, nsrc))) return rc;
@@ -661,6 +673,17 @@ int plan_execute(Plan& p, const double* fvals, double* pot, double* grad, double
         timeinfo[6] = ms[4] * 1e-3;  // near field
     }
     return BFGT_OK;
+}
+
+int plan_targets(Plan& p, int ntarg, const double* targs, int ifpghtarg, const double* pot, const double* grad,
+                 const double* hess, double* pote, double* grade, double* hesse) {
+    if (ntarg < 0) return set_error(BFGT_ERR_ARG, "ntarg must be >= 0");
+    if (ifpghtarg < 1 || ifpghtarg > 3) return set_error(BFGT_ERR_TARGETS, "ifpghtarg must be 1, 2 or 3");
+    if (ntarg == 0) return BFGT_OK;
+    if (!targs || !pot || !pote || (ifpghtarg >= 2 && (!grad || !grade)) || (ifpghtarg >= 3 && (!hess || !hesse)))
+        return set_error(BFGT_ERR_ARG, "null device array");
+    BFGT_HIP_CHECK(hipSetDevice(p.device));
+    return launch_targets(p, ntarg, targs, ifpghtarg, pot, grad, hess, pote, grade, hesse);
 }
 
 int plan_adjoint(Plan& p, const double* gpot, const double* ggrad, const double* ghess, double* fbar,
@@ -735,6 +758,13 @@ int bfgt_plan_execute(bfgt_plan_t plan, const double* d_fvals, double* d_pot, do
                       double* timeinfo) {
     if (!plan) return set_error(BFGT_ERR_ARG, "null plan");
     return bfgt::plan_execute(plan->p, d_fvals, d_pot, d_grad, d_hess, timeinfo);
+}
+
+int bfgt_plan_eval_targets(bfgt_plan_t plan, int ntarg, const double* d_targs, int ifpghtarg, const double* d_pot,
+                           const double* d_grad, const double* d_hess, double* d_pote, double* d_grade,
+                           double* d_hesse) {
+    if (!plan) return set_error(BFGT_ERR_ARG, "null plan");
+    return bfgt::plan_targets(plan->p, ntarg, d_targs, ifpghtarg, d_pot, d_grad, d_hess, d_pote, d_grade, d_hesse);
 }
 
 int bfgt_plan_adjoint(bfgt_plan_t plan, const double* d_gpot, const double* d_ggrad, const double* d_ghess,
@@ -866,29 +896,31 @@ int bfgt_hip_eval(int nd, int ndim, double delta, double eps, int ipoly, int ipe
                   const double* boxsize, const double* fvals, int ifpgh, double* pot, double* grad, double* hess,
                   int ifnewtree, int ntarg, const double* targs, int ifpghtarg, double* pote, double* grade,
                   double* hesse, double* timeinfo, int precision) {
-    (void)ntarg;
-    (void)targs;
-    (void)pote;
-    (void)grade;
-    (void)hesse;
     auto t0 = std::chrono::steady_clock::now();
     if (ifnewtree != 0) return set_error(BFGT_ERR_IFNEWTREE, "ifnewtree=1 is not supported");
-    if (ifpghtarg > 0) return set_error(BFGT_ERR_TARGETS, "extra targets (ifpghtarg>0) are not supported");
+    if (ifpghtarg < 0 || ifpghtarg > 3) return set_error(BFGT_ERR_TARGETS, "ifpghtarg must be 0..3");
+    if (ifpghtarg > 0 && ntarg < 0) return set_error(BFGT_ERR_ARG, "ntarg must be >= 0");
+    const bool with_targets = ifpghtarg > 0 && ntarg > 0;
     if (npbox != norder * norder) return set_error(BFGT_ERR_NORDER, "npbox must equal norder**2");
     if (ifpgh < 1 || ifpgh > 3) return set_error(BFGT_ERR_IFPGH, "ifpgh must be 1, 2 or 3");
     if (!fvals || !pot || (ifpgh >= 2 && !grad) || (ifpgh >= 3 && !hess))
         return set_error(BFGT_ERR_ARG, "null output array");
+    if (with_targets && (!targs || !pote || (ifpghtarg >= 2 && !grade) || (ifpghtarg >= 3 && !hesse)))
+        return set_error(BFGT_ERR_ARG, "null target array");
     if (ndim != 2) return set_error(BFGT_ERR_NDIM, "only ndim=2 is supported");
     if (ipoly != 0 && ipoly != 1) return set_error(BFGT_ERR_IPOLY, "ipoly must be 0 or 1");
     if (norder < 1 || norder > bfgt::kMaxOrder) return set_error(BFGT_ERR_NORDER, "norder must be in 1..32");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return set_error(BFGT_ERR_NODEVICE, "no GPU visible");
+    // with targets asking for more channels than ifpgh, the transform computes
+    // them into zeroed scratch arrays the caller never sees (bfgt/boxfgt.f:207-277)
+    const int ifpgh2 = with_targets ? std::max(ifpgh, ifpghtarg) : ifpgh;
     hipStream_t stream = nullptr;
     if (hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess)
         return set_error(BFGT_ERR_HIP, "hipStreamCreateWithFlags failed");
     bfgt_plan_t plan = nullptr;
     int rc = bfgt_plan_create(&plan, nd, ndim, delta, eps, ipoly, iperiod, norder, nboxes, nlevels, ltree, itree,
-                              iptr, centers, boxsize, ifpgh, precision, stream);
+                              iptr, centers, boxsize, ifpgh2, precision, stream);
     if (rc) {
         hipStreamDestroy(stream);
         return rc;
@@ -896,12 +928,18 @@ int bfgt_hip_eval(int nd, int ndim, double delta, double eps, int ipoly, int ipe
     plan->p.own_stream = true;  // released with the plan
     bfgt::Plan& p = plan->p;
     size_t nv = (size_t)nboxes * npbox * nd;
+    size_t nt = with_targets ? (size_t)ntarg : 0;
     double *dfv = nullptr, *dpot = nullptr, *dgrad = nullptr, *dhess = nullptr;
+    double *dtg = nullptr, *dpe = nullptr, *dge = nullptr, *dhe = nullptr;
     auto fail = [&](int code) {
         hipFree(dfv);
         hipFree(dpot);
         hipFree(dgrad);
         hipFree(dhess);
+        hipFree(dtg);
+        hipFree(dpe);
+        hipFree(dge);
+        hipFree(dhe);
         bfgt_plan_destroy(plan);
         return code;
     };
@@ -914,18 +952,48 @@ int bfgt_hip_eval(int nd, int ndim, double delta, double eps, int ipoly, int ipe
     BFGT_EVAL_CHECK(hipMalloc(&dpot, nv * sizeof(double)));
     BFGT_EVAL_CHECK(hipMemcpyAsync(dfv, fvals, nv * sizeof(double), hipMemcpyHostToDevice, p.stream));
     BFGT_EVAL_CHECK(hipMemcpyAsync(dpot, pot, nv * sizeof(double), hipMemcpyHostToDevice, p.stream));
-    if (ifpgh >= 2) {
+    if (ifpgh2 >= 2) {
         BFGT_EVAL_CHECK(hipMalloc(&dgrad, 2 * nv * sizeof(double)));
-        BFGT_EVAL_CHECK(hipMemcpyAsync(dgrad, grad, 2 * nv * sizeof(double), hipMemcpyHostToDevice, p.stream));
+        if (ifpgh >= 2)
+            BFGT_EVAL_CHECK(hipMemcpyAsync(dgrad, grad, 2 * nv * sizeof(double), hipMemcpyHostToDevice, p.stream));
+        else
+            BFGT_EVAL_CHECK(hipMemsetAsync(dgrad, 0, 2 * nv * sizeof(double), p.stream));
     }
-    if (ifpgh >= 3) {
+    if (ifpgh2 >= 3) {
         BFGT_EVAL_CHECK(hipMalloc(&dhess, 3 * nv * sizeof(double)));
-        BFGT_EVAL_CHECK(hipMemcpyAsync(dhess, hess, 3 * nv * sizeof(double), hipMemcpyHostToDevice, p.stream));
+        if (ifpgh >= 3)
+            BFGT_EVAL_CHECK(hipMemcpyAsync(dhess, hess, 3 * nv * sizeof(double), hipMemcpyHostToDevice, p.stream));
+        else
+            BFGT_EVAL_CHECK(hipMemsetAsync(dhess, 0, 3 * nv * sizeof(double), p.stream));
+    }
+    if (with_targets) {
+        BFGT_EVAL_CHECK(hipMalloc(&dtg, 2 * nt * sizeof(double)));
+        BFGT_EVAL_CHECK(hipMalloc(&dpe, nt * nd * sizeof(double)));
+        if (ifpghtarg >= 2) BFGT_EVAL_CHECK(hipMalloc(&dge, 2 * nt * nd * sizeof(double)));
+        if (ifpghtarg >= 3) BFGT_EVAL_CHECK(hipMalloc(&dhe, 3 * nt * nd * sizeof(double)));
+        BFGT_EVAL_CHECK(hipMemcpyAsync(dtg, targs, 2 * nt * sizeof(double), hipMemcpyHostToDevice, p.stream));
     }
     auto t1 = std::chrono::steady_clock::now();
     double ti[20];
     rc = bfgt::plan_execute(p, dfv, dpot, dgrad, dhess, ti);
     if (rc) return fail(rc);
+    double t_targets = 0;
+    if (with_targets) {
+        hipEvent_t e0 = p.ev[6], e1 = p.ev[7];
+        BFGT_EVAL_CHECK(hipEventRecord(e0, p.stream));
+        rc = bfgt::plan_targets(p, ntarg, dtg, ifpghtarg, dpot, dgrad, dhess, dpe, dge, dhe);
+        if (rc) return fail(rc);
+        BFGT_EVAL_CHECK(hipEventRecord(e1, p.stream));
+        BFGT_EVAL_CHECK(hipMemcpyAsync(pote, dpe, nt * nd * sizeof(double), hipMemcpyDeviceToHost, p.stream));
+        if (ifpghtarg >= 2)
+            BFGT_EVAL_CHECK(hipMemcpyAsync(grade, dge, 2 * nt * nd * sizeof(double), hipMemcpyDeviceToHost, p.stream));
+        if (ifpghtarg >= 3)
+            BFGT_EVAL_CHECK(hipMemcpyAsync(hesse, dhe, 3 * nt * nd * sizeof(double), hipMemcpyDeviceToHost, p.stream));
+        BFGT_EVAL_CHECK(hipEventSynchronize(e1));
+        float ms = 0;
+        BFGT_EVAL_CHECK(hipEventElapsedTime(&ms, e0, e1));
+        t_targets = ms * 1e-3;
+    }
     BFGT_EVAL_CHECK(hipMemcpyAsync(pot, dpot, nv * sizeof(double), hipMemcpyDeviceToHost, p.stream));
     if (ifpgh >= 2)
         BFGT_EVAL_CHECK(hipMemcpyAsync(grad, dgrad, 2 * nv * sizeof(double), hipMemcpyDeviceToHost, p.stream));
@@ -936,6 +1004,7 @@ int bfgt_hip_eval(int nd, int ndim, double delta, double eps, int ipoly, int ipe
     if (timeinfo) {
         for (int i = 0; i < 20; i++) timeinfo[i] = ti[i];
         timeinfo[0] = std::chrono::duration<double>(t1 - t0).count();
+        timeinfo[11] = t_targets;  // extra targets (slot 12 of the reference)
     }
     return fail(BFGT_OK);
 }

@@ -26,7 +26,7 @@ extern "C" {
 #define BFGT_ERR_IPOLY 2     /* ipoly not 0 (Legendre) or 1 (Chebyshev) */
 #define BFGT_ERR_IFPGH 3     /* ifpgh not in 1..3 */
 #define BFGT_ERR_IFNEWTREE 4 /* ifnewtree=1 (tree refinement inside the FGT) is not supported */
-#define BFGT_ERR_TARGETS 5   /* ifpghtarg > 0 (extra targets) is not supported */
+#define BFGT_ERR_TARGETS 5   /* ifpghtarg outside 0..3 */
 #define BFGT_ERR_NORDER 6    /* norder < 1, norder > 32, or npbox != norder**2 */
 #define BFGT_ERR_NPW 7       /* plane-wave length above 200 (the reference's ws/ts(200)) */
 #define BFGT_ERR_TREE 8      /* inconsistent tree arrays */
@@ -48,9 +48,13 @@ extern "C" {
  * Side effects as in the reference: the colleague lists in itree are
  * recomputed (bfgt/boxfgt.f:521-528); timeinfo(1..20) is zeroed and filled
  * (slot 1 set-up, 2 leaf->PW, 3 merge, 4 translation, 5 split (fused into 6),
- * 6 PW->pot/grad/hess, 7 near field).  ifnewtree must be 0 and ifpghtarg
- * 0; pote/grade/hesse/targs are not touched.  `precision` selects
- * BFGT_PREC_FP64 or BFGT_PREC_SPLIT32.
+ * 6 PW->pot/grad/hess, 7 near field, 12 extra targets).  ifnewtree must be
+ * 0.  With ifpghtarg = 1..3 and ntarg > 0 the leaf pot (and grad, hess) are
+ * interpolated at targs(2,ntarg) into pote(nd,ntarg), grade(nd,2,ntarg),
+ * hesse(nd,3,ntarg), which are overwritten (bfgtmain STEP 11,
+ * bfgt/boxfgt.f:1494-1563); channels above ifpgh are computed into scratch
+ * arrays and never written to grad/hess (bfgt/boxfgt.f:207-277).
+ * `precision` selects BFGT_PREC_FP64 or BFGT_PREC_SPLIT32.
  */
 int bfgt_hip_eval(int nd, int ndim, double delta, double eps, int ipoly, int iperiod, int norder, int npbox,
                   int nboxes, int nlevels, int ltree, int *itree, const int *iptr, const double *centers,
@@ -78,6 +82,19 @@ int bfgt_plan_create(bfgt_plan_t *plan, int nd, int ndim, double delta, double e
  * non-NULL, is filled after a stream synchronisation. */
 int bfgt_plan_execute(bfgt_plan_t plan, const double *d_fvals, double *d_pot, double *d_grad, double *d_hess,
                       double *timeinfo);
+
+/* Extra targets on device arrays: interpolate the leaf arrays pot (and grad,
+ * hess for ifpghtarg >= 2, 3) -- normally the outputs of bfgt_plan_execute --
+ * at d_targs(2,ntarg) into d_pote(nd,ntarg), d_grade(nd,2,ntarg),
+ * d_hesse(nd,3,ntarg) (overwritten).  Each target is located in its leaf by
+ * descending the tree with ties to the plus side and evaluated with the
+ * leaf's tensor-polynomial interpolant.  Replaces the extra-target step of
+ * bfgtmain (bfgt/boxfgt.f:1494-1563: treedata_trans_nd +
+ * treedata_evalt_nd, common/tree_data_routs_nd.f:16-90, 710-806).
+ * Asynchronous on the plan's stream; allocates nothing. */
+int bfgt_plan_eval_targets(bfgt_plan_t plan, int ntarg, const double *d_targs, int ifpghtarg, const double *d_pot,
+                           const double *d_grad, const double *d_hess, double *d_pote, double *d_grade,
+                           double *d_hesse);
 
 /* Adjoint (transposed operator) of bfgt_plan_execute, on device arrays,
  * asynchronous on the plan's stream; allocates nothing.  No counterpart in

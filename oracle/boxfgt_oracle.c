@@ -1235,3 +1235,97 @@ int oracle_boxfgt(int nd, double delta, double eps, int ipoly, int iperiod, int 
     free(tab); free(tabx); free(tabxx); free(ind);
     return 0;
 }
+
+/* ------------------------------------------------------------------ */
+/* Extra targets: bfgtmain STEP 11 with iffast=0 (bfgt/boxfgt.f:1494-1563) */
+/* ------------------------------------------------------------------ */
+
+/* Leaf of the point (x, y): descend from the root choosing the child on the
+ * plus side of the centre when the offset is >= 0 (sort_pts_to_children and
+ * find_childbox_ind, common/pts_tree_nd.f:650-741, driven level by level by
+ * pts_tree_sort :1226-1253).  Child order mm, pm, mp, pp. */
+static int locate_leaf(const tree_t *T, double x, double y) {
+    int b = 0;
+    while (T->nchild[b] > 0) {
+        int k = (x - T->centers[2 * b] >= 0 ? 1 : 0) + (y - T->centers[2 * b + 1] >= 0 ? 2 : 0);
+        b = T->ichild[4 * b + k] - 1;
+    }
+    return b;
+}
+
+/* One channel set (nf functions per point) of one leaf: values -> tensor
+ * coefficients (ortho_trans_2d itype=0, common/tensor_prod_routs_nd.f:103-180:
+ * x transform, then y transform), then the expansion at the scaled target
+ * (ortho_evalt_2d: sum over x degree first, then y). */
+static void eval_leaf(int nf, int n, int ipoly, const double *umat, const double *vals, double xs, double ys,
+                      double *out) {
+    double px[MAXN], py[MAXN];
+    if (ipoly == 0) {
+        oracle_legepols(xs, n - 1, px);
+        oracle_legepols(ys, n - 1, py);
+    } else {
+        oracle_chebpols(xs, n - 1, px);
+        oracle_chebpols(ys, n - 1, py);
+    }
+    double *work = (double *)malloc(sizeof(double) * nf * n * n);
+    double *coef = (double *)malloc(sizeof(double) * nf * n * n);
+    for (int j = 0; j < n; j++)
+        for (int k = 0; k < n; k++)
+            for (int f = 0; f < nf; f++) {
+                double dd = 0;
+                for (int k1 = 0; k1 < n; k1++) dd += umat[k + n * k1] * vals[f + nf * (k1 + n * j)];
+                work[f + nf * (k + n * j)] = dd;
+            }
+    for (int i = 0; i < n; i++)
+        for (int k = 0; k < n; k++)
+            for (int f = 0; f < nf; f++) {
+                double dd = 0;
+                for (int i1 = 0; i1 < n; i1++) dd += umat[i + n * i1] * work[f + nf * (k + n * i1)];
+                coef[f + nf * (k + n * i)] = dd;
+            }
+    for (int f = 0; f < nf; f++) {
+        double pot = 0;
+        for (int j = 0; j < n; j++) {
+            double pp = 0;
+            for (int k = 0; k < n; k++) pp += coef[f + nf * (k + n * j)] * px[k];
+            pot += pp * py[j];
+        }
+        out[f] = pot;
+    }
+    free(work);
+    free(coef);
+}
+
+/*
+ * oracle_eval_targets: the leaf arrays pot (nd,npbox,*), grad (nd,2,npbox,*),
+ * hess (nd,3,npbox,*) interpolated at ntarg targets (2,ntarg) by their
+ * leaves' tensor polynomial expansions (treedata_trans_nd + treedata_evalt_nd,
+ * common/tree_data_routs_nd.f:16-90, 710-806); pote (nd,ntarg),
+ * grade (nd,2,ntarg), hesse (nd,3,ntarg) are overwritten for the channels
+ * ifpghtarg asks for.
+ */
+int oracle_eval_targets(int nd, int ipoly, int norder, int nlevels, int *itree, const int *iptr,
+                        const double *centers, const double *boxsize, int ntarg, const double *targs,
+                        int ifpghtarg, const double *pot, const double *grad, const double *hess, double *pote,
+                        double *grade, double *hesse) {
+    if (norder < 1 || norder > MAXN) return 3;
+    if (ifpghtarg < 1 || ifpghtarg > 3) return 5;
+    int n = norder, npbox = n * n;
+    tree_t T;
+    tree_view(&T, nlevels, itree, iptr, centers, boxsize);
+    double xq[MAXN], wq[MAXN], umat[MAXN * MAXN];
+    if (ipoly == 0) oracle_legeexps(n, xq, wq, umat);
+    else oracle_chebexps(n, xq, wq, umat);
+    for (int it = 0; it < ntarg; it++) {
+        double x = targs[2 * it], y = targs[2 * it + 1];
+        int b = locate_leaf(&T, x, y);
+        double sc = 2.0 / boxsize[T.ilevel[b]];
+        double xs = (x - centers[2 * b]) * sc, ys = (y - centers[2 * b + 1]) * sc;
+        eval_leaf(nd, n, ipoly, umat, pot + (size_t)b * npbox * nd, xs, ys, pote + (size_t)it * nd);
+        if (ifpghtarg >= 2)
+            eval_leaf(2 * nd, n, ipoly, umat, grad + (size_t)b * npbox * 2 * nd, xs, ys, grade + (size_t)it * 2 * nd);
+        if (ifpghtarg >= 3)
+            eval_leaf(3 * nd, n, ipoly, umat, hess + (size_t)b * npbox * 3 * nd, xs, ys, hesse + (size_t)it * 3 * nd);
+    }
+    return 0;
+}

@@ -57,6 +57,11 @@ def load_oracle() -> ctypes.CDLL:
             ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int,
             ctypes.c_int, ctypes.c_int, ctypes.c_int, _ip, _ip, _dp, _dp, _dp, ctypes.c_int, _dp, _dp, _dp, _dp, _ip,
         ]
+        lib.oracle_eval_targets.restype = ctypes.c_int
+        lib.oracle_eval_targets.argtypes = [
+            ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _ip, _ip, _dp, _dp, ctypes.c_int, _dp,
+            ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp,
+        ]
         for name in ("oracle_legeexps", "oracle_chebexps"):
             f = getattr(lib, name)
             f.restype = None
@@ -101,14 +106,55 @@ class FGTResult:
     timeinfo: np.ndarray
     itree: np.ndarray
     info: Optional[dict] = None
+    pote: Optional[np.ndarray] = None   # (ntarg, nd)
+    grade: Optional[np.ndarray] = None  # (ntarg, 2, nd)
+    hesse: Optional[np.ndarray] = None  # (ntarg, 3, nd)
 
 
 def _alloc(nboxes, npbox, nd):
     return (np.zeros((nboxes, npbox, nd)), np.zeros((nboxes, npbox, 2, nd)), np.zeros((nboxes, npbox, 3, nd)))
 
 
-def oracle_boxfgt(tree, fvals, nd, delta, eps, ipoly, iperiod, norder, ifpgh) -> FGTResult:
-    """Run the C restatement on a copy of the tree arrays."""
+def oracle_eval_targets(tree, itree, nd, ipoly, norder, targs, ifpghtarg, pot, grad, hess):
+    """Leaf arrays interpolated at targets (ntarg, 2): (pote, grade, hesse)."""
+    lib = load_oracle()
+    iptr = np.ascontiguousarray(tree.iptr, dtype=np.int32)
+    centers = np.ascontiguousarray(tree.centers, dtype=np.float64)
+    boxsize = np.ascontiguousarray(tree.boxsize, dtype=np.float64)
+    tg = np.ascontiguousarray(targs, dtype=np.float64)
+    nt = tg.shape[0]
+    pote, grade, hesse = np.zeros((nt, nd)), np.zeros((nt, 2, nd)), np.zeros((nt, 3, nd))
+    g = np.ascontiguousarray(grad) if grad is not None else np.zeros(1)
+    h = np.ascontiguousarray(hess) if hess is not None else np.zeros(1)
+    rc = lib.oracle_eval_targets(nd, ipoly, norder, tree.nlevels, _p(itree, _ip), _p(iptr, _ip), _p(centers),
+                                 _p(boxsize), nt, _p(tg), ifpghtarg, _p(np.ascontiguousarray(pot)), _p(g), _p(h),
+                                 _p(pote), _p(grade), _p(hesse))
+    if rc != 0:
+        raise RuntimeError(f"oracle_eval_targets failed with status {rc}")
+    return pote, (grade if ifpghtarg >= 2 else None), (hesse if ifpghtarg >= 3 else None)
+
+
+def oracle_boxfgt(tree, fvals, nd, delta, eps, ipoly, iperiod, norder, ifpgh, targs=None, ifpghtarg=0,
+                  pot0=None) -> FGTResult:
+    """Run the C restatement on a copy of the tree arrays.
+
+    With targets, follows boxfgt's dispatch (bfgt/boxfgt.f:207-277): when
+    ifpghtarg > ifpgh the transform runs with ifpghtarg on zeroed scratch
+    arrays for the extra channels, which the caller's grad/hess never see;
+    the targets then interpolate pot (accumulated onto pot0) and the
+    gradient/Hessian arrays the transform wrote (bfgt/boxfgt.f:1494-1563)."""
+    if targs is not None and ifpghtarg > 0:
+        ifpgh2 = max(ifpgh, ifpghtarg)
+        r = oracle_boxfgt(tree, fvals, nd, delta, eps, ipoly, iperiod, norder, ifpgh2)
+        if pot0 is not None:
+            r.pot += pot0
+        r.pote, r.grade, r.hesse = oracle_eval_targets(tree, r.itree, nd, ipoly, norder, targs, ifpghtarg, r.pot,
+                                                       r.grad, r.hess)
+        if ifpgh < 2:
+            r.grad = np.zeros_like(r.grad)
+        if ifpgh < 3:
+            r.hess = np.zeros_like(r.hess)
+        return r
     lib = load_oracle()
     itree = np.ascontiguousarray(tree.itree, dtype=np.int32).copy()
     iptr = np.ascontiguousarray(tree.iptr, dtype=np.int32)
@@ -130,8 +176,10 @@ def oracle_boxfgt(tree, fvals, nd, delta, eps, ipoly, iperiod, norder, ifpgh) ->
     return FGTResult(pot, grad, hess, ti, itree, {k: int(v) for k, v in zip(keys, info)})
 
 
-def reference_boxfgt(tree, fvals, nd, delta, eps, ipoly, iperiod, norder, ifpgh, nthreads: Optional[int] = None) -> FGTResult:
-    """Run the reference Fortran boxfgt (ifnewtree=0, no extra targets) on a copy of the tree."""
+def reference_boxfgt(tree, fvals, nd, delta, eps, ipoly, iperiod, norder, ifpgh, nthreads: Optional[int] = None,
+                     targs=None, ifpghtarg=0, pot0=None) -> FGTResult:
+    """Run the reference Fortran boxfgt (ifnewtree=0) on a copy of the tree,
+    optionally with extra targets (ntarg, 2) and pot pre-loaded with pot0."""
     lib = load_reference()
     if nthreads is not None:
         try:
@@ -150,24 +198,35 @@ def reference_boxfgt(tree, fvals, nd, delta, eps, ipoly, iperiod, norder, ifpgh,
     fv = np.ascontiguousarray(fvals, dtype=np.float64)
     assert fv.shape == (nboxes, npbox, nd)
     pot, grad, hess = _alloc(nboxes, npbox, nd)
+    if pot0 is not None:
+        pot += pot0
     ti = np.zeros(20)
-    targs = np.zeros(2)
-    pote = np.zeros(max(nd, 1))
-    grade = np.zeros(2 * max(nd, 1))
-    hesse = np.zeros(3 * max(nd, 1))
+    if targs is not None and ifpghtarg > 0:
+        tg = np.ascontiguousarray(targs, dtype=np.float64).copy()
+        nt = tg.shape[0]
+    else:
+        tg, nt, ifpghtarg = np.zeros((1, 2)), 0, 0
+    pote = np.zeros((max(nt, 1), nd))
+    grade = np.zeros((max(nt, 1), 2, nd))
+    hesse = np.zeros((max(nt, 1), 3, nd))
     args = [
         c_int(nd), c_int(2), c_dbl(delta), c_dbl(eps), c_int(ipoly), c_int(iperiod), c_int(norder), c_int(npbox),
         c_int(nboxes), c_int(tree.nlevels), c_int(itree.shape[0]),
     ]
     refs = [ctypes.byref(a) for a in args]
     refs += [_p(itree, _ip), _p(iptr, _ip), _p(centers), _p(boxsize), _p(fv)]
-    ifpgh_c, ifnew, ntarg, ifpt = c_int(ifpgh), c_int(0), c_int(0), c_int(0)
-    refs += [ctypes.byref(ifpgh_c), _p(pot), _p(grad), _p(hess), ctypes.byref(ifnew), ctypes.byref(ntarg), _p(targs),
+    ifpgh_c, ifnew, ntarg, ifpt = c_int(ifpgh), c_int(0), c_int(nt), c_int(ifpghtarg)
+    refs += [ctypes.byref(ifpgh_c), _p(pot), _p(grad), _p(hess), ctypes.byref(ifnew), ctypes.byref(ntarg), _p(tg),
              ctypes.byref(ifpt), _p(pote), _p(grade), _p(hesse), _p(ti)]
     lib.boxfgt_(*refs)
     if args[5].value != iperiod:
         raise RuntimeError("reference did not restore iperiod")
-    return FGTResult(pot, grad, hess, ti, itree)
+    r = FGTResult(pot, grad, hess, ti, itree)
+    if nt > 0:
+        r.pote = pote[:nt]
+        r.grade = grade[:nt] if ifpghtarg >= 2 else None
+        r.hesse = hesse[:nt] if ifpghtarg >= 3 else None
+    return r
 
 
 # -- 1-D pieces, for pinning the oracle against closed forms ----------------

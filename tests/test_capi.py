@@ -55,7 +55,8 @@ def _eval(**over):
 
 @pytest.mark.parametrize("over,code", [
     ({"ifnewtree": 1}, 4),
-    ({"ifpghtarg": 1}, 5),
+    ({"ifpghtarg": 4}, 5),
+    ({"ifpghtarg": -1}, 5),
     ({"npbox": 15}, 6),
     ({"ndim": 3}, 1),
     ({"ipoly": 2}, 2),
