@@ -218,7 +218,7 @@ __device__ __host__ inline int round_up(int a, int b) { return (a + b - 1) / b *
 // the middle y frequency of an odd expansion (:591-595); the adjoint of the
 // forward transform is NTAB = 1 with the conjugated values-to-PW table and
 // w = 1, writing the leaf-value cotangent.
-template <int NTAB>
+template <int NTAB, int NTP>
 __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_down_mfma(
     int n, int P, int ps, int nd, int nleaf, const PwLeaf* __restrict__ leaves, const cplx* __restrict__ tab,
     int lstride, int ymid_half, const cplx* __restrict__ phase, const int* __restrict__ levbase,
@@ -242,39 +242,49 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_down_mfma(
         v4d out[NOUT];
 #pragma unroll
         for (int o = 0; o < NOUT; o++) out[o] = v4d{0, 0, 0, 0};
-        for (int tM = 0; tM < nT; tM++) {
-            const int m2 = 16 * tM + cc;
-            v4d FFr[NTAB], FFi[NTAB];
+        // NTP y-frequency tiles per pass share the x-table and phase operands
+        for (int tg = 0; tg < nT; tg += NTP) {
+            v4d FFr[NTP][NTAB], FFi[NTP][NTAB];
 #pragma unroll
-            for (int t = 0; t < NTAB; t++) FFr[t] = FFi[t] = v4d{0, 0, 0, 0};
+            for (int p = 0; p < NTP; p++)
+#pragma unroll
+                for (int t = 0; t < NTAB; t++) FFr[p][t] = FFi[p][t] = v4d{0, 0, 0, 0};
             for (int s = 0; s < nks1; s++) {
                 const int m = 4 * s + q;
-                const Fold sd = F[m * H16 + m2];
+                Fold sd[NTP];
+#pragma unroll
+                for (int p = 0; p < NTP; p++) sd[p] = F[m * H16 + 16 * (tg + p) + cc];
                 const cplx px = phx[m];
 #pragma unroll
                 for (int t = 0; t < NTAB; t++) {
                     cplx e = cmul(E[t * tstride + m * 16 + cc], px);
-                    FFr[t] = mfma(sd.S.x, e.x, FFr[t]);
-                    FFr[t] = mfma(-sd.D.y, e.y, FFr[t]);
-                    FFi[t] = mfma(sd.S.y, e.x, FFi[t]);
-                    FFi[t] = mfma(sd.D.x, e.y, FFi[t]);
+#pragma unroll
+                    for (int p = 0; p < NTP; p++) {
+                        FFr[p][t] = mfma(sd[p].S.x, e.x, FFr[p][t]);
+                        FFi[p][t] = mfma(sd[p].S.y, e.x, FFi[p][t]);
+                        FFr[p][t] = mfma(-sd[p].D.y, e.y, FFr[p][t]);
+                        FFi[p][t] = mfma(sd[p].D.x, e.y, FFi[p][t]);
+                    }
                 }
             }
-            // element s of FF holds FF^T[16 tM + 4s + q][cc]: the B operand of K-step s
+            // element s of FF[p] holds FF^T[16 (tg+p) + 4s + q][cc]: the B operand of K-step s
 #pragma unroll
-            for (int s = 0; s < 4; s++) {
-                const int m2s = 16 * tM + 4 * s + q;
-                cplx pyw = phy[m2s];
-                const double w = (ymid_half && m2s >= P2) ? 0.5 : 1.0;
-                pyw.x *= w;
-                pyw.y *= w;
-                cplx eu[NTAB];
+            for (int p = 0; p < NTP; p++) {
 #pragma unroll
-                for (int u = 0; u < NTAB; u++) eu[u] = cmul(E[u * tstride + m2s * 16 + cc], pyw);
+                for (int s = 0; s < 4; s++) {
+                    const int m2s = 16 * (tg + p) + 4 * s + q;
+                    if (16 * (tg + p) + 4 * s >= H) break;  // rows of this step are all padding (uniform)
+                    cplx pyw = phy[m2s];
+                    const double w = (ymid_half && m2s >= P2) ? 0.5 : 1.0;
+                    pyw.x *= w;
+                    pyw.y *= w;
+                    cplx eu[NTAB];
 #pragma unroll
-                for (int o = 0; o < NOUT; o++) {
-                    out[o] = mfma(eu[chan_y(o)].x, FFr[chan_x(o)][s], out[o]);
-                    out[o] = mfma(-eu[chan_y(o)].y, FFi[chan_x(o)][s], out[o]);
+                    for (int u = 0; u < NTAB; u++) eu[u] = cmul(E[u * tstride + m2s * 16 + cc], pyw);
+#pragma unroll
+                    for (int o = 0; o < NOUT; o++) out[o] = mfma(eu[chan_y(o)].x, FFr[p][chan_x(o)][s], out[o]);
+#pragma unroll
+                    for (int o = 0; o < NOUT; o++) out[o] = mfma(-eu[chan_y(o)].y, FFi[p][chan_x(o)][s], out[o]);
                 }
             }
         }
@@ -957,15 +967,24 @@ static int launch_down_vec(Plan& p, double* pot, double* grad, double* hess, siz
     return BFGT_OK;
 }
 
-template <int NTAB>
-static int launch_down_mfma(Plan& p, const cplx* tab, int lstride, int ymid_half, double* pot, double* grad,
-                            double* hess) {
+template <int NTAB, int NTP>
+static int launch_down_mfma2(Plan& p, const cplx* tab, int lstride, int ymid_half, double* pot, double* grad,
+                             double* hess) {
     int blocks = (p.npwleaf + kWavesPerBlock - 1) / kWavesPerBlock;
-    k_down_mfma<NTAB><<<blocks, kWave * kWavesPerBlock, 0, p.stream>>>(
+    k_down_mfma<NTAB, NTP><<<blocks, kWave * kWavesPerBlock, 0, p.stream>>>(
         p.n, p.npw, p.pstride, p.nd, p.npwleaf, p.d_pwleaf, tab, lstride, ymid_half, p.d_phase, p.d_levbase,
         reinterpret_cast<const Fold*>(p.d_loc), pot, grad, hess);
     BFGT_HIP_CHECK(hipGetLastError());
     return BFGT_OK;
+}
+
+// two y tiles per pass when their count is even
+template <int NTAB>
+static int launch_down_mfma(Plan& p, const cplx* tab, int lstride, int ymid_half, double* pot, double* grad,
+                            double* hess) {
+    int nT = round_up((p.npw + 1) / 2, 16) / 16;
+    if (nT % 2 == 0) return launch_down_mfma2<NTAB, 2>(p, tab, lstride, ymid_half, pot, grad, hess);
+    return launch_down_mfma2<NTAB, 1>(p, tab, lstride, ymid_half, pot, grad, hess);
 }
 
 template <int NTAB>
