@@ -207,8 +207,10 @@ class BoxFGT:
             shapes[5] = ((1 << max(level - self.info["nlevstart"], 0), npw), np.complex128)
             shapes[6] = ((self.info["n_ancestors"], self.nd, npw, H), np.complex128)
             # local expansions as stored: mirror-folded (S, D) on the MFMA path
-            shapes[7] = (((self.info["n_ancestors"], self.nd, H, H, 2), np.complex128) if self.info["mfma"]
+            mh4, h16 = -(-H // 4) * 4, -(-H // 16) * 16
+            shapes[7] = (((self.info["n_ancestors"], self.nd, mh4, h16, 2), np.complex128) if self.info["mfma"]
                          else shapes[6])
+        shapes[8] = ((self.info["n_near_targets"] + 1,), np.int32)
         shape, dt = shapes[which]
         out = np.zeros(shape, dtype=dt)
         check(lib().bfgt_plan_get_table(self._h, which, level, out.ctypes.data_as(ctypes.c_void_p), out.nbytes))

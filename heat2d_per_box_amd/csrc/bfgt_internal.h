@@ -121,6 +121,8 @@ struct Plan {
     int* d_near_ptr = nullptr;     // CSR [nnear_t+1]
     NearPair* d_near = nullptr;
     TreeNode* d_tnodes = nullptr;  // [nboxes], for extra targets
+    double* d_tnear = nullptr;     // [level][13][3][16][16] zero-padded near tables (n <= 16), [target][source]
+    double* d_tnearT = nullptr;    // the same, [source][target]
 
     // zero-padded PW tables of the matrix-core kernels (16 point rows or
     // columns, pstride frequencies), so that every load in their inner loops
@@ -172,6 +174,8 @@ int launch_reduce(Plan& p);
 int launch_m2l(Plan& p);
 int launch_downward(Plan& p, double* pot, double* grad, double* hess);
 int launch_near(Plan& p, const double* fvals, double* pot, double* grad, double* hess);
+bool near_use_mfma(const Plan& p);
+int build_near_padded(Plan& p);
 
 // extra targets (bfgt_targets.hip)
 int launch_targets(Plan& p, int ntarg, const double* targs, int ifpghtarg, const double* pot, const double* grad,

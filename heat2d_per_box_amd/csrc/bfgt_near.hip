@@ -204,12 +204,258 @@ __global__ __launch_bounds__(kNearThreads) void k_near_adj(int n, int nd, int nt
     }
 }
 
+// ---------------------------------------------------------------------------
+// matrix-core near field (n <= 16)
+// ---------------------------------------------------------------------------
+//
+// The tables are read from zero-padded copies, tnear[level][13][3][16][16]
+// ([k+6][t][target k][source j]) for the adjoint and its transpose tnearT
+// ([k+6][t][source j][target k]) for the forward, so that every table load
+// of 64 lanes covers four 16-entry rows; out-of-range rows and columns add
+// exact zeros; leaf values are read at clamped indices (finite values that
+// only meet those zeros).  v_mfma_f64_16x16x4f64 lane layout as in the PW
+// kernels: A holds A[c][q], B holds B[q][c], element r of the accumulator
+// holds D[4r + q][c] (lane l, q = l/16, c = l%16).
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+__device__ inline v4d mfma(double a, double b, v4d c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+constexpr int near_chan(int ntab) { return ntab == 1 ? 1 : (ntab == 2 ? 3 : 6); }
+constexpr int near_x(int o) { return o == 1 || o == 4 ? 1 : (o == 3 ? 2 : 0); }
+constexpr int near_y(int o) { return o == 2 || o == 4 ? 1 : (o == 5 ? 2 : 0); }
+constexpr int kNearWaves = 4;
+
+// One wave per (target, density).  For each (source, offsets) pair:
+//   FF_t^T = F^T Tx_t^T           (x contraction, K = source x index)
+//   out_o += Ty_{y(o)} FF_{x(o)}   (y contraction, K = source y index)
+// where element s of FF_t^T's accumulator is the B operand of K-step s of
+// the second product.  Channels o: pot (0,0), gx (1,0), gy (0,1), hxx (2,0),
+// hxy (1,1), hyy (0,2) as (x table, y table).  NKS = ceil(n/4).
+template <int NKS, int NTAB>
+__global__ __launch_bounds__(64 * kNearWaves) void k_near_mfma(
+    int n, int nd, int nitems, const int* __restrict__ tgt, const int* __restrict__ tlev,
+    const int* __restrict__ ptr, const NearPair* __restrict__ pairs, const double* __restrict__ tnearT,
+    const double* __restrict__ fvals, double* __restrict__ pot, double* __restrict__ grad,
+    double* __restrict__ hess) {
+    constexpr int NOUT = near_chan(NTAB);
+    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+    // wave-uniform: schedule entries are then read with scalar loads
+    const int item = __builtin_amdgcn_readfirstlane(blockIdx.x * kNearWaves + wave);
+    if (item >= nitems) return;  // no block-wide barriers in this kernel
+    const int it = item / nd, d = item % nd;
+    const int q = lane >> 4, c = lane & 15;
+    const int b = tgt[it];
+    const int p0 = ptr[it], p1 = ptr[it + 1];
+    const double* T = tnearT + (size_t)tlev[it] * 13 * 3 * 256;
+    const int nn = n * n;
+    const int cl = min(c, n - 1);
+    v4d out[NOUT];
+#pragma unroll
+    for (int o = 0; o < NOUT; o++) out[o] = v4d{0, 0, 0, 0};
+    // two pairs per iteration so that their loads overlap (a repeated last
+    // pair of an odd count contributes with weight zero)
+    for (int p = p0; p < p1; p += 2) {
+        const bool two = p + 1 < p1;
+        const NearPair pr[2] = {pairs[p], pairs[two ? p + 1 : p]};
+        v4d FF[2][NTAB];
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const double* fs = fvals + (size_t)pr[h].src * nn * nd + d;
+            const double* Tx = T + (size_t)(pr[h].ix + 6) * 3 * 256;
+            const double wgt = (h == 0 || two) ? 1.0 : 0.0;
+#pragma unroll
+            for (int t = 0; t < NTAB; t++) FF[h][t] = v4d{0, 0, 0, 0};
+#pragma unroll
+            for (int s = 0; s < NKS; s++) {
+                const int j1 = 4 * s + q;
+                // A = F^T: A[j2 = c][j1]; B = Tx_t^T: B[j1][k1 = c] = Tx_t[c][j1] (transposed table)
+                const double a = wgt * fs[(size_t)(min(j1, n - 1) + n * cl) * nd];
+#pragma unroll
+                for (int t = 0; t < NTAB; t++) FF[h][t] = mfma(a, Tx[t * 256 + j1 * 16 + c], FF[h][t]);
+            }
+        }
+        // FF_t^T: element s of lane (q, c) = FF_t^T[j2 = 4s + q][k1 = c]
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const double* Ty = T + (size_t)(pr[h].iy + 6) * 3 * 256;
+#pragma unroll
+            for (int s = 0; s < NKS; s++) {
+                const int j2 = 4 * s + q;
+                double ty[NTAB];
+#pragma unroll
+                for (int u = 0; u < NTAB; u++) ty[u] = Ty[u * 256 + j2 * 16 + c];  // A[k2 = c][j2] = Ty[c][j2]
+#pragma unroll
+                for (int o = 0; o < NOUT; o++) out[o] = mfma(ty[near_y(o)], FF[h][near_x(o)][s], out[o]);
+            }
+        }
+    }
+    // element r of lane (q, c) = out[k1 = c][k2 = 4r + q]
+    if (c < n && p1 > p0) {
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int k2 = 4 * r + q;
+            if (k2 < n) {
+                const size_t pb = (size_t)b * nn + c + (size_t)n * k2;
+                pot[pb * nd + d] += out[0][r];
+                if constexpr (NTAB >= 2) {
+                    grad[(pb * 2 + 0) * nd + d] += out[1][r];
+                    grad[(pb * 2 + 1) * nd + d] += out[2][r];
+                }
+                if constexpr (NTAB >= 3) {
+                    hess[(pb * 3 + 0) * nd + d] += out[3][r];
+                    hess[(pb * 3 + 1) * nd + d] += out[4][r];
+                    hess[(pb * 3 + 2) * nd + d] += out[5][r];
+                }
+            }
+        }
+    }
+}
+
+// Adjoint, one wave per (source, density) over the (target, offsets) pairs
+// that read it:
+//   W_v   = sum_{o: x(o) = v} G_o Ty_{y(o)}   (K = target y index)
+//   fbar += sum_v Tx_v^T W_v                (K = target x index)
+// with element s of W_v's accumulator the B operand of K-step s.
+template <int NKS, int NTAB>
+__global__ __launch_bounds__(64 * kNearWaves) void k_near_adj_mfma(
+    int n, int nd, int nitems, const int* __restrict__ src, const int* __restrict__ ptr,
+    const NearPairT* __restrict__ pairs, const double* __restrict__ tnear, const double* __restrict__ gpot,
+    const double* __restrict__ ggrad, const double* __restrict__ ghess, double* __restrict__ fbar) {
+    constexpr int NOUT = near_chan(NTAB);
+    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+    // wave-uniform: schedule entries are then read with scalar loads
+    const int item = __builtin_amdgcn_readfirstlane(blockIdx.x * kNearWaves + wave);
+    if (item >= nitems) return;  // no block-wide barriers in this kernel
+    const int is = item / nd, d = item % nd;
+    const int q = lane >> 4, c = lane & 15;
+    const int sb = src[is];
+    const int p0 = ptr[is], p1 = ptr[is + 1];
+    const int nn = n * n;
+    const int cl = min(c, n - 1);
+    v4d acc = {0, 0, 0, 0};
+    for (int p = p0; p < p1; p++) {
+        const NearPairT pr = pairs[p];
+        const double* T = tnear + (size_t)pr.lev * 13 * 3 * 256;
+        const double* Tx = T + (size_t)(pr.ix + 6) * 3 * 256;
+        const double* Ty = T + (size_t)(pr.iy + 6) * 3 * 256;
+        const size_t tb = (size_t)pr.tgt * nn;
+        v4d W[NTAB];
+#pragma unroll
+        for (int v = 0; v < NTAB; v++) W[v] = v4d{0, 0, 0, 0};
+#pragma unroll
+        for (int s = 0; s < NKS; s++) {
+            const int k2 = 4 * s + q;
+            // A = G_o: A[k1 = c][k2]; B = Ty_u: B[k2][j2 = c] = Ty_u[k2][c]
+            const size_t pb = tb + cl + (size_t)n * min(k2, n - 1);
+            double ty[NTAB];
+#pragma unroll
+            for (int u = 0; u < NTAB; u++) ty[u] = Ty[u * 256 + k2 * 16 + c];
+#pragma unroll
+            for (int o = 0; o < NOUT; o++) {
+                double g;
+                switch (o) {
+                    case 0: g = gpot[pb * nd + d]; break;
+                    case 1: g = ggrad[(pb * 2 + 0) * nd + d]; break;
+                    case 2: g = ggrad[(pb * 2 + 1) * nd + d]; break;
+                    case 3: g = ghess[(pb * 3 + 0) * nd + d]; break;
+                    case 4: g = ghess[(pb * 3 + 1) * nd + d]; break;
+                    default: g = ghess[(pb * 3 + 2) * nd + d]; break;
+                }
+                W[near_x(o)] = mfma(g, ty[near_y(o)], W[near_x(o)]);
+            }
+        }
+        // W_v: element s of lane (q, c) = W_v[k1 = 4s + q][j2 = c]
+#pragma unroll
+        for (int s = 0; s < NKS; s++) {
+            const int k1 = 4 * s + q;
+#pragma unroll
+            for (int v = 0; v < NTAB; v++) acc = mfma(Tx[v * 256 + k1 * 16 + c], W[v][s], acc);  // A[j1 = c][k1]
+        }
+    }
+    // element r of lane (q, c) = fbar[j1 = 4r + q][j2 = c]
+    if (c < n && p1 > p0) {
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int j1 = 4 * r + q;
+            if (j1 < n) fbar[((size_t)sb * nn + j1 + (size_t)n * c) * nd + d] += acc[r];
+        }
+    }
+}
+
+// Zero-padded copies of the masked near-field tables, [level][13][3][16][16]
+// as [target][source] (tnear) and [source][target] (tnearT).
+__global__ void k_pad_near(int n, int nlev, const double* __restrict__ tloc, double* __restrict__ tnear,
+                           double* __restrict__ tnearT) {
+    const size_t tot = (size_t)nlev * 13 * 3 * 256;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < tot; e += (size_t)gridDim.x * blockDim.x) {
+        const size_t blk = e / 256;
+        const int r = (int)(e % 256), k = r / 16, j = r % 16;
+        const double v = (k < n && j < n) ? tloc[blk * n * n + k * n + j] : 0.0;
+        tnear[e] = v;
+        tnearT[blk * 256 + j * 16 + k] = v;
+    }
+}
+
+template <int NKS, int NTAB>
+int launch_near_mfma2(Plan& p, const double* fvals, double* pot, double* grad, double* hess) {
+    int nitems = p.nnear_t * p.nd;
+    int blocks = (nitems + kNearWaves - 1) / kNearWaves;
+    k_near_mfma<NKS, NTAB><<<blocks, 64 * kNearWaves, 0, p.stream>>>(p.n, p.nd, nitems, p.d_near_tgt, p.d_near_lev,
+                                                                   p.d_near_ptr, p.d_near, p.d_tnearT, fvals, pot,
+                                                                   grad, hess);
+    BFGT_HIP_CHECK(hipGetLastError());
+    return BFGT_OK;
+}
+
+template <int NKS>
+int launch_near_mfma1(Plan& p, const double* fvals, double* pot, double* grad, double* hess) {
+    if (p.ifpgh == 1) return launch_near_mfma2<NKS, 1>(p, fvals, pot, grad, hess);
+    if (p.ifpgh == 2) return launch_near_mfma2<NKS, 2>(p, fvals, pot, grad, hess);
+    return launch_near_mfma2<NKS, 3>(p, fvals, pot, grad, hess);
+}
+
+template <int NKS, int NTAB>
+int launch_near_adj_mfma2(Plan& p, const double* gpot, const double* ggrad, const double* ghess, double* fbar) {
+    int nitems = p.nnearT_s * p.nd;
+    int blocks = (nitems + kNearWaves - 1) / kNearWaves;
+    k_near_adj_mfma<NKS, NTAB><<<blocks, 64 * kNearWaves, 0, p.stream>>>(
+        p.n, p.nd, nitems, p.d_nearT_src, p.d_nearT_ptr, p.d_nearT, p.d_tnear, gpot, ggrad, ghess, fbar);
+    BFGT_HIP_CHECK(hipGetLastError());
+    return BFGT_OK;
+}
+
+template <int NKS>
+int launch_near_adj_mfma1(Plan& p, const double* gpot, const double* ggrad, const double* ghess, double* fbar) {
+    if (p.ifpgh == 1) return launch_near_adj_mfma2<NKS, 1>(p, gpot, ggrad, ghess, fbar);
+    if (p.ifpgh == 2) return launch_near_adj_mfma2<NKS, 2>(p, gpot, ggrad, ghess, fbar);
+    return launch_near_adj_mfma2<NKS, 3>(p, gpot, ggrad, ghess, fbar);
+}
+
 }  // namespace
+
+bool near_use_mfma(const Plan& p) { return p.n <= 16; }
+
+int build_near_padded(Plan& p) {
+    if (!near_use_mfma(p)) return BFGT_OK;
+    int nlev = p.nlevels + 1;
+    size_t tot = (size_t)nlev * 13 * 3 * 256;
+    int blocks = (int)((tot + 255) / 256 < 4096 ? (tot + 255) / 256 : 4096);
+    k_pad_near<<<blocks, 256, 0, p.stream>>>(p.n, nlev, p.d_tloc, p.d_tnear, p.d_tnearT);
+    BFGT_HIP_CHECK(hipGetLastError());
+    return BFGT_OK;
+}
 
 size_t near_smem_bytes(int n, int ntab) { return sizeof(double) * (size_t)n * n * (1 + ntab); }
 
 int launch_near(Plan& p, const double* fvals, double* pot, double* grad, double* hess) {
     if (p.nnear_t == 0) return BFGT_OK;
+    if (near_use_mfma(p)) {
+        switch ((p.n + 3) / 4) {
+            case 1: return launch_near_mfma1<1>(p, fvals, pot, grad, hess);
+            case 2: return launch_near_mfma1<2>(p, fvals, pot, grad, hess);
+            case 3: return launch_near_mfma1<3>(p, fvals, pot, grad, hess);
+            default: return launch_near_mfma1<4>(p, fvals, pot, grad, hess);
+        }
+    }
     int ntab = p.ifpgh;
     size_t sh = near_smem_bytes(p.n, ntab);
     k_near<<<p.nnear_t, kNearThreads, sh, p.stream>>>(p.n, p.nd, ntab, p.d_near_tgt, p.d_near_lev, p.d_near_ptr,
@@ -220,6 +466,14 @@ int launch_near(Plan& p, const double* fvals, double* pot, double* grad, double*
 
 int launch_near_adj(Plan& p, const double* gpot, const double* ggrad, const double* ghess, double* fbar) {
     if (p.nnearT_s == 0) return BFGT_OK;
+    if (near_use_mfma(p)) {
+        switch ((p.n + 3) / 4) {
+            case 1: return launch_near_adj_mfma1<1>(p, gpot, ggrad, ghess, fbar);
+            case 2: return launch_near_adj_mfma1<2>(p, gpot, ggrad, ghess, fbar);
+            case 3: return launch_near_adj_mfma1<3>(p, gpot, ggrad, ghess, fbar);
+            default: return launch_near_adj_mfma1<4>(p, gpot, ggrad, ghess, fbar);
+        }
+    }
     int ntab = p.ifpgh;
     int nout = ntab == 1 ? 1 : (ntab == 2 ? 3 : 6);
     size_t sh = sizeof(double) * (size_t)p.n * p.n * (nout + ntab);

@@ -394,6 +394,11 @@ int plan_create(Plan& p, int nd, int ndim, double delta, double eps, int ipoly, 
         if ((rc = build_pw_nodes(p))) return rc;
     }
     if ((rc = build_tables(p))) return rc;
+    if (near_use_mfma(p)) {
+        if ((rc = dalloc(p, p.d_tnear, (size_t)(nlevels + 1) * 13 * 3 * 256))) return rc;
+        if ((rc = dalloc(p, p.d_tnearT, (size_t)(nlevels + 1) * 13 * 3 * 256))) return rc;
+        if ((rc = build_near_padded(p))) return rc;
+    }
     if (p.has_pw && use_mfma(p)) {
         size_t per = (size_t)(nlevels + 1) * 16 * p.pstride;
         if ((rc = dalloc(p, p.d_tup_f, per))) return rc;
@@ -881,6 +886,11 @@ int bfgt_plan_get_table(bfgt_plan_t plan, int which, int level, void* host_out, 
             sz = sizeof(bfgt::cplx) * (size_t)p.nanc *
                  (bfgt::use_mfma(p) ? 2 * bfgt::fold_len(p) : (size_t)p.nd * p.nexp);
             src = p.d_loc;
+            break;
+        case 8:
+            // near-field schedule: CSR pointer over the target leaves (nnear_t + 1 ints)
+            sz = sizeof(int) * (size_t)(p.nnear_t + 1);
+            src = p.nnear_t > 0 ? (const void*)p.d_near_ptr : nullptr;
             break;
         default:
             return set_error(BFGT_ERR_ARG, "unknown table");

@@ -114,7 +114,10 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_up_mfma(
     const cplx* __restrict__ phase, const int* __restrict__ levbase, cplx* __restrict__ partial) {
     constexpr int NOUT = nchan(NTAB);
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
-    const int item = blockIdx.x * kWavesPerBlock + wave;
+    // the multi-channel (adjoint) instances read their schedule with scalar
+    // loads; the single-channel forward measured faster with vector loads
+    const int item = NTAB > 1 ? __builtin_amdgcn_readfirstlane(blockIdx.x * kWavesPerBlock + wave)
+                              : blockIdx.x * kWavesPerBlock + wave;
     if (item >= nitems) return;  // no block-wide barriers in this kernel
     const int ci = item / nTN, tN = item % nTN;
     const int q = lane >> 4, cc = lane & 15;
@@ -225,7 +228,8 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_down_mfma(
     const Fold* __restrict__ fold, double* __restrict__ pot, double* __restrict__ grad, double* __restrict__ hess) {
     constexpr int NOUT = nchan(NTAB);
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
-    const int item = blockIdx.x * kWavesPerBlock + wave;
+    // wave-uniform: schedule entries are then read with scalar loads
+    const int item = __builtin_amdgcn_readfirstlane(blockIdx.x * kWavesPerBlock + wave);
     if (item >= nleaf) return;  // no block-wide barriers in this kernel
     const int q = lane >> 4, cc = lane & 15;
     const int H = (P + 1) / 2, P2 = P / 2, npbox = n * n;
@@ -437,7 +441,8 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_down_split(
     double* __restrict__ pot, double* __restrict__ grad, double* __restrict__ hess) {
     constexpr int NOUT = nchan(NTAB);
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
-    const int item = blockIdx.x * kWavesPerBlock + wave;
+    // wave-uniform: schedule entries are then read with scalar loads
+    const int item = __builtin_amdgcn_readfirstlane(blockIdx.x * kWavesPerBlock + wave);
     if (item >= nleaf) return;  // no block-wide barriers in this kernel
     const int q = lane >> 4, cc = lane & 15;
     const int H = (P + 1) / 2, P2 = P / 2, npbox = n * n;

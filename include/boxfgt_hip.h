@@ -144,8 +144,10 @@ int bfgt_plan_query(bfgt_plan_t plan, int *ivals, double *dvals);
  * mp expansions of the last execution (n_ancestors x nd x npw x H complex;
  * `level` ignored), 7 its loc expansions as stored: on the matrix-core path
  * (dvals[4] = 1) mirror-folded pairs S = L(m,.) + L(npw-1-m,.),
- * D = L(m,.) - L(npw-1-m,.) (n_ancestors x nd x ceil(npw/2) x H x 2 complex),
- * otherwise n_ancestors x nd x npw x H complex. */
+ * D = L(m,.) - L(npw-1-m,.) (n_ancestors x nd x Mh4 x H16 x 2 complex, zero
+ * padded, Mh4 = 4 ceil(H/4), H16 = 16 ceil(H/16)),
+ * otherwise n_ancestors x nd x npw x H complex; 8 the near-field schedule's
+ * CSR pointer over its target leaves (n_near_targets + 1 ints). */
 int bfgt_plan_get_table(bfgt_plan_t plan, int which, int level, void *host_out, size_t nbytes);
 
 const char *bfgt_last_error(void);
