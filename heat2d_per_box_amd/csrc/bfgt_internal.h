@@ -162,6 +162,18 @@ struct Plan {
     std::vector<void*> allocations;
 };
 
+// The dispatcher deals workgroups round-robin to the 8 XCDs of the chip,
+// each with its own L2.  xcd_block maps physical workgroup b of nb to a
+// logical index such that every XCD runs one contiguous range of logical
+// workgroups (a bijection on [0, nb)), so neighbouring work items -- leaves
+// of one cutoff box, neighbouring cutoff boxes -- share an L2.
+constexpr int kXcds = 8;
+__device__ inline int xcd_block(int b, int nb) {
+    const int q = nb / kXcds, r = nb % kXcds;
+    const int x = b % kXcds;
+    return x * q + (x < r ? x : r) + b / kXcds;
+}
+
 // status helpers
 int set_error(int code, const std::string& msg);
 

@@ -635,7 +635,10 @@ int plan_create(Plan& p, int nd, int ndim, double delta, double eps, int ipoly, 
         if ((rc = dalloc(p, p.d_partial, (size_t)p.nchunks * len))) return rc;
         if ((rc = dalloc(p, p.d_gpart, (size_t)p.ngslot * len))) return rc;
         if ((rc = dalloc(p, p.d_mp, (size_t)p.nanc * len))) return rc;
-        if ((rc = dalloc(p, p.d_loc, (size_t)p.nanc * std::max(loc_buffer_cplx(p), 2 * fold_len(p))))) return rc;
+        size_t nloc = (size_t)p.nanc * std::max(loc_buffer_cplx(p), 2 * fold_len(p));
+        if ((rc = dalloc(p, p.d_loc, nloc))) return rc;
+        // the folded layout's padding is never written by the translation: zero it once
+        BFGT_HIP_CHECK(hipMemsetAsync(p.d_loc, 0, nloc * sizeof(cplx), p.stream));
     }
     BFGT_HIP_CHECK(hipStreamSynchronize(p.stream));
     auto t1 = std::chrono::steady_clock::now();

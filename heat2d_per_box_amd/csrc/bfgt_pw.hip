@@ -228,8 +228,9 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_down_mfma(
     const Fold* __restrict__ fold, double* __restrict__ pot, double* __restrict__ grad, double* __restrict__ hess) {
     constexpr int NOUT = nchan(NTAB);
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
-    // wave-uniform: schedule entries are then read with scalar loads
-    const int item = __builtin_amdgcn_readfirstlane(blockIdx.x * kWavesPerBlock + wave);
+    // wave-uniform (schedule entries read with scalar loads), and XCD-contiguous:
+    // the leaves of one cutoff box read its folded expansion from one L2
+    const int item = __builtin_amdgcn_readfirstlane(xcd_block(blockIdx.x, gridDim.x) * kWavesPerBlock + wave);
     if (item >= nleaf) return;  // no block-wide barriers in this kernel
     const int q = lane >> 4, cc = lane & 15;
     const int H = (P + 1) / 2, P2 = P / 2, npbox = n * n;
@@ -765,35 +766,36 @@ __global__ void k_m2l(int nd, int nexp, const int* __restrict__ m2l_ptr, const i
         loc[(size_t)a * len + e] = m2l_at(wshift, mp, nexp, len, a, e, j0, j1, m2l, e % nexp, 1.0);
 }
 
-// Same translation, written mirror-folded and zero-padded (struct Fold) for
-// the MFMA downward pass.
-__global__ void k_m2l_fold(int nd, int P, const int* __restrict__ m2l_ptr, const int2* __restrict__ m2l,
+// Same translation, written mirror-folded (struct Fold) for the MFMA
+// downward pass.  Only the H x H valid entries of the padded layout are
+// written: the padding is zeroed once when the plan is built and never
+// touched again.  Workgroups are mapped XCD-contiguously, so the expansions
+// of neighbouring cutoff boxes are read from one L2.
+__global__ void k_m2l_fold(int nd, int P, int gy, const int* __restrict__ m2l_ptr, const int2* __restrict__ m2l,
                            const cplx* __restrict__ wshift, double csign, const cplx* __restrict__ mp,
                            Fold* __restrict__ fold) {
-    const int a = blockIdx.x;
+    const int lb = xcd_block(blockIdx.x, gridDim.x);
+    const int a = lb / gy;
     const int H = (P + 1) / 2, nexp = P * H;
     const int Mh4 = round_up(H, 4), H16 = round_up(H, 16);
     const int len = nd * nexp;
-    const int flen = nd * Mh4 * H16;
+    const int e = (lb % gy) * blockDim.x + threadIdx.x;
+    if (e >= nd * H * H) return;
     const int j0 = m2l_ptr[a], j1 = m2l_ptr[a + 1];
-    for (int e = blockIdx.y * blockDim.x + threadIdx.x; e < flen; e += gridDim.y * blockDim.x) {
-        int d = e / (Mh4 * H16), r = e % (Mh4 * H16);
-        int m = r / H16, k2 = r % H16;
-        Fold f{{0, 0}, {0, 0}};
-        if (m < H && k2 < H) {
-            int q1 = m * H + k2, mm = P - 1 - m;
-            cplx l1 = m2l_at(wshift, mp, nexp, len, a, d * nexp + q1, j0, j1, m2l, q1, csign);
-            if (mm != m) {
-                int q2 = mm * H + k2;
-                cplx l2 = m2l_at(wshift, mp, nexp, len, a, d * nexp + q2, j0, j1, m2l, q2, csign);
-                f.S = cplx{l1.x + l2.x, l1.y + l2.y};
-                f.D = cplx{l1.x - l2.x, l1.y - l2.y};
-            } else {
-                f.S = l1;
-            }
-        }
-        fold[(size_t)a * flen + e] = f;
+    const int d = e / (H * H), r = e % (H * H);
+    const int m = r / H, k2 = r % H;
+    const int q1 = m * H + k2, mm = P - 1 - m;
+    cplx l1 = m2l_at(wshift, mp, nexp, len, a, d * nexp + q1, j0, j1, m2l, q1, csign);
+    Fold f{{0, 0}, {0, 0}};
+    if (mm != m) {
+        int q2 = mm * H + k2;
+        cplx l2 = m2l_at(wshift, mp, nexp, len, a, d * nexp + q2, j0, j1, m2l, q2, csign);
+        f.S = cplx{l1.x + l2.x, l1.y + l2.y};
+        f.D = cplx{l1.x - l2.x, l1.y - l2.y};
+    } else {
+        f.S = l1;
     }
+    fold[(size_t)a * nd * Mh4 * H16 + ((size_t)d * Mh4 + m) * H16 + k2] = f;
 }
 
 }  // namespace
@@ -945,10 +947,10 @@ int launch_reduce(Plan& p) {
 }
 
 static int launch_m2l_fold(Plan& p, const int* ptr, const int2* list, double csign) {
-    int flen = (int)fold_len(p);
-    int gy = (flen + 255) / 256;
-    k_m2l_fold<<<dim3(p.nanc, gy), 256, 0, p.stream>>>(p.nd, p.npw, ptr, list, p.d_wshift, csign, p.d_mp,
-                                                       reinterpret_cast<Fold*>(p.d_loc));
+    int H = (p.npw + 1) / 2;
+    int gy = (p.nd * H * H + 255) / 256;
+    k_m2l_fold<<<p.nanc * gy, 256, 0, p.stream>>>(p.nd, p.npw, gy, ptr, list, p.d_wshift, csign, p.d_mp,
+                                                  reinterpret_cast<Fold*>(p.d_loc));
     BFGT_HIP_CHECK(hipGetLastError());
     return BFGT_OK;
 }
