@@ -32,6 +32,8 @@ EXPORTS = [
     "bfgt_hip_eval",
     "bfgt_plan_create",
     "bfgt_plan_execute",
+    "bfgt_plan_execute_scaled",
+    "bfgt_plan_leaf_points",
     "bfgt_plan_adjoint",
     "bfgt_plan_eval_targets",
     "bfgt_plan_destroy",
@@ -79,6 +81,10 @@ def lib() -> ctypes.CDLL:
     L.bfgt_plan_eval_targets.argtypes = [_vp, _i, _vp, _i, _vp, _vp, _vp, _vp, _vp, _vp]
     L.bfgt_plan_adjoint.restype = _i
     L.bfgt_plan_adjoint.argtypes = [_vp, _vp, _vp, _vp, _vp, _dp]
+    L.bfgt_plan_execute_scaled.restype = _i
+    L.bfgt_plan_execute_scaled.argtypes = [_vp, _d, _vp, _vp, _vp, _vp, _dp]
+    L.bfgt_plan_leaf_points.restype = _i
+    L.bfgt_plan_leaf_points.argtypes = [_vp, _vp]
     L.bfgt_plan_destroy.restype = _i
     L.bfgt_plan_destroy.argtypes = [_vp]
     L.bfgt_plan_query.restype = _i

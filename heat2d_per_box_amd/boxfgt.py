@@ -145,15 +145,30 @@ class BoxFGT:
             raise ValueError(f"{name} has shape {tuple(t.shape)}, expected {shape}")
         return ctypes.c_void_p(t.data_ptr())
 
-    def __call__(self, fvals, pot, grad=None, hess=None, timeinfo: bool = False):
+    def __call__(self, fvals, pot, grad=None, hess=None, timeinfo: bool = False, scale: float = 1.0):
+        """pot (+grad, hess) += scale * A fvals on device tensors."""
         nb, npb, nd = self.nboxes, self.npbox, self.nd
         f = self._check_tensor(fvals, (nb, npb, nd), "fvals")
         p = self._check_tensor(pot, (nb, npb, nd), "pot")
         g = self._check_tensor(grad, (nb, npb, 2, nd), "grad")
         h = self._check_tensor(hess, (nb, npb, 3, nd), "hess")
         ti = np.zeros(20) if timeinfo else None
-        check(lib().bfgt_plan_execute(self._h, f, p, g, h, _ptr(ti) if ti is not None else None))
+        tip = _ptr(ti) if ti is not None else None
+        if scale == 1.0:
+            check(lib().bfgt_plan_execute(self._h, f, p, g, h, tip))
+        else:
+            check(lib().bfgt_plan_execute_scaled(self._h, float(scale), f, p, g, h, tip))
         return ti
+
+    def leaf_points(self, xy=None):
+        """Device tensor (nboxes, npbox, 2) of the leaf tensor-grid nodes (zeros on non-leaf boxes)."""
+        import torch
+
+        if xy is None:
+            xy = torch.zeros((self.nboxes, self.npbox, 2), dtype=torch.float64, device="cuda")
+        x = self._check_tensor(xy, (self.nboxes, self.npbox, 2), "xy")
+        check(lib().bfgt_plan_leaf_points(self._h, x))
+        return xy
 
     def eval_targets(self, targs, ifpghtarg: int, pot, pote, grad=None, hess=None, grade=None, hesse=None):
         """Interpolate the leaf arrays at targets (ntarg, 2) into pote (ntarg, nd),

@@ -78,6 +78,7 @@ struct Plan {
     int nleafbox = 0;
     hipStream_t stream = nullptr;
     bool own_stream = false;
+    double out_scale = 1.0;  // factor on the outputs of the current execution
     int device = 0;
 
     // host copies used for scheduling
@@ -188,6 +189,9 @@ int launch_downward(Plan& p, double* pot, double* grad, double* hess);
 int launch_near(Plan& p, const double* fvals, double* pot, double* grad, double* hess);
 bool near_use_mfma(const Plan& p);
 int build_near_padded(Plan& p);
+
+// leaf-node coordinates (bfgt_targets.hip)
+int launch_leaf_points(Plan& p, double* xy);
 
 // extra targets (bfgt_targets.hip)
 int launch_targets(Plan& p, int ntarg, const double* targs, int ifpghtarg, const double* pot, const double* grad,

@@ -26,7 +26,8 @@ __global__ __launch_bounds__(kNearThreads) void k_near(int n, int nd, int ntab, 
                                                        const NearPair* __restrict__ pairs,
                                                        const double* __restrict__ tloc,
                                                        const double* __restrict__ fvals, double* __restrict__ pot,
-                                                       double* __restrict__ grad, double* __restrict__ hess) {
+                                                       double* __restrict__ grad, double* __restrict__ hess,
+                                                       double oscale) {
     extern __shared__ double smem[];
     const int nn = n * n;
     double* sf = smem;       // nn
@@ -100,15 +101,15 @@ __global__ __launch_bounds__(kNearThreads) void k_near(int n, int nd, int ntab, 
             int q = tid + r * kNearThreads;
             if (q < nn && p1 > p0) {
                 size_t pb = (size_t)b * nn + q;
-                pot[pb * nd + d] += acc[r][0];
+                pot[pb * nd + d] += oscale * acc[r][0];
                 if (ntab >= 2) {
-                    grad[(pb * 2 + 0) * nd + d] += acc[r][1];
-                    grad[(pb * 2 + 1) * nd + d] += acc[r][2];
+                    grad[(pb * 2 + 0) * nd + d] += oscale * acc[r][1];
+                    grad[(pb * 2 + 1) * nd + d] += oscale * acc[r][2];
                 }
                 if (ntab >= 3) {
-                    hess[(pb * 3 + 0) * nd + d] += acc[r][3];
-                    hess[(pb * 3 + 1) * nd + d] += acc[r][4];
-                    hess[(pb * 3 + 2) * nd + d] += acc[r][5];
+                    hess[(pb * 3 + 0) * nd + d] += oscale * acc[r][3];
+                    hess[(pb * 3 + 1) * nd + d] += oscale * acc[r][4];
+                    hess[(pb * 3 + 2) * nd + d] += oscale * acc[r][5];
                 }
             }
         }
@@ -235,7 +236,7 @@ __global__ __launch_bounds__(64 * kNearWaves) void k_near_mfma(
     int n, int nd, int nitems, const int* __restrict__ tgt, const int* __restrict__ tlev,
     const int* __restrict__ ptr, const NearPair* __restrict__ pairs, const double* __restrict__ tnearT,
     const double* __restrict__ fvals, double* __restrict__ pot, double* __restrict__ grad,
-    double* __restrict__ hess) {
+    double* __restrict__ hess, double oscale) {
     constexpr int NOUT = near_chan(NTAB);
     const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
     // wave-uniform: schedule entries are then read with scalar loads
@@ -295,15 +296,15 @@ __global__ __launch_bounds__(64 * kNearWaves) void k_near_mfma(
             const int k2 = 4 * r + q;
             if (k2 < n) {
                 const size_t pb = (size_t)b * nn + c + (size_t)n * k2;
-                pot[pb * nd + d] += out[0][r];
+                pot[pb * nd + d] += oscale * out[0][r];
                 if constexpr (NTAB >= 2) {
-                    grad[(pb * 2 + 0) * nd + d] += out[1][r];
-                    grad[(pb * 2 + 1) * nd + d] += out[2][r];
+                    grad[(pb * 2 + 0) * nd + d] += oscale * out[1][r];
+                    grad[(pb * 2 + 1) * nd + d] += oscale * out[2][r];
                 }
                 if constexpr (NTAB >= 3) {
-                    hess[(pb * 3 + 0) * nd + d] += out[3][r];
-                    hess[(pb * 3 + 1) * nd + d] += out[4][r];
-                    hess[(pb * 3 + 2) * nd + d] += out[5][r];
+                    hess[(pb * 3 + 0) * nd + d] += oscale * out[3][r];
+                    hess[(pb * 3 + 1) * nd + d] += oscale * out[4][r];
+                    hess[(pb * 3 + 2) * nd + d] += oscale * out[5][r];
                 }
             }
         }
@@ -401,7 +402,7 @@ int launch_near_mfma2(Plan& p, const double* fvals, double* pot, double* grad, d
     int blocks = (nitems + kNearWaves - 1) / kNearWaves;
     k_near_mfma<NKS, NTAB><<<blocks, 64 * kNearWaves, 0, p.stream>>>(p.n, p.nd, nitems, p.d_near_tgt, p.d_near_lev,
                                                                    p.d_near_ptr, p.d_near, p.d_tnearT, fvals, pot,
-                                                                   grad, hess);
+                                                                   grad, hess, p.out_scale);
     BFGT_HIP_CHECK(hipGetLastError());
     return BFGT_OK;
 }
@@ -459,7 +460,7 @@ int launch_near(Plan& p, const double* fvals, double* pot, double* grad, double*
     int ntab = p.ifpgh;
     size_t sh = near_smem_bytes(p.n, ntab);
     k_near<<<p.nnear_t, kNearThreads, sh, p.stream>>>(p.n, p.nd, ntab, p.d_near_tgt, p.d_near_lev, p.d_near_ptr,
-                                                      p.d_near, p.d_tloc, fvals, pot, grad, hess);
+                                                      p.d_near, p.d_tloc, fvals, pot, grad, hess, p.out_scale);
     BFGT_HIP_CHECK(hipGetLastError());
     return BFGT_OK;
 }

@@ -646,10 +646,13 @@ int plan_create(Plan& p, int nd, int ndim, double delta, double eps, int ipoly, 
     return BFGT_OK;
 }
 
-int plan_execute(Plan& p, const double* fvals, double* pot, double* grad, double* hess, double* timeinfo) {
+int plan_execute(Plan& p, const double* fvals, double* pot, double* grad, double* hess, double* timeinfo,
+                 double scale) {
     if (!fvals || !pot || (p.ifpgh >= 2 && !grad) || (p.ifpgh >= 3 && !hess))
         return set_error(BFGT_ERR_ARG, "null device array");
+    if (!std::isfinite(scale)) return set_error(BFGT_ERR_ARG, "scale must be finite");
     BFGT_HIP_CHECK(hipSetDevice(p.device));
+    p.out_scale = scale;
     int rc;
     hipEvent_t* ev = p.ev;
     if (p.profiling && p.prof_count < p.prof_max) ev = p.prof_ev.data() + 6 * p.prof_count++;
@@ -701,6 +704,7 @@ int plan_adjoint(Plan& p, const double* gpot, const double* ggrad, const double*
     if (!p.has_adjoint)
         return set_error(BFGT_ERR_RESOURCE, "the adjoint needs leaf order <= 16 and at most 128 plane waves");
     BFGT_HIP_CHECK(hipSetDevice(p.device));
+    p.out_scale = 1.0;
     int rc;
     hipEvent_t* ev = p.ev;
     BFGT_HIP_CHECK(hipEventRecord(ev[0], p.stream));
@@ -765,7 +769,19 @@ int bfgt_plan_create(bfgt_plan_t* out, int nd, int ndim, double delta, double ep
 int bfgt_plan_execute(bfgt_plan_t plan, const double* d_fvals, double* d_pot, double* d_grad, double* d_hess,
                       double* timeinfo) {
     if (!plan) return set_error(BFGT_ERR_ARG, "null plan");
-    return bfgt::plan_execute(plan->p, d_fvals, d_pot, d_grad, d_hess, timeinfo);
+    return bfgt::plan_execute(plan->p, d_fvals, d_pot, d_grad, d_hess, timeinfo, 1.0);
+}
+
+int bfgt_plan_execute_scaled(bfgt_plan_t plan, double scale, const double* d_fvals, double* d_pot, double* d_grad,
+                             double* d_hess, double* timeinfo) {
+    if (!plan) return set_error(BFGT_ERR_ARG, "null plan");
+    return bfgt::plan_execute(plan->p, d_fvals, d_pot, d_grad, d_hess, timeinfo, scale);
+}
+
+int bfgt_plan_leaf_points(bfgt_plan_t plan, double* d_xy) {
+    if (!plan || !d_xy) return set_error(BFGT_ERR_ARG, "null argument");
+    BFGT_HIP_CHECK(hipSetDevice(plan->p.device));
+    return bfgt::launch_leaf_points(plan->p, d_xy);
 }
 
 int bfgt_plan_eval_targets(bfgt_plan_t plan, int ntarg, const double* d_targs, int ifpghtarg, const double* d_pot,
@@ -988,7 +1004,7 @@ int bfgt_hip_eval(int nd, int ndim, double delta, double eps, int ipoly, int ipe
     }
     auto t1 = std::chrono::steady_clock::now();
     double ti[20];
-    rc = bfgt::plan_execute(p, dfv, dpot, dgrad, dhess, ti);
+    rc = bfgt::plan_execute(p, dfv, dpot, dgrad, dhess, ti, 1.0);
     if (rc) return fail(rc);
     double t_targets = 0;
     if (with_targets) {

@@ -225,7 +225,8 @@ template <int NTAB, int NTP>
 __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_down_mfma(
     int n, int P, int ps, int nd, int nleaf, const PwLeaf* __restrict__ leaves, const cplx* __restrict__ tab,
     int lstride, int ymid_half, const cplx* __restrict__ phase, const int* __restrict__ levbase,
-    const Fold* __restrict__ fold, double* __restrict__ pot, double* __restrict__ grad, double* __restrict__ hess) {
+    const Fold* __restrict__ fold, double* __restrict__ pot, double* __restrict__ grad, double* __restrict__ hess,
+    double oscale) {
     constexpr int NOUT = nchan(NTAB);
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
     // wave-uniform (schedule entries read with scalar loads), and XCD-contiguous:
@@ -300,15 +301,16 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_down_mfma(
                 int k2 = drow(q, r);
                 if (k2 < n) {
                     size_t pb = (size_t)lf.box * npbox + cc + (size_t)n * k2;
-                    pot[pb * nd + d] += 2.0 * out[0][r];
+                    const double w2 = 2.0 * oscale;
+                    pot[pb * nd + d] += w2 * out[0][r];
                     if constexpr (NTAB >= 2) {
-                        grad[(pb * 2 + 0) * nd + d] += 2.0 * out[1][r];
-                        grad[(pb * 2 + 1) * nd + d] += 2.0 * out[2][r];
+                        grad[(pb * 2 + 0) * nd + d] += w2 * out[1][r];
+                        grad[(pb * 2 + 1) * nd + d] += w2 * out[2][r];
                     }
                     if constexpr (NTAB >= 3) {
-                        hess[(pb * 3 + 0) * nd + d] += 2.0 * out[3][r];
-                        hess[(pb * 3 + 1) * nd + d] += 2.0 * out[4][r];
-                        hess[(pb * 3 + 2) * nd + d] += 2.0 * out[5][r];
+                        hess[(pb * 3 + 0) * nd + d] += w2 * out[3][r];
+                        hess[(pb * 3 + 1) * nd + d] += w2 * out[4][r];
+                        hess[(pb * 3 + 2) * nd + d] += w2 * out[5][r];
                     }
                 }
             }
@@ -439,7 +441,7 @@ template <int NTAB>
 __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_down_split(
     int n, int P, int ps, int nd, int nleaf, const PwLeaf* __restrict__ leaves, const cplx* __restrict__ tab,
     int lstride, const cplx* __restrict__ phase, const int* __restrict__ levbase, const Fold* __restrict__ fold,
-    double* __restrict__ pot, double* __restrict__ grad, double* __restrict__ hess) {
+    double* __restrict__ pot, double* __restrict__ grad, double* __restrict__ hess, double oscale) {
     constexpr int NOUT = nchan(NTAB);
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
     // wave-uniform: schedule entries are then read with scalar loads
@@ -509,15 +511,16 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_down_split(
                 int k2 = drow32(q, r);
                 if (k2 < n) {
                     size_t pb = (size_t)lf.box * npbox + cc + (size_t)n * k2;
-                    pot[pb * nd + d] += 2.0 * (double)out[0][r];
+                    const double w2 = 2.0 * oscale;
+                    pot[pb * nd + d] += w2 * (double)out[0][r];
                     if constexpr (NTAB >= 2) {
-                        grad[(pb * 2 + 0) * nd + d] += 2.0 * (double)out[1][r];
-                        grad[(pb * 2 + 1) * nd + d] += 2.0 * (double)out[2][r];
+                        grad[(pb * 2 + 0) * nd + d] += w2 * (double)out[1][r];
+                        grad[(pb * 2 + 1) * nd + d] += w2 * (double)out[2][r];
                     }
                     if constexpr (NTAB >= 3) {
-                        hess[(pb * 3 + 0) * nd + d] += 2.0 * (double)out[3][r];
-                        hess[(pb * 3 + 1) * nd + d] += 2.0 * (double)out[4][r];
-                        hess[(pb * 3 + 2) * nd + d] += 2.0 * (double)out[5][r];
+                        hess[(pb * 3 + 0) * nd + d] += w2 * (double)out[3][r];
+                        hess[(pb * 3 + 1) * nd + d] += w2 * (double)out[4][r];
+                        hess[(pb * 3 + 2) * nd + d] += w2 * (double)out[5][r];
                     }
                 }
             }
@@ -606,7 +609,8 @@ __global__ __launch_bounds__(kVecThreads) void k_down_vec(int n, int npw, int nd
                                                           const PwLeaf* __restrict__ leaves,
                                                           const cplx* __restrict__ tpw, const double* __restrict__ ts,
                                                           const cplx* __restrict__ loc, double* __restrict__ pot,
-                                                          double* __restrict__ grad, double* __restrict__ hess) {
+                                                          double* __restrict__ grad, double* __restrict__ hess,
+                                                          double oscale) {
     extern __shared__ double smem[];
     const int H = (npw + 1) / 2;
     const int npw2 = npw / 2;
@@ -681,15 +685,16 @@ __global__ __launch_bounds__(kVecThreads) void k_down_vec(int n, int npw, int nd
             int q = tid + r * kVecThreads;
             if (q < npbox) {
                 size_t pb = (size_t)lf.box * npbox + q;
-                pot[pb * nd + d] += 2.0 * acc[r][0];
+                const double w2 = 2.0 * oscale;
+                pot[pb * nd + d] += w2 * acc[r][0];
                 if (ntab >= 2) {
-                    grad[(pb * 2 + 0) * nd + d] += 2.0 * acc[r][1];
-                    grad[(pb * 2 + 1) * nd + d] += 2.0 * acc[r][2];
+                    grad[(pb * 2 + 0) * nd + d] += w2 * acc[r][1];
+                    grad[(pb * 2 + 1) * nd + d] += w2 * acc[r][2];
                 }
                 if (ntab >= 3) {
-                    hess[(pb * 3 + 0) * nd + d] += 2.0 * acc[r][3];
-                    hess[(pb * 3 + 1) * nd + d] += 2.0 * acc[r][4];
-                    hess[(pb * 3 + 2) * nd + d] += 2.0 * acc[r][5];
+                    hess[(pb * 3 + 0) * nd + d] += w2 * acc[r][3];
+                    hess[(pb * 3 + 1) * nd + d] += w2 * acc[r][4];
+                    hess[(pb * 3 + 2) * nd + d] += w2 * acc[r][5];
                 }
             }
         }
@@ -969,7 +974,7 @@ static int launch_down_vec(Plan& p, double* pot, double* grad, double* hess, siz
     if (sh > 65536)
         BFGT_HIP_CHECK(hipFuncSetAttribute((const void*)k_down_vec<NQ>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
     k_down_vec<NQ><<<p.npwleaf, kVecThreads, sh, p.stream>>>(p.n, p.npw, p.nd, p.ifpgh, p.d_pwleaf, p.d_tpw, p.d_ts,
-                                                              p.d_loc, pot, grad, hess);
+                                                              p.d_loc, pot, grad, hess, p.out_scale);
     BFGT_HIP_CHECK(hipGetLastError());
     return BFGT_OK;
 }
@@ -980,7 +985,7 @@ static int launch_down_mfma2(Plan& p, const cplx* tab, int lstride, int ymid_hal
     int blocks = (p.npwleaf + kWavesPerBlock - 1) / kWavesPerBlock;
     k_down_mfma<NTAB, NTP><<<blocks, kWave * kWavesPerBlock, 0, p.stream>>>(
         p.n, p.npw, p.pstride, p.nd, p.npwleaf, p.d_pwleaf, tab, lstride, ymid_half, p.d_phase, p.d_levbase,
-        reinterpret_cast<const Fold*>(p.d_loc), pot, grad, hess);
+        reinterpret_cast<const Fold*>(p.d_loc), pot, grad, hess, p.out_scale);
     BFGT_HIP_CHECK(hipGetLastError());
     return BFGT_OK;
 }
@@ -999,7 +1004,7 @@ static int launch_down_split(Plan& p, double* pot, double* grad, double* hess) {
     int blocks = (p.npwleaf + kWavesPerBlock - 1) / kWavesPerBlock;
     k_down_split<NTAB><<<blocks, kWave * kWavesPerBlock, 0, p.stream>>>(
         p.n, p.npw, p.pstride, p.nd, p.npwleaf, p.d_pwleaf, p.d_tdn_f, 3 * 16 * p.pstride, p.d_phase, p.d_levbase,
-        reinterpret_cast<const Fold*>(p.d_loc), pot, grad, hess);
+        reinterpret_cast<const Fold*>(p.d_loc), pot, grad, hess, p.out_scale);
     BFGT_HIP_CHECK(hipGetLastError());
     return BFGT_OK;
 }

@@ -120,7 +120,37 @@ int launch_targets_n(Plan& p, int ntarg, const double* targs, int ifpghtarg, con
     return BFGT_OK;
 }
 
+// Tensor-grid node coordinates of every leaf, point i + n j at
+// (cx + x_i b/2, cy + x_j b/2) with x the node rule on [-1, 1]: the points
+// at which the callers evaluate their forcing (hpots/hpots2dadap.f:296-306,
+// xtarg = centers + xref * boxsize/2).  Non-leaf boxes get zeros.
+__global__ void k_leaf_points(int nboxes, int n, const TreeNode* __restrict__ nodes, const double* __restrict__ xq,
+                              const double* __restrict__ boxsize, double* __restrict__ xy) {
+    const size_t tot = (size_t)nboxes * n * n;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < tot; e += (size_t)gridDim.x * blockDim.x) {
+        const int b = (int)(e / ((size_t)n * n));
+        const int pt = (int)(e % ((size_t)n * n));
+        const TreeNode nb = nodes[b];
+        double x = 0.0, y = 0.0;
+        if (nb.child[0] < 0) {
+            const double h = 0.5 * boxsize[nb.level];
+            x = nb.cx + xq[pt % n] * h;
+            y = nb.cy + xq[pt / n] * h;
+        }
+        xy[2 * e] = x;
+        xy[2 * e + 1] = y;
+    }
+}
+
 }  // namespace
+
+int launch_leaf_points(Plan& p, double* xy) {
+    size_t tot = (size_t)p.nboxes * p.n * p.n;
+    int blocks = (int)((tot + 255) / 256 < 4096 ? (tot + 255) / 256 : 4096);
+    k_leaf_points<<<blocks, 256, 0, p.stream>>>(p.nboxes, p.n, p.d_tnodes, p.d_nodes, p.d_boxsize, xy);
+    BFGT_HIP_CHECK(hipGetLastError());
+    return BFGT_OK;
+}
 
 int launch_targets(Plan& p, int ntarg, const double* targs, int ifpghtarg, const double* pot, const double* grad,
                    const double* hess, double* pote, double* grade, double* hesse) {

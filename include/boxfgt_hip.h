@@ -114,6 +114,21 @@ int bfgt_plan_eval_targets(bfgt_plan_t plan, int ntarg, const double *d_targs, i
 int bfgt_plan_adjoint(bfgt_plan_t plan, const double *d_gpot, const double *d_ggrad, const double *d_ghess,
                       double *d_fbar, double *timeinfo);
 
+/* pot (+grad, hess) += scale * A fvals: bfgt_plan_execute with the outputs
+ * scaled before they are accumulated.  The callers' time-quadrature loop
+ * multiplies every boxfgt result by dt w_j / (4 pi (dt - t_j)) before adding
+ * it to the volume potential (hpots/hpots2dadap.f:343-354); with one plan per
+ * quadrature node this is one call per node into the same device array, with
+ * no temporary and no host round trip.  Allocates nothing. */
+int bfgt_plan_execute_scaled(bfgt_plan_t plan, double scale, const double *d_fvals, double *d_pot, double *d_grad,
+                             double *d_hess, double *timeinfo);
+
+/* Device coordinates d_xy(2, npbox, nboxes) of the tensor-grid nodes of every
+ * leaf (zeros for non-leaf boxes), point i + n*j at centre + (x_i, x_j) *
+ * boxsize/2 -- where the callers evaluate their forcing term
+ * (hpots/hpots2dadap.f:296-306).  Asynchronous on the plan's stream. */
+int bfgt_plan_leaf_points(bfgt_plan_t plan, double *d_xy);
+
 int bfgt_plan_destroy(bfgt_plan_t plan);
 
 /* Per-stage device timing over a window of executions.  After
