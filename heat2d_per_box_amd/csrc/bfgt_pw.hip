@@ -136,46 +136,66 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_up_mfma(
         for (int p = 0; p < 4; p++)
 #pragma unroll
             for (int t = 0; t < NTM; t++) acc[p][t] = v4d{0, 0, 0, 0};
-        for (int li = 0; li < ch.count; li++) {
-            const PwLeaf lf = leaves[ch.first + li];
-            const cplx* T = tab + (size_t)lf.level * lstride;
-            const cplx* ph = phase + levbase[lf.level];
-            const size_t pb0 = (size_t)lf.box * npbox;
-            // y phase of this lane's column (upward shift: conjugate of the table)
-            cplx py = ph[lf.jy * ps + k2];
-            py.x *= wy;
-            py.y *= -wy;
-            v4d Gr[NTAB], Gi[NTAB];
+        // LP leaves per pass: one leaf's y-to-x dependency and load latency are
+        // covered by the other's work (measured faster for n <= 8 only; at
+        // n = 16 the second leaf's operands crowd the registers).  A pass past
+        // the end of the chunk repeats its last leaf with weight zero (exact
+        // zeros; the sums keep their order).
+        constexpr int LP = (NTAB == 1 && NKS <= 2) ? 2 : 1;
+        for (int li = 0; li < ch.count; li += LP) {
+            PwLeaf lf[LP];
+            double wl[LP];
 #pragma unroll
-            for (int v = 0; v < NTAB; v++) Gr[v] = Gi[v] = v4d{0, 0, 0, 0};
+            for (int h = 0; h < LP; h++) {
+                lf[h] = leaves[ch.first + min(li + h, ch.count - 1)];
+                wl[h] = li + h < ch.count ? 1.0 : 0.0;
+            }
+            v4d Gr[LP][NTAB], Gi[LP][NTAB];
 #pragma unroll
-            for (int s = 0; s < NKS; s++) {
-                const int m2 = 4 * s + q;
-                cplx ty[NTAB];
+            for (int h = 0; h < LP; h++) {
+                const cplx* T = tab + (size_t)lf[h].level * lstride;
+                const cplx* ph = phase + levbase[lf[h].level];
+                const size_t pb0 = (size_t)lf[h].box * npbox;
+                // y phase of this lane's column (upward shift: conjugate of the table)
+                cplx py = ph[lf[h].jy * ps + k2];
+                py.x *= wy;
+                py.y *= -wy;
 #pragma unroll
-                for (int u = 0; u < NTAB; u++) ty[u] = cmul(T[u * tstride + m2 * ps + k2], py);
+                for (int v = 0; v < NTAB; v++) Gr[h][v] = Gi[h][v] = v4d{0, 0, 0, 0};
 #pragma unroll
-                for (int o = 0; o < NOUT; o++) {
-                    double a = leaf_in(in, o, pb0 + pidx[s], nd, d);
-                    Gr[chan_x(o)] = mfma(a, ty[chan_y(o)].x, Gr[chan_x(o)]);
-                    Gi[chan_x(o)] = mfma(a, ty[chan_y(o)].y, Gi[chan_x(o)]);
+                for (int s = 0; s < NKS; s++) {
+                    const int m2 = 4 * s + q;
+                    cplx ty[NTAB];
+#pragma unroll
+                    for (int u = 0; u < NTAB; u++) ty[u] = cmul(T[u * tstride + m2 * ps + k2], py);
+#pragma unroll
+                    for (int o = 0; o < NOUT; o++) {
+                        double a = wl[h] * leaf_in(in, o, pb0 + pidx[s], nd, d);
+                        Gr[h][chan_x(o)] = mfma(a, ty[chan_y(o)].x, Gr[h][chan_x(o)]);
+                        Gi[h][chan_x(o)] = mfma(a, ty[chan_y(o)].y, Gi[h][chan_x(o)]);
+                    }
                 }
             }
 #pragma unroll
             for (int t = 0; t < NTM; t++) {
                 const int k1 = 16 * t + cc;
-                cplx px = ph[lf.jx * ps + k1];
-                px.y = -px.y;
 #pragma unroll
-                for (int v = 0; v < NTAB; v++) {
+                for (int h = 0; h < LP; h++) {
+                    const cplx* T = tab + (size_t)lf[h].level * lstride;
+                    const cplx* ph = phase + levbase[lf[h].level];
+                    cplx px = ph[lf[h].jx * ps + k1];
+                    px.y = -px.y;
 #pragma unroll
-                    for (int s = 0; s < NKS; s++) {
-                        const int m1 = 4 * s + q;
-                        cplx tx = cmul(T[v * tstride + m1 * ps + k1], px);
-                        acc[0][t] = mfma(tx.x, Gr[v][s], acc[0][t]);
-                        acc[1][t] = mfma(tx.y, Gi[v][s], acc[1][t]);
-                        acc[2][t] = mfma(tx.x, Gi[v][s], acc[2][t]);
-                        acc[3][t] = mfma(tx.y, Gr[v][s], acc[3][t]);
+                    for (int v = 0; v < NTAB; v++) {
+#pragma unroll
+                        for (int s = 0; s < NKS; s++) {
+                            const int m1 = 4 * s + q;
+                            cplx tx = cmul(T[v * tstride + m1 * ps + k1], px);
+                            acc[0][t] = mfma(tx.x, Gr[h][v][s], acc[0][t]);
+                            acc[1][t] = mfma(tx.y, Gi[h][v][s], acc[1][t]);
+                            acc[2][t] = mfma(tx.x, Gi[h][v][s], acc[2][t]);
+                            acc[3][t] = mfma(tx.y, Gr[h][v][s], acc[3][t]);
+                        }
                     }
                 }
             }
