@@ -104,6 +104,8 @@ struct Plan {
     // schedules
     int npwleaf = 0, nchunks = 0, nanc = 0, nm2l = 0;
     PwLeaf* d_pwleaf = nullptr;
+    int npair = 0;                 // PW leaves paired by (cutoff box, level) for n <= 8
+    int2* d_pwpair = nullptr;      // (leaf, partner leaf or -1)
     UpChunk* d_chunks = nullptr;
     int* d_anc_box = nullptr;      // ancestor index -> box
     int* d_anc_chunk = nullptr;    // CSR over chunks [nanc+1]

@@ -485,6 +485,16 @@ int plan_create(Plan& p, int nd, int ndim, double delta, double eps, int ipoly, 
         }
     }
     p.npwleaf = (int)leaves.size();
+    // pairs of PW leaves sharing a cutoff box and a level (leaves are grouped by
+    // box and sorted by level within it): for n <= 8 one 16-wide MFMA point
+    // dimension holds both leaves' points
+    std::vector<int2> pairs_pw;
+    for (int i = 0; i < p.npwleaf;) {
+        const bool two = i + 1 < p.npwleaf && leaves[i + 1].anc == leaves[i].anc && leaves[i + 1].level == leaves[i].level;
+        pairs_pw.push_back(make_int2(i, two ? i + 1 : -1));
+        i += two ? 2 : 1;
+    }
+    p.npair = (int)pairs_pw.size();
     p.nchunks = (int)chunks.size();
     p.nm2l = (int)m2l.size();
     p.nred1 = (int)red1.size();
@@ -601,6 +611,7 @@ int plan_create(Plan& p, int nd, int ndim, double delta, double eps, int ipoly, 
     p.nnearT_s = (int)nsrc.size();
 
     if ((rc = upload(p, p.d_pwleaf, leaves))) return rc;
+    if ((rc = upload(p, p.d_pwpair, pairs_pw))) return rc;
     if ((rc = upload(p, p.d_chunks, chunks))) return rc;
     if ((rc = upload(p, p.d_anc_box, anc_box))) return rc;
     if ((rc = upload(p, p.d_anc_chunk, anc_chunk))) return rc;

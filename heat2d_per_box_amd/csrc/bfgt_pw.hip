@@ -338,6 +338,115 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_down_mfma(
     }
 }
 
+// Downward pass for leaf orders n <= 8: one wave per PAIR of PW leaves with
+// the same cutoff box and level (a missing partner is a zero-weight copy).
+// The 16-wide point dimension holds leaf 0's points in lanes 0-7 and leaf 1's
+// in lanes 8-15: the x contraction shares the folded expansion (same box)
+// and the tables (same level) and differs only in the phase each lane's leaf
+// applies; the y contraction's output tile holds the two leaves' diagonal
+// 8 x 8 blocks (the off-diagonal blocks mix the leaves and are not stored).
+template <int NTAB, int NTP>
+__global__ __launch_bounds__(kWave * kWavesPerBlock) void k_down_pair(
+    int n, int P, int ps, int nd, int npair, const int2* __restrict__ pairs, const PwLeaf* __restrict__ leaves,
+    const cplx* __restrict__ tab, int lstride, int ymid_half, const cplx* __restrict__ phase,
+    const int* __restrict__ levbase, const Fold* __restrict__ fold, double* __restrict__ pot,
+    double* __restrict__ grad, double* __restrict__ hess, double oscale) {
+    constexpr int NOUT = nchan(NTAB);
+    const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    const int item = __builtin_amdgcn_readfirstlane(xcd_block(blockIdx.x, gridDim.x) * kWavesPerBlock + wave);
+    if (item >= npair) return;  // no block-wide barriers in this kernel
+    const int q = lane >> 4, cc = lane & 15;
+    const int half = cc >> 3, ci = cc & 7;
+    const int H = (P + 1) / 2, P2 = P / 2, npbox = n * n;
+    const int Mh4 = round_up(H, 4), H16 = round_up(H, 16);
+    const int nks1 = Mh4 / 4, nT = H16 / 16;
+    const int tstride = 16 * ps;
+    const int2 pr = pairs[item];
+    const bool two = pr.y >= 0;
+    const PwLeaf lf0 = leaves[pr.x];
+    const PwLeaf lfh = leaves[(half && two) ? pr.y : pr.x];  // this lane's leaf
+    const double wh = (half && !two) ? 0.0 : 1.0;          // zero weight for a missing partner
+    const cplx* E = tab + (size_t)lf0.level * lstride;     // [t][m][i], shared level
+    const cplx* ph = phase + levbase[lf0.level];
+    const cplx* phx = ph + lfh.jx * ps;
+    const cplx* phy = ph + lfh.jy * ps;
+    for (int d = 0; d < nd; d++) {
+        const Fold* F = fold + ((size_t)lf0.anc * nd + d) * Mh4 * H16;
+        v4d out[NOUT];
+#pragma unroll
+        for (int o = 0; o < NOUT; o++) out[o] = v4d{0, 0, 0, 0};
+        for (int tg = 0; tg < nT; tg += NTP) {
+            v4d FFr[NTP][NTAB], FFi[NTP][NTAB];
+#pragma unroll
+            for (int p = 0; p < NTP; p++)
+#pragma unroll
+                for (int t = 0; t < NTAB; t++) FFr[p][t] = FFi[p][t] = v4d{0, 0, 0, 0};
+            for (int s = 0; s < nks1; s++) {
+                const int m = 4 * s + q;
+                Fold sd[NTP];
+#pragma unroll
+                for (int p = 0; p < NTP; p++) sd[p] = F[m * H16 + 16 * (tg + p) + cc];
+                cplx px = phx[m];
+                px.x *= wh;
+                px.y *= wh;
+#pragma unroll
+                for (int t = 0; t < NTAB; t++) {
+                    cplx e = cmul(E[t * tstride + m * 16 + ci], px);
+#pragma unroll
+                    for (int p = 0; p < NTP; p++) {
+                        FFr[p][t] = mfma(sd[p].S.x, e.x, FFr[p][t]);
+                        FFi[p][t] = mfma(sd[p].S.y, e.x, FFi[p][t]);
+                        FFr[p][t] = mfma(-sd[p].D.y, e.y, FFr[p][t]);
+                        FFi[p][t] = mfma(sd[p].D.x, e.y, FFi[p][t]);
+                    }
+                }
+            }
+#pragma unroll
+            for (int p = 0; p < NTP; p++) {
+#pragma unroll
+                for (int s = 0; s < 4; s++) {
+                    const int m2s = 16 * (tg + p) + 4 * s + q;
+                    if (16 * (tg + p) + 4 * s >= H) break;  // rows of this step are all padding (uniform)
+                    cplx pyw = phy[m2s];  // row k2' = cc: this lane's leaf
+                    const double w = (ymid_half && m2s >= P2) ? 0.5 : 1.0;
+                    pyw.x *= w;
+                    pyw.y *= w;
+                    cplx eu[NTAB];
+#pragma unroll
+                    for (int u = 0; u < NTAB; u++) eu[u] = cmul(E[u * tstride + m2s * 16 + ci], pyw);
+#pragma unroll
+                    for (int o = 0; o < NOUT; o++) out[o] = mfma(eu[chan_y(o)].x, FFr[p][chan_x(o)][s], out[o]);
+#pragma unroll
+                    for (int o = 0; o < NOUT; o++) out[o] = mfma(-eu[chan_y(o)].y, FFi[p][chan_x(o)][s], out[o]);
+                }
+            }
+        }
+        // out^T holds [k2' = drow(q, r)][k1' = cc]; a leaf's block has k2' and cc in the same half
+        if (ci < n && (half == 0 || two)) {
+            const PwLeaf& lfo = lfh;
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int k2p = drow(q, r);
+                const int k2 = k2p & 7;
+                if ((k2p >> 3) == half && k2 < n) {
+                    size_t pb = (size_t)lfo.box * npbox + ci + (size_t)n * k2;
+                    const double w2 = 2.0 * oscale;
+                    pot[pb * nd + d] += w2 * out[0][r];
+                    if constexpr (NTAB >= 2) {
+                        grad[(pb * 2 + 0) * nd + d] += w2 * out[1][r];
+                        grad[(pb * 2 + 1) * nd + d] += w2 * out[2][r];
+                    }
+                    if constexpr (NTAB >= 3) {
+                        hess[(pb * 3 + 0) * nd + d] += w2 * out[3][r];
+                        hess[(pb * 3 + 1) * nd + d] += w2 * out[4][r];
+                        hess[(pb * 3 + 2) * nd + d] += w2 * out[5][r];
+                    }
+                }
+            }
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // split-FP32 variants of the leaf transforms (v_mfma_f32_16x16x4f32)
 // ---------------------------------------------------------------------------
@@ -1010,11 +1119,26 @@ static int launch_down_mfma2(Plan& p, const cplx* tab, int lstride, int ymid_hal
     return BFGT_OK;
 }
 
-// two y tiles per pass when their count is even
+template <int NTAB, int NTP>
+static int launch_down_pair(Plan& p, const cplx* tab, int lstride, int ymid_half, double* pot, double* grad,
+                            double* hess) {
+    int blocks = (p.npair + kWavesPerBlock - 1) / kWavesPerBlock;
+    k_down_pair<NTAB, NTP><<<blocks, kWave * kWavesPerBlock, 0, p.stream>>>(
+        p.n, p.npw, p.pstride, p.nd, p.npair, p.d_pwpair, p.d_pwleaf, tab, lstride, ymid_half, p.d_phase,
+        p.d_levbase, reinterpret_cast<const Fold*>(p.d_loc), pot, grad, hess, p.out_scale);
+    BFGT_HIP_CHECK(hipGetLastError());
+    return BFGT_OK;
+}
+
+// two y tiles per pass when their count is even; leaf pairs for n <= 8
 template <int NTAB>
 static int launch_down_mfma(Plan& p, const cplx* tab, int lstride, int ymid_half, double* pot, double* grad,
                             double* hess) {
     int nT = round_up((p.npw + 1) / 2, 16) / 16;
+    if (p.n <= 8) {
+        if (nT % 2 == 0) return launch_down_pair<NTAB, 2>(p, tab, lstride, ymid_half, pot, grad, hess);
+        return launch_down_pair<NTAB, 1>(p, tab, lstride, ymid_half, pot, grad, hess);
+    }
     if (nT % 2 == 0) return launch_down_mfma2<NTAB, 2>(p, tab, lstride, ymid_half, pot, grad, hess);
     return launch_down_mfma2<NTAB, 1>(p, tab, lstride, ymid_half, pot, grad, hess);
 }
