@@ -481,6 +481,8 @@ int plan_create(Plan& p, int nd, int ndim, double delta, double eps, int ipoly, 
                 if (k < 0) return set_error(BFGT_ERR_TREE, "PW-list partner is not adjacent");
                 m2l.push_back(make_int2(jb - a0, k));
             }
+            if ((int)m2l.size() - m2l_ptr.back() > 8)
+                return set_error(BFGT_ERR_TREE, "more than 8 PW-list partners (colleague list corrupt)");
             m2l_ptr.push_back((int)m2l.size());
         }
     }
@@ -583,6 +585,7 @@ int plan_create(Plan& p, int nd, int ndim, double delta, double eps, int ipoly, 
             for (int j = m2l_ptr[a]; j < m2l_ptr[a + 1]; j++) byc[m2l[j].x].push_back(make_int2(a, m2l[j].y));
         m2lT_ptr.push_back(0);
         for (int c = 0; c < p.nanc; c++) {
+            if (byc[c].size() > 8) return set_error(BFGT_ERR_TREE, "more than 8 transposed PW-list partners");
             m2lT.insert(m2lT.end(), byc[c].begin(), byc[c].end());
             m2lT_ptr.push_back((int)m2lT.size());
         }

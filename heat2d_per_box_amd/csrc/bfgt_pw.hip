@@ -915,19 +915,37 @@ __global__ void k_m2l_fold(int nd, int P, int gy, const int* __restrict__ m2l_pt
     const int len = nd * nexp;
     const int e = (lb % gy) * blockDim.x + threadIdx.x;
     if (e >= nd * H * H) return;
-    const int j0 = m2l_ptr[a], j1 = m2l_ptr[a + 1];
     const int d = e / (H * H), r = e % (H * H);
     const int m = r / H, k2 = r % H;
-    const int q1 = m * H + k2, mm = P - 1 - m;
-    cplx l1 = m2l_at(wshift, mp, nexp, len, a, d * nexp + q1, j0, j1, m2l, q1, csign);
-    Fold f{{0, 0}, {0, 0}};
+    const int mm = P - 1 - m;
+    const int q1 = m * H + k2, q2 = mm * H + k2;
+    const size_t o1 = (size_t)d * nexp + q1, o2 = (size_t)d * nexp + q2;
+    // the shift to a neighbour offset (kx, ky) in {-1,0,1}^2 factors as
+    // ex^kx(m) * ey^ky(k2) with ex = the (1,0) shift and ey = the (0,1) shift
+    // (an offset of -1 is the conjugate): three loads replace one per partner
+    const cplx ex1 = wshift[(size_t)5 * nexp + m * H], ex2 = wshift[(size_t)5 * nexp + mm * H];
+    const cplx ey = wshift[(size_t)7 * nexp + k2];
+    cplx l1 = mp[(size_t)a * len + o1], l2 = mp[(size_t)a * len + o2];
+    const int j0 = m2l_ptr[a], j1 = m2l_ptr[a + 1];
+    for (int j = j0; j < j1; j++) {
+        const int2 sj = m2l[j];
+        const int kx = sj.y % 3 - 1, ky = sj.y / 3 - 1;
+        const cplx wy = ky == 0 ? cplx{1.0, 0.0} : cplx{ey.x, ky * ey.y};
+        const cplx wx1 = kx == 0 ? cplx{1.0, 0.0} : cplx{ex1.x, kx * ex1.y};
+        const cplx wx2 = kx == 0 ? cplx{1.0, 0.0} : cplx{ex2.x, kx * ex2.y};
+        cplx w1 = cmul(wx1, wy), w2 = cmul(wx2, wy);
+        w1.y *= csign;
+        w2.y *= csign;
+        l1 = cfma(w1, mp[(size_t)sj.x * len + o1], l1);
+        l2 = cfma(w2, mp[(size_t)sj.x * len + o2], l2);
+    }
+    Fold f;
     if (mm != m) {
-        int q2 = mm * H + k2;
-        cplx l2 = m2l_at(wshift, mp, nexp, len, a, d * nexp + q2, j0, j1, m2l, q2, csign);
         f.S = cplx{l1.x + l2.x, l1.y + l2.y};
         f.D = cplx{l1.x - l2.x, l1.y - l2.y};
     } else {
         f.S = l1;
+        f.D = cplx{0, 0};
     }
     fold[(size_t)a * nd * Mh4 * H16 + ((size_t)d * Mh4 + m) * H16 + k2] = f;
 }
