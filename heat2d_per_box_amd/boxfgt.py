@@ -132,7 +132,8 @@ class BoxFGT:
                 "precision"]
         self.info = PlanInfo({k: int(v) for k, v in zip(keys, iv)})
         self.info.update({"delta0": dv[0], "pmax": dv[1], "t_plan_host": dv[2], "t_tables": dv[3],
-                          "mfma": bool(dv[4]), "n_m2l": int(dv[5]), "n_near_sources": int(dv[6])})
+                          "mfma": bool(dv[4]), "n_m2l": int(dv[5]), "n_near_sources": int(dv[6]),
+                          "translation_group": int(dv[7])})
 
     def _check_tensor(self, t, shape, name):
         import torch

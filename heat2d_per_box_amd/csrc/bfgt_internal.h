@@ -144,6 +144,13 @@ struct Plan {
     bool has_adjoint = false;
     int* d_m2lT_ptr = nullptr;     // CSR [nanc+1]
     int2* d_m2lT = nullptr;        // (target ancestor, shift index)
+    // both lists regrouped by m2l_group (1 or 4) consecutive cutoff boxes: CSR
+    // [ngroups+1] of (source, one 4-bit shift index per target, 15 = none)
+    int m2l_group = 1;
+    int* d_m2lg_ptr = nullptr;
+    int2* d_m2lg = nullptr;
+    int* d_m2lgT_ptr = nullptr;
+    int2* d_m2lgT = nullptr;
     int nnearT_s = 0;
     int* d_nearT_src = nullptr;    // source boxes
     int* d_nearT_ptr = nullptr;    // CSR [nnearT_s+1]

@@ -208,6 +208,35 @@ int pwshift_index(int iperiod, const double* tc, const double* sc, double bs0, d
     return (jx[0] + 1) + 3 * (jx[1] + 1);
 }
 
+// Translation lists regrouped by G consecutive cutoff boxes (G = 4: the four
+// children of one parent, since every refined box has four consecutive
+// children): one entry per distinct source expansion, holding the shift index
+// of its contribution to each target of the group in a 4-bit field (15 =
+// none, 4 = the box itself).  A source that reaches one target under two
+// shifts (periodic images on a coarse level) gets a second entry.
+void group_lists(int nanc, int G, const std::vector<int>& ptr, const std::vector<int2>& list, std::vector<int>& gptr,
+                 std::vector<int2>& glist) {
+    gptr.assign(1, 0);
+    glist.clear();
+    for (int g0 = 0; g0 < nanc; g0 += G) {
+        const size_t base = glist.size();
+        auto add = [&](int src, int slot, int k) {
+            const int sh = 4 * slot;
+            for (size_t i = base; i < glist.size(); i++)
+                if (glist[i].x == src && ((glist[i].y >> sh) & 15) == 15) {
+                    glist[i].y = (glist[i].y & ~(15 << sh)) | (k << sh);
+                    return;
+                }
+            glist.push_back(make_int2(src, (0xFFFF & ~(15 << sh)) | (k << sh)));
+        };
+        for (int t = 0; t < G && g0 + t < nanc; t++) {
+            add(g0 + t, t, 4);
+            for (int j = ptr[g0 + t]; j < ptr[g0 + t + 1]; j++) add(list[j].x, t, list[j].y);
+        }
+        gptr.push_back((int)glist.size());
+    }
+}
+
 // near-field table offsets (gnd_find_loctab_ind, bfgt/bfgt_local.f:1729-1765)
 void loctab_index(int iperiod, const double* tc, const double* sc, double sbs, double bs0, int* ixy) {
     double bs = sbs / 4;
@@ -640,6 +669,20 @@ int plan_create(Plan& p, int nd, int ndim, double delta, double eps, int ipoly, 
     }
     if ((rc = upload(p, p.d_m2lT_ptr, m2lT_ptr))) return rc;
     if ((rc = upload(p, p.d_m2lT, m2lT))) return rc;
+    if (p.has_pw) {
+        std::vector<int> gptr;
+        std::vector<int2> glist;
+        // groups of four read each shared neighbour once, but give a quarter
+        // of the workgroups: used only where that still fills the chip
+        const int H = (p.npw + 1) / 2;
+        p.m2l_group = (long)((p.nanc + 3) / 4) * ((p.nd * H * H + 255) / 256) >= 2048 ? 4 : 1;
+        group_lists(p.nanc, p.m2l_group, m2l_ptr, m2l, gptr, glist);
+        if ((rc = upload(p, p.d_m2lg_ptr, gptr))) return rc;
+        if ((rc = upload(p, p.d_m2lg, glist))) return rc;
+        group_lists(p.nanc, p.m2l_group, m2lT_ptr, m2lT, gptr, glist);
+        if ((rc = upload(p, p.d_m2lgT_ptr, gptr))) return rc;
+        if ((rc = upload(p, p.d_m2lgT, glist))) return rc;
+    }
     if ((rc = upload(p, p.d_nearT_src, nsrc))) return rc;
     if ((rc = upload(p, p.d_nearT_ptr, nsptr))) return rc;
     if ((rc = upload(p, p.d_nearT, pairsT))) return rc;
@@ -868,7 +911,7 @@ int bfgt_plan_query(bfgt_plan_t plan, int* iv, double* dv) {
     }
     if (dv) {
         double v[8] = {p.delta0, p.pmax, p.t_plan_host, p.t_tables, bfgt::use_mfma(p) ? 1.0 : 0.0, (double)p.nm2l,
-                       (double)p.nnearT_s, 0};
+                       (double)p.nnearT_s, (double)p.m2l_group};
         std::memcpy(dv, v, sizeof(v));
     }
     return BFGT_OK;

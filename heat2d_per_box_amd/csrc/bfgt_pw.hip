@@ -901,15 +901,18 @@ __global__ void k_m2l(int nd, int nexp, const int* __restrict__ m2l_ptr, const i
 }
 
 // Same translation, written mirror-folded (struct Fold) for the MFMA
-// downward pass.  Only the H x H valid entries of the padded layout are
+// downward pass.  A workgroup serves G consecutive cutoff boxes (G = 4: four
+// siblings, whose neighbours overlap) and reads each distinct source
+// expansion once.  Only the H x H valid entries of the padded layout are
 // written: the padding is zeroed once when the plan is built and never
 // touched again.  Workgroups are mapped XCD-contiguously, so the expansions
 // of neighbouring cutoff boxes are read from one L2.
-__global__ void k_m2l_fold(int nd, int P, int gy, const int* __restrict__ m2l_ptr, const int2* __restrict__ m2l,
-                           const cplx* __restrict__ wshift, double csign, const cplx* __restrict__ mp,
-                           Fold* __restrict__ fold) {
+template <int G>
+__global__ void k_m2l_fold(int nd, int P, int gy, int nanc, const int* __restrict__ gptr,
+                           const int2* __restrict__ glist, const cplx* __restrict__ wshift, double csign,
+                           const cplx* __restrict__ mp, Fold* __restrict__ fold) {
     const int lb = xcd_block(blockIdx.x, gridDim.x);
-    const int a = lb / gy;
+    const int g = lb / gy;
     const int H = (P + 1) / 2, nexp = P * H;
     const int Mh4 = round_up(H, 4), H16 = round_up(H, 16);
     const int len = nd * nexp;
@@ -925,29 +928,43 @@ __global__ void k_m2l_fold(int nd, int P, int gy, const int* __restrict__ m2l_pt
     // (an offset of -1 is the conjugate): three loads replace one per partner
     const cplx ex1 = wshift[(size_t)5 * nexp + m * H], ex2 = wshift[(size_t)5 * nexp + mm * H];
     const cplx ey = wshift[(size_t)7 * nexp + k2];
-    cplx l1 = mp[(size_t)a * len + o1], l2 = mp[(size_t)a * len + o2];
-    const int j0 = m2l_ptr[a], j1 = m2l_ptr[a + 1];
+    cplx l1[G], l2[G];
+#pragma unroll
+    for (int t = 0; t < G; t++) l1[t] = l2[t] = cplx{0.0, 0.0};
+    // each distinct source of the group's G boxes is read once
+    const int j0 = gptr[g], j1 = gptr[g + 1];
     for (int j = j0; j < j1; j++) {
-        const int2 sj = m2l[j];
-        const int kx = sj.y % 3 - 1, ky = sj.y / 3 - 1;
-        const cplx wy = ky == 0 ? cplx{1.0, 0.0} : cplx{ey.x, ky * ey.y};
-        const cplx wx1 = kx == 0 ? cplx{1.0, 0.0} : cplx{ex1.x, kx * ex1.y};
-        const cplx wx2 = kx == 0 ? cplx{1.0, 0.0} : cplx{ex2.x, kx * ex2.y};
-        cplx w1 = cmul(wx1, wy), w2 = cmul(wx2, wy);
-        w1.y *= csign;
-        w2.y *= csign;
-        l1 = cfma(w1, mp[(size_t)sj.x * len + o1], l1);
-        l2 = cfma(w2, mp[(size_t)sj.x * len + o2], l2);
+        const int2 sj = glist[j];
+        const cplx v1 = mp[(size_t)sj.x * len + o1], v2 = mp[(size_t)sj.x * len + o2];
+#pragma unroll
+        for (int t = 0; t < G; t++) {
+            const int k = (sj.y >> (4 * t)) & 15;
+            if (k == 15) continue;
+            const int kx = k % 3 - 1, ky = k / 3 - 1;
+            const cplx wy = ky == 0 ? cplx{1.0, 0.0} : cplx{ey.x, ky * ey.y};
+            const cplx wx1 = kx == 0 ? cplx{1.0, 0.0} : cplx{ex1.x, kx * ex1.y};
+            const cplx wx2 = kx == 0 ? cplx{1.0, 0.0} : cplx{ex2.x, kx * ex2.y};
+            cplx w1 = cmul(wx1, wy), w2 = cmul(wx2, wy);
+            w1.y *= csign;
+            w2.y *= csign;
+            l1[t] = cfma(w1, v1, l1[t]);
+            l2[t] = cfma(w2, v2, l2[t]);
+        }
     }
-    Fold f;
-    if (mm != m) {
-        f.S = cplx{l1.x + l2.x, l1.y + l2.y};
-        f.D = cplx{l1.x - l2.x, l1.y - l2.y};
-    } else {
-        f.S = l1;
-        f.D = cplx{0, 0};
+#pragma unroll
+    for (int t = 0; t < G; t++) {
+        const int a = G * g + t;
+        if (a >= nanc) break;
+        Fold f;
+        if (mm != m) {
+            f.S = cplx{l1[t].x + l2[t].x, l1[t].y + l2[t].y};
+            f.D = cplx{l1[t].x - l2[t].x, l1[t].y - l2[t].y};
+        } else {
+            f.S = l1[t];
+            f.D = cplx{0, 0};
+        }
+        fold[(size_t)a * nd * Mh4 * H16 + ((size_t)d * Mh4 + m) * H16 + k2] = f;
     }
-    fold[(size_t)a * nd * Mh4 * H16 + ((size_t)d * Mh4 + m) * H16 + k2] = f;
 }
 
 }  // namespace
@@ -1098,17 +1115,19 @@ int launch_reduce(Plan& p) {
     return BFGT_OK;
 }
 
-static int launch_m2l_fold(Plan& p, const int* ptr, const int2* list, double csign) {
+static int launch_m2l_fold(Plan& p, const int* gptr, const int2* glist, double csign) {
     int H = (p.npw + 1) / 2;
     int gy = (p.nd * H * H + 255) / 256;
-    k_m2l_fold<<<p.nanc * gy, 256, 0, p.stream>>>(p.nd, p.npw, gy, ptr, list, p.d_wshift, csign, p.d_mp,
-                                                  reinterpret_cast<Fold*>(p.d_loc));
+    const int G = p.m2l_group, ngroups = (p.nanc + G - 1) / G;
+    auto kern = G == 4 ? k_m2l_fold<4> : k_m2l_fold<1>;
+    kern<<<ngroups * gy, 256, 0, p.stream>>>(p.nd, p.npw, gy, p.nanc, gptr, glist, p.d_wshift, csign, p.d_mp,
+                                             reinterpret_cast<Fold*>(p.d_loc));
     BFGT_HIP_CHECK(hipGetLastError());
     return BFGT_OK;
 }
 
 int launch_m2l(Plan& p) {
-    if (use_mfma(p)) return launch_m2l_fold(p, p.d_m2l_ptr, p.d_m2l, 1.0);
+    if (use_mfma(p)) return launch_m2l_fold(p, p.d_m2lg_ptr, p.d_m2lg, 1.0);
     int len = p.nd * p.nexp;
     int gy = (len + 255) / 256;
     k_m2l<<<dim3(p.nanc, gy), 256, 0, p.stream>>>(p.nd, p.nexp, p.d_m2l_ptr, p.d_m2l, p.d_wshift, p.d_mp, p.d_loc);
@@ -1204,7 +1223,7 @@ int launch_adj_eval(Plan& p, const double* gpot, const double* ggrad, const doub
 }
 
 // local-expansion cotangent (in d_mp) -> multipole cotangent, mirror-folded in d_loc
-int launch_adj_m2l(Plan& p) { return launch_m2l_fold(p, p.d_m2lT_ptr, p.d_m2lT, -1.0); }
+int launch_adj_m2l(Plan& p) { return launch_m2l_fold(p, p.d_m2lgT_ptr, p.d_m2lgT, -1.0); }
 
 // multipole cotangent -> leaf-value cotangent at PW leaves (accumulated)
 int launch_adj_transform(Plan& p, double* fbar) {

@@ -146,7 +146,8 @@ int bfgt_plan_profile_end(bfgt_plan_t plan, double *stage_seconds, int *nsteps);
  * n_pw_leaves, n_chunks, n_ancestors, n_near_targets, n_near_pairs, has_pw,
  * has_near, iperiod_eff, nboxes, precision}; dvals[0..7] = {delta0, pmax,
  * t_plan_host_s, t_tables_s, matrix_core_path (1.0 / 0.0), n_translation_pairs,
- * n_near_sources, 0}. */
+ * n_near_sources, translation_group (cutoff boxes per translation workgroup:
+ * 1 or 4)}. */
 int bfgt_plan_query(bfgt_plan_t plan, int *ivals, double *dvals);
 
 /* Copy one device table of one level to host memory (parity tests of the FP64

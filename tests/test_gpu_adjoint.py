@@ -94,6 +94,14 @@ def test_dot_product_identity(device, name, scale, kw, ifpgh):
     assert err < 1e-13, (err, info)
 
 
+def test_dot_product_identity_grouped_translation(device):
+    """The transposed translation lists in groups of four (2,676 cutoff boxes)."""
+    w = make_workload("c3", "full", delta=1e-5, eps=1e-9)
+    err, info = _dot_identity(w.tree, w.nd, w.delta, w.eps, w.ipoly, w.iperiod, w.norder, 3)
+    assert info["translation_group"] == 4
+    assert err < 1e-13, (err, info)
+
+
 @pytest.mark.parametrize("seed", range(4))
 def test_dot_product_random_trees(device, seed):
     rng = np.random.default_rng(700 + seed)

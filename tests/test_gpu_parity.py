@@ -82,6 +82,16 @@ def test_workload_matches_oracle(device, name, scale, kw):
     assert_close(compare(w.tree, got, ref, w.ifpgh))
 
 
+def test_grouped_translation_matches_oracle(device):
+    """2,676 cutoff boxes: the translation runs in groups of four siblings,
+    each distinct neighbour expansion read once per group."""
+    w = make_workload("c3", "full", delta=1e-5, eps=1e-9)
+    ref = O.oracle_boxfgt(w.tree, w.fvals, w.nd, w.delta, w.eps, w.ipoly, w.iperiod, w.norder, w.ifpgh)
+    got = run_gpu(w.tree, w.fvals, w.nd, w.delta, w.eps, w.ipoly, w.iperiod, w.norder, w.ifpgh)
+    assert got["plan"].info["translation_group"] == 4
+    assert_close(compare(w.tree, got, ref, w.ifpgh))
+
+
 @pytest.mark.skipif(not O.reference_available(), reason="reference library not built")
 @pytest.mark.parametrize("name,kw", [("c1", {}), ("c3", {"delta": 1e-5, "eps": 1e-9}), ("c3", {"delta": 1e-1, "eps": 1e-9})])
 def test_workload_matches_reference(device, name, kw):
