@@ -33,7 +33,8 @@ struct PwLeaf {
     double dx, dy;
 };
 
-// A group of PW leaves sharing one ancestor; partial sums go to slot `slot`.
+// A group of PW leaves sharing one ancestor; its sum goes to slot `slot` of the
+// expansion buffer (the ancestor's own mp slot when it is its only chunk).
 struct UpChunk {
     int anc;
     int first;  // into the PwLeaf array
@@ -113,7 +114,7 @@ struct Plan {
     // directly when an ancestor has one group), level 2 sums the groups of
     // ancestors with several
     int nred1 = 0, nred2 = 0, ngslot = 0;
-    int4* d_red1 = nullptr;        // (anc, first chunk, count, dest slot or -1 for mp)
+    int4* d_red1 = nullptr;        // (anc, first chunk slot, count, dest group slot or -1 for mp)
     int4* d_red2 = nullptr;        // (anc, first group slot, count, 0)
     cplx* d_gpart = nullptr;       // [ngslot][nd][nexp]
     int* d_m2l_ptr = nullptr;      // CSR [nanc+1]
@@ -157,8 +158,10 @@ struct Plan {
     NearPairT* d_nearT = nullptr;
 
     // work buffers
-    cplx* d_partial = nullptr;  // [nchunks][nd][nexp]
-    cplx* d_mp = nullptr;       // [nanc][nd][nexp]
+    // [nmpslot][nd][nexp]: the cutoff boxes' mp in slots [0, nanc), then the
+    // chunk partial sums of boxes with more than one chunk
+    cplx* d_mp = nullptr;
+    int nmpslot = 0;
     cplx* d_loc = nullptr;      // [nanc][nd][nexp]
 
     // timing

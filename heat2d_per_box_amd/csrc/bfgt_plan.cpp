@@ -486,16 +486,31 @@ int plan_create(Plan& p, int nd, int ndim, double delta, double eps, int ipoly, 
             anc_chunk.push_back((int)chunks.size());
             anc_box.push_back(a0 + a);
         }
-        // two-level merge schedule of the chunk partial sums
+        // chunk output slots of the expansion buffer: the only chunk of a
+        // cutoff box is written straight into the box's mp slot (slots
+        // [0, nanc)); the chunks of a box with several get slots from nanc on
+        // and are merged.  A box with no chunk keeps the zero mp written when
+        // the plan is built.
+        int nslot = p.nanc;
+        for (int a = 0; a < p.nanc; a++) {
+            int c0 = anc_chunk[a], c1 = anc_chunk[a + 1];
+            if (c1 - c0 == 1) chunks[c0].slot = a;
+            else
+                for (int c = c0; c < c1; c++) chunks[c].slot = nslot++;
+        }
+        p.nmpslot = nslot;
+        // two-level merge schedule of the chunk partial sums (consecutive slots)
         for (int a = 0; a < p.nanc; a++) {
             int c0 = anc_chunk[a], c1 = anc_chunk[a + 1], nc = c1 - c0;
+            if (nc <= 1) continue;
             if (nc <= 8) {
-                red1.push_back(make_int4(a, c0, nc, -1));
+                red1.push_back(make_int4(a, chunks[c0].slot, nc, -1));
                 continue;
             }
             int G = std::max(8, (int)std::ceil(std::sqrt((double)nc)));
             int g0 = ngslot;
-            for (int c = c0; c < c1; c += G) red1.push_back(make_int4(a, c, std::min(G, c1 - c), ngslot++));
+            for (int c = c0; c < c1; c += G)
+                red1.push_back(make_int4(a, chunks[c].slot, std::min(G, c1 - c), ngslot++));
             red2.push_back(make_int4(a, g0, ngslot - g0, 0));
         }
         // PW translation lists at the cutoff level (gnd_compute_listpw)
@@ -689,9 +704,9 @@ int plan_create(Plan& p, int nd, int ndim, double delta, double eps, int ipoly, 
     p.has_adjoint = !p.has_pw || use_mfma(p);
     if (p.has_pw) {
         size_t len = (size_t)nd * p.nexp;
-        if ((rc = dalloc(p, p.d_partial, (size_t)p.nchunks * len))) return rc;
         if ((rc = dalloc(p, p.d_gpart, (size_t)p.ngslot * len))) return rc;
-        if ((rc = dalloc(p, p.d_mp, (size_t)p.nanc * len))) return rc;
+        if ((rc = dalloc(p, p.d_mp, (size_t)p.nmpslot * len))) return rc;
+        BFGT_HIP_CHECK(hipMemsetAsync(p.d_mp, 0, (size_t)p.nmpslot * len * sizeof(cplx), p.stream));
         size_t nloc = (size_t)p.nanc * std::max(loc_buffer_cplx(p), 2 * fold_len(p));
         if ((rc = dalloc(p, p.d_loc, nloc))) return rc;
         // the folded layout's padding is never written by the translation: zero it once

@@ -855,8 +855,9 @@ __device__ inline cplx sum_blocks(const cplx* __restrict__ src, size_t len, int 
     return cplx{(r0 + r1) + (r2 + r3), (i0 + i1) + (i2 + i3)};
 }
 
-// Merge, level 1: each group of chunk partial sums of one cutoff box is summed
-// into the box's mp (dest < 0) or into a group slot.
+// Merge, level 1: each group of chunk partial sums of one cutoff box (slots
+// past the mp slots of the same buffer) is summed into the box's mp (dest <
+// 0) or into a group slot.
 __global__ void k_reduce1(int len, const int4* __restrict__ groups, const cplx* __restrict__ partial,
                           cplx* __restrict__ gpart, cplx* __restrict__ mp) {
     const int4 g = groups[blockIdx.x];
@@ -999,7 +1000,7 @@ static int launch_up_vec(Plan& p, const double* fvals, size_t sh) {
     if (sh > 65536)
         BFGT_HIP_CHECK(hipFuncSetAttribute((const void*)k_up_vec<E>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
     k_up_vec<E><<<p.nchunks, kVecThreads, sh, p.stream>>>(p.n, p.npw, p.nd, fvals, p.d_pwleaf, p.d_chunks, p.d_tpoly,
-                                                          p.d_ts, p.d_partial);
+                                                          p.d_ts, p.d_mp);
     BFGT_HIP_CHECK(hipGetLastError());
     return BFGT_OK;
 }
@@ -1017,7 +1018,7 @@ static int launch_up_mfma3(Plan& p, const UpArgs& u, int nTN) {
     int blocks = (nitems + kWavesPerBlock - 1) / kWavesPerBlock;
     k_up_mfma<NKS, NTM, NTAB><<<blocks, kWave * kWavesPerBlock, 0, p.stream>>>(
         p.n, p.npw, p.pstride, p.nd, nTN, nitems, u.in, p.d_pwleaf, p.d_chunks, u.tab, u.lstride, u.ymid_half,
-        p.d_phase, p.d_levbase, p.d_partial);
+        p.d_phase, p.d_levbase, p.d_mp);
     BFGT_HIP_CHECK(hipGetLastError());
     return BFGT_OK;
 }
@@ -1058,7 +1059,7 @@ static int launch_up_split2(Plan& p, const double* fvals, int nTN) {
     int blocks = (nitems + kWavesPerBlock - 1) / kWavesPerBlock;
     k_up_split<NKS, NTM><<<blocks, kWave * kWavesPerBlock, 0, p.stream>>>(
         p.n, p.npw, p.pstride, p.nd, nTN, nitems, fvals, p.d_pwleaf, p.d_chunks, p.d_tup_f, 16 * p.pstride,
-        p.d_phase, p.d_levbase, p.d_partial);
+        p.d_phase, p.d_levbase, p.d_mp);
     BFGT_HIP_CHECK(hipGetLastError());
     return BFGT_OK;
 }
@@ -1105,7 +1106,7 @@ int launch_reduce(Plan& p) {
     int len = p.nd * p.nexp;
     int gy = (len + 255) / 256;
     if (p.nred1 > 0) {
-        k_reduce1<<<dim3(p.nred1, gy), 256, 0, p.stream>>>(len, p.d_red1, p.d_partial, p.d_gpart, p.d_mp);
+        k_reduce1<<<dim3(p.nred1, gy), 256, 0, p.stream>>>(len, p.d_red1, p.d_mp, p.d_gpart, p.d_mp);
         BFGT_HIP_CHECK(hipGetLastError());
     }
     if (p.nred2 > 0) {
