@@ -470,15 +470,24 @@ int plan_create(Plan& p, int nd, int ndim, double delta, double eps, int ipoly, 
                 lf.dy = (centers[2 * b + 1] - centers[2 * a + 1]) / sq;
                 by_anc[a - a0].push_back(lf);
             }
+        // leaves per chunk: 8, or 16 where the upward pass still has at least
+        // eight waves per SIMD (fewer chunk sums to write and merge)
+        int chunk_leaves = kChunkLeaves;
+        if (use_mfma(p)) {
+            const long nTN = ((p.npw + 1) / 2 + 15) / 16;
+            long waves16 = 0;
+            for (int a = 0; a < p.nanc; a++) waves16 += nTN * (((long)by_anc[a].size() + 15) / 16);
+            if (waves16 >= 8 * 1024) chunk_leaves = 2 * kChunkLeaves;
+        }
         anc_chunk.push_back(0);
         for (int a = 0; a < p.nanc; a++) {
             auto& v = by_anc[a];
             std::stable_sort(v.begin(), v.end(), [](const PwLeaf& x, const PwLeaf& y) { return x.level < y.level; });
-            for (size_t s = 0; s < v.size(); s += kChunkLeaves) {
+            for (size_t s = 0; s < v.size(); s += chunk_leaves) {
                 UpChunk c;
                 c.anc = a;
                 c.first = (int)(leaves.size() + s);
-                c.count = (int)std::min<size_t>(kChunkLeaves, v.size() - s);
+                c.count = (int)std::min<size_t>(chunk_leaves, v.size() - s);
                 c.slot = (int)chunks.size();
                 chunks.push_back(c);
             }

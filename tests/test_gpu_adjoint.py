@@ -102,6 +102,14 @@ def test_dot_product_identity_grouped_translation(device):
     assert err < 1e-13, (err, info)
 
 
+def test_dot_product_identity_full_size_c5(device):
+    """16-leaf upward chunks in the adjoint's chunk sums (104,317 leaves)."""
+    w = make_workload("c5", "full")
+    err, info = _dot_identity(w.tree, w.nd, w.delta, w.eps, w.ipoly, w.iperiod, w.norder, 2)
+    assert info["n_chunks"] < info["n_pw_leaves"] / 8
+    assert err < 1e-13, (err, info)
+
+
 @pytest.mark.parametrize("seed", range(4))
 def test_dot_product_random_trees(device, seed):
     rng = np.random.default_rng(700 + seed)

@@ -92,6 +92,18 @@ def test_grouped_translation_matches_oracle(device):
     assert_close(compare(w.tree, got, ref, w.ifpgh))
 
 
+def test_full_size_c5_matches_oracle(device):
+    """The 104,317-leaf n = 16 workload at full size: 16-leaf upward chunks
+    (fewer chunk sums than one per 8 leaves) and grouped translation."""
+    w = make_workload("c5", "full")
+    print(f"case c5/full: leaves={w.tree.nleaves}", flush=True)
+    ref = O.oracle_boxfgt(w.tree, w.fvals, w.nd, w.delta, w.eps, w.ipoly, w.iperiod, w.norder, w.ifpgh)
+    got = run_gpu(w.tree, w.fvals, w.nd, w.delta, w.eps, w.ipoly, w.iperiod, w.norder, w.ifpgh)
+    info = got["plan"].info
+    assert info["n_chunks"] < info["n_pw_leaves"] / 8
+    assert_close(compare(w.tree, got, ref, w.ifpgh))
+
+
 @pytest.mark.skipif(not O.reference_available(), reason="reference library not built")
 @pytest.mark.parametrize("name,kw", [("c1", {}), ("c3", {"delta": 1e-5, "eps": 1e-9}), ("c3", {"delta": 1e-1, "eps": 1e-9})])
 def test_workload_matches_reference(device, name, kw):
