@@ -33,10 +33,15 @@ HBM_PEAK_GBS = 8000.0
 
 def algorithmic_work(plan, w):
     """Per-launch algorithmic FLOPs (FP64) or bytes of each stage, from the plan's
-    schedule: leaf transforms 4n^2H + 8nPH per PW leaf, evaluation
-    8*ntab*nPH + 4c n^2 H (c = 1, 3, 6 products for pot, pg, pgh), near field
-    {4, 10, 18} n^3 per source/target pair; the merge and translation stages
-    move expansions (16 B per complex coefficient)."""
+    schedule: the upward pass 4n^2H per PW leaf (y contraction) + 8nPH per x
+    contraction, the evaluation 8*ntab*nPH per x contraction + 4c n^2 H per
+    leaf (c = 1, 3, 6 products for pot, pg, pgh); an x contraction serves a
+    whole column of leaves (same cutoff box, level and x cell) where the
+    kernels share it (plan.info xc_up / xc_down), else one leaf.  Near field
+    {4, 10, 18} n^3 per source/target pair.  The merge reads the chunk sums
+    it adds and writes the sums (16 B per complex coefficient); the
+    translation reads each cutoff box's expansion once (the partners' reads
+    are meant to hit in L2) and writes the H x H folded pairs (32 B each)."""
     info = plan.info
     n, nd = w.norder, w.nd
     P, nexp = info["npw"], info["nexp"]
@@ -44,13 +49,11 @@ def algorithmic_work(plan, w):
     L = info["n_pw_leaves"]
     ntab = w.ifpgh
     c = {1: 1, 2: 3, 3: 6}[w.ifpgh]
-    up = nd * L * (4 * n * n * H + 8 * n * P * H)
-    down = nd * L * (8 * ntab * n * P * H + 4 * c * n * n * H)
+    up = nd * (L * 4 * n * n * H + info["xc_up"] * 8 * n * P * H)
+    down = nd * (info["xc_down"] * 8 * ntab * n * P * H + L * 4 * c * n * n * H)
     near = nd * info["n_near_pairs"] * {1: 4, 2: 10, 3: 18}[w.ifpgh] * n**3
-    merge_bytes = 16 * nd * nexp * (info["n_chunks"] + info["n_ancestors"])
-    # translation: every cutoff box reads its own and its PW-list partners' mp once (list
-    # lengths from the plan) and writes its local expansion
-    translate_bytes = 16 * nd * nexp * (info["n_ancestors"] * 2 + info.get("n_m2l", 0))
+    merge_bytes = 16 * nd * nexp * (info["merge_reads"] + info["merge_writes"])
+    translate_bytes = info["n_ancestors"] * nd * (16 * nexp + 32 * H * H)
     return {"leaf_to_pw": ("flop", up), "merge": ("byte", merge_bytes), "translate": ("byte", translate_bytes),
             "pw_to_values": ("flop", down), "near_field": ("flop", near)}
 

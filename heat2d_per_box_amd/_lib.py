@@ -38,6 +38,7 @@ EXPORTS = [
     "bfgt_plan_eval_targets",
     "bfgt_plan_destroy",
     "bfgt_plan_query",
+    "bfgt_plan_work",
     "bfgt_plan_get_table",
     "bfgt_plan_profile_begin",
     "bfgt_plan_profile_end",
@@ -89,6 +90,8 @@ def lib() -> ctypes.CDLL:
     L.bfgt_plan_destroy.argtypes = [_vp]
     L.bfgt_plan_query.restype = _i
     L.bfgt_plan_query.argtypes = [_vp, _ip, _dp]
+    L.bfgt_plan_work.restype = _i
+    L.bfgt_plan_work.argtypes = [_vp, ctypes.POINTER(ctypes.c_longlong), _i]
     L.bfgt_plan_get_table.restype = _i
     L.bfgt_plan_get_table.argtypes = [_vp, _i, _i, _vp, ctypes.c_size_t]
     L.bfgt_plan_profile_begin.restype = _i

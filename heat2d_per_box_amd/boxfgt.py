@@ -134,6 +134,11 @@ class BoxFGT:
         self.info.update({"delta0": dv[0], "pmax": dv[1], "t_plan_host": dv[2], "t_tables": dv[3],
                           "mfma": bool(dv[4]), "n_m2l": int(dv[5]), "n_near_sources": int(dv[6]),
                           "translation_group": int(dv[7])})
+        wk = (ctypes.c_longlong * 7)()
+        check(lib().bfgt_plan_work(self._h, wk, 7))
+        self.info.update({"xc_up": int(wk[0]), "xc_down": int(wk[1]), "xc_adj_eval": int(wk[3]),
+                          "xc_adj_transform": int(wk[4]), "merge_reads": int(wk[5]),
+                          "merge_writes": int(wk[6])})
 
     def _check_tensor(self, t, shape, name):
         import torch

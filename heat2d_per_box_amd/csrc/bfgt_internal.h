@@ -68,6 +68,9 @@ struct TreeNode {
     int pad;
 };
 
+// leaves per column group of the downward pass (leaf order > 8)
+constexpr int kColLeaves = 4;
+
 struct Plan {
     // parameters
     int nd = 1, ipoly = 0, iperiod_in = 1, iperiod_eff = 1, n = 8, npbox = 64, ifpgh = 1;
@@ -107,6 +110,10 @@ struct Plan {
     PwLeaf* d_pwleaf = nullptr;
     int npair = 0;                 // PW leaves paired by (cutoff box, level) for n <= 8
     int2* d_pwpair = nullptr;      // (leaf, partner leaf or -1)
+    int ncolgrp = 0;               // runs of PW leaves with one cutoff box, level and x cell
+    long long nxcol_up = 0;        // column pieces of the upward chunks (x contractions)
+    long long merge_reads = 0;     // expansions the merge sums (chunk and group slots)
+    int2* d_colgrp = nullptr;      // (first leaf, count)
     UpChunk* d_chunks = nullptr;
     int* d_anc_box = nullptr;      // ancestor index -> box
     int* d_anc_chunk = nullptr;    // CSR over chunks [nanc+1]

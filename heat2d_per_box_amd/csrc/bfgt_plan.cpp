@@ -482,14 +482,38 @@ int plan_create(Plan& p, int nd, int ndim, double delta, double eps, int ipoly, 
         anc_chunk.push_back(0);
         for (int a = 0; a < p.nanc; a++) {
             auto& v = by_anc[a];
-            std::stable_sort(v.begin(), v.end(), [](const PwLeaf& x, const PwLeaf& y) { return x.level < y.level; });
-            for (size_t s = 0; s < v.size(); s += chunk_leaves) {
+            // by level, then x cell, then y cell: the leaves of one column
+            // (same box, level and jx) are consecutive
+            std::stable_sort(v.begin(), v.end(), [](const PwLeaf& x, const PwLeaf& y) {
+                if (x.level != y.level) return x.level < y.level;
+                if (x.jx != y.jx) return x.jx < y.jx;
+                return x.jy < y.jy;
+            });
+            // chunks of whole columns up to chunk_leaves leaves (a longer
+            // column is cut): the upward pass contracts x once per column
+            // piece of a chunk
+            const size_t cap = (size_t)chunk_leaves;
+            for (size_t s = 0; s < v.size();) {
+                size_t e = s;
+                while (e < v.size()) {
+                    size_t ce = e + 1;
+                    while (ce < v.size() && v[ce].level == v[e].level && v[ce].jx == v[e].jx) ce++;
+                    if (ce - s <= cap) {
+                        e = ce;
+                        continue;
+                    }
+                    if (e == s) e = s + cap;
+                    break;
+                }
+                for (size_t i = s; i < e; i++)
+                    if (i == s || v[i].level != v[i - 1].level || v[i].jx != v[i - 1].jx) p.nxcol_up++;
                 UpChunk c;
                 c.anc = a;
                 c.first = (int)(leaves.size() + s);
-                c.count = (int)std::min<size_t>(chunk_leaves, v.size() - s);
+                c.count = (int)(e - s);
                 c.slot = (int)chunks.size();
                 chunks.push_back(c);
+                s = e;
             }
             leaves.insert(leaves.end(), v.begin(), v.end());
             anc_chunk.push_back((int)chunks.size());
@@ -550,9 +574,33 @@ int plan_create(Plan& p, int nd, int ndim, double delta, double eps, int ipoly, 
         i += two ? 2 : 1;
     }
     p.npair = (int)pairs_pw.size();
+    // column groups for the downward pass: runs of leaves with the same box,
+    // level and x cell, cut to at most kColLeaves leaves (one wave each)
+    std::vector<int2> colgrp;
+    {
+        std::vector<int2> cols;
+        for (int i = 0; i < p.npwleaf;) {
+            int j = i + 1;
+            while (j < p.npwleaf && leaves[j].anc == leaves[i].anc && leaves[j].level == leaves[i].level &&
+                   leaves[j].jx == leaves[i].jx)
+                j++;
+            cols.push_back(make_int2(i, j - i));
+            i = j;
+        }
+        auto cut = [&](int cap) {
+            colgrp.clear();
+            for (const int2& c : cols)
+                for (int k = 0; k < c.y; k += cap) colgrp.push_back(make_int2(c.x + k, std::min(cap, c.y - k)));
+        };
+        cut(kColLeaves);
+    }
+    p.ncolgrp = (int)colgrp.size();
     p.nchunks = (int)chunks.size();
     p.nm2l = (int)m2l.size();
     p.nred1 = (int)red1.size();
+    p.merge_reads = 0;
+    for (const int4& r : red1) p.merge_reads += r.z;
+    for (const int4& r : red2) p.merge_reads += r.z;
     p.nred2 = (int)red2.size();
     p.ngslot = ngslot;
 
@@ -668,6 +716,7 @@ int plan_create(Plan& p, int nd, int ndim, double delta, double eps, int ipoly, 
 
     if ((rc = upload(p, p.d_pwleaf, leaves))) return rc;
     if ((rc = upload(p, p.d_pwpair, pairs_pw))) return rc;
+    if ((rc = upload(p, p.d_colgrp, colgrp))) return rc;
     if ((rc = upload(p, p.d_chunks, chunks))) return rc;
     if ((rc = upload(p, p.d_anc_box, anc_box))) return rc;
     if ((rc = upload(p, p.d_anc_chunk, anc_chunk))) return rc;
@@ -938,6 +987,26 @@ int bfgt_plan_query(bfgt_plan_t plan, int* iv, double* dv) {
                        (double)p.nnearT_s, (double)p.m2l_group};
         std::memcpy(dv, v, sizeof(v));
     }
+    return BFGT_OK;
+}
+
+int bfgt_plan_work(bfgt_plan_t plan, long long* counts, int ncounts) {
+    if (!plan || (!counts && ncounts > 0)) return set_error(BFGT_ERR_ARG, "null argument");
+    const bfgt::Plan& p = plan->p;
+    const long long L = p.npwleaf;
+    const bool fp64_mfma = p.has_pw && bfgt::use_mfma(p) && p.precision != BFGT_PREC_SPLIT32;
+    const bool big = p.n > 8;
+    long long v[7] = {
+        fp64_mfma && big ? p.nxcol_up : L,                       // forward upward pass
+        fp64_mfma && big ? p.ncolgrp : L,                        // forward downward pass
+        L,                                                       // PW leaves
+        p.has_pw && (big || p.ifpgh > 1) ? p.nxcol_up : L,       // adjoint evaluation
+        p.has_pw && big ? p.ncolgrp : L,                         // adjoint leaf transform
+        p.merge_reads,                                           // expansions read by the merge
+        (long long)p.nred1 + p.nred2,                            // expansions written by the merge
+    };
+    for (int i = 0; i < ncounts && i < 7; i++) counts[i] = v[i];
+    for (int i = 7; i < ncounts; i++) counts[i] = 0;
     return BFGT_OK;
 }
 

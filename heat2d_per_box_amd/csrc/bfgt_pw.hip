@@ -142,6 +142,16 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_up_mfma(
         // the end of the chunk repeats its last leaf with weight zero (exact
         // zeros; the sums keep their order).
         constexpr int LP = (NTAB == 1 && NKS <= 2) ? 2 : 1;
+        // With one leaf per pass, the y contractions G of the leaves of one
+        // column (same level and x cell, consecutive in the chunk) are summed
+        // and contracted in x once: sum_l (T px)^T F_l (T py_l) =
+        // (T px)^T sum_l F_l (T py_l).
+        constexpr bool COL = LP == 1;
+        v4d Gr[LP][NTAB], Gi[LP][NTAB];
+#pragma unroll
+        for (int h = 0; h < LP; h++)
+#pragma unroll
+            for (int v = 0; v < NTAB; v++) Gr[h][v] = Gi[h][v] = v4d{0, 0, 0, 0};
         for (int li = 0; li < ch.count; li += LP) {
             PwLeaf lf[LP];
             double wl[LP];
@@ -150,7 +160,13 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_up_mfma(
                 lf[h] = leaves[ch.first + min(li + h, ch.count - 1)];
                 wl[h] = li + h < ch.count ? 1.0 : 0.0;
             }
-            v4d Gr[LP][NTAB], Gi[LP][NTAB];
+            // the x contraction runs at the end of a column (always without COL)
+            bool flush = true;
+            if constexpr (COL) {
+                const PwLeaf nx = leaves[ch.first + min(li + 1, ch.count - 1)];
+                flush = li + 1 == ch.count || nx.level != lf[0].level || nx.jx != lf[0].jx;
+                flush = __builtin_amdgcn_readfirstlane(flush ? 1 : 0) != 0;
+            }
 #pragma unroll
             for (int h = 0; h < LP; h++) {
                 const cplx* T = tab + (size_t)lf[h].level * lstride;
@@ -160,8 +176,6 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_up_mfma(
                 cplx py = ph[lf[h].jy * ps + k2];
                 py.x *= wy;
                 py.y *= -wy;
-#pragma unroll
-                for (int v = 0; v < NTAB; v++) Gr[h][v] = Gi[h][v] = v4d{0, 0, 0, 0};
 #pragma unroll
                 for (int s = 0; s < NKS; s++) {
                     const int m2 = 4 * s + q;
@@ -176,6 +190,7 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_up_mfma(
                     }
                 }
             }
+            if (!flush) continue;
 #pragma unroll
             for (int t = 0; t < NTM; t++) {
                 const int k1 = 16 * t + cc;
@@ -199,6 +214,10 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_up_mfma(
                     }
                 }
             }
+#pragma unroll
+            for (int h = 0; h < LP; h++)
+#pragma unroll
+                for (int v = 0; v < NTAB; v++) Gr[h][v] = Gi[h][v] = v4d{0, 0, 0, 0};
         }
         cplx* out = partial + ((size_t)ch.slot * nd + d) * nexp;
 #pragma unroll
@@ -228,8 +247,7 @@ struct Fold {
 
 __device__ __host__ inline int round_up(int a, int b) { return (a + b - 1) / b * b; }
 
-// Downward pass.  One wave per PW leaf; y frequencies are processed in tiles
-// of 16:
+// Downward pass.  y frequencies are processed in tiles of 16:
 //   FF_t^T = S^T a_t + i D^T b_t      a_t + i b_t = E_t diag(px), the x
 //                                     contraction folded with
 //                                     E_t(npw-1-m) = conj E_t(m)
@@ -241,34 +259,39 @@ __device__ __host__ inline int round_up(int a, int b) { return (a + b - 1) / b *
 // the middle y frequency of an odd expansion (:591-595); the adjoint of the
 // forward transform is NTAB = 1 with the conjugated values-to-PW table and
 // w = 1, writing the leaf-value cotangent.
-template <int NTAB, int NTP>
-__global__ __launch_bounds__(kWave * kWavesPerBlock) void k_down_mfma(
-    int n, int P, int ps, int nd, int nleaf, const PwLeaf* __restrict__ leaves, const cplx* __restrict__ tab,
-    int lstride, int ymid_half, const cplx* __restrict__ phase, const int* __restrict__ levbase,
-    const Fold* __restrict__ fold, double* __restrict__ pot, double* __restrict__ grad, double* __restrict__ hess,
-    double oscale) {
+//
+// Downward pass over column groups: the PW leaves of one cutoff box, level
+// and x cell jx share the x contraction FF (it depends only on the folded
+// expansion, the level's tables and the x phase), so one wave forms FF once
+// and then runs the y contraction and the output update for the group's
+// leaves, LG at a time (the y-table loads of a step serve all LG leaves; a
+// missing leaf is a zero-phase copy that is not stored).  groups[i] =
+// (first leaf, count) over the leaf array sorted by (box, level, jx, jy).  Per leaf the operands and products are those of the one-leaf
+// formulation: FF_t^T = S^T a_t + i D^T b_t, out_o^T += Re[(E_y diag(py w))^T
+// FF_x(o)^T].
+template <int NTAB, int NTP, int LG>
+__global__ __launch_bounds__(kWave * kWavesPerBlock) void k_down_col(
+    int n, int P, int ps, int nd, int ngroups, const int2* __restrict__ groups, const PwLeaf* __restrict__ leaves,
+    const cplx* __restrict__ tab, int lstride, int ymid_half, const cplx* __restrict__ phase,
+    const int* __restrict__ levbase, const Fold* __restrict__ fold, double* __restrict__ pot,
+    double* __restrict__ grad, double* __restrict__ hess, double oscale) {
     constexpr int NOUT = nchan(NTAB);
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
-    // wave-uniform (schedule entries read with scalar loads), and XCD-contiguous:
-    // the leaves of one cutoff box read its folded expansion from one L2
     const int item = __builtin_amdgcn_readfirstlane(xcd_block(blockIdx.x, gridDim.x) * kWavesPerBlock + wave);
-    if (item >= nleaf) return;  // no block-wide barriers in this kernel
+    if (item >= ngroups) return;  // no block-wide barriers in this kernel
+    const int2 grp = groups[item];
     const int q = lane >> 4, cc = lane & 15;
     const int H = (P + 1) / 2, P2 = P / 2, npbox = n * n;
     const int Mh4 = round_up(H, 4), H16 = round_up(H, 16);
     const int nks1 = Mh4 / 4, nT = H16 / 16;
     const int tstride = 16 * ps;
-    const PwLeaf lf = leaves[item];
-    const cplx* E = tab + (size_t)lf.level * lstride;  // [t][m][i]
-    const cplx* ph = phase + levbase[lf.level];
-    const cplx* phx = ph + lf.jx * ps;
-    const cplx* phy = ph + lf.jy * ps;
+    const PwLeaf l0 = leaves[grp.x];
+    const cplx* E = tab + (size_t)l0.level * lstride;  // [t][m][i]
+    const cplx* ph = phase + levbase[l0.level];
+    const cplx* phx = ph + l0.jx * ps;
+    const double w2 = 2.0 * oscale;
     for (int d = 0; d < nd; d++) {
-        const Fold* F = fold + ((size_t)lf.anc * nd + d) * Mh4 * H16;
-        v4d out[NOUT];
-#pragma unroll
-        for (int o = 0; o < NOUT; o++) out[o] = v4d{0, 0, 0, 0};
-        // NTP y-frequency tiles per pass share the x-table and phase operands
+        const Fold* F = fold + ((size_t)l0.anc * nd + d) * Mh4 * H16;
         for (int tg = 0; tg < nT; tg += NTP) {
             v4d FFr[NTP][NTAB], FFi[NTP][NTAB];
 #pragma unroll
@@ -293,6 +316,22 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_down_mfma(
                     }
                 }
             }
+            for (int b = 0; b < grp.y; b += LG) {
+            const cplx* phy[LG];
+            int box[LG];
+            double wg[LG];
+#pragma unroll
+            for (int g = 0; g < LG; g++) {
+                const PwLeaf lf = leaves[grp.x + min(b + g, grp.y - 1)];
+                phy[g] = ph + lf.jy * ps;
+                box[g] = lf.box;
+                wg[g] = b + g < grp.y ? 1.0 : 0.0;
+            }
+            v4d out[LG][NOUT];
+#pragma unroll
+            for (int g = 0; g < LG; g++)
+#pragma unroll
+                for (int o = 0; o < NOUT; o++) out[g][o] = v4d{0, 0, 0, 0};
             // element s of FF[p] holds FF^T[16 (tg+p) + 4s + q][cc]: the B operand of K-step s
 #pragma unroll
             for (int p = 0; p < NTP; p++) {
@@ -300,39 +339,51 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_down_mfma(
                 for (int s = 0; s < 4; s++) {
                     const int m2s = 16 * (tg + p) + 4 * s + q;
                     if (16 * (tg + p) + 4 * s >= H) break;  // rows of this step are all padding (uniform)
-                    cplx pyw = phy[m2s];
                     const double w = (ymid_half && m2s >= P2) ? 0.5 : 1.0;
-                    pyw.x *= w;
-                    pyw.y *= w;
-                    cplx eu[NTAB];
+                    cplx ey[NTAB];
 #pragma unroll
-                    for (int u = 0; u < NTAB; u++) eu[u] = cmul(E[u * tstride + m2s * 16 + cc], pyw);
+                    for (int u = 0; u < NTAB; u++) ey[u] = E[u * tstride + m2s * 16 + cc];
 #pragma unroll
-                    for (int o = 0; o < NOUT; o++) out[o] = mfma(eu[chan_y(o)].x, FFr[p][chan_x(o)][s], out[o]);
+                    for (int g = 0; g < LG; g++) {
+                        cplx pyw = phy[g][m2s];
+                        pyw.x *= w * wg[g];
+                        pyw.y *= w * wg[g];
+                        cplx eu[NTAB];
 #pragma unroll
-                    for (int o = 0; o < NOUT; o++) out[o] = mfma(-eu[chan_y(o)].y, FFi[p][chan_x(o)][s], out[o]);
+                        for (int u = 0; u < NTAB; u++) eu[u] = cmul(ey[u], pyw);
+#pragma unroll
+                        for (int o = 0; o < NOUT; o++)
+                            out[g][o] = mfma(eu[chan_y(o)].x, FFr[p][chan_x(o)][s], out[g][o]);
+#pragma unroll
+                        for (int o = 0; o < NOUT; o++)
+                            out[g][o] = mfma(-eu[chan_y(o)].y, FFi[p][chan_x(o)][s], out[g][o]);
+                    }
                 }
             }
-        }
-        // out^T holds [k2 = drow(q, r)][k1 = cc]
-        if (cc < n) {
+            // out^T holds [k2 = drow(q, r)][k1 = cc]
+            if (cc < n) {
 #pragma unroll
-            for (int r = 0; r < 4; r++) {
-                int k2 = drow(q, r);
-                if (k2 < n) {
-                    size_t pb = (size_t)lf.box * npbox + cc + (size_t)n * k2;
-                    const double w2 = 2.0 * oscale;
-                    pot[pb * nd + d] += w2 * out[0][r];
-                    if constexpr (NTAB >= 2) {
-                        grad[(pb * 2 + 0) * nd + d] += w2 * out[1][r];
-                        grad[(pb * 2 + 1) * nd + d] += w2 * out[2][r];
-                    }
-                    if constexpr (NTAB >= 3) {
-                        hess[(pb * 3 + 0) * nd + d] += w2 * out[3][r];
-                        hess[(pb * 3 + 1) * nd + d] += w2 * out[4][r];
-                        hess[(pb * 3 + 2) * nd + d] += w2 * out[5][r];
+                for (int g = 0; g < LG; g++) {
+                    if (b + g >= grp.y) break;
+#pragma unroll
+                    for (int r = 0; r < 4; r++) {
+                        int k2 = drow(q, r);
+                        if (k2 < n) {
+                            size_t pb = (size_t)box[g] * npbox + cc + (size_t)n * k2;
+                            pot[pb * nd + d] += w2 * out[g][0][r];
+                            if constexpr (NTAB >= 2) {
+                                grad[(pb * 2 + 0) * nd + d] += w2 * out[g][1][r];
+                                grad[(pb * 2 + 1) * nd + d] += w2 * out[g][2][r];
+                            }
+                            if constexpr (NTAB >= 3) {
+                                hess[(pb * 3 + 0) * nd + d] += w2 * out[g][3][r];
+                                hess[(pb * 3 + 1) * nd + d] += w2 * out[g][4][r];
+                                hess[(pb * 3 + 2) * nd + d] += w2 * out[g][5][r];
+                            }
+                        }
                     }
                 }
+            }
             }
         }
     }
@@ -565,7 +616,7 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_up_split(
     }
 }
 
-// Forward downward pass, split-FP32 (same tiling as k_down_mfma).
+// Forward downward pass, split-FP32 (one leaf per wave, the same two products).
 template <int NTAB>
 __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_down_split(
     int n, int P, int ps, int nd, int nleaf, const PwLeaf* __restrict__ leaves, const cplx* __restrict__ tab,
@@ -1147,12 +1198,14 @@ static int launch_down_vec(Plan& p, double* pot, double* grad, double* hess, siz
 }
 
 template <int NTAB, int NTP>
-static int launch_down_mfma2(Plan& p, const cplx* tab, int lstride, int ymid_half, double* pot, double* grad,
-                             double* hess) {
-    int blocks = (p.npwleaf + kWavesPerBlock - 1) / kWavesPerBlock;
-    k_down_mfma<NTAB, NTP><<<blocks, kWave * kWavesPerBlock, 0, p.stream>>>(
-        p.n, p.npw, p.pstride, p.nd, p.npwleaf, p.d_pwleaf, tab, lstride, ymid_half, p.d_phase, p.d_levbase,
-        reinterpret_cast<const Fold*>(p.d_loc), pot, grad, hess, p.out_scale);
+static int launch_down_col(Plan& p, const cplx* tab, int lstride, int ymid_half, double* pot, double* grad,
+                           double* hess) {
+    int blocks = (p.ncolgrp + kWavesPerBlock - 1) / kWavesPerBlock;
+    // leaves per y-contraction pass: as many as the registers hold
+    constexpr int LG = NTAB == 1 ? 4 : (NTAB == 2 ? 2 : 1);
+    k_down_col<NTAB, NTP, LG><<<blocks, kWave * kWavesPerBlock, 0, p.stream>>>(
+        p.n, p.npw, p.pstride, p.nd, p.ncolgrp, p.d_colgrp, p.d_pwleaf, tab, lstride, ymid_half, p.d_phase,
+        p.d_levbase, reinterpret_cast<const Fold*>(p.d_loc), pot, grad, hess, p.out_scale);
     BFGT_HIP_CHECK(hipGetLastError());
     return BFGT_OK;
 }
@@ -1168,7 +1221,8 @@ static int launch_down_pair(Plan& p, const cplx* tab, int lstride, int ymid_half
     return BFGT_OK;
 }
 
-// two y tiles per pass when their count is even; leaf pairs for n <= 8
+// two y tiles per pass when their count is even; leaf pairs for n <= 8,
+// column groups above
 template <int NTAB>
 static int launch_down_mfma(Plan& p, const cplx* tab, int lstride, int ymid_half, double* pot, double* grad,
                             double* hess) {
@@ -1177,8 +1231,8 @@ static int launch_down_mfma(Plan& p, const cplx* tab, int lstride, int ymid_half
         if (nT % 2 == 0) return launch_down_pair<NTAB, 2>(p, tab, lstride, ymid_half, pot, grad, hess);
         return launch_down_pair<NTAB, 1>(p, tab, lstride, ymid_half, pot, grad, hess);
     }
-    if (nT % 2 == 0) return launch_down_mfma2<NTAB, 2>(p, tab, lstride, ymid_half, pot, grad, hess);
-    return launch_down_mfma2<NTAB, 1>(p, tab, lstride, ymid_half, pot, grad, hess);
+    if (nT % 2 == 0) return launch_down_col<NTAB, 2>(p, tab, lstride, ymid_half, pot, grad, hess);
+    return launch_down_col<NTAB, 1>(p, tab, lstride, ymid_half, pot, grad, hess);
 }
 
 template <int NTAB>

@@ -150,6 +150,17 @@ int bfgt_plan_profile_end(bfgt_plan_t plan, double *stage_seconds, int *nsteps);
  * 1 or 4)}. */
 int bfgt_plan_query(bfgt_plan_t plan, int *ivals, double *dvals);
 
+/* Work counts of the plan's schedule, per density (no counterpart in the
+ * reference; they give the algorithmic FLOPs of the leaf passes):
+ * counts[0] x contractions of the upward pass, [1] x contractions of the
+ * downward pass, [2] PW leaves, [3] and [4] the same two counts for the
+ * adjoint's evaluation and leaf transform, [5] expansions read and [6]
+ * expansions written by the merge (a cutoff box whose leaves fit one upward
+ * chunk is written by the upward pass directly).  A contraction serves a
+ * whole column (PW leaves with one cutoff box, level and x cell) where the
+ * kernels share it, else one leaf.  Entries past 7 are set to 0. */
+int bfgt_plan_work(bfgt_plan_t plan, long long *counts, int ncounts);
+
 /* Copy one device table of one level to host memory (parity tests of the FP64
  * table generation).  which: 0 values->PW (n x npw complex, [m][k]),
  * 1 PW->pot/d/dd (3 x npw x n complex, [t][k][i]), 2 near-field masked

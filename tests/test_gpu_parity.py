@@ -92,6 +92,25 @@ def test_grouped_translation_matches_oracle(device):
     assert_close(compare(w.tree, got, ref, w.ifpgh))
 
 
+@pytest.mark.parametrize("iperiod,n,delta,ifpgh", [(0, 12, 1e-3, 3), (1, 16, 1e-3, 2), (1, 12, 3e-4, 1),
+                                                    (0, 16, 1e-2, 3)])
+def test_column_passes_match_oracle(device, iperiod, n, delta, ifpgh):
+    """Leaf orders > 8 share each x contraction over a column (same cutoff
+    box, level and x cell) in both leaf passes; eps = 1e-12 gives 3 tiles of
+    16 y frequencies (npw = 66, 72) besides the 2-tile cases."""
+    g = GaussianSum(np.array([[0.1, -0.2], [-0.3, 0.25]]), np.array([0.03, 0.06]), np.array([1.0, -0.5]))
+    fun = lambda p: np.stack([g(p), np.cos(2 * p[..., 0]) * g(p)], axis=-1)  # noqa: E731
+    t = build_adaptive(fun, 2, n, 0, 1e-7, 6, iperiod=iperiod, min_level=2, eta=0.0)
+    fv = t.sample(fun, n, 0, 2)
+    ref = O.oracle_boxfgt(t, fv, 2, delta, 1e-12, 0, iperiod, n, ifpgh)
+    got = run_gpu(t, fv, 2, delta, 1e-12, 0, iperiod, n, ifpgh)
+    info = got["plan"].info
+    print(f"iperiod={iperiod} n={n} delta={delta}: npw={info['npw']} leaves={info['n_pw_leaves']} "
+          f"x contractions up/down {info['xc_up']}/{info['xc_down']}", flush=True)
+    assert info["xc_up"] < info["n_pw_leaves"] and info["xc_down"] < info["n_pw_leaves"]
+    assert_close(compare(t, got, ref, ifpgh))
+
+
 def test_full_size_c5_matches_oracle(device):
     """The 104,317-leaf n = 16 workload at full size: 16-leaf upward chunks
     (fewer chunk sums than one per 8 leaves) and grouped translation."""
