@@ -112,6 +112,8 @@ struct Plan {
     int2* d_pwpair = nullptr;      // (leaf, partner leaf or -1)
     int ncolgrp = 0;               // runs of PW leaves with one cutoff box, level and x cell
     long long nxcol_up = 0;        // column pieces of the upward chunks (x contractions)
+    int2* d_upcols = nullptr;      // column pieces (first leaf, count), chunk by chunk
+    int4* d_upcc = nullptr;        // per chunk: (first column piece, count, slot, 0)
     long long merge_reads = 0;     // expansions the merge sums (chunk and group slots)
     int2* d_colgrp = nullptr;      // (first leaf, count)
     UpChunk* d_chunks = nullptr;

@@ -440,6 +440,8 @@ int plan_create(Plan& p, int nd, int ndim, double delta, double eps, int ipoly, 
     // PW schedule: leaves grouped by their cutoff-level ancestor
     std::vector<PwLeaf> leaves;
     std::vector<UpChunk> chunks;
+    std::vector<int> chunk_cols;  // first column piece of each chunk
+    std::vector<int2> upcols;
     std::vector<int> anc_box, anc_chunk;
     std::vector<int> m2l_ptr;
     std::vector<int2> m2l;
@@ -505,8 +507,14 @@ int plan_create(Plan& p, int nd, int ndim, double delta, double eps, int ipoly, 
                     if (e == s) e = s + cap;
                     break;
                 }
-                for (size_t i = s; i < e; i++)
-                    if (i == s || v[i].level != v[i - 1].level || v[i].jx != v[i - 1].jx) p.nxcol_up++;
+                // the chunk's column pieces
+                chunk_cols.push_back((int)upcols.size());
+                for (size_t i = s; i < e;) {
+                    size_t j = i + 1;
+                    while (j < e && v[j].level == v[i].level && v[j].jx == v[i].jx) j++;
+                    upcols.push_back(make_int2((int)(leaves.size() + i), (int)(j - i)));
+                    i = j;
+                }
                 UpChunk c;
                 c.anc = a;
                 c.first = (int)(leaves.size() + s);
@@ -532,6 +540,7 @@ int plan_create(Plan& p, int nd, int ndim, double delta, double eps, int ipoly, 
                 for (int c = c0; c < c1; c++) chunks[c].slot = nslot++;
         }
         p.nmpslot = nslot;
+        p.nxcol_up = (long long)upcols.size();
         // two-level merge schedule of the chunk partial sums (consecutive slots)
         for (int a = 0; a < p.nanc; a++) {
             int c0 = anc_chunk[a], c1 = anc_chunk[a + 1], nc = c1 - c0;
@@ -717,6 +726,15 @@ int plan_create(Plan& p, int nd, int ndim, double delta, double eps, int ipoly, 
     if ((rc = upload(p, p.d_pwleaf, leaves))) return rc;
     if ((rc = upload(p, p.d_pwpair, pairs_pw))) return rc;
     if ((rc = upload(p, p.d_colgrp, colgrp))) return rc;
+    {
+        std::vector<int4> upcc(chunks.size());
+        for (size_t c = 0; c < chunks.size(); c++) {
+            const int c0 = chunk_cols[c], c1 = c + 1 < chunks.size() ? chunk_cols[c + 1] : (int)upcols.size();
+            upcc[c] = make_int4(c0, c1 - c0, chunks[c].slot, 0);
+        }
+        if ((rc = upload(p, p.d_upcc, upcc))) return rc;
+        if ((rc = upload(p, p.d_upcols, upcols))) return rc;
+    }
     if ((rc = upload(p, p.d_chunks, chunks))) return rc;
     if ((rc = upload(p, p.d_anc_box, anc_box))) return rc;
     if ((rc = upload(p, p.d_anc_chunk, anc_chunk))) return rc;

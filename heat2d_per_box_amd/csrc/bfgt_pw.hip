@@ -142,16 +142,6 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_up_mfma(
         // the end of the chunk repeats its last leaf with weight zero (exact
         // zeros; the sums keep their order).
         constexpr int LP = (NTAB == 1 && NKS <= 2) ? 2 : 1;
-        // With one leaf per pass, the y contractions G of the leaves of one
-        // column (same level and x cell, consecutive in the chunk) are summed
-        // and contracted in x once: sum_l (T px)^T F_l (T py_l) =
-        // (T px)^T sum_l F_l (T py_l).
-        constexpr bool COL = LP == 1;
-        v4d Gr[LP][NTAB], Gi[LP][NTAB];
-#pragma unroll
-        for (int h = 0; h < LP; h++)
-#pragma unroll
-            for (int v = 0; v < NTAB; v++) Gr[h][v] = Gi[h][v] = v4d{0, 0, 0, 0};
         for (int li = 0; li < ch.count; li += LP) {
             PwLeaf lf[LP];
             double wl[LP];
@@ -160,13 +150,7 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_up_mfma(
                 lf[h] = leaves[ch.first + min(li + h, ch.count - 1)];
                 wl[h] = li + h < ch.count ? 1.0 : 0.0;
             }
-            // the x contraction runs at the end of a column (always without COL)
-            bool flush = true;
-            if constexpr (COL) {
-                const PwLeaf nx = leaves[ch.first + min(li + 1, ch.count - 1)];
-                flush = li + 1 == ch.count || nx.level != lf[0].level || nx.jx != lf[0].jx;
-                flush = __builtin_amdgcn_readfirstlane(flush ? 1 : 0) != 0;
-            }
+            v4d Gr[LP][NTAB], Gi[LP][NTAB];
 #pragma unroll
             for (int h = 0; h < LP; h++) {
                 const cplx* T = tab + (size_t)lf[h].level * lstride;
@@ -176,6 +160,8 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_up_mfma(
                 cplx py = ph[lf[h].jy * ps + k2];
                 py.x *= wy;
                 py.y *= -wy;
+#pragma unroll
+                for (int v = 0; v < NTAB; v++) Gr[h][v] = Gi[h][v] = v4d{0, 0, 0, 0};
 #pragma unroll
                 for (int s = 0; s < NKS; s++) {
                     const int m2 = 4 * s + q;
@@ -190,7 +176,6 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_up_mfma(
                     }
                 }
             }
-            if (!flush) continue;
 #pragma unroll
             for (int t = 0; t < NTM; t++) {
                 const int k1 = 16 * t + cc;
@@ -214,12 +199,122 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_up_mfma(
                     }
                 }
             }
-#pragma unroll
-            for (int h = 0; h < LP; h++)
-#pragma unroll
-                for (int v = 0; v < NTAB; v++) Gr[h][v] = Gi[h][v] = v4d{0, 0, 0, 0};
         }
         cplx* out = partial + ((size_t)ch.slot * nd + d) * nexp;
+#pragma unroll
+        for (int t = 0; t < NTM; t++) {
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                int k1 = 16 * t + drow(q, r);
+                if (k1 < Mlo && k2 < H) {
+                    double AC = acc[0][t][r], BD = acc[1][t][r], AD = acc[2][t][r], BC = acc[3][t][r];
+                    out[k1 * H + k2] = cplx{AC - BD, AD + BC};
+                    int km = P - 1 - k1;
+                    if (km > k1) out[km * H + k2] = cplx{AC + BD, AD - BC};
+                }
+            }
+        }
+    }
+}
+
+// Upward pass over columns (leaf order > 8, and the adjoint of the
+// evaluation): one wave per (chunk, y-frequency tile tN).  chunks[i] = (first
+// column piece, count, partial-sum slot, 0) and cols[c] = (first leaf, count)
+// of a column piece: PW leaves of one cutoff box, level and x cell.  The y
+// contractions of a column's leaves are summed in G, LPC leaves per pass (the
+// level's table rows are read once for all of them; a pass past the end of
+// the column repeats its last leaf with weight zero), and contracted in x
+// once:  sum_l (T px)^T F_l (T py_l) = (T px)^T sum_l F_l (T py_l).  Same
+// tables, weights and accumulators as k_up_mfma.
+template <int NKS, int NTM, int NTAB, int LPC>
+__global__ __launch_bounds__(kWave * kWavesPerBlock) void k_up_col(
+    int n, int P, int ps, int nd, int nTN, int nitems, LeafIn in, const PwLeaf* __restrict__ leaves,
+    const int4* __restrict__ chunks, const int2* __restrict__ cols, const cplx* __restrict__ tab, int lstride,
+    int ymid_half, const cplx* __restrict__ phase, const int* __restrict__ levbase, cplx* __restrict__ partial) {
+    constexpr int NOUT = nchan(NTAB);
+    const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    // wave-uniform: the schedule is read with scalar loads
+    const int item = __builtin_amdgcn_readfirstlane(blockIdx.x * kWavesPerBlock + wave);
+    if (item >= nitems) return;  // no block-wide barriers in this kernel
+    const int ci = item / nTN, tN = item % nTN;
+    const int q = lane >> 4, cc = lane & 15;
+    const int H = (P + 1) / 2, Mlo = (P + 1) / 2, nexp = P * H, npbox = n * n;
+    const int tstride = 16 * ps;
+    const int k2 = 16 * tN + cc;
+    const double wy = (ymid_half && k2 >= P / 2) ? 0.5 : 1.0;
+    const int ccl = min(cc, n - 1);
+    int pidx[NKS];  // clamped point index of (cc, m2 = 4s + q)
+#pragma unroll
+    for (int s = 0; s < NKS; s++) pidx[s] = ccl + n * min(4 * s + q, n - 1);
+    const int4 ch = chunks[ci];
+    for (int d = 0; d < nd; d++) {
+        v4d acc[4][NTM];
+#pragma unroll
+        for (int p = 0; p < 4; p++)
+#pragma unroll
+            for (int t = 0; t < NTM; t++) acc[p][t] = v4d{0, 0, 0, 0};
+        for (int c = ch.x; c < ch.x + ch.y; c++) {
+            const int2 col = cols[c];
+            const PwLeaf l0 = leaves[col.x];
+            const cplx* T = tab + (size_t)l0.level * lstride;
+            const cplx* ph = phase + levbase[l0.level];
+            v4d Gr[NTAB], Gi[NTAB];
+#pragma unroll
+            for (int v = 0; v < NTAB; v++) Gr[v] = Gi[v] = v4d{0, 0, 0, 0};
+            for (int li = 0; li < col.y; li += LPC) {
+                size_t pb0[LPC];
+                cplx py[LPC];
+                double wl[LPC];
+#pragma unroll
+                for (int h = 0; h < LPC; h++) {
+                    const PwLeaf lf = leaves[col.x + min(li + h, col.y - 1)];
+                    pb0[h] = (size_t)lf.box * npbox;
+                    wl[h] = li + h < col.y ? 1.0 : 0.0;
+                    // y phase of this lane's column (upward shift: conjugate of the table)
+                    py[h] = ph[lf.jy * ps + k2];
+                    py[h].x *= wy * wl[h];
+                    py[h].y *= -wy * wl[h];
+                }
+#pragma unroll
+                for (int s = 0; s < NKS; s++) {
+                    const int m2 = 4 * s + q;
+                    cplx tt[NTAB];
+#pragma unroll
+                    for (int u = 0; u < NTAB; u++) tt[u] = T[u * tstride + m2 * ps + k2];
+#pragma unroll
+                    for (int h = 0; h < LPC; h++) {
+                        cplx ty[NTAB];
+#pragma unroll
+                        for (int u = 0; u < NTAB; u++) ty[u] = cmul(tt[u], py[h]);
+#pragma unroll
+                        for (int o = 0; o < NOUT; o++) {
+                            const double a = leaf_in(in, o, pb0[h] + pidx[s], nd, d);
+                            Gr[chan_x(o)] = mfma(a, ty[chan_y(o)].x, Gr[chan_x(o)]);
+                            Gi[chan_x(o)] = mfma(a, ty[chan_y(o)].y, Gi[chan_x(o)]);
+                        }
+                    }
+                }
+            }
+#pragma unroll
+            for (int t = 0; t < NTM; t++) {
+                const int k1 = 16 * t + cc;
+                cplx px = ph[l0.jx * ps + k1];
+                px.y = -px.y;
+#pragma unroll
+                for (int v = 0; v < NTAB; v++) {
+#pragma unroll
+                    for (int s = 0; s < NKS; s++) {
+                        const int m1 = 4 * s + q;
+                        cplx tx = cmul(T[v * tstride + m1 * ps + k1], px);
+                        acc[0][t] = mfma(tx.x, Gr[v][s], acc[0][t]);
+                        acc[1][t] = mfma(tx.y, Gi[v][s], acc[1][t]);
+                        acc[2][t] = mfma(tx.x, Gi[v][s], acc[2][t]);
+                        acc[3][t] = mfma(tx.y, Gr[v][s], acc[3][t]);
+                    }
+                }
+            }
+        }
+        cplx* out = partial + ((size_t)ch.z * nd + d) * nexp;
 #pragma unroll
         for (int t = 0; t < NTM; t++) {
 #pragma unroll
@@ -1063,13 +1158,22 @@ struct UpArgs {
     int lstride, ymid_half, ntab;
 };
 
+// one table, n <= 8: two leaves per pass (k_up_mfma); otherwise column
+// sums (k_up_col), four leaves per pass with one table, two with more
 template <int NKS, int NTM, int NTAB>
 static int launch_up_mfma3(Plan& p, const UpArgs& u, int nTN) {
     int nitems = p.nchunks * nTN;
     int blocks = (nitems + kWavesPerBlock - 1) / kWavesPerBlock;
-    k_up_mfma<NKS, NTM, NTAB><<<blocks, kWave * kWavesPerBlock, 0, p.stream>>>(
-        p.n, p.npw, p.pstride, p.nd, nTN, nitems, u.in, p.d_pwleaf, p.d_chunks, u.tab, u.lstride, u.ymid_half,
-        p.d_phase, p.d_levbase, p.d_mp);
+    if constexpr (NTAB == 1 && NKS <= 2) {
+        k_up_mfma<NKS, NTM, NTAB><<<blocks, kWave * kWavesPerBlock, 0, p.stream>>>(
+            p.n, p.npw, p.pstride, p.nd, nTN, nitems, u.in, p.d_pwleaf, p.d_chunks, u.tab, u.lstride, u.ymid_half,
+            p.d_phase, p.d_levbase, p.d_mp);
+    } else {
+        constexpr int LPC = NTAB == 1 ? 4 : 2;
+        k_up_col<NKS, NTM, NTAB, LPC><<<blocks, kWave * kWavesPerBlock, 0, p.stream>>>(
+            p.n, p.npw, p.pstride, p.nd, nTN, nitems, u.in, p.d_pwleaf, p.d_upcc, p.d_upcols, u.tab, u.lstride,
+            u.ymid_half, p.d_phase, p.d_levbase, p.d_mp);
+    }
     BFGT_HIP_CHECK(hipGetLastError());
     return BFGT_OK;
 }
