@@ -393,12 +393,20 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_down_col(
             for (int p = 0; p < NTP; p++)
 #pragma unroll
                 for (int t = 0; t < NTAB; t++) FFr[p][t] = FFi[p][t] = v4d{0, 0, 0, 0};
-            for (int s = 0; s < nks1; s++) {
+            // two K-steps per pass (a step past the end reads the last rows
+            // with weight zero)
+            for (int s0 = 0; s0 < nks1; s0 += 2) {
+#pragma unroll
+              for (int s1 = 0; s1 < 2; s1++) {
+                const int s = min(s0 + s1, nks1 - 1);
+                const double ws = s0 + s1 < nks1 ? 1.0 : 0.0;
                 const int m = 4 * s + q;
                 Fold sd[NTP];
 #pragma unroll
                 for (int p = 0; p < NTP; p++) sd[p] = F[m * H16 + 16 * (tg + p) + cc];
-                const cplx px = phx[m];
+                cplx px = phx[m];
+                px.x *= ws;
+                px.y *= ws;
 #pragma unroll
                 for (int t = 0; t < NTAB; t++) {
                     cplx e = cmul(E[t * tstride + m * 16 + cc], px);
@@ -410,6 +418,7 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_down_col(
                         FFi[p][t] = mfma(sd[p].D.x, e.y, FFi[p][t]);
                     }
                 }
+              }
             }
             for (int b = 0; b < grp.y; b += LG) {
             const cplx* phy[LG];
@@ -455,28 +464,35 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_down_col(
                     }
                 }
             }
-            // out^T holds [k2 = drow(q, r)][k1 = cc]
+            // out^T holds [k2 = drow(q, r)][k1 = cc].  The outputs are read for
+            // all (leaf, r) at clamped points first, so that one memory round
+            // trip serves the whole batch, then added and stored where valid.
+            double* dst[NOUT];
+            dst[0] = pot;
+            if constexpr (NTAB >= 2) dst[1] = dst[2] = grad;
+            if constexpr (NTAB >= 3) dst[3] = dst[4] = dst[5] = hess;
+            constexpr int stride_o[6] = {1, 2, 2, 3, 3, 3}, comp_o[6] = {0, 0, 1, 0, 1, 2};
+            double old[LG][4][NOUT];
+#pragma unroll
+            for (int g = 0; g < LG; g++)
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const size_t pb = (size_t)box[g] * npbox + min(cc, n - 1) + (size_t)n * min(drow(q, r), n - 1);
+#pragma unroll
+                    for (int o = 0; o < NOUT; o++) old[g][r][o] = dst[o][(pb * stride_o[o] + comp_o[o]) * nd + d];
+                }
             if (cc < n) {
 #pragma unroll
                 for (int g = 0; g < LG; g++) {
                     if (b + g >= grp.y) break;
 #pragma unroll
-                    for (int r = 0; r < 4; r++) {
-                        int k2 = drow(q, r);
-                        if (k2 < n) {
-                            size_t pb = (size_t)box[g] * npbox + cc + (size_t)n * k2;
-                            pot[pb * nd + d] += w2 * out[g][0][r];
-                            if constexpr (NTAB >= 2) {
-                                grad[(pb * 2 + 0) * nd + d] += w2 * out[g][1][r];
-                                grad[(pb * 2 + 1) * nd + d] += w2 * out[g][2][r];
-                            }
-                            if constexpr (NTAB >= 3) {
-                                hess[(pb * 3 + 0) * nd + d] += w2 * out[g][3][r];
-                                hess[(pb * 3 + 1) * nd + d] += w2 * out[g][4][r];
-                                hess[(pb * 3 + 2) * nd + d] += w2 * out[g][5][r];
-                            }
+                    for (int r = 0; r < 4; r++)
+                        if (drow(q, r) < n) {
+                            const size_t pb = (size_t)box[g] * npbox + cc + (size_t)n * drow(q, r);
+#pragma unroll
+                            for (int o = 0; o < NOUT; o++)
+                                dst[o][(pb * stride_o[o] + comp_o[o]) * nd + d] = old[g][r][o] + w2 * out[g][o][r];
                         }
-                    }
                 }
             }
             }
@@ -1159,7 +1175,7 @@ struct UpArgs {
 };
 
 // one table, n <= 8: two leaves per pass (k_up_mfma); otherwise column
-// sums (k_up_col), four leaves per pass with one table, two with more
+// sums (k_up_col), two leaves per pass with one table, one with more
 template <int NKS, int NTM, int NTAB>
 static int launch_up_mfma3(Plan& p, const UpArgs& u, int nTN) {
     int nitems = p.nchunks * nTN;
@@ -1169,7 +1185,7 @@ static int launch_up_mfma3(Plan& p, const UpArgs& u, int nTN) {
             p.n, p.npw, p.pstride, p.nd, nTN, nitems, u.in, p.d_pwleaf, p.d_chunks, u.tab, u.lstride, u.ymid_half,
             p.d_phase, p.d_levbase, p.d_mp);
     } else {
-        constexpr int LPC = NTAB == 1 ? 4 : 2;
+        constexpr int LPC = NTAB == 1 ? 2 : 1;
         k_up_col<NKS, NTM, NTAB, LPC><<<blocks, kWave * kWavesPerBlock, 0, p.stream>>>(
             p.n, p.npw, p.pstride, p.nd, nTN, nitems, u.in, p.d_pwleaf, p.d_upcc, p.d_upcols, u.tab, u.lstride,
             u.ymid_half, p.d_phase, p.d_levbase, p.d_mp);
