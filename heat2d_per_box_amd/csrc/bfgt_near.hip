@@ -289,22 +289,31 @@ __global__ __launch_bounds__(64 * kNearWaves) void k_near_mfma(
             }
         }
     }
-    // element r of lane (q, c) = out[k1 = c][k2 = 4r + q]
-    if (c < n && p1 > p0) {
+    // element r of lane (q, c) = out[k1 = c][k2 = 4r + q].  The outputs are
+    // read at clamped points for all r first (one memory round trip), then
+    // added and stored where valid.
+    if (p1 > p0) {
+        double* dst[NOUT];
+        dst[0] = pot;
+        if constexpr (NTAB >= 2) dst[1] = dst[2] = grad;
+        if constexpr (NTAB >= 3) dst[3] = dst[4] = dst[5] = hess;
+        constexpr int stride_o[6] = {1, 2, 2, 3, 3, 3}, comp_o[6] = {0, 0, 1, 0, 1, 2};
+        double old[4][NOUT];
 #pragma unroll
         for (int r = 0; r < 4; r++) {
-            const int k2 = 4 * r + q;
-            if (k2 < n) {
-                const size_t pb = (size_t)b * nn + c + (size_t)n * k2;
-                pot[pb * nd + d] += oscale * out[0][r];
-                if constexpr (NTAB >= 2) {
-                    grad[(pb * 2 + 0) * nd + d] += oscale * out[1][r];
-                    grad[(pb * 2 + 1) * nd + d] += oscale * out[2][r];
-                }
-                if constexpr (NTAB >= 3) {
-                    hess[(pb * 3 + 0) * nd + d] += oscale * out[3][r];
-                    hess[(pb * 3 + 1) * nd + d] += oscale * out[4][r];
-                    hess[(pb * 3 + 2) * nd + d] += oscale * out[5][r];
+            const size_t pb = (size_t)b * nn + min(c, n - 1) + (size_t)n * min(4 * r + q, n - 1);
+#pragma unroll
+            for (int o = 0; o < NOUT; o++) old[r][o] = dst[o][(pb * stride_o[o] + comp_o[o]) * nd + d];
+        }
+        if (c < n) {
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int k2 = 4 * r + q;
+                if (k2 < n) {
+                    const size_t pb = (size_t)b * nn + c + (size_t)n * k2;
+#pragma unroll
+                    for (int o = 0; o < NOUT; o++)
+                        dst[o][(pb * stride_o[o] + comp_o[o]) * nd + d] = old[r][o] + oscale * out[o][r];
                 }
             }
         }

@@ -795,24 +795,31 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_down_split(
                 }
             }
         }
-        // register r of out^T holds [k2 = 4q + r][k1 = cc]
+        // register r of out^T holds [k2 = 4q + r][k1 = cc].  The outputs are
+        // read at clamped points for all r first (one memory round trip), then
+        // added and stored where valid.
+        const double w2 = 2.0 * oscale;
+        double* dst[NOUT];
+        dst[0] = pot;
+        if constexpr (NTAB >= 2) dst[1] = dst[2] = grad;
+        if constexpr (NTAB >= 3) dst[3] = dst[4] = dst[5] = hess;
+        constexpr int stride_o[6] = {1, 2, 2, 3, 3, 3}, comp_o[6] = {0, 0, 1, 0, 1, 2};
+        double old[4][NOUT];
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const size_t pb = (size_t)lf.box * npbox + min(cc, n - 1) + (size_t)n * min(drow32(q, r), n - 1);
+#pragma unroll
+            for (int o = 0; o < NOUT; o++) old[r][o] = dst[o][(pb * stride_o[o] + comp_o[o]) * nd + d];
+        }
         if (cc < n) {
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                int k2 = drow32(q, r);
+                const int k2 = drow32(q, r);
                 if (k2 < n) {
-                    size_t pb = (size_t)lf.box * npbox + cc + (size_t)n * k2;
-                    const double w2 = 2.0 * oscale;
-                    pot[pb * nd + d] += w2 * (double)out[0][r];
-                    if constexpr (NTAB >= 2) {
-                        grad[(pb * 2 + 0) * nd + d] += w2 * (double)out[1][r];
-                        grad[(pb * 2 + 1) * nd + d] += w2 * (double)out[2][r];
-                    }
-                    if constexpr (NTAB >= 3) {
-                        hess[(pb * 3 + 0) * nd + d] += w2 * (double)out[3][r];
-                        hess[(pb * 3 + 1) * nd + d] += w2 * (double)out[4][r];
-                        hess[(pb * 3 + 2) * nd + d] += w2 * (double)out[5][r];
-                    }
+                    const size_t pb = (size_t)lf.box * npbox + cc + (size_t)n * k2;
+#pragma unroll
+                    for (int o = 0; o < NOUT; o++)
+                        dst[o][(pb * stride_o[o] + comp_o[o]) * nd + d] = old[r][o] + w2 * (double)out[o][r];
                 }
             }
         }
