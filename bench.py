@@ -58,6 +58,20 @@ def algorithmic_work(plan, w):
             "pw_to_values": ("flop", down), "near_field": ("flop", near)}
 
 
+def per_leaf_work(plan, w):
+    """The same two leaf passes counted with one x contraction per PW leaf
+    (nd * L * [4nH(n + 2P)] upward, nd * L * [8 ntab nPH + 4c n^2 H] downward):
+    the work of the leaf-by-leaf algorithm, which the column sharing avoids."""
+    info = plan.info
+    n, nd = w.norder, w.nd
+    P = info["npw"]
+    H = (P + 1) // 2
+    L = info["n_pw_leaves"]
+    c = {1: 1, 2: 3, 3: 6}[w.ifpgh]
+    return {"leaf_to_pw": nd * L * 4 * n * H * (n + 2 * P),
+            "pw_to_values": nd * L * (8 * w.ifpgh * n * P * H + 4 * c * n * n * H)}
+
+
 def measured_traffic(workload, precision, kernel):
     """HBM bytes per launch of `kernel` recorded from rocprofv3 counters in
     profiles/boxfgt_traffic.txt for this workload and precision, else None."""
@@ -226,6 +240,13 @@ def main():
         roof["traffic_detail"] = tr
     roof["algorithmic_per_launch"] = amount
     roof["avg_launch_ms"] = stage_ms[dom]
+    plw = per_leaf_work(plan, w)
+    if dom in plw:
+        # the leaf-by-leaf operation count over the same time: how fast the
+        # kernel does the per-leaf algorithm's work (above the peak when the
+        # column sharing removes more than the remaining inefficiency)
+        roof["per_leaf_formula_per_launch"] = plw[dom]
+        roof["per_leaf_formula_tflops"] = plw[dom] / dur / 1e12
     # every stage against its own bound
     stages_roof = {}
     for k in names:
