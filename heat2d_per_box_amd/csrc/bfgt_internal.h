@@ -70,6 +70,8 @@ struct TreeNode {
 
 // leaves per column group of the downward pass (leaf order > 8)
 constexpr int kColLeaves = 4;
+// quads (2 x 2 leaf blocks) per column-pair group of the downward pass (n <= 8)
+constexpr int kQuadsPerGroup = 4;
 
 struct Plan {
     // parameters
@@ -116,6 +118,10 @@ struct Plan {
     int4* d_upcc = nullptr;        // per chunk: (first column piece, count, slot, 0)
     long long merge_reads = 0;     // expansions the merge sums (chunk and group slots)
     int2* d_colgrp = nullptr;      // (first leaf, count)
+    int nqgroup = 0;               // n <= 8: column pairs evaluated as 2 x 2 leaf blocks
+    int4* d_qgroups = nullptr;     // (first quad, count, a leaf of column A, a leaf of column B or -1)
+    int4* d_quads = nullptr;       // leaves (A, jyA), (B, jyA), (A, jyB), (B, jyB) or -1
+    long long nxcol_quad = 0;      // columns of the groups (x contractions of n points)
     UpChunk* d_chunks = nullptr;
     int* d_anc_box = nullptr;      // ancestor index -> box
     int* d_anc_chunk = nullptr;    // CSR over chunks [nanc+1]

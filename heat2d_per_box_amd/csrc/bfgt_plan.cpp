@@ -604,6 +604,63 @@ int plan_create(Plan& p, int nd, int ndim, double delta, double eps, int ipoly, 
         cut(kColLeaves);
     }
     p.ncolgrp = (int)colgrp.size();
+    // for n <= 8: pairs of columns (same box and level) whose leaves are
+    // evaluated as 2 x 2 blocks -- quads (A, jyA), (B, jyA), (A, jyB), (B,
+    // jyB) of leaf indices (-1 where no leaf) over consecutive y cells of
+    // the union of the two columns' rows -- and groups of at most
+    // kQuadsPerGroup quads (first quad, count, a leaf of column A, a leaf of
+    // column B or -1), fewer where that is needed for 2048 groups
+    std::vector<int4> quads, qgroups;
+    std::vector<int> qcol_first;  // first quad of each column pair
+    if (p.n <= 8) {
+        std::vector<int2> cols;  // (first leaf, count) of each column
+        for (int i = 0; i < p.npwleaf;) {
+            int j = i + 1;
+            while (j < p.npwleaf && leaves[j].anc == leaves[i].anc && leaves[j].level == leaves[i].level &&
+                   leaves[j].jx == leaves[i].jx)
+                j++;
+            cols.push_back(make_int2(i, j - i));
+            i = j;
+        }
+        for (size_t c = 0; c < cols.size();) {
+            const int2 ca = cols[c];
+            const PwLeaf& la = leaves[ca.x];
+            const bool pair = c + 1 < cols.size() && leaves[cols[c + 1].x].anc == la.anc &&
+                              leaves[cols[c + 1].x].level == la.level;
+            const int2 cb = pair ? cols[c + 1] : make_int2(0, 0);
+            // rows: the union of the y cells, each with the leaf of A and of B
+            std::vector<int3> rows;  // (jy, leaf in A or -1, leaf in B or -1)
+            int ia = 0, ib = 0;
+            while (ia < ca.y || ib < cb.y) {
+                const int ja = ia < ca.y ? leaves[ca.x + ia].jy : INT32_MAX;
+                const int jb = ib < cb.y ? leaves[cb.x + ib].jy : INT32_MAX;
+                const int jy = std::min(ja, jb);
+                rows.push_back(make_int3(jy, ja == jy ? ca.x + ia++ : -1, jb == jy ? cb.x + ib++ : -1));
+            }
+            qcol_first.push_back((int)quads.size());
+            qcol_first.push_back(ca.x);
+            qcol_first.push_back(pair ? cb.x : -1);
+            for (size_t r = 0; r < rows.size(); r += 2) {
+                const int3 ra = rows[r], rb = r + 1 < rows.size() ? rows[r + 1] : make_int3(0, -1, -1);
+                quads.push_back(make_int4(ra.y, ra.z, rb.y, rb.z));
+            }
+            c += pair ? 2 : 1;
+        }
+        auto cut = [&](int cap) {
+            qgroups.clear();
+            for (size_t i = 0; i < qcol_first.size(); i += 3) {
+                const int q0 = qcol_first[i], q1 = i + 3 < qcol_first.size() ? qcol_first[i + 3] : (int)quads.size();
+                for (int k = q0; k < q1; k += cap)
+                    qgroups.push_back(make_int4(k, std::min(cap, q1 - k), qcol_first[i + 1], qcol_first[i + 2]));
+            }
+        };
+        for (int cap = kQuadsPerGroup; cap >= 1; cap /= 2) {
+            cut(cap);
+            if (qgroups.size() >= 2048) break;
+        }
+        for (const int4& gq : qgroups) p.nxcol_quad += gq.w >= 0 ? 2 : 1;
+    }
+    p.nqgroup = (int)qgroups.size();
     p.nchunks = (int)chunks.size();
     p.nm2l = (int)m2l.size();
     p.nred1 = (int)red1.size();
@@ -726,6 +783,8 @@ int plan_create(Plan& p, int nd, int ndim, double delta, double eps, int ipoly, 
     if ((rc = upload(p, p.d_pwleaf, leaves))) return rc;
     if ((rc = upload(p, p.d_pwpair, pairs_pw))) return rc;
     if ((rc = upload(p, p.d_colgrp, colgrp))) return rc;
+    if ((rc = upload(p, p.d_quads, quads))) return rc;
+    if ((rc = upload(p, p.d_qgroups, qgroups))) return rc;
     {
         std::vector<int4> upcc(chunks.size());
         for (size_t c = 0; c < chunks.size(); c++) {
@@ -1014,12 +1073,16 @@ int bfgt_plan_work(bfgt_plan_t plan, long long* counts, int ncounts) {
     const long long L = p.npwleaf;
     const bool fp64_mfma = p.has_pw && bfgt::use_mfma(p) && p.precision != BFGT_PREC_SPLIT32;
     const bool big = p.n > 8;
+    // x contractions of the downward pass: one per column group (n > 8), one
+    // per column of a column pair evaluated as 2 x 2 blocks (n <= 8, when
+    // there are enough pairs), else one per leaf
+    const long long xc_down = big ? p.ncolgrp : (p.nqgroup >= 1024 ? p.nxcol_quad : L);
     long long v[7] = {
         fp64_mfma && big ? p.nxcol_up : L,                       // forward upward pass
-        fp64_mfma && big ? p.ncolgrp : L,                        // forward downward pass
+        fp64_mfma ? xc_down : L,                                 // forward downward pass
         L,                                                       // PW leaves
         p.has_pw && (big || p.ifpgh > 1) ? p.nxcol_up : L,       // adjoint evaluation
-        p.has_pw && big ? p.ncolgrp : L,                         // adjoint leaf transform
+        p.has_pw ? xc_down : L,                                  // adjoint leaf transform
         p.merge_reads,                                           // expansions read by the merge
         (long long)p.nred1 + p.nred2,                            // expansions written by the merge
     };

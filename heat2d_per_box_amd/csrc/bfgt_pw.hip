@@ -500,6 +500,128 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_down_col(
     }
 }
 
+// Downward pass for leaf orders n <= 8 over column pairs: the PW leaves of
+// one cutoff box and level in x cells jxA and jxB fill the two 8-wide halves
+// of the x contraction's point dimension, so FF is formed once per group;
+// each y-contraction product then evaluates a 2 x 2 block of leaves (a
+// quad): rows 0-7 of its point dimension are the y cell jyA, rows 8-15 jyB.
+// qgroups[i] = (first quad, count, a leaf of column A, a leaf of column B or
+// -1); quads[k] = leaves (A, jyA), (B, jyA), (A, jyB), (B, jyB) or -1.  A
+// missing column or row gets zero phases; missing leaves are not stored.
+template <int NTAB, int NTP>
+__global__ __launch_bounds__(kWave * kWavesPerBlock) void k_down_quad(
+    int n, int P, int ps, int nd, int ngroups, const int4* __restrict__ qgroups, const int4* __restrict__ quads,
+    const PwLeaf* __restrict__ leaves, const cplx* __restrict__ tab, int lstride, int ymid_half,
+    const cplx* __restrict__ phase, const int* __restrict__ levbase, const Fold* __restrict__ fold,
+    double* __restrict__ pot, double* __restrict__ grad, double* __restrict__ hess, double oscale) {
+    constexpr int NOUT = nchan(NTAB);
+    const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    const int item = __builtin_amdgcn_readfirstlane(xcd_block(blockIdx.x, gridDim.x) * kWavesPerBlock + wave);
+    if (item >= ngroups) return;  // no block-wide barriers in this kernel
+    const int4 g = qgroups[item];
+    const int q = lane >> 4, cc = lane & 15;
+    const int half = cc >> 3, ci = cc & 7;
+    const int H = (P + 1) / 2, P2 = P / 2, npbox = n * n;
+    const int Mh4 = round_up(H, 4), H16 = round_up(H, 16);
+    const int nks1 = Mh4 / 4, nT = H16 / 16;
+    const int tstride = 16 * ps;
+    const PwLeaf la = leaves[g.z];
+    const int jxb = g.w >= 0 ? leaves[g.w].jx : la.jx;
+    const cplx* E = tab + (size_t)la.level * lstride;  // [t][m][i]
+    const cplx* ph = phase + levbase[la.level];
+    // this lane's column: its x phase, zero in an absent column B
+    const cplx* phx = ph + (half ? jxb : la.jx) * ps;
+    const double wx = (half && g.w < 0) ? 0.0 : 1.0;
+    const double w2 = 2.0 * oscale;
+    constexpr int stride_o[6] = {1, 2, 2, 3, 3, 3}, comp_o[6] = {0, 0, 1, 0, 1, 2};
+    double* dst[NOUT];
+    dst[0] = pot;
+    if constexpr (NTAB >= 2) dst[1] = dst[2] = grad;
+    if constexpr (NTAB >= 3) dst[3] = dst[4] = dst[5] = hess;
+    for (int d = 0; d < nd; d++) {
+        const Fold* F = fold + ((size_t)la.anc * nd + d) * Mh4 * H16;
+        for (int tg = 0; tg < nT; tg += NTP) {
+            v4d FFr[NTP][NTAB], FFi[NTP][NTAB];
+#pragma unroll
+            for (int p = 0; p < NTP; p++)
+#pragma unroll
+                for (int t = 0; t < NTAB; t++) FFr[p][t] = FFi[p][t] = v4d{0, 0, 0, 0};
+            // two K-steps per pass (a step past the end reads the last rows
+            // with weight zero)
+            for (int s0 = 0; s0 < nks1; s0 += 2) {
+#pragma unroll
+                for (int s1 = 0; s1 < 2; s1++) {
+                    const int s = min(s0 + s1, nks1 - 1);
+                    const double ws = s0 + s1 < nks1 ? wx : 0.0;
+                    const int m = 4 * s + q;
+                    Fold sd[NTP];
+#pragma unroll
+                    for (int p = 0; p < NTP; p++) sd[p] = F[m * H16 + 16 * (tg + p) + cc];
+                    cplx px = phx[m];
+                    px.x *= ws;
+                    px.y *= ws;
+#pragma unroll
+                    for (int t = 0; t < NTAB; t++) {
+                        const cplx e = cmul(E[t * tstride + m * 16 + ci], px);
+#pragma unroll
+                        for (int p = 0; p < NTP; p++) {
+                            FFr[p][t] = mfma(sd[p].S.x, e.x, FFr[p][t]);
+                            FFi[p][t] = mfma(sd[p].S.y, e.x, FFi[p][t]);
+                            FFr[p][t] = mfma(-sd[p].D.y, e.y, FFr[p][t]);
+                            FFi[p][t] = mfma(sd[p].D.x, e.y, FFi[p][t]);
+                        }
+                    }
+                }
+            }
+            for (int k = g.x; k < g.x + g.y; k++) {
+                const int4 qd = quads[k];
+                // this lane's row of the y contraction: y cell jyA (half 0) or jyB
+                const int ra = qd.x >= 0 ? qd.x : qd.y, rb = qd.z >= 0 ? qd.z : qd.w;
+                const int lrow = half ? (rb >= 0 ? rb : ra) : ra;
+                const double wy = (half && rb < 0) ? 0.0 : 1.0;
+                const cplx* phy = ph + leaves[lrow].jy * ps;
+                v4d out[NOUT];
+#pragma unroll
+                for (int o = 0; o < NOUT; o++) out[o] = v4d{0, 0, 0, 0};
+#pragma unroll
+                for (int p = 0; p < NTP; p++) {
+#pragma unroll
+                    for (int s = 0; s < 4; s++) {
+                        const int m2s = 16 * (tg + p) + 4 * s + q;
+                        if (16 * (tg + p) + 4 * s >= H) break;  // rows of this step are all padding (uniform)
+                        cplx pyw = phy[m2s];
+                        const double w = ((ymid_half && m2s >= P2) ? 0.5 : 1.0) * wy;
+                        pyw.x *= w;
+                        pyw.y *= w;
+                        cplx eu[NTAB];
+#pragma unroll
+                        for (int u = 0; u < NTAB; u++) eu[u] = cmul(E[u * tstride + m2s * 16 + ci], pyw);
+#pragma unroll
+                        for (int o = 0; o < NOUT; o++) out[o] = mfma(eu[chan_y(o)].x, FFr[p][chan_x(o)][s], out[o]);
+#pragma unroll
+                        for (int o = 0; o < NOUT; o++) out[o] = mfma(-eu[chan_y(o)].y, FFi[p][chan_x(o)][s], out[o]);
+                    }
+                }
+                // element r of lane (q, cc) = D[row 4r + q][col cc]: leaf (column
+                // cc >> 3, row (4r + q) >> 3 = r >> 1), point (ci, (4r + q) & 7)
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const int lid = (r >> 1) ? (half ? qd.w : qd.z) : (half ? qd.y : qd.x);
+                    const int yp = drow(q, r) & 7;
+                    if (lid >= 0 && ci < n && yp < n) {
+                        const size_t pb = (size_t)leaves[lid].box * npbox + ci + (size_t)n * yp;
+#pragma unroll
+                        for (int o = 0; o < NOUT; o++) {
+                            double& v = dst[o][(pb * stride_o[o] + comp_o[o]) * nd + d];
+                            v += w2 * out[o][r];
+                        }
+                    }
+                }
+            }
+        }
+    }
+}
+
 // Downward pass for leaf orders n <= 8: one wave per PAIR of PW leaves with
 // the same cutoff box and level (a missing partner is a zero-weight copy).
 // The 16-wide point dimension holds leaf 0's points in lanes 0-7 and leaf 1's
@@ -1348,15 +1470,31 @@ static int launch_down_pair(Plan& p, const cplx* tab, int lstride, int ymid_half
     return BFGT_OK;
 }
 
-// two y tiles per pass when their count is even; leaf pairs for n <= 8,
-// column groups above
+template <int NTAB, int NTP>
+static int launch_down_quad(Plan& p, const cplx* tab, int lstride, int ymid_half, double* pot, double* grad,
+                            double* hess) {
+    int blocks = (p.nqgroup + kWavesPerBlock - 1) / kWavesPerBlock;
+    k_down_quad<NTAB, NTP><<<blocks, kWave * kWavesPerBlock, 0, p.stream>>>(
+        p.n, p.npw, p.pstride, p.nd, p.nqgroup, p.d_qgroups, p.d_quads, p.d_pwleaf, tab, lstride, ymid_half,
+        p.d_phase, p.d_levbase, reinterpret_cast<const Fold*>(p.d_loc), pot, grad, hess, p.out_scale);
+    BFGT_HIP_CHECK(hipGetLastError());
+    return BFGT_OK;
+}
+
+// two y tiles per pass when their count is even; for n <= 8 2 x 2 leaf
+// blocks over column pairs (leaf pairs where that leaves fewer than 1024
+// waves), column groups above
 template <int NTAB>
 static int launch_down_mfma(Plan& p, const cplx* tab, int lstride, int ymid_half, double* pot, double* grad,
                             double* hess) {
     int nT = round_up((p.npw + 1) / 2, 16) / 16;
-    if (p.n <= 8) {
+    if (p.n <= 8 && p.nqgroup < 1024) {
         if (nT % 2 == 0) return launch_down_pair<NTAB, 2>(p, tab, lstride, ymid_half, pot, grad, hess);
         return launch_down_pair<NTAB, 1>(p, tab, lstride, ymid_half, pot, grad, hess);
+    }
+    if (p.n <= 8) {
+        if (nT % 2 == 0) return launch_down_quad<NTAB, 2>(p, tab, lstride, ymid_half, pot, grad, hess);
+        return launch_down_quad<NTAB, 1>(p, tab, lstride, ymid_half, pot, grad, hess);
     }
     if (nT % 2 == 0) return launch_down_col<NTAB, 2>(p, tab, lstride, ymid_half, pot, grad, hess);
     return launch_down_col<NTAB, 1>(p, tab, lstride, ymid_half, pot, grad, hess);

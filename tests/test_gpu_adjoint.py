@@ -94,11 +94,16 @@ def test_dot_product_identity(device, name, scale, kw, ifpgh):
     assert err < 1e-13, (err, info)
 
 
-def test_dot_product_identity_grouped_translation(device):
-    """The transposed translation lists in groups of four (2,676 cutoff boxes)."""
-    w = make_workload("c3", "full", delta=1e-5, eps=1e-9)
+@pytest.mark.parametrize("delta", [1e-5, 1e-3])
+def test_dot_product_identity_full_size_c3(device, delta):
+    """Full-size c3: the transposed translation lists in groups of four
+    (2,676 cutoff boxes at delta = 1e-5) and the leaf transform's transpose
+    as 2 x 2 leaf blocks over column pairs."""
+    w = make_workload("c3", "full", delta=delta, eps=1e-9)
     err, info = _dot_identity(w.tree, w.nd, w.delta, w.eps, w.ipoly, w.iperiod, w.norder, 3)
-    assert info["translation_group"] == 4
+    assert info["xc_adj_transform"] < info["n_pw_leaves"]
+    if delta == 1e-5:
+        assert info["translation_group"] == 4
     assert err < 1e-13, (err, info)
 
 
