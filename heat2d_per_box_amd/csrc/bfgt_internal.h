@@ -70,6 +70,8 @@ struct TreeNode {
 
 // leaves per column group of the downward pass (leaf order > 8)
 constexpr int kColLeaves = 4;
+// waves (chunks) of one upward workgroup that sum the chunks of a cutoff box
+constexpr int kUpReduceWaves = 4;
 // quads (2 x 2 leaf blocks) per column-pair group of the downward pass (n <= 8)
 constexpr int kQuadsPerGroup = 4;
 
@@ -115,7 +117,16 @@ struct Plan {
     int ncolgrp = 0;               // runs of PW leaves with one cutoff box, level and x cell
     long long nxcol_up = 0;        // column pieces of the upward chunks (x contractions)
     int2* d_upcols = nullptr;      // column pieces (first leaf, count), chunk by chunk
-    int4* d_upcc = nullptr;        // per chunk: (first column piece, count, slot, 0)
+    int4* d_upcc = nullptr;        // per chunk: (first column piece, count, slot, 1)
+    // the same with the chunks of one cutoff box summed within each upward
+    // workgroup (kUpReduceWaves consecutive chunks): (first column piece,
+    // count, destination slot, chunks in the piece at its first chunk else 0)
+    int4* d_upccb = nullptr;
+    int nred1b = 0, nred2b = 0;    // the merge of those pieces
+    int4* d_red1b = nullptr;
+    int4* d_red2b = nullptr;
+    long long merge_reads_b = 0;
+    bool merge_pieces = false;     // the last upward pass summed chunks per workgroup
     long long merge_reads = 0;     // expansions the merge sums (chunk and group slots)
     int2* d_colgrp = nullptr;      // (first leaf, count)
     int nqgroup = 0;               // n <= 8: column pairs evaluated as 2 x 2 leaf blocks

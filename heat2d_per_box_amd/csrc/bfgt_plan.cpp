@@ -445,8 +445,8 @@ int plan_create(Plan& p, int nd, int ndim, double delta, double eps, int ipoly, 
     std::vector<int> anc_box, anc_chunk;
     std::vector<int> m2l_ptr;
     std::vector<int2> m2l;
-    std::vector<int4> red1, red2;
-    int ngslot = 0;
+    std::vector<int4> red1, red2, red1b, red2b, upccb;
+    int ngslot = 0, ngslotb = 0;
     if (p.has_pw) {
         int L0 = p.nlevstart;
         int a0 = t.first(L0);
@@ -555,6 +555,42 @@ int plan_create(Plan& p, int nd, int ndim, double delta, double eps, int ipoly, 
                 red1.push_back(make_int4(a, chunks[c].slot, std::min(G, c1 - c), ngslot++));
             red2.push_back(make_int4(a, g0, ngslot - g0, 0));
         }
+        // The same merge when the upward pass sums, inside each workgroup of
+        // four consecutive chunks, the chunks of one cutoff box (a "piece")
+        // before writing: a box whose chunks form one piece is written
+        // directly; the pieces of the others get consecutive slots.  Per
+        // chunk: (first column piece, count, destination slot, chunks in its
+        // piece if it is the piece's first chunk, else 0).
+        upccb.resize(chunks.size());
+        int nslotb = p.nanc;
+        for (int a = 0; a < p.nanc; a++) {
+            const int c0 = anc_chunk[a], c1 = anc_chunk[a + 1];
+            std::vector<int2> pieces;  // (first chunk, count)
+            for (int c = c0; c < c1;) {
+                int e = c + 1;
+                while (e < c1 && e / kUpReduceWaves == c / kUpReduceWaves) e++;
+                pieces.push_back(make_int2(c, e - c));
+                c = e;
+            }
+            const int np = (int)pieces.size(), first = np == 1 ? a : nslotb;
+            for (int k = 0; k < np; k++)
+                for (int c = pieces[k].x; c < pieces[k].x + pieces[k].y; c++) {
+                    const int cf = chunk_cols[c];
+                    const int cl = c + 1 < (int)chunk_cols.size() ? chunk_cols[c + 1] : (int)upcols.size();
+                    upccb[c] = make_int4(cf, cl - cf, np == 1 ? a : first + k, c == pieces[k].x ? pieces[k].y : 0);
+                }
+            if (np == 1) continue;
+            nslotb += np;
+            if (np <= 8) {
+                red1b.push_back(make_int4(a, first, np, -1));
+                continue;
+            }
+            int G = std::max(8, (int)std::ceil(std::sqrt((double)np)));
+            int g0 = ngslotb;
+            for (int k = 0; k < np; k += G) red1b.push_back(make_int4(a, first + k, std::min(G, np - k), ngslotb++));
+            red2b.push_back(make_int4(a, g0, ngslotb - g0, 0));
+        }
+        p.nmpslot = std::max(p.nmpslot, nslotb);
         // PW translation lists at the cutoff level (gnd_compute_listpw)
         double xmin = boxsize[L0];
         m2l_ptr.push_back(0);
@@ -669,6 +705,11 @@ int plan_create(Plan& p, int nd, int ndim, double delta, double eps, int ipoly, 
     for (const int4& r : red2) p.merge_reads += r.z;
     p.nred2 = (int)red2.size();
     p.ngslot = ngslot;
+    p.nred1b = (int)red1b.size();
+    p.nred2b = (int)red2b.size();
+    p.merge_reads_b = 0;
+    for (const int4& r : red1b) p.merge_reads_b += r.z;
+    for (const int4& r : red2b) p.merge_reads_b += r.z;
 
     // near-field schedule (bfgt/boxfgt.f:923-1071), pairs with empty tables dropped
     std::vector<int> ntgt, nlev, nptr;
@@ -789,9 +830,10 @@ int plan_create(Plan& p, int nd, int ndim, double delta, double eps, int ipoly, 
         std::vector<int4> upcc(chunks.size());
         for (size_t c = 0; c < chunks.size(); c++) {
             const int c0 = chunk_cols[c], c1 = c + 1 < chunks.size() ? chunk_cols[c + 1] : (int)upcols.size();
-            upcc[c] = make_int4(c0, c1 - c0, chunks[c].slot, 0);
+            upcc[c] = make_int4(c0, c1 - c0, chunks[c].slot, 1);
         }
         if ((rc = upload(p, p.d_upcc, upcc))) return rc;
+        if ((rc = upload(p, p.d_upccb, upccb))) return rc;
         if ((rc = upload(p, p.d_upcols, upcols))) return rc;
     }
     if ((rc = upload(p, p.d_chunks, chunks))) return rc;
@@ -799,6 +841,8 @@ int plan_create(Plan& p, int nd, int ndim, double delta, double eps, int ipoly, 
     if ((rc = upload(p, p.d_anc_chunk, anc_chunk))) return rc;
     if ((rc = upload(p, p.d_red1, red1))) return rc;
     if ((rc = upload(p, p.d_red2, red2))) return rc;
+    if ((rc = upload(p, p.d_red1b, red1b))) return rc;
+    if ((rc = upload(p, p.d_red2b, red2b))) return rc;
     if ((rc = upload(p, p.d_m2l_ptr, m2l_ptr))) return rc;
     if ((rc = upload(p, p.d_m2l, m2l))) return rc;
     if ((rc = upload(p, p.d_near_tgt, ntgt))) return rc;
@@ -839,7 +883,7 @@ int plan_create(Plan& p, int nd, int ndim, double delta, double eps, int ipoly, 
     p.has_adjoint = !p.has_pw || use_mfma(p);
     if (p.has_pw) {
         size_t len = (size_t)nd * p.nexp;
-        if ((rc = dalloc(p, p.d_gpart, (size_t)p.ngslot * len))) return rc;
+        if ((rc = dalloc(p, p.d_gpart, (size_t)std::max(p.ngslot, ngslotb) * len))) return rc;
         if ((rc = dalloc(p, p.d_mp, (size_t)p.nmpslot * len))) return rc;
         BFGT_HIP_CHECK(hipMemsetAsync(p.d_mp, 0, (size_t)p.nmpslot * len * sizeof(cplx), p.stream));
         size_t nloc = (size_t)p.nanc * std::max(loc_buffer_cplx(p), 2 * fold_len(p));
@@ -1077,14 +1121,17 @@ int bfgt_plan_work(bfgt_plan_t plan, long long* counts, int ncounts) {
     // per column of a column pair evaluated as 2 x 2 blocks (n <= 8, when
     // there are enough pairs), else one per leaf
     const long long xc_down = big ? p.ncolgrp : (p.nqgroup >= 1024 ? p.nxcol_quad : L);
+    // the forward upward pass sums chunks per workgroup (k_up_col with one
+    // table and at most two x tiles) and then merges workgroup pieces
+    const bool pieces = fp64_mfma && big && (p.npw + 1) / 2 <= 32;
     long long v[7] = {
         fp64_mfma && big ? p.nxcol_up : L,                       // forward upward pass
         fp64_mfma ? xc_down : L,                                 // forward downward pass
         L,                                                       // PW leaves
         p.has_pw && (big || p.ifpgh > 1) ? p.nxcol_up : L,       // adjoint evaluation
         p.has_pw ? xc_down : L,                                  // adjoint leaf transform
-        p.merge_reads,                                           // expansions read by the merge
-        (long long)p.nred1 + p.nred2,                            // expansions written by the merge
+        pieces ? p.merge_reads_b : p.merge_reads,                // expansions read by the merge
+        pieces ? (long long)p.nred1b + p.nred2b : (long long)p.nred1 + p.nred2,  // ... and written
     };
     for (int i = 0; i < ncounts && i < 7; i++) counts[i] = v[i];
     for (int i = 7; i < ncounts; i++) counts[i] = 0;

@@ -218,25 +218,31 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_up_mfma(
 }
 
 // Upward pass over columns (leaf order > 8, and the adjoint of the
-// evaluation): one wave per (chunk, y-frequency tile tN).  chunks[i] = (first
-// column piece, count, partial-sum slot, 0) and cols[c] = (first leaf, count)
-// of a column piece: PW leaves of one cutoff box, level and x cell.  The y
+// evaluation): one wave per (chunk, y-frequency tile tN), workgroup b serving
+// tile b % nTN of the kUpReduceWaves consecutive chunks b / nTN.  chunks[i] =
+// (first column piece, count, destination slot, run) and cols[c] = (first
+// leaf, count) of a column piece: PW leaves of one cutoff box, level and x
+// cell.  With BR the waves of a workgroup whose chunks belong to one cutoff
+// box add their sums through LDS and the first of them (run = their count)
+// writes; without BR every chunk writes its own slot (run = 1).  The y
 // contractions of a column's leaves are summed in G, LPC leaves per pass (the
 // level's table rows are read once for all of them; a pass past the end of
 // the column repeats its last leaf with weight zero), and contracted in x
 // once:  sum_l (T px)^T F_l (T py_l) = (T px)^T sum_l F_l (T py_l).  Same
 // tables, weights and accumulators as k_up_mfma.
-template <int NKS, int NTM, int NTAB, int LPC>
-__global__ __launch_bounds__(kWave * kWavesPerBlock) void k_up_col(
-    int n, int P, int ps, int nd, int nTN, int nitems, LeafIn in, const PwLeaf* __restrict__ leaves,
+template <int NKS, int NTM, int NTAB, int LPC, bool BR>
+__global__ __launch_bounds__(kWave * kUpReduceWaves) void k_up_col(
+    int n, int P, int ps, int nd, int nTN, int nchunks, LeafIn in, const PwLeaf* __restrict__ leaves,
     const int4* __restrict__ chunks, const int2* __restrict__ cols, const cplx* __restrict__ tab, int lstride,
     int ymid_half, const cplx* __restrict__ phase, const int* __restrict__ levbase, cplx* __restrict__ partial) {
     constexpr int NOUT = nchan(NTAB);
+    constexpr int NACC = 16 * NTM;  // accumulator doubles per lane
+    __shared__ double red[BR ? kUpReduceWaves * NACC * kWave : 1];
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
     // wave-uniform: the schedule is read with scalar loads
-    const int item = __builtin_amdgcn_readfirstlane(blockIdx.x * kWavesPerBlock + wave);
-    if (item >= nitems) return;  // no block-wide barriers in this kernel
-    const int ci = item / nTN, tN = item % nTN;
+    const int tN = blockIdx.x % nTN;
+    const int ci = __builtin_amdgcn_readfirstlane((blockIdx.x / nTN) * kUpReduceWaves + wave);
+    const bool valid = ci < nchunks;  // waves past the end still meet the barriers
     const int q = lane >> 4, cc = lane & 15;
     const int H = (P + 1) / 2, Mlo = (P + 1) / 2, nexp = P * H, npbox = n * n;
     const int tstride = 16 * ps;
@@ -246,14 +252,14 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_up_col(
     int pidx[NKS];  // clamped point index of (cc, m2 = 4s + q)
 #pragma unroll
     for (int s = 0; s < NKS; s++) pidx[s] = ccl + n * min(4 * s + q, n - 1);
-    const int4 ch = chunks[ci];
+    const int4 ch = chunks[valid ? ci : 0];
     for (int d = 0; d < nd; d++) {
         v4d acc[4][NTM];
 #pragma unroll
         for (int p = 0; p < 4; p++)
 #pragma unroll
             for (int t = 0; t < NTM; t++) acc[p][t] = v4d{0, 0, 0, 0};
-        for (int c = ch.x; c < ch.x + ch.y; c++) {
+        for (int c = ch.x; valid && c < ch.x + ch.y; c++) {
             const int2 col = cols[c];
             const PwLeaf l0 = leaves[col.x];
             const cplx* T = tab + (size_t)l0.level * lstride;
@@ -314,6 +320,29 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_up_col(
                 }
             }
         }
+        if constexpr (BR) {
+            // sum the accumulators of the run of waves that share this
+            // chunk's destination (consecutive waves of the workgroup)
+            double* mine = red + wave * NACC * kWave;
+#pragma unroll
+            for (int p = 0; p < 4; p++)
+#pragma unroll
+                for (int t = 0; t < NTM; t++)
+#pragma unroll
+                    for (int r = 0; r < 4; r++) mine[((p * NTM + t) * 4 + r) * kWave + lane] = acc[p][t][r];
+            __syncthreads();
+            for (int w = wave + 1; w < wave + ch.w; w++) {
+                const double* other = red + w * NACC * kWave;
+#pragma unroll
+                for (int p = 0; p < 4; p++)
+#pragma unroll
+                    for (int t = 0; t < NTM; t++)
+#pragma unroll
+                        for (int r = 0; r < 4; r++) acc[p][t][r] += other[((p * NTM + t) * 4 + r) * kWave + lane];
+            }
+            __syncthreads();
+        }
+        if (!valid || ch.w == 0) continue;
         cplx* out = partial + ((size_t)ch.z * nd + d) * nexp;
 #pragma unroll
         for (int t = 0; t < NTM; t++) {
@@ -1307,17 +1336,25 @@ struct UpArgs {
 // sums (k_up_col), two leaves per pass with one table, one with more
 template <int NKS, int NTM, int NTAB>
 static int launch_up_mfma3(Plan& p, const UpArgs& u, int nTN) {
-    int nitems = p.nchunks * nTN;
-    int blocks = (nitems + kWavesPerBlock - 1) / kWavesPerBlock;
     if constexpr (NTAB == 1 && NKS <= 2) {
+        const int nitems = p.nchunks * nTN;
+        const int blocks = (nitems + kWavesPerBlock - 1) / kWavesPerBlock;
+        p.merge_pieces = false;
         k_up_mfma<NKS, NTM, NTAB><<<blocks, kWave * kWavesPerBlock, 0, p.stream>>>(
             p.n, p.npw, p.pstride, p.nd, nTN, nitems, u.in, p.d_pwleaf, p.d_chunks, u.tab, u.lstride, u.ymid_half,
             p.d_phase, p.d_levbase, p.d_mp);
     } else {
+        // chunks of one cutoff box summed per workgroup where the
+        // accumulators fit LDS (two x tiles: 16 KB per wave) and the
+        // registers leave room for it (one table; with the adjoint's two or
+        // three the reducing form measured slower)
         constexpr int LPC = NTAB == 1 ? 2 : 1;
-        k_up_col<NKS, NTM, NTAB, LPC><<<blocks, kWave * kWavesPerBlock, 0, p.stream>>>(
-            p.n, p.npw, p.pstride, p.nd, nTN, nitems, u.in, p.d_pwleaf, p.d_upcc, p.d_upcols, u.tab, u.lstride,
-            u.ymid_half, p.d_phase, p.d_levbase, p.d_mp);
+        constexpr bool BR = NTM <= 2 && NTAB == 1;
+        const int nblk = nTN * ((p.nchunks + kUpReduceWaves - 1) / kUpReduceWaves);
+        k_up_col<NKS, NTM, NTAB, LPC, BR><<<nblk, kWave * kUpReduceWaves, 0, p.stream>>>(
+            p.n, p.npw, p.pstride, p.nd, nTN, p.nchunks, u.in, p.d_pwleaf, BR ? p.d_upccb : p.d_upcc, p.d_upcols,
+            u.tab, u.lstride, u.ymid_half, p.d_phase, p.d_levbase, p.d_mp);
+        p.merge_pieces = BR;
     }
     BFGT_HIP_CHECK(hipGetLastError());
     return BFGT_OK;
@@ -1386,6 +1423,7 @@ static int launch_up_split(Plan& p, const double* fvals) {
 }
 
 int launch_upward(Plan& p, const double* fvals) {
+    p.merge_pieces = false;
     if (p.nchunks == 0) return BFGT_OK;
     if (use_mfma(p) && p.precision == BFGT_PREC_SPLIT32) return launch_up_split(p, fvals);
     if (use_mfma(p)) {
@@ -1402,15 +1440,20 @@ int launch_upward(Plan& p, const double* fvals) {
     return set_error(BFGT_ERR_RESOURCE, "plane-wave expansion too long for the leaf kernels");
 }
 
+// the merge schedule of the upward pass that ran: per chunk, or per
+// workgroup piece
 int launch_reduce(Plan& p) {
     int len = p.nd * p.nexp;
     int gy = (len + 255) / 256;
-    if (p.nred1 > 0) {
-        k_reduce1<<<dim3(p.nred1, gy), 256, 0, p.stream>>>(len, p.d_red1, p.d_mp, p.d_gpart, p.d_mp);
+    const int n1 = p.merge_pieces ? p.nred1b : p.nred1, n2 = p.merge_pieces ? p.nred2b : p.nred2;
+    if (n1 > 0) {
+        k_reduce1<<<dim3(n1, gy), 256, 0, p.stream>>>(len, p.merge_pieces ? p.d_red1b : p.d_red1, p.d_mp,
+                                                       p.d_gpart, p.d_mp);
         BFGT_HIP_CHECK(hipGetLastError());
     }
-    if (p.nred2 > 0) {
-        k_reduce2<<<dim3(p.nred2, gy), 256, 0, p.stream>>>(len, p.d_red2, p.d_gpart, p.d_mp);
+    if (n2 > 0) {
+        k_reduce2<<<dim3(n2, gy), 256, 0, p.stream>>>(len, p.merge_pieces ? p.d_red2b : p.d_red2, p.d_gpart,
+                                                       p.d_mp);
         BFGT_HIP_CHECK(hipGetLastError());
     }
     return BFGT_OK;
