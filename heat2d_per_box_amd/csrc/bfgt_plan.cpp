@@ -579,7 +579,7 @@ int plan_create(Plan& p, int nd, int ndim, double delta, double eps, int ipoly, 
                     const int cl = c + 1 < (int)chunk_cols.size() ? chunk_cols[c + 1] : (int)upcols.size();
                     upccb[c] = make_int4(cf, cl - cf, np == 1 ? a : first + k, c == pieces[k].x ? pieces[k].y : 0);
                 }
-            if (np == 1) continue;
+            if (np <= 1) continue;  // written directly, or no PW leaves (mp stays zero)
             nslotb += np;
             if (np <= 8) {
                 red1b.push_back(make_int4(a, first, np, -1));
@@ -1121,9 +1121,9 @@ int bfgt_plan_work(bfgt_plan_t plan, long long* counts, int ncounts) {
     // per column of a column pair evaluated as 2 x 2 blocks (n <= 8, when
     // there are enough pairs), else one per leaf
     const long long xc_down = big ? p.ncolgrp : (p.nqgroup >= 1024 ? p.nxcol_quad : L);
-    // the forward upward pass sums chunks per workgroup (k_up_col with one
-    // table and at most two x tiles) and then merges workgroup pieces
-    const bool pieces = fp64_mfma && big && (p.npw + 1) / 2 <= 32;
+    // the forward upward pass sums chunks per workgroup (at most two x
+    // tiles) and then merges workgroup pieces
+    const bool pieces = fp64_mfma && (p.npw + 1) / 2 <= 32 && p.nred1 + p.nred2 > 0;
     long long v[7] = {
         fp64_mfma && big ? p.nxcol_up : L,                       // forward upward pass
         fp64_mfma ? xc_down : L,                                 // forward downward pass

@@ -107,19 +107,23 @@ __device__ inline double leaf_in(const LeafIn& in, int o, size_t pb, int nd, int
 // odd expansion.  Leaf values are read at clamped point indices (finite
 // values) and meet zero table rows beyond n.  NKS = ceil(n/4),
 // NTM = ceil(ceil(npw/2)/16).
-template <int NKS, int NTM, int NTAB>
-__global__ __launch_bounds__(kWave * kWavesPerBlock) void k_up_mfma(
-    int n, int P, int ps, int nd, int nTN, int nitems, LeafIn in, const PwLeaf* __restrict__ leaves,
-    const UpChunk* __restrict__ chunks, const cplx* __restrict__ tab, int lstride, int ymid_half,
-    const cplx* __restrict__ phase, const int* __restrict__ levbase, cplx* __restrict__ partial) {
+template <int NKS, int NTM, int NTAB, bool BR>
+__global__ __launch_bounds__(kWave * kUpReduceWaves) void k_up_mfma(
+    int n, int P, int ps, int nd, int nTN, int nchunks, LeafIn in, const PwLeaf* __restrict__ leaves,
+    const UpChunk* __restrict__ chunks, const int4* __restrict__ dstrun, const cplx* __restrict__ tab,
+    int lstride, int ymid_half, const cplx* __restrict__ phase, const int* __restrict__ levbase,
+    cplx* __restrict__ partial) {
     constexpr int NOUT = nchan(NTAB);
+    constexpr int NACC = 16 * NTM;  // accumulator doubles per lane
+    __shared__ double red[BR ? kUpReduceWaves * NACC * kWave : 1];
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
-    // the multi-channel (adjoint) instances read their schedule with scalar
-    // loads; the single-channel forward measured faster with vector loads
-    const int item = NTAB > 1 ? __builtin_amdgcn_readfirstlane(blockIdx.x * kWavesPerBlock + wave)
-                              : blockIdx.x * kWavesPerBlock + wave;
-    if (item >= nitems) return;  // no block-wide barriers in this kernel
-    const int ci = item / nTN, tN = item % nTN;
+    // workgroup b serves y tile b % nTN of the kUpReduceWaves consecutive
+    // chunks b / nTN; with BR the chunks of one cutoff box in the workgroup
+    // add their sums through LDS and the first writes (dstrun = (., .,
+    // destination slot, run)), else every chunk writes its own slot
+    const int tN = blockIdx.x % nTN;
+    const int ci = (blockIdx.x / nTN) * kUpReduceWaves + wave;
+    const bool valid = ci < nchunks;  // waves past the end still meet the barriers
     const int q = lane >> 4, cc = lane & 15;
     const int H = (P + 1) / 2, Mlo = (P + 1) / 2, nexp = P * H, npbox = n * n;
     const int tstride = 16 * ps;
@@ -129,7 +133,13 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_up_mfma(
     int pidx[NKS];  // clamped point index of (cc, m2 = 4s + q)
 #pragma unroll
     for (int s = 0; s < NKS; s++) pidx[s] = ccl + n * min(4 * s + q, n - 1);
-    const UpChunk ch = chunks[ci];
+    const UpChunk ch = chunks[valid ? ci : 0];
+    int dst = ch.slot, run = 1;
+    if constexpr (BR) {
+        const int4 dr = dstrun[valid ? ci : 0];
+        dst = dr.z;
+        run = dr.w;
+    }
     for (int d = 0; d < nd; d++) {
         v4d acc[4][NTM];
 #pragma unroll
@@ -142,7 +152,7 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_up_mfma(
         // the end of the chunk repeats its last leaf with weight zero (exact
         // zeros; the sums keep their order).
         constexpr int LP = (NTAB == 1 && NKS <= 2) ? 2 : 1;
-        for (int li = 0; li < ch.count; li += LP) {
+        for (int li = 0; valid && li < ch.count; li += LP) {
             PwLeaf lf[LP];
             double wl[LP];
 #pragma unroll
@@ -200,7 +210,28 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_up_mfma(
                 }
             }
         }
-        cplx* out = partial + ((size_t)ch.slot * nd + d) * nexp;
+        if constexpr (BR) {
+            double* mine = red + wave * NACC * kWave;
+#pragma unroll
+            for (int p = 0; p < 4; p++)
+#pragma unroll
+                for (int t = 0; t < NTM; t++)
+#pragma unroll
+                    for (int r = 0; r < 4; r++) mine[((p * NTM + t) * 4 + r) * kWave + lane] = acc[p][t][r];
+            __syncthreads();
+            for (int w = wave + 1; w < wave + run; w++) {
+                const double* other = red + w * NACC * kWave;
+#pragma unroll
+                for (int p = 0; p < 4; p++)
+#pragma unroll
+                    for (int t = 0; t < NTM; t++)
+#pragma unroll
+                        for (int r = 0; r < 4; r++) acc[p][t][r] += other[((p * NTM + t) * 4 + r) * kWave + lane];
+            }
+            __syncthreads();
+        }
+        if (!valid || run == 0) continue;
+        cplx* out = partial + ((size_t)dst * nd + d) * nexp;
 #pragma unroll
         for (int t = 0; t < NTM; t++) {
 #pragma unroll
@@ -1333,29 +1364,30 @@ struct UpArgs {
 };
 
 // one table, n <= 8: two leaves per pass (k_up_mfma); otherwise column
-// sums (k_up_col), two leaves per pass with one table, one with more
+// sums (k_up_col), two leaves per pass with one table, one with more; with
+// one table and at most two x tiles the chunks of a cutoff box are summed
+// per workgroup (merge over workgroup pieces)
 template <int NKS, int NTM, int NTAB>
 static int launch_up_mfma3(Plan& p, const UpArgs& u, int nTN) {
+    // chunks of one cutoff box summed per workgroup where there is a merge
+    // to shrink, the accumulators fit LDS (two x tiles: 16 KB per wave) and
+    // the registers leave room for it (one table; with the adjoint's two or
+    // three tables the reducing form measured slower)
+    const bool br = NTM <= 2 && NTAB == 1 && p.nred1 + p.nred2 > 0;
+    const int nblk = nTN * ((p.nchunks + kUpReduceWaves - 1) / kUpReduceWaves);
     if constexpr (NTAB == 1 && NKS <= 2) {
-        const int nitems = p.nchunks * nTN;
-        const int blocks = (nitems + kWavesPerBlock - 1) / kWavesPerBlock;
-        p.merge_pieces = false;
-        k_up_mfma<NKS, NTM, NTAB><<<blocks, kWave * kWavesPerBlock, 0, p.stream>>>(
-            p.n, p.npw, p.pstride, p.nd, nTN, nitems, u.in, p.d_pwleaf, p.d_chunks, u.tab, u.lstride, u.ymid_half,
-            p.d_phase, p.d_levbase, p.d_mp);
+        auto kern = br ? k_up_mfma<NKS, NTM, NTAB, NTM <= 2> : k_up_mfma<NKS, NTM, NTAB, false>;
+        kern<<<nblk, kWave * kUpReduceWaves, 0, p.stream>>>(p.n, p.npw, p.pstride, p.nd, nTN, p.nchunks, u.in,
+                                                             p.d_pwleaf, p.d_chunks, p.d_upccb, u.tab, u.lstride,
+                                                             u.ymid_half, p.d_phase, p.d_levbase, p.d_mp);
     } else {
-        // chunks of one cutoff box summed per workgroup where the
-        // accumulators fit LDS (two x tiles: 16 KB per wave) and the
-        // registers leave room for it (one table; with the adjoint's two or
-        // three the reducing form measured slower)
         constexpr int LPC = NTAB == 1 ? 2 : 1;
-        constexpr bool BR = NTM <= 2 && NTAB == 1;
-        const int nblk = nTN * ((p.nchunks + kUpReduceWaves - 1) / kUpReduceWaves);
-        k_up_col<NKS, NTM, NTAB, LPC, BR><<<nblk, kWave * kUpReduceWaves, 0, p.stream>>>(
-            p.n, p.npw, p.pstride, p.nd, nTN, p.nchunks, u.in, p.d_pwleaf, BR ? p.d_upccb : p.d_upcc, p.d_upcols,
-            u.tab, u.lstride, u.ymid_half, p.d_phase, p.d_levbase, p.d_mp);
-        p.merge_pieces = BR;
+        auto kern = br ? k_up_col<NKS, NTM, NTAB, LPC, NTM <= 2 && NTAB == 1> : k_up_col<NKS, NTM, NTAB, LPC, false>;
+        kern<<<nblk, kWave * kUpReduceWaves, 0, p.stream>>>(p.n, p.npw, p.pstride, p.nd, nTN, p.nchunks, u.in,
+                                                             p.d_pwleaf, br ? p.d_upccb : p.d_upcc, p.d_upcols, u.tab,
+                                                             u.lstride, u.ymid_half, p.d_phase, p.d_levbase, p.d_mp);
     }
+    p.merge_pieces = br;
     BFGT_HIP_CHECK(hipGetLastError());
     return BFGT_OK;
 }
