@@ -126,6 +126,15 @@ struct Plan {
     int4* d_red1b = nullptr;
     int4* d_red2b = nullptr;
     long long merge_reads_b = 0;
+    // n <= 8 upward pass: per chunk (first segment, count, destination,
+    // run) for either merge; segments (first quad, count, first leaf of
+    // column piece A, of B or -1) pair a chunk's column pieces of one level,
+    // quads as for the downward pass
+    int4* d_upqcc = nullptr;
+    int4* d_upqccb = nullptr;
+    int4* d_upsegs = nullptr;
+    int4* d_upquads = nullptr;
+    long long nxcol_upq = 0;       // column pieces in those segments (x contractions of n points)
     bool merge_pieces = false;     // the last upward pass summed chunks per workgroup
     long long merge_reads = 0;     // expansions the merge sums (chunk and group slots)
     int2* d_colgrp = nullptr;      // (first leaf, count)

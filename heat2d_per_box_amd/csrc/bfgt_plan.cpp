@@ -442,6 +442,8 @@ int plan_create(Plan& p, int nd, int ndim, double delta, double eps, int ipoly, 
     std::vector<UpChunk> chunks;
     std::vector<int> chunk_cols;  // first column piece of each chunk
     std::vector<int2> upcols;
+    std::vector<int> chunk_segs;  // n <= 8: first column-pair segment of each chunk
+    std::vector<int4> upsegs, upquads;
     std::vector<int> anc_box, anc_chunk;
     std::vector<int> m2l_ptr;
     std::vector<int2> m2l;
@@ -473,9 +475,12 @@ int plan_create(Plan& p, int nd, int ndim, double delta, double eps, int ipoly, 
                 by_anc[a - a0].push_back(lf);
             }
         // leaves per chunk: 8, or 16 where the upward pass still has at least
-        // eight waves per SIMD (fewer chunk sums to write and merge)
+        // eight waves per SIMD (fewer chunk sums to write and merge); 16 for
+        // n <= 8, where the chunk's columns are evaluated in pairs
         int chunk_leaves = kChunkLeaves;
-        if (use_mfma(p)) {
+        if (use_mfma(p) && p.n <= 8) {
+            chunk_leaves = 2 * kChunkLeaves;
+        } else if (use_mfma(p)) {
             const long nTN = ((p.npw + 1) / 2 + 15) / 16;
             long waves16 = 0;
             for (int a = 0; a < p.nanc; a++) waves16 += nTN * (((long)by_anc[a].size() + 15) / 16);
@@ -508,12 +513,40 @@ int plan_create(Plan& p, int nd, int ndim, double delta, double eps, int ipoly, 
                     break;
                 }
                 // the chunk's column pieces
-                chunk_cols.push_back((int)upcols.size());
+                const int cp0 = (int)upcols.size();
+                chunk_cols.push_back(cp0);
                 for (size_t i = s; i < e;) {
                     size_t j = i + 1;
                     while (j < e && v[j].level == v[i].level && v[j].jx == v[i].jx) j++;
                     upcols.push_back(make_int2((int)(leaves.size() + i), (int)(j - i)));
                     i = j;
+                }
+                // n <= 8: consecutive column pieces of one level in pairs,
+                // as 2 x 2 leaf blocks (quads, as for the downward pass)
+                chunk_segs.push_back((int)upsegs.size());
+                if (p.n <= 8) {
+                    const int base = (int)leaves.size(), cp1 = (int)upcols.size();
+                    for (int k = cp0; k < cp1;) {
+                        const int2 ca = upcols[k];
+                        const bool two = k + 1 < cp1 && v[upcols[k + 1].x - base].level == v[ca.x - base].level;
+                        const int2 cb = two ? upcols[k + 1] : make_int2(0, 0);
+                        std::vector<int3> rows;  // (jy, leaf in A or -1, leaf in B or -1)
+                        int ia = 0, ib = 0;
+                        while (ia < ca.y || ib < cb.y) {
+                            const int ja = ia < ca.y ? v[ca.x - base + ia].jy : INT32_MAX;
+                            const int jb = ib < cb.y ? v[cb.x - base + ib].jy : INT32_MAX;
+                            const int jy = std::min(ja, jb);
+                            rows.push_back(make_int3(jy, ja == jy ? ca.x + ia++ : -1, jb == jy ? cb.x + ib++ : -1));
+                        }
+                        const int q0 = (int)upquads.size();
+                        for (size_t r = 0; r < rows.size(); r += 2) {
+                            const int3 ra = rows[r], rb = r + 1 < rows.size() ? rows[r + 1] : make_int3(0, -1, -1);
+                            upquads.push_back(make_int4(ra.y, ra.z, rb.y, rb.z));
+                        }
+                        upsegs.push_back(make_int4(q0, (int)upquads.size() - q0, ca.x, two ? cb.x : -1));
+                        p.nxcol_upq += two ? 2 : 1;
+                        k += two ? 2 : 1;
+                    }
                 }
                 UpChunk c;
                 c.anc = a;
@@ -834,6 +867,19 @@ int plan_create(Plan& p, int nd, int ndim, double delta, double eps, int ipoly, 
         }
         if ((rc = upload(p, p.d_upcc, upcc))) return rc;
         if ((rc = upload(p, p.d_upccb, upccb))) return rc;
+        if (p.n <= 8) {
+            // per chunk (first segment, count, destination, run), both merges
+            std::vector<int4> sa(chunks.size()), sb(chunks.size());
+            for (size_t c = 0; c < chunks.size(); c++) {
+                const int s0 = chunk_segs[c], s1 = c + 1 < chunks.size() ? chunk_segs[c + 1] : (int)upsegs.size();
+                sa[c] = make_int4(s0, s1 - s0, chunks[c].slot, 1);
+                sb[c] = make_int4(s0, s1 - s0, upccb[c].z, upccb[c].w);
+            }
+            if ((rc = upload(p, p.d_upqcc, sa))) return rc;
+            if ((rc = upload(p, p.d_upqccb, sb))) return rc;
+            if ((rc = upload(p, p.d_upsegs, upsegs))) return rc;
+            if ((rc = upload(p, p.d_upquads, upquads))) return rc;
+        }
         if ((rc = upload(p, p.d_upcols, upcols))) return rc;
     }
     if ((rc = upload(p, p.d_chunks, chunks))) return rc;
@@ -1125,10 +1171,10 @@ int bfgt_plan_work(bfgt_plan_t plan, long long* counts, int ncounts) {
     // tiles) and then merges workgroup pieces
     const bool pieces = fp64_mfma && (p.npw + 1) / 2 <= 32 && p.nred1 + p.nred2 > 0;
     long long v[7] = {
-        fp64_mfma && big ? p.nxcol_up : L,                       // forward upward pass
+        fp64_mfma ? (big ? p.nxcol_up : p.nxcol_upq) : L,        // forward upward pass
         fp64_mfma ? xc_down : L,                                 // forward downward pass
         L,                                                       // PW leaves
-        p.has_pw && (big || p.ifpgh > 1) ? p.nxcol_up : L,       // adjoint evaluation
+        p.has_pw ? (big || p.ifpgh > 1 ? p.nxcol_up : p.nxcol_upq) : L,  // adjoint evaluation
         p.has_pw ? xc_down : L,                                  // adjoint leaf transform
         pieces ? p.merge_reads_b : p.merge_reads,                // expansions read by the merge
         pieces ? (long long)p.nred1b + p.nred2b : (long long)p.nred1 + p.nred2,  // ... and written

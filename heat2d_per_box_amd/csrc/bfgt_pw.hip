@@ -93,7 +93,7 @@ __device__ inline double leaf_in(const LeafIn& in, int o, size_t pb, int nd, int
 // every load is unconditional and in bounds, and padding rows or columns
 // either contribute exact zeros or land in outputs that are not stored.
 
-// Upward pass.  One wave per (chunk, y-frequency tile tN): for every leaf of
+// Upward pass.  One wave per (chunk, y-frequency tile tN): for the leaves of
 // the chunk
 //   G_v  = sum_o F_o (T_y(o) diag py w)    (rows m1 < 16, columns k2 of the tile)
 //   out += sum_v (T_v diag px)^T G_v       with T(m, npw-1-k) = conj T(m, k), so
@@ -106,107 +106,96 @@ __device__ inline double leaf_in(const LeafIn& in, int o, size_t pb, int nd, int
 // conjugated evaluation tables and w = 1/2 on the middle y frequency of an
 // odd expansion.  Leaf values are read at clamped point indices (finite
 // values) and meet zero table rows beyond n.  NKS = ceil(n/4),
-// NTM = ceil(ceil(npw/2)/16).
-template <int NKS, int NTM, int NTAB, bool BR>
-__global__ __launch_bounds__(kWave * kUpReduceWaves) void k_up_mfma(
-    int n, int P, int ps, int nd, int nTN, int nchunks, LeafIn in, const PwLeaf* __restrict__ leaves,
-    const UpChunk* __restrict__ chunks, const int4* __restrict__ dstrun, const cplx* __restrict__ tab,
-    int lstride, int ymid_half, const cplx* __restrict__ phase, const int* __restrict__ levbase,
-    cplx* __restrict__ partial) {
-    constexpr int NOUT = nchan(NTAB);
+// NTM = ceil(ceil(npw/2)/16).  Workgroup b serves y tile b % nTN of the
+// kUpReduceWaves consecutive chunks b / nTN; with BR the chunks of one
+// cutoff box in the workgroup add their sums through LDS and the first of
+// them writes (chunk entries (., ., destination slot, run)), else every
+// chunk writes its own slot.
+
+// Upward pass for leaf orders n <= 8, one table: the column pieces of a
+// chunk in pairs (a segment: column pieces A and B of one level, x cells jxA
+// and jxB) and their leaves as 2 x 2 blocks (quads, as in the downward
+// pass).  Per quad the y contraction G += F (T diag py) runs with the two
+// columns in the two 8-wide halves of G's rows (x points) and the quad's two
+// y cells in the two halves of the 16-long K dimension (y points), so one
+// product serves four leaves; per segment the x contraction out += (T diag
+// px)^T G runs once with the two columns in the two halves of its K
+// dimension.  Missing leaves read as zero; ymid_half weights the middle y
+// frequency by 1/2 (the adjoint of the evaluation).
+template <int NTM, bool BR>
+__global__ __launch_bounds__(kWave * kUpReduceWaves) void k_up_quad(
+    int n, int P, int ps, int nd, int nTN, int nchunks, const double* __restrict__ fvals,
+    const PwLeaf* __restrict__ leaves, const int4* __restrict__ chunks, const int4* __restrict__ segs,
+    const int4* __restrict__ quads, const cplx* __restrict__ tab, int lstride, int ymid_half,
+    const cplx* __restrict__ phase, const int* __restrict__ levbase, cplx* __restrict__ partial) {
     constexpr int NACC = 16 * NTM;  // accumulator doubles per lane
     __shared__ double red[BR ? kUpReduceWaves * NACC * kWave : 1];
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
-    // workgroup b serves y tile b % nTN of the kUpReduceWaves consecutive
-    // chunks b / nTN; with BR the chunks of one cutoff box in the workgroup
-    // add their sums through LDS and the first writes (dstrun = (., .,
-    // destination slot, run)), else every chunk writes its own slot
     const int tN = blockIdx.x % nTN;
-    const int ci = (blockIdx.x / nTN) * kUpReduceWaves + wave;
+    const int ci = __builtin_amdgcn_readfirstlane((blockIdx.x / nTN) * kUpReduceWaves + wave);
     const bool valid = ci < nchunks;  // waves past the end still meet the barriers
     const int q = lane >> 4, cc = lane & 15;
+    const int half = cc >> 3, xi = min(cc & 7, n - 1);
     const int H = (P + 1) / 2, Mlo = (P + 1) / 2, nexp = P * H, npbox = n * n;
-    const int tstride = 16 * ps;
     const int k2 = 16 * tN + cc;
     const double wy = (ymid_half && k2 >= P / 2) ? 0.5 : 1.0;
-    const int ccl = min(cc, n - 1);
-    int pidx[NKS];  // clamped point index of (cc, m2 = 4s + q)
-#pragma unroll
-    for (int s = 0; s < NKS; s++) pidx[s] = ccl + n * min(4 * s + q, n - 1);
-    const UpChunk ch = chunks[valid ? ci : 0];
-    int dst = ch.slot, run = 1;
-    if constexpr (BR) {
-        const int4 dr = dstrun[valid ? ci : 0];
-        dst = dr.z;
-        run = dr.w;
-    }
+    const int4 ch = chunks[valid ? ci : 0];
     for (int d = 0; d < nd; d++) {
         v4d acc[4][NTM];
 #pragma unroll
         for (int p = 0; p < 4; p++)
 #pragma unroll
             for (int t = 0; t < NTM; t++) acc[p][t] = v4d{0, 0, 0, 0};
-        // LP leaves per pass: one leaf's y-to-x dependency and load latency are
-        // covered by the other's work (measured faster for n <= 8 only; at
-        // n = 16 the second leaf's operands crowd the registers).  A pass past
-        // the end of the chunk repeats its last leaf with weight zero (exact
-        // zeros; the sums keep their order).
-        constexpr int LP = (NTAB == 1 && NKS <= 2) ? 2 : 1;
-        for (int li = 0; valid && li < ch.count; li += LP) {
-            PwLeaf lf[LP];
-            double wl[LP];
+        for (int sg = ch.x; valid && sg < ch.x + ch.y; sg++) {
+            const int4 seg = segs[sg];
+            const PwLeaf la = leaves[seg.z];
+            const int jxb = seg.w >= 0 ? leaves[seg.w].jx : la.jx;
+            const double wb = seg.w >= 0 ? 1.0 : 0.0;
+            const cplx* T = tab + (size_t)la.level * lstride;
+            const cplx* ph = phase + levbase[la.level];
+            v4d Gr = v4d{0, 0, 0, 0}, Gi = v4d{0, 0, 0, 0};
+            for (int k = seg.x; k < seg.x + seg.y; k++) {
+                const int4 qd = quads[k];
+                // rows of the quad: y cells of (A or B) in row A and in row B
+                const int ra = qd.x >= 0 ? qd.x : qd.y, rb = qd.z >= 0 ? qd.z : qd.w;
+                const int jya = leaves[ra].jy, jyb = rb >= 0 ? leaves[rb].jy : jya;
+                // this lane's leaves (its x half) in rows A and B, zero weight if absent
+                const int lfa = half ? qd.y : qd.x, lfb = half ? qd.w : qd.z;
+                const size_t pba = (size_t)leaves[lfa >= 0 ? lfa : ra].box * npbox;
+                const size_t pbb = (size_t)leaves[lfb >= 0 ? lfb : ra].box * npbox;
+                const double wa = lfa >= 0 ? 1.0 : 0.0, wbr = lfb >= 0 ? 1.0 : 0.0;
+                cplx pya = ph[jya * ps + k2], pyb = ph[jyb * ps + k2];
+                pya.x *= wy;  // upward shift: conjugate of the table
+                pya.y *= -wy;
+                pyb.x *= wy;
+                pyb.y *= -wy;
+                if (rb < 0) pyb = cplx{0.0, 0.0};
 #pragma unroll
-            for (int h = 0; h < LP; h++) {
-                lf[h] = leaves[ch.first + min(li + h, ch.count - 1)];
-                wl[h] = li + h < ch.count ? 1.0 : 0.0;
-            }
-            v4d Gr[LP][NTAB], Gi[LP][NTAB];
-#pragma unroll
-            for (int h = 0; h < LP; h++) {
-                const cplx* T = tab + (size_t)lf[h].level * lstride;
-                const cplx* ph = phase + levbase[lf[h].level];
-                const size_t pb0 = (size_t)lf[h].box * npbox;
-                // y phase of this lane's column (upward shift: conjugate of the table)
-                cplx py = ph[lf[h].jy * ps + k2];
-                py.x *= wy;
-                py.y *= -wy;
-#pragma unroll
-                for (int v = 0; v < NTAB; v++) Gr[h][v] = Gi[h][v] = v4d{0, 0, 0, 0};
-#pragma unroll
-                for (int s = 0; s < NKS; s++) {
-                    const int m2 = 4 * s + q;
-                    cplx ty[NTAB];
-#pragma unroll
-                    for (int u = 0; u < NTAB; u++) ty[u] = cmul(T[u * tstride + m2 * ps + k2], py);
-#pragma unroll
-                    for (int o = 0; o < NOUT; o++) {
-                        double a = wl[h] * leaf_in(in, o, pb0 + pidx[s], nd, d);
-                        Gr[h][chan_x(o)] = mfma(a, ty[chan_y(o)].x, Gr[h][chan_x(o)]);
-                        Gi[h][chan_x(o)] = mfma(a, ty[chan_y(o)].y, Gi[h][chan_x(o)]);
-                    }
+                for (int s = 0; s < 4; s++) {
+                    const int yp = 4 * (s & 1) + q;  // y point of this K index
+                    const cplx py = (s >> 1) ? pyb : pya;
+                    const cplx ty = cmul(T[min(yp, 15) * ps + k2], py);
+                    const double a = ((s >> 1) ? wbr : wa) *
+                                     fvals[(((s >> 1) ? pbb : pba) + xi + (size_t)n * min(yp, n - 1)) * nd + d];
+                    Gr = mfma(a, ty.x, Gr);
+                    Gi = mfma(a, ty.y, Gi);
                 }
             }
 #pragma unroll
             for (int t = 0; t < NTM; t++) {
                 const int k1 = 16 * t + cc;
+                cplx pxa = ph[la.jx * ps + k1], pxb = ph[jxb * ps + k1];
+                pxa.y = -pxa.y;
+                pxb.x *= wb;
+                pxb.y *= -wb;
 #pragma unroll
-                for (int h = 0; h < LP; h++) {
-                    const cplx* T = tab + (size_t)lf[h].level * lstride;
-                    const cplx* ph = phase + levbase[lf[h].level];
-                    cplx px = ph[lf[h].jx * ps + k1];
-                    px.y = -px.y;
-#pragma unroll
-                    for (int v = 0; v < NTAB; v++) {
-#pragma unroll
-                        for (int s = 0; s < NKS; s++) {
-                            const int m1 = 4 * s + q;
-                            cplx tx = cmul(T[v * tstride + m1 * ps + k1], px);
-                            acc[0][t] = mfma(tx.x, Gr[h][v][s], acc[0][t]);
-                            acc[1][t] = mfma(tx.y, Gi[h][v][s], acc[1][t]);
-                            acc[2][t] = mfma(tx.x, Gi[h][v][s], acc[2][t]);
-                            acc[3][t] = mfma(tx.y, Gr[h][v][s], acc[3][t]);
-                        }
-                    }
+                for (int s = 0; s < 4; s++) {
+                    const int xp = 4 * (s & 1) + q;  // x point of this K index, column s >> 1
+                    const cplx tx = cmul(T[xp * ps + k1], (s >> 1) ? pxb : pxa);
+                    acc[0][t] = mfma(tx.x, Gr[s], acc[0][t]);
+                    acc[1][t] = mfma(tx.y, Gi[s], acc[1][t]);
+                    acc[2][t] = mfma(tx.x, Gi[s], acc[2][t]);
+                    acc[3][t] = mfma(tx.y, Gr[s], acc[3][t]);
                 }
             }
         }
@@ -219,7 +208,7 @@ __global__ __launch_bounds__(kWave * kUpReduceWaves) void k_up_mfma(
 #pragma unroll
                     for (int r = 0; r < 4; r++) mine[((p * NTM + t) * 4 + r) * kWave + lane] = acc[p][t][r];
             __syncthreads();
-            for (int w = wave + 1; w < wave + run; w++) {
+            for (int w = wave + 1; w < wave + ch.w; w++) {
                 const double* other = red + w * NACC * kWave;
 #pragma unroll
                 for (int p = 0; p < 4; p++)
@@ -230,8 +219,8 @@ __global__ __launch_bounds__(kWave * kUpReduceWaves) void k_up_mfma(
             }
             __syncthreads();
         }
-        if (!valid || run == 0) continue;
-        cplx* out = partial + ((size_t)dst * nd + d) * nexp;
+        if (!valid || ch.w == 0) continue;
+        cplx* out = partial + ((size_t)ch.z * nd + d) * nexp;
 #pragma unroll
         for (int t = 0; t < NTM; t++) {
 #pragma unroll
@@ -259,8 +248,7 @@ __global__ __launch_bounds__(kWave * kUpReduceWaves) void k_up_mfma(
 // contractions of a column's leaves are summed in G, LPC leaves per pass (the
 // level's table rows are read once for all of them; a pass past the end of
 // the column repeats its last leaf with weight zero), and contracted in x
-// once:  sum_l (T px)^T F_l (T py_l) = (T px)^T sum_l F_l (T py_l).  Same
-// tables, weights and accumulators as k_up_mfma.
+// once:  sum_l (T px)^T F_l (T py_l) = (T px)^T sum_l F_l (T py_l).
 template <int NKS, int NTM, int NTAB, int LPC, bool BR>
 __global__ __launch_bounds__(kWave * kUpReduceWaves) void k_up_col(
     int n, int P, int ps, int nd, int nTN, int nchunks, LeafIn in, const PwLeaf* __restrict__ leaves,
@@ -833,7 +821,8 @@ __device__ inline v4f mfma2(Split a, float b, v4f c) {
 }
 __device__ inline int drow32(int q, int r) { return 4 * q + r; }
 
-// Forward upward pass, split-FP32 (same tiling as k_up_mfma with NTAB = 1).
+// Forward upward pass, split-FP32 (one wave per (chunk, y tile), one leaf at
+// a time, the same two products).
 template <int NKS, int NTM>
 __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_up_split(
     int n, int P, int ps, int nd, int nTN, int nitems, const double* __restrict__ fvals,
@@ -1363,7 +1352,8 @@ struct UpArgs {
     int lstride, ymid_half, ntab;
 };
 
-// one table, n <= 8: two leaves per pass (k_up_mfma); otherwise column
+// one table, n <= 8: 2 x 2 leaf blocks over column pairs (k_up_quad);
+// otherwise column
 // sums (k_up_col), two leaves per pass with one table, one with more; with
 // one table and at most two x tiles the chunks of a cutoff box are summed
 // per workgroup (merge over workgroup pieces)
@@ -1376,10 +1366,12 @@ static int launch_up_mfma3(Plan& p, const UpArgs& u, int nTN) {
     const bool br = NTM <= 2 && NTAB == 1 && p.nred1 + p.nred2 > 0;
     const int nblk = nTN * ((p.nchunks + kUpReduceWaves - 1) / kUpReduceWaves);
     if constexpr (NTAB == 1 && NKS <= 2) {
-        auto kern = br ? k_up_mfma<NKS, NTM, NTAB, NTM <= 2> : k_up_mfma<NKS, NTM, NTAB, false>;
-        kern<<<nblk, kWave * kUpReduceWaves, 0, p.stream>>>(p.n, p.npw, p.pstride, p.nd, nTN, p.nchunks, u.in,
-                                                             p.d_pwleaf, p.d_chunks, p.d_upccb, u.tab, u.lstride,
-                                                             u.ymid_half, p.d_phase, p.d_levbase, p.d_mp);
+        // 2 x 2 leaf blocks (the forward transform, or the adjoint's one-table pass)
+        auto kq = br ? k_up_quad<NTM, NTM <= 2> : k_up_quad<NTM, false>;
+        kq<<<nblk, kWave * kUpReduceWaves, 0, p.stream>>>(p.n, p.npw, p.pstride, p.nd, nTN, p.nchunks, u.in.pot,
+                                                            p.d_pwleaf, br ? p.d_upqccb : p.d_upqcc, p.d_upsegs,
+                                                            p.d_upquads, u.tab, u.lstride, u.ymid_half, p.d_phase,
+                                                            p.d_levbase, p.d_mp);
     } else {
         constexpr int LPC = NTAB == 1 ? 2 : 1;
         auto kern = br ? k_up_col<NKS, NTM, NTAB, LPC, NTM <= 2 && NTAB == 1> : k_up_col<NKS, NTM, NTAB, LPC, false>;
