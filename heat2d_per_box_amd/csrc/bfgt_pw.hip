@@ -131,8 +131,11 @@ __global__ __launch_bounds__(kWave * kUpReduceWaves) void k_up_quad(
     constexpr int NACC = 16 * NTM;  // accumulator doubles per lane
     __shared__ double red[BR ? kUpReduceWaves * NACC * kWave : 1];
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
-    const int tN = blockIdx.x % nTN;
-    const int ci = __builtin_amdgcn_readfirstlane((blockIdx.x / nTN) * kUpReduceWaves + wave);
+    // XCD-contiguous: the y tiles of one chunk run on one XCD (one L2), so
+    // the chunk's leaf values come from HBM once
+    const int lb = xcd_block(blockIdx.x, gridDim.x);
+    const int tN = lb % nTN;
+    const int ci = __builtin_amdgcn_readfirstlane((lb / nTN) * kUpReduceWaves + wave);
     const bool valid = ci < nchunks;  // waves past the end still meet the barriers
     const int q = lane >> 4, cc = lane & 15;
     const int half = cc >> 3, xi = min(cc & 7, n - 1);
@@ -259,8 +262,11 @@ __global__ __launch_bounds__(kWave * kUpReduceWaves) void k_up_col(
     __shared__ double red[BR ? kUpReduceWaves * NACC * kWave : 1];
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
     // wave-uniform: the schedule is read with scalar loads
-    const int tN = blockIdx.x % nTN;
-    const int ci = __builtin_amdgcn_readfirstlane((blockIdx.x / nTN) * kUpReduceWaves + wave);
+    // XCD-contiguous: the y tiles of one chunk run on one XCD (one L2), so
+    // the chunk's leaf values come from HBM once
+    const int lb = xcd_block(blockIdx.x, gridDim.x);
+    const int tN = lb % nTN;
+    const int ci = __builtin_amdgcn_readfirstlane((lb / nTN) * kUpReduceWaves + wave);
     const bool valid = ci < nchunks;  // waves past the end still meet the barriers
     const int q = lane >> 4, cc = lane & 15;
     const int H = (P + 1) / 2, Mlo = (P + 1) / 2, nexp = P * H, npbox = n * n;
