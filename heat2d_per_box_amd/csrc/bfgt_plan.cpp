@@ -336,7 +336,10 @@ int plan_create(Plan& p, int nd, int ndim, double delta, double eps, int ipoly, 
     BFGT_HIP_CHECK(hipGetDevice(&p.device));
     p.stream = (hipStream_t)stream;  // NULL: the device's default stream
     p.own_stream = false;
-    for (auto& e : p.ev) BFGT_HIP_CHECK(hipEventCreate(&e));
+    // stage-timing events without the system-scope fence at record time (its
+    // cache writeback would cost every execution a few microseconds per
+    // event); results are handed over through stream order, never via them
+    for (auto& e : p.ev) BFGT_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
 
     // cutoff level and PW length (bfgt/boxfgt.f:133-167)
     double dcut = std::sqrt(delta * std::log(1.0 / eps));
@@ -1107,7 +1110,7 @@ int bfgt_plan_profile_begin(bfgt_plan_t plan, int max_steps) {
     BFGT_HIP_CHECK(hipSetDevice(p.device));
     while ((int)p.prof_ev.size() < 6 * max_steps) {
         hipEvent_t e;
-        BFGT_HIP_CHECK(hipEventCreate(&e));
+        BFGT_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
         p.prof_ev.push_back(e);
     }
     p.prof_max = max_steps;
