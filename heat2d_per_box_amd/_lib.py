@@ -41,6 +41,7 @@ EXPORTS = [
     "bfgt_plan_work",
     "bfgt_plan_get_table",
     "bfgt_plan_profile_begin",
+    "bfgt_plan_profile_stage",
     "bfgt_plan_profile_end",
     "bfgt_last_error",
     "bfgt_device_count",
@@ -96,6 +97,8 @@ def lib() -> ctypes.CDLL:
     L.bfgt_plan_get_table.argtypes = [_vp, _i, _i, _vp, ctypes.c_size_t]
     L.bfgt_plan_profile_begin.restype = _i
     L.bfgt_plan_profile_begin.argtypes = [_vp, _i]
+    L.bfgt_plan_profile_stage.restype = _i
+    L.bfgt_plan_profile_stage.argtypes = [_vp, _i, _i]
     L.bfgt_plan_profile_end.restype = _i
     L.bfgt_plan_profile_end.argtypes = [_vp, _dp, _ip]
     L.bfgt_last_error.restype = ctypes.c_char_p

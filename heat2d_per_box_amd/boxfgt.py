@@ -202,9 +202,11 @@ class BoxFGT:
         check(lib().bfgt_plan_adjoint(self._h, gp, gg, gh, fb, _ptr(ti) if ti is not None else None))
         return ti
 
-    def profile_begin(self, max_steps: int) -> None:
-        """Record per-stage HIP events for the next ``max_steps`` calls."""
-        check(lib().bfgt_plan_profile_begin(self._h, int(max_steps)))
+    def profile_begin(self, max_steps: int, stage: int = -1) -> None:
+        """Record per-stage HIP events for the next ``max_steps`` calls: around
+        every stage (``stage=-1``) or only around one (0 leaf->PW, 1 merge,
+        2 translate, 4 PW->values, 5 near field)."""
+        check(lib().bfgt_plan_profile_stage(self._h, int(max_steps), int(stage)))
 
     def profile_end(self):
         """Summed seconds per stage over the profiled calls and their count.

@@ -203,6 +203,7 @@ struct Plan {
     hipEvent_t ev[10] = {};
     std::vector<hipEvent_t> prof_ev;  // 6 events per profiled execution
     int prof_max = 0, prof_count = 0;
+    int prof_stage = -1;  // -1: every stage, else the one stage whose two events are recorded
     bool profiling = false;
     double t_plan_host = 0;
     double t_tables = 0;

@@ -946,6 +946,14 @@ int plan_create(Plan& p, int nd, int ndim, double delta, double eps, int ipoly, 
     return BFGT_OK;
 }
 
+// profiled stage s (0 leaf values -> PW, 1 merge, 2 translation, 4 PW ->
+// values, 5 near field) lies between stage boundaries *lo and *hi
+static void stage_marks(int s, int* lo, int* hi) {
+    static const int b[6][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 3}, {3, 4}, {4, 5}};
+    *lo = b[s][0];
+    *hi = b[s][1];
+}
+
 int plan_execute(Plan& p, const double* fvals, double* pot, double* grad, double* hess, double* timeinfo,
                  double scale) {
     if (!fvals || !pot || (p.ifpgh >= 2 && !grad) || (p.ifpgh >= 3 && !hess))
@@ -954,23 +962,37 @@ int plan_execute(Plan& p, const double* fvals, double* pot, double* grad, double
     BFGT_HIP_CHECK(hipSetDevice(p.device));
     p.out_scale = scale;
     int rc;
-    hipEvent_t* ev = p.ev;
-    if (p.profiling && p.prof_count < p.prof_max) ev = p.prof_ev.data() + 6 * p.prof_count++;
-    BFGT_HIP_CHECK(hipEventRecord(ev[0], p.stream));
+    // stage boundaries 0..5 (before the upward pass, after it, after the
+    // merge, the translation, the downward pass and the near field); a plain
+    // execution records none, a profiled one all six or only the two around
+    // the profiled stage
+    hipEvent_t* ev = timeinfo ? p.ev : nullptr;
+    int mark_lo = 0, mark_hi = 5;
+    if (!timeinfo && p.profiling && p.prof_count < p.prof_max) {
+        ev = p.prof_ev.data() + 6 * p.prof_count++;
+        if (p.prof_stage >= 0) stage_marks(p.prof_stage, &mark_lo, &mark_hi);
+    }
+    auto mark = [&](int i) -> int {
+        if (ev && (i == mark_lo || i == mark_hi || (mark_lo == 0 && mark_hi == 5)))
+            BFGT_HIP_CHECK(hipEventRecord(ev[i], p.stream));
+        return BFGT_OK;
+    };
+    if ((rc = mark(0))) return rc;
     if (p.has_pw) {
         if ((rc = launch_upward(p, fvals))) return rc;
-        BFGT_HIP_CHECK(hipEventRecord(ev[1], p.stream));
+        if ((rc = mark(1))) return rc;
         if ((rc = launch_reduce(p))) return rc;
-        BFGT_HIP_CHECK(hipEventRecord(ev[2], p.stream));
+        if ((rc = mark(2))) return rc;
         if ((rc = launch_m2l(p))) return rc;
-        BFGT_HIP_CHECK(hipEventRecord(ev[3], p.stream));
+        if ((rc = mark(3))) return rc;
         if ((rc = launch_downward(p, pot, grad, hess))) return rc;
-        BFGT_HIP_CHECK(hipEventRecord(ev[4], p.stream));
+        if ((rc = mark(4))) return rc;
     } else {
-        for (int i = 1; i <= 4; i++) BFGT_HIP_CHECK(hipEventRecord(ev[i], p.stream));
+        for (int i = 1; i <= 4; i++)
+            if ((rc = mark(i))) return rc;
     }
     if ((rc = launch_near(p, fvals, pot, grad, hess))) return rc;
-    BFGT_HIP_CHECK(hipEventRecord(ev[5], p.stream));
+    if ((rc = mark(5))) return rc;
     if (timeinfo) {
         BFGT_HIP_CHECK(hipEventSynchronize(ev[5]));
         float ms[5];
@@ -1006,22 +1028,27 @@ int plan_adjoint(Plan& p, const double* gpot, const double* ggrad, const double*
     BFGT_HIP_CHECK(hipSetDevice(p.device));
     p.out_scale = 1.0;
     int rc;
-    hipEvent_t* ev = p.ev;
-    BFGT_HIP_CHECK(hipEventRecord(ev[0], p.stream));
+    hipEvent_t* ev = timeinfo ? p.ev : nullptr;  // stage events only when timings are asked for
+    auto mark = [&](int i) -> int {
+        if (ev) BFGT_HIP_CHECK(hipEventRecord(ev[i], p.stream));
+        return BFGT_OK;
+    };
+    if ((rc = mark(0))) return rc;
     if (p.has_pw) {
         if ((rc = launch_adj_eval(p, gpot, ggrad, ghess))) return rc;
-        BFGT_HIP_CHECK(hipEventRecord(ev[1], p.stream));
+        if ((rc = mark(1))) return rc;
         if ((rc = launch_reduce(p))) return rc;
-        BFGT_HIP_CHECK(hipEventRecord(ev[2], p.stream));
+        if ((rc = mark(2))) return rc;
         if ((rc = launch_adj_m2l(p))) return rc;
-        BFGT_HIP_CHECK(hipEventRecord(ev[3], p.stream));
+        if ((rc = mark(3))) return rc;
         if ((rc = launch_adj_transform(p, fbar))) return rc;
-        BFGT_HIP_CHECK(hipEventRecord(ev[4], p.stream));
+        if ((rc = mark(4))) return rc;
     } else {
-        for (int i = 1; i <= 4; i++) BFGT_HIP_CHECK(hipEventRecord(ev[i], p.stream));
+        for (int i = 1; i <= 4; i++)
+            if ((rc = mark(i))) return rc;
     }
     if ((rc = launch_near_adj(p, gpot, ggrad, ghess, fbar))) return rc;
-    BFGT_HIP_CHECK(hipEventRecord(ev[5], p.stream));
+    if ((rc = mark(5))) return rc;
     if (timeinfo) {
         BFGT_HIP_CHECK(hipEventSynchronize(ev[5]));
         float ms[5];
@@ -1104,8 +1131,12 @@ int bfgt_plan_destroy(bfgt_plan_t plan) {
     return BFGT_OK;
 }
 
-int bfgt_plan_profile_begin(bfgt_plan_t plan, int max_steps) {
+int bfgt_plan_profile_begin(bfgt_plan_t plan, int max_steps) { return bfgt_plan_profile_stage(plan, max_steps, -1); }
+
+int bfgt_plan_profile_stage(bfgt_plan_t plan, int max_steps, int stage) {
     if (!plan || max_steps < 1) return set_error(BFGT_ERR_ARG, "invalid profiling request");
+    if (stage != -1 && (stage < 0 || stage > 5 || stage == 3))
+        return set_error(BFGT_ERR_ARG, "profiled stage must be -1 (all), 0, 1, 2, 4 or 5");
     bfgt::Plan& p = plan->p;
     BFGT_HIP_CHECK(hipSetDevice(p.device));
     while ((int)p.prof_ev.size() < 6 * max_steps) {
@@ -1115,6 +1146,7 @@ int bfgt_plan_profile_begin(bfgt_plan_t plan, int max_steps) {
     }
     p.prof_max = max_steps;
     p.prof_count = 0;
+    p.prof_stage = stage;
     p.profiling = true;
     return BFGT_OK;
 }
@@ -1126,18 +1158,15 @@ int bfgt_plan_profile_end(bfgt_plan_t plan, double* stage_seconds, int* nsteps) 
     double acc[6] = {0, 0, 0, 0, 0, 0};
     for (int s = 0; s < p.prof_count; s++) {
         hipEvent_t* ev = p.prof_ev.data() + 6 * s;
-        BFGT_HIP_CHECK(hipEventSynchronize(ev[5]));
-        float ms;
-        BFGT_HIP_CHECK(hipEventElapsedTime(&ms, ev[0], ev[1]));
-        acc[0] += ms * 1e-3;
-        BFGT_HIP_CHECK(hipEventElapsedTime(&ms, ev[1], ev[2]));
-        acc[1] += ms * 1e-3;
-        BFGT_HIP_CHECK(hipEventElapsedTime(&ms, ev[2], ev[3]));
-        acc[2] += ms * 1e-3;
-        BFGT_HIP_CHECK(hipEventElapsedTime(&ms, ev[3], ev[4]));
-        acc[4] += ms * 1e-3;
-        BFGT_HIP_CHECK(hipEventElapsedTime(&ms, ev[4], ev[5]));
-        acc[5] += ms * 1e-3;
+        for (int k = 0; k < 6; k++) {
+            if (k == 3 || (p.prof_stage >= 0 && k != p.prof_stage)) continue;
+            int lo, hi;
+            bfgt::stage_marks(k, &lo, &hi);
+            BFGT_HIP_CHECK(hipEventSynchronize(ev[hi]));
+            float ms;
+            BFGT_HIP_CHECK(hipEventElapsedTime(&ms, ev[lo], ev[hi]));
+            acc[k] += ms * 1e-3;
+        }
     }
     if (stage_seconds) std::memcpy(stage_seconds, acc, sizeof(acc));
     if (nsteps) *nsteps = p.prof_count;

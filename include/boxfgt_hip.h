@@ -138,8 +138,14 @@ int bfgt_plan_destroy(bfgt_plan_t plan);
  * seconds per stage: [0] leaf values -> PW (merge folded in), [1] merge of
  * partial expansions, [2] mp -> loc translation, [3] split (0: folded into
  * [4]), [4] loc -> pot/grad/hess, [5] near field -- the timeinfo(2..7) slots of
- * the reference (bfgt/boxfgt.f:748-1076) -- and the number of executions seen. */
+ * the reference (bfgt/boxfgt.f:748-1076) -- and the number of executions seen.
+ * bfgt_plan_profile_stage(plan, max_steps, stage) records only the two events
+ * around one stage (0, 1, 2, 4 or 5; -1 = every stage, as profile_begin), so
+ * that the other stages run back to back as in an unprofiled execution; the
+ * other entries of stage_seconds are then 0.  Executions outside a profiling
+ * window, and executions called without timeinfo, record no events at all. */
 int bfgt_plan_profile_begin(bfgt_plan_t plan, int max_steps);
+int bfgt_plan_profile_stage(bfgt_plan_t plan, int max_steps, int stage);
 int bfgt_plan_profile_end(bfgt_plan_t plan, double *stage_seconds, int *nsteps);
 
 /* Plan facts: ivals[0..15] = {npwlevel, npw, nexp, nlevstart, nlevend, nleafbox,

@@ -276,3 +276,37 @@ def test_npw_limit_is_an_error(device):
     with pytest.raises(BoxFGTError) as ei:
         BoxFGT(t, 1, 1e-4, 1e-9, 0, 8, 1)
     assert ei.value.code == 7
+
+
+def test_stage_profiling_windows(device):
+    """Profiling windows: all stages, then one stage only (its two events alone),
+    then calls outside any window; none of them changes the results."""
+    import torch
+    from heat2d_per_box_amd.boxfgt import BoxFGTError
+
+    w = make_workload("c3", "tiny", delta=1e-5, eps=1e-9)
+    ref = O.oracle_boxfgt(w.tree, w.fvals, w.nd, w.delta, w.eps, w.ipoly, w.iperiod, w.norder, w.ifpgh)
+    got = run_gpu(w.tree, w.fvals, w.nd, w.delta, w.eps, w.ipoly, w.iperiod, w.norder, w.ifpgh)
+    plan = got["plan"]
+    fv = torch.from_numpy(np.ascontiguousarray(w.fvals)).cuda()
+    nb, npb, nd = w.fvals.shape
+    pot = torch.zeros((nb, npb, nd), dtype=torch.float64, device="cuda")
+    grad = torch.zeros((nb, npb, 2, nd), dtype=torch.float64, device="cuda")
+    hess = torch.zeros((nb, npb, 3, nd), dtype=torch.float64, device="cuda")
+    plan.profile_begin(2)
+    for _ in range(3):  # the third call is past the window
+        plan(fv, pot, grad, hess)
+    s_all, n_all = plan.profile_end()
+    assert n_all == 2 and np.all(s_all >= 0) and s_all[3] == 0
+    assert s_all[0] > 0 and s_all[4] > 0 and s_all[5] > 0
+    plan.profile_begin(4, 4)
+    for _ in range(2):
+        plan(fv, pot, grad, hess)
+    s_one, n_one = plan.profile_end()
+    assert n_one == 2 and s_one[4] > 0
+    assert np.all(np.delete(s_one, 4) == 0)
+    with pytest.raises(BoxFGTError):
+        plan.profile_begin(1, 3)
+    torch.cuda.synchronize()
+    out = {"pot": pot.cpu().numpy(), "grad": grad.cpu().numpy(), "hess": hess.cpu().numpy()}
+    assert_close(compare(w.tree, out, ref, w.ifpgh, scale=5.0))
