@@ -248,7 +248,7 @@ def main():
     else:
         roof = {"bound": "hbm", "achieved": amount / dur / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s"}
     roof["frac"] = roof["achieved"] / roof["peak"]
-    kern = {"leaf_to_pw": "k_up", "merge": "k_reduce", "translate": "k_m2l", "pw_to_values": "k_down",
+    kern = {"leaf_to_pw": "k_up", "merge": "k_merge", "translate": "k_m2l", "pw_to_values": "k_down",
             "near_field": "k_near"}
     roof["kernel"] = kern[dom]
     tr = measured_traffic(f"{args.config}/{args.scale}", args.precision, kern[dom])

@@ -122,9 +122,8 @@ struct Plan {
     // workgroup (kUpReduceWaves consecutive chunks): (first column piece,
     // count, destination slot, chunks in the piece at its first chunk else 0)
     int4* d_upccb = nullptr;
-    int nred1b = 0, nred2b = 0;    // the merge of those pieces
-    int4* d_red1b = nullptr;
-    int4* d_red2b = nullptr;
+    int nmergeb = 0;               // the merge of those pieces (as d_merge)
+    int4* d_mergeb = nullptr;
     long long merge_reads_b = 0;
     // n <= 8 upward pass: per chunk (first segment, count, destination,
     // run) for either merge; segments (first quad, count, first leaf of
@@ -136,7 +135,7 @@ struct Plan {
     int4* d_upquads = nullptr;
     long long nxcol_upq = 0;       // column pieces in those segments (x contractions of n points)
     bool merge_pieces = false;     // the last upward pass summed chunks per workgroup
-    long long merge_reads = 0;     // expansions the merge sums (chunk and group slots)
+    long long merge_reads = 0;     // expansions the merge sums
     int2* d_colgrp = nullptr;      // (first leaf, count)
     int nqgroup = 0;               // n <= 8: column pairs evaluated as 2 x 2 leaf blocks
     int4* d_qgroups = nullptr;     // (first quad, count, a leaf of column A, a leaf of column B or -1)
@@ -145,13 +144,10 @@ struct Plan {
     UpChunk* d_chunks = nullptr;
     int* d_anc_box = nullptr;      // ancestor index -> box
     int* d_anc_chunk = nullptr;    // CSR over chunks [nanc+1]
-    // merge of chunk partials: level 1 sums groups of chunks (into mp
-    // directly when an ancestor has one group), level 2 sums the groups of
-    // ancestors with several
-    int nred1 = 0, nred2 = 0, ngslot = 0;
-    int4* d_red1 = nullptr;        // (anc, first chunk slot, count, dest group slot or -1 for mp)
-    int4* d_red2 = nullptr;        // (anc, first group slot, count, 0)
-    cplx* d_gpart = nullptr;       // [ngslot][nd][nexp]
+    // merge of chunk partial sums: one entry per cutoff box with several
+    // (anc, first slot, count, 0), the slots consecutive past the mp slots
+    int nmerge = 0;
+    int4* d_merge = nullptr;
     int* d_m2l_ptr = nullptr;      // CSR [nanc+1]
     int2* d_m2l = nullptr;         // (source ancestor, shift index)
     int nnear_t = 0, nnear_p = 0;

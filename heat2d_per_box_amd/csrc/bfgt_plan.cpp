@@ -450,8 +450,7 @@ int plan_create(Plan& p, int nd, int ndim, double delta, double eps, int ipoly, 
     std::vector<int> anc_box, anc_chunk;
     std::vector<int> m2l_ptr;
     std::vector<int2> m2l;
-    std::vector<int4> red1, red2, red1b, red2b, upccb;
-    int ngslot = 0, ngslotb = 0;
+    std::vector<int4> merge, mergeb, upccb;
     if (p.has_pw) {
         int L0 = p.nlevstart;
         int a0 = t.first(L0);
@@ -577,19 +576,10 @@ int plan_create(Plan& p, int nd, int ndim, double delta, double eps, int ipoly, 
         }
         p.nmpslot = nslot;
         p.nxcol_up = (long long)upcols.size();
-        // two-level merge schedule of the chunk partial sums (consecutive slots)
+        // merge schedule of the chunk partial sums (consecutive slots)
         for (int a = 0; a < p.nanc; a++) {
             int c0 = anc_chunk[a], c1 = anc_chunk[a + 1], nc = c1 - c0;
-            if (nc <= 1) continue;
-            if (nc <= 8) {
-                red1.push_back(make_int4(a, chunks[c0].slot, nc, -1));
-                continue;
-            }
-            int G = std::max(8, (int)std::ceil(std::sqrt((double)nc)));
-            int g0 = ngslot;
-            for (int c = c0; c < c1; c += G)
-                red1.push_back(make_int4(a, chunks[c].slot, std::min(G, c1 - c), ngslot++));
-            red2.push_back(make_int4(a, g0, ngslot - g0, 0));
+            if (nc >= 2) merge.push_back(make_int4(a, chunks[c0].slot, nc, 0));
         }
         // The same merge when the upward pass sums, inside each workgroup of
         // four consecutive chunks, the chunks of one cutoff box (a "piece")
@@ -617,14 +607,7 @@ int plan_create(Plan& p, int nd, int ndim, double delta, double eps, int ipoly, 
                 }
             if (np <= 1) continue;  // written directly, or no PW leaves (mp stays zero)
             nslotb += np;
-            if (np <= 8) {
-                red1b.push_back(make_int4(a, first, np, -1));
-                continue;
-            }
-            int G = std::max(8, (int)std::ceil(std::sqrt((double)np)));
-            int g0 = ngslotb;
-            for (int k = 0; k < np; k += G) red1b.push_back(make_int4(a, first + k, std::min(G, np - k), ngslotb++));
-            red2b.push_back(make_int4(a, g0, ngslotb - g0, 0));
+            mergeb.push_back(make_int4(a, first, np, 0));
         }
         p.nmpslot = std::max(p.nmpslot, nslotb);
         // PW translation lists at the cutoff level (gnd_compute_listpw)
@@ -735,17 +718,12 @@ int plan_create(Plan& p, int nd, int ndim, double delta, double eps, int ipoly, 
     p.nqgroup = (int)qgroups.size();
     p.nchunks = (int)chunks.size();
     p.nm2l = (int)m2l.size();
-    p.nred1 = (int)red1.size();
+    p.nmerge = (int)merge.size();
     p.merge_reads = 0;
-    for (const int4& r : red1) p.merge_reads += r.z;
-    for (const int4& r : red2) p.merge_reads += r.z;
-    p.nred2 = (int)red2.size();
-    p.ngslot = ngslot;
-    p.nred1b = (int)red1b.size();
-    p.nred2b = (int)red2b.size();
+    for (const int4& r : merge) p.merge_reads += r.z;
+    p.nmergeb = (int)mergeb.size();
     p.merge_reads_b = 0;
-    for (const int4& r : red1b) p.merge_reads_b += r.z;
-    for (const int4& r : red2b) p.merge_reads_b += r.z;
+    for (const int4& r : mergeb) p.merge_reads_b += r.z;
 
     // near-field schedule (bfgt/boxfgt.f:923-1071), pairs with empty tables dropped
     std::vector<int> ntgt, nlev, nptr;
@@ -888,10 +866,8 @@ int plan_create(Plan& p, int nd, int ndim, double delta, double eps, int ipoly, 
     if ((rc = upload(p, p.d_chunks, chunks))) return rc;
     if ((rc = upload(p, p.d_anc_box, anc_box))) return rc;
     if ((rc = upload(p, p.d_anc_chunk, anc_chunk))) return rc;
-    if ((rc = upload(p, p.d_red1, red1))) return rc;
-    if ((rc = upload(p, p.d_red2, red2))) return rc;
-    if ((rc = upload(p, p.d_red1b, red1b))) return rc;
-    if ((rc = upload(p, p.d_red2b, red2b))) return rc;
+    if ((rc = upload(p, p.d_merge, merge))) return rc;
+    if ((rc = upload(p, p.d_mergeb, mergeb))) return rc;
     if ((rc = upload(p, p.d_m2l_ptr, m2l_ptr))) return rc;
     if ((rc = upload(p, p.d_m2l, m2l))) return rc;
     if ((rc = upload(p, p.d_near_tgt, ntgt))) return rc;
@@ -932,7 +908,6 @@ int plan_create(Plan& p, int nd, int ndim, double delta, double eps, int ipoly, 
     p.has_adjoint = !p.has_pw || use_mfma(p);
     if (p.has_pw) {
         size_t len = (size_t)nd * p.nexp;
-        if ((rc = dalloc(p, p.d_gpart, (size_t)std::max(p.ngslot, ngslotb) * len))) return rc;
         if ((rc = dalloc(p, p.d_mp, (size_t)p.nmpslot * len))) return rc;
         BFGT_HIP_CHECK(hipMemsetAsync(p.d_mp, 0, (size_t)p.nmpslot * len * sizeof(cplx), p.stream));
         size_t nloc = (size_t)p.nanc * std::max(loc_buffer_cplx(p), 2 * fold_len(p));
@@ -1201,7 +1176,7 @@ int bfgt_plan_work(bfgt_plan_t plan, long long* counts, int ncounts) {
     const long long xc_down = big ? p.ncolgrp : (p.nqgroup >= 1024 ? p.nxcol_quad : L);
     // the forward upward pass sums chunks per workgroup (at most two x
     // tiles) and then merges workgroup pieces
-    const bool pieces = fp64_mfma && (p.npw + 1) / 2 <= 32 && p.nred1 + p.nred2 > 0;
+    const bool pieces = fp64_mfma && (p.npw + 1) / 2 <= 32 && p.nmerge > 0;
     long long v[7] = {
         fp64_mfma ? (big ? p.nxcol_up : p.nxcol_upq) : L,        // forward upward pass
         fp64_mfma ? xc_down : L,                                 // forward downward pass
@@ -1209,7 +1184,7 @@ int bfgt_plan_work(bfgt_plan_t plan, long long* counts, int ncounts) {
         p.has_pw ? (big || p.ifpgh > 1 ? p.nxcol_up : p.nxcol_upq) : L,  // adjoint evaluation
         p.has_pw ? xc_down : L,                                  // adjoint leaf transform
         pieces ? p.merge_reads_b : p.merge_reads,                // expansions read by the merge
-        pieces ? (long long)p.nred1b + p.nred2b : (long long)p.nred1 + p.nred2,  // ... and written
+        pieces ? (long long)p.nmergeb : (long long)p.nmerge,    // ... and written
     };
     for (int i = 0; i < ncounts && i < 7; i++) counts[i] = v[i];
     for (int i = 7; i < ncounts; i++) counts[i] = 0;

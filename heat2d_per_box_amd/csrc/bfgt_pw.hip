@@ -1180,46 +1180,51 @@ __global__ __launch_bounds__(kVecThreads) void k_down_vec(int n, int npw, int nd
 // expansion-level stages
 // ---------------------------------------------------------------------------
 
-// Sum of `count` consecutive expansion blocks of length len starting at
-// block `first`, element e; four independent accumulators keep several loads
-// in flight.  Summation order is fixed, so results are deterministic.
-__device__ inline cplx sum_blocks(const cplx* __restrict__ src, size_t len, int first, int count, int e) {
-    double r0 = 0, i0 = 0, r1 = 0, i1 = 0, r2 = 0, i2 = 0, r3 = 0, i3 = 0;
-    int c = 0;
-    for (; c + 4 <= count; c += 4) {
-        cplx a = src[(size_t)(first + c) * len + e];
-        cplx b = src[(size_t)(first + c + 1) * len + e];
-        cplx d = src[(size_t)(first + c + 2) * len + e];
-        cplx f = src[(size_t)(first + c + 3) * len + e];
-        r0 += a.x; i0 += a.y; r1 += b.x; i1 += b.y; r2 += d.x; i2 += d.y; r3 += f.x; i3 += f.y;
+// Merge (bfgt/boxfgt.f:765-786 shifts the children's expansions to their
+// parents level by level up to the cutoff; here the shifts are folded into
+// the leaf phases and only the sum remains): every cutoff box with several
+// partial expansions sums them into its mp slot.  boxes[i] = (box, first slot, count, 0), the partial sums
+// in consecutive slots of the same buffer.  A workgroup serves kMergeCols
+// consecutive coefficients of one box: its kMergeRows rows each add every
+// kMergeRows-th partial sum (loads of different rows in flight together, so a
+// box with many partial sums takes count / kMergeRows round trips), and the
+// rows are added through LDS in a fixed order (deterministic results).
+constexpr int kMergeCols = 32, kMergeRows = 8;
+__global__ __launch_bounds__(kMergeCols * kMergeRows) void k_merge(int len, int ntile,
+                                                                   const int4* __restrict__ boxes,
+                                                                   cplx* __restrict__ mp) {
+    __shared__ cplx part[kMergeRows][kMergeCols];
+    const int4 g = boxes[blockIdx.x / ntile];
+    const int col = threadIdx.x % kMergeCols, row = threadIdx.x / kMergeCols;
+    const int e = (blockIdx.x % ntile) * kMergeCols + col;
+    double r0 = 0, i0 = 0, r1 = 0, i1 = 0;
+    if (e < len) {
+        int c = row;
+        for (; c + kMergeRows < g.z; c += 2 * kMergeRows) {
+            const cplx a = mp[(size_t)(g.y + c) * len + e];
+            const cplx b = mp[(size_t)(g.y + c + kMergeRows) * len + e];
+            r0 += a.x;
+            i0 += a.y;
+            r1 += b.x;
+            i1 += b.y;
+        }
+        if (c < g.z) {
+            const cplx a = mp[(size_t)(g.y + c) * len + e];
+            r0 += a.x;
+            i0 += a.y;
+        }
     }
-    for (; c < count; c++) {
-        cplx a = src[(size_t)(first + c) * len + e];
-        r0 += a.x;
-        i0 += a.y;
+    part[row][col] = cplx{r0 + r1, i0 + i1};
+    __syncthreads();
+    if (row == 0 && e < len) {
+        cplx t = part[0][col];
+#pragma unroll
+        for (int k = 1; k < kMergeRows; k++) {
+            t.x += part[k][col].x;
+            t.y += part[k][col].y;
+        }
+        mp[(size_t)g.x * len + e] = t;
     }
-    return cplx{(r0 + r1) + (r2 + r3), (i0 + i1) + (i2 + i3)};
-}
-
-// Merge, level 1: each group of chunk partial sums of one cutoff box (slots
-// past the mp slots of the same buffer) is summed into the box's mp (dest <
-// 0) or into a group slot.
-__global__ void k_reduce1(int len, const int4* __restrict__ groups, const cplx* __restrict__ partial,
-                          cplx* __restrict__ gpart, cplx* __restrict__ mp) {
-    const int4 g = groups[blockIdx.x];
-    for (int e = blockIdx.y * blockDim.x + threadIdx.x; e < len; e += gridDim.y * blockDim.x) {
-        cplx s = sum_blocks(partial, len, g.y, g.z, e);
-        if (g.w < 0) mp[(size_t)g.x * len + e] = s;
-        else gpart[(size_t)g.w * len + e] = s;
-    }
-}
-
-// Merge, level 2: cutoff boxes with several groups sum their group slots.
-__global__ void k_reduce2(int len, const int4* __restrict__ boxes, const cplx* __restrict__ gpart,
-                          cplx* __restrict__ mp) {
-    const int4 g = boxes[blockIdx.x];
-    for (int e = blockIdx.y * blockDim.x + threadIdx.x; e < len; e += gridDim.y * blockDim.x)
-        mp[(size_t)g.x * len + e] = sum_blocks(gpart, len, g.y, g.z, e);
 }
 
 // loc = mp(self) + sum over the PW list of shift(offset) * mp(source)
@@ -1369,7 +1374,7 @@ static int launch_up_mfma3(Plan& p, const UpArgs& u, int nTN) {
     // to shrink, the accumulators fit LDS (two x tiles: 16 KB per wave) and
     // the registers leave room for it (one table; with the adjoint's two or
     // three tables the reducing form measured slower)
-    const bool br = NTM <= 2 && NTAB == 1 && p.nred1 + p.nred2 > 0;
+    const bool br = NTM <= 2 && NTAB == 1 && p.nmerge > 0;
     const int nblk = nTN * ((p.nchunks + kUpReduceWaves - 1) / kUpReduceWaves);
     if constexpr (NTAB == 1 && NKS <= 2) {
         // 2 x 2 leaf blocks (the forward transform, or the adjoint's one-table pass)
@@ -1473,17 +1478,12 @@ int launch_upward(Plan& p, const double* fvals) {
 // the merge schedule of the upward pass that ran: per chunk, or per
 // workgroup piece
 int launch_reduce(Plan& p) {
-    int len = p.nd * p.nexp;
-    int gy = (len + 255) / 256;
-    const int n1 = p.merge_pieces ? p.nred1b : p.nred1, n2 = p.merge_pieces ? p.nred2b : p.nred2;
-    if (n1 > 0) {
-        k_reduce1<<<dim3(n1, gy), 256, 0, p.stream>>>(len, p.merge_pieces ? p.d_red1b : p.d_red1, p.d_mp,
-                                                       p.d_gpart, p.d_mp);
-        BFGT_HIP_CHECK(hipGetLastError());
-    }
-    if (n2 > 0) {
-        k_reduce2<<<dim3(n2, gy), 256, 0, p.stream>>>(len, p.merge_pieces ? p.d_red2b : p.d_red2, p.d_gpart,
-                                                       p.d_mp);
+    const int len = p.nd * p.nexp;
+    const int ntile = (len + kMergeCols - 1) / kMergeCols;
+    const int nbox = p.merge_pieces ? p.nmergeb : p.nmerge;
+    if (nbox > 0) {
+        k_merge<<<nbox * ntile, kMergeCols * kMergeRows, 0, p.stream>>>(
+            len, ntile, p.merge_pieces ? p.d_mergeb : p.d_merge, p.d_mp);
         BFGT_HIP_CHECK(hipGetLastError());
     }
     return BFGT_OK;
