@@ -1283,11 +1283,10 @@ __global__ void k_m2l_fold(int nd, int P, int gy, int nanc, const int* __restric
     cplx l1[G], l2[G];
 #pragma unroll
     for (int t = 0; t < G; t++) l1[t] = l2[t] = cplx{0.0, 0.0};
-    // each distinct source of the group's G boxes is read once
+    // each distinct source of the group's G boxes is read once; the
+    // expansions of four sources are loaded before any is used
     const int j0 = gptr[g], j1 = gptr[g + 1];
-    for (int j = j0; j < j1; j++) {
-        const int2 sj = glist[j];
-        const cplx v1 = mp[(size_t)sj.x * len + o1], v2 = mp[(size_t)sj.x * len + o2];
+    auto add = [&](const int2 sj, const cplx v1, const cplx v2) {
 #pragma unroll
         for (int t = 0; t < G; t++) {
             const int k = (sj.y >> (4 * t)) & 15;
@@ -1302,6 +1301,24 @@ __global__ void k_m2l_fold(int nd, int P, int gy, int nanc, const int* __restric
             l1[t] = cfma(w1, v1, l1[t]);
             l2[t] = cfma(w2, v2, l2[t]);
         }
+    };
+    int j = j0;
+    for (; j + 4 <= j1; j += 4) {
+        int2 sj[4];
+        cplx v1[4], v2[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) sj[u] = glist[j + u];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            v1[u] = mp[(size_t)sj[u].x * len + o1];
+            v2[u] = mp[(size_t)sj[u].x * len + o2];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) add(sj[u], v1[u], v2[u]);
+    }
+    for (; j < j1; j++) {
+        const int2 sj = glist[j];
+        add(sj, mp[(size_t)sj.x * len + o1], mp[(size_t)sj.x * len + o2]);
     }
 #pragma unroll
     for (int t = 0; t < G; t++) {
