@@ -72,6 +72,9 @@ struct TreeNode {
 constexpr int kColLeaves = 4;
 // waves (chunks) of one upward workgroup that sum the chunks of a cutoff box
 constexpr int kUpReduceWaves = 4;
+// below this many (chunk, y tile) waves the upward pass puts each density on
+// its own workgroups
+constexpr int kUpSplitWaves = 2048;
 // quads (2 x 2 leaf blocks) per column-pair group of the downward pass (n <= 8)
 constexpr int kQuadsPerGroup = 4;
 

@@ -610,6 +610,10 @@ int plan_create(Plan& p, int nd, int ndim, double delta, double eps, int ipoly, 
             mergeb.push_back(make_int4(a, first, np, 0));
         }
         p.nmpslot = std::max(p.nmpslot, nslotb);
+        // the upward kernels' LDS holds the sums of waves 1.. of a workgroup
+        // only: the first chunk of every workgroup must start a piece
+        for (size_t c = 0; c < upccb.size(); c += kUpReduceWaves)
+            if (upccb[c].w < 1) return set_error(BFGT_ERR_RESOURCE, "upward piece schedule: workgroup starts mid-piece");
         // PW translation lists at the cutoff level (gnd_compute_listpw)
         double xmin = boxsize[L0];
         m2l_ptr.push_back(0);
@@ -664,7 +668,8 @@ int plan_create(Plan& p, int nd, int ndim, double delta, double eps, int ipoly, 
     // jyB) of leaf indices (-1 where no leaf) over consecutive y cells of
     // the union of the two columns' rows -- and groups of at most
     // kQuadsPerGroup quads (first quad, count, a leaf of column A, a leaf of
-    // column B or -1), fewer where that is needed for 2048 groups
+    // column B or -1), fewer where that is needed for 2048 (group, density)
+    // waves
     std::vector<int4> quads, qgroups;
     std::vector<int> qcol_first;  // first quad of each column pair
     if (p.n <= 8) {
@@ -711,7 +716,7 @@ int plan_create(Plan& p, int nd, int ndim, double delta, double eps, int ipoly, 
         };
         for (int cap = kQuadsPerGroup; cap >= 1; cap /= 2) {
             cut(cap);
-            if (qgroups.size() >= 2048) break;
+            if (qgroups.size() * nd >= 2048) break;
         }
         for (const int4& gq : qgroups) p.nxcol_quad += gq.w >= 0 ? 2 : 1;
     }
@@ -1173,7 +1178,7 @@ int bfgt_plan_work(bfgt_plan_t plan, long long* counts, int ncounts) {
     // x contractions of the downward pass: one per column group (n > 8), one
     // per column of a column pair evaluated as 2 x 2 blocks (n <= 8, when
     // there are enough pairs), else one per leaf
-    const long long xc_down = big ? p.ncolgrp : (p.nqgroup >= 1024 ? p.nxcol_quad : L);
+    const long long xc_down = big ? p.ncolgrp : ((long long)p.nqgroup * p.nd >= 1024 ? p.nxcol_quad : L);
     // the forward upward pass sums chunks per workgroup (at most two x
     // tiles) and then merges workgroup pieces
     const bool pieces = fp64_mfma && (p.npw + 1) / 2 <= 32 && p.nmerge > 0;

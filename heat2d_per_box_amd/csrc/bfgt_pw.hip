@@ -122,20 +122,26 @@ __device__ inline double leaf_in(const LeafIn& in, int o, size_t pb, int nd, int
 // px)^T G runs once with the two columns in the two halves of its K
 // dimension.  Missing leaves read as zero; ymid_half weights the middle y
 // frequency by 1/2 (the adjoint of the evaluation).
-template <int NTM, bool BR>
+template <int NTM, bool BR, bool DS>
 __global__ __launch_bounds__(kWave * kUpReduceWaves) void k_up_quad(
     int n, int P, int ps, int nd, int nTN, int nchunks, const double* __restrict__ fvals,
     const PwLeaf* __restrict__ leaves, const int4* __restrict__ chunks, const int4* __restrict__ segs,
     const int4* __restrict__ quads, const cplx* __restrict__ tab, int lstride, int ymid_half,
     const cplx* __restrict__ phase, const int* __restrict__ levbase, cplx* __restrict__ partial) {
     constexpr int NACC = 16 * NTM;  // accumulator doubles per lane
-    __shared__ double red[BR ? kUpReduceWaves * NACC * kWave : 1];
+    // the sums of waves 1.. of a workgroup (wave 0 always starts a piece and
+    // only reads)
+    __shared__ double red[BR ? (kUpReduceWaves - 1) * NACC * kWave : 1];
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
     // XCD-contiguous: the y tiles of one chunk run on one XCD (one L2), so
     // the chunk's leaf values come from HBM once
     const int lb = xcd_block(blockIdx.x, gridDim.x);
     const int tN = lb % nTN;
-    const int ci = __builtin_amdgcn_readfirstlane((lb / nTN) * kUpReduceWaves + wave);
+    // DS: one density per workgroup (the y tiles and densities of a chunk on
+    // one XCD); else the densities in turn
+    const int nds = DS ? nd : 1;
+    const int d0 = DS ? (lb / nTN) % nd : 0;
+    const int ci = __builtin_amdgcn_readfirstlane((lb / nTN / nds) * kUpReduceWaves + wave);
     const bool valid = ci < nchunks;  // waves past the end still meet the barriers
     const int q = lane >> 4, cc = lane & 15;
     const int half = cc >> 3, xi = min(cc & 7, n - 1);
@@ -143,7 +149,7 @@ __global__ __launch_bounds__(kWave * kUpReduceWaves) void k_up_quad(
     const int k2 = 16 * tN + cc;
     const double wy = (ymid_half && k2 >= P / 2) ? 0.5 : 1.0;
     const int4 ch = chunks[valid ? ci : 0];
-    for (int d = 0; d < nd; d++) {
+    for (int d = d0; d < (DS ? d0 + 1 : nd); d++) {
         v4d acc[4][NTM];
 #pragma unroll
         for (int p = 0; p < 4; p++)
@@ -203,16 +209,19 @@ __global__ __launch_bounds__(kWave * kUpReduceWaves) void k_up_quad(
             }
         }
         if constexpr (BR) {
-            double* mine = red + wave * NACC * kWave;
+            // the waves that continue a piece hand their sums to its first wave
+            if (valid && ch.w == 0) {
+                double* mine = red + (wave - 1) * NACC * kWave;
 #pragma unroll
-            for (int p = 0; p < 4; p++)
+                for (int p = 0; p < 4; p++)
 #pragma unroll
-                for (int t = 0; t < NTM; t++)
+                    for (int t = 0; t < NTM; t++)
 #pragma unroll
-                    for (int r = 0; r < 4; r++) mine[((p * NTM + t) * 4 + r) * kWave + lane] = acc[p][t][r];
+                        for (int r = 0; r < 4; r++) mine[((p * NTM + t) * 4 + r) * kWave + lane] = acc[p][t][r];
+            }
             __syncthreads();
-            for (int w = wave + 1; w < wave + ch.w; w++) {
-                const double* other = red + w * NACC * kWave;
+            for (int w = wave + 1; valid && w < wave + ch.w; w++) {
+                const double* other = red + (w - 1) * NACC * kWave;
 #pragma unroll
                 for (int p = 0; p < 4; p++)
 #pragma unroll
@@ -220,7 +229,7 @@ __global__ __launch_bounds__(kWave * kUpReduceWaves) void k_up_quad(
 #pragma unroll
                         for (int r = 0; r < 4; r++) acc[p][t][r] += other[((p * NTM + t) * 4 + r) * kWave + lane];
             }
-            __syncthreads();
+            if constexpr (!DS) __syncthreads();  // the slots are refilled for the next density
         }
         if (!valid || ch.w == 0) continue;
         cplx* out = partial + ((size_t)ch.z * nd + d) * nexp;
@@ -252,21 +261,27 @@ __global__ __launch_bounds__(kWave * kUpReduceWaves) void k_up_quad(
 // level's table rows are read once for all of them; a pass past the end of
 // the column repeats its last leaf with weight zero), and contracted in x
 // once:  sum_l (T px)^T F_l (T py_l) = (T px)^T sum_l F_l (T py_l).
-template <int NKS, int NTM, int NTAB, int LPC, bool BR>
+template <int NKS, int NTM, int NTAB, int LPC, bool BR, bool DS>
 __global__ __launch_bounds__(kWave * kUpReduceWaves) void k_up_col(
     int n, int P, int ps, int nd, int nTN, int nchunks, LeafIn in, const PwLeaf* __restrict__ leaves,
     const int4* __restrict__ chunks, const int2* __restrict__ cols, const cplx* __restrict__ tab, int lstride,
     int ymid_half, const cplx* __restrict__ phase, const int* __restrict__ levbase, cplx* __restrict__ partial) {
     constexpr int NOUT = nchan(NTAB);
     constexpr int NACC = 16 * NTM;  // accumulator doubles per lane
-    __shared__ double red[BR ? kUpReduceWaves * NACC * kWave : 1];
+    // the sums of waves 1.. of a workgroup (wave 0 always starts a piece and
+    // only reads)
+    __shared__ double red[BR ? (kUpReduceWaves - 1) * NACC * kWave : 1];
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
     // wave-uniform: the schedule is read with scalar loads
     // XCD-contiguous: the y tiles of one chunk run on one XCD (one L2), so
     // the chunk's leaf values come from HBM once
     const int lb = xcd_block(blockIdx.x, gridDim.x);
     const int tN = lb % nTN;
-    const int ci = __builtin_amdgcn_readfirstlane((lb / nTN) * kUpReduceWaves + wave);
+    // DS: one density per workgroup (the y tiles and densities of a chunk on
+    // one XCD); else the densities in turn
+    const int nds = DS ? nd : 1;
+    const int d0 = DS ? (lb / nTN) % nd : 0;
+    const int ci = __builtin_amdgcn_readfirstlane((lb / nTN / nds) * kUpReduceWaves + wave);
     const bool valid = ci < nchunks;  // waves past the end still meet the barriers
     const int q = lane >> 4, cc = lane & 15;
     const int H = (P + 1) / 2, Mlo = (P + 1) / 2, nexp = P * H, npbox = n * n;
@@ -278,7 +293,7 @@ __global__ __launch_bounds__(kWave * kUpReduceWaves) void k_up_col(
 #pragma unroll
     for (int s = 0; s < NKS; s++) pidx[s] = ccl + n * min(4 * s + q, n - 1);
     const int4 ch = chunks[valid ? ci : 0];
-    for (int d = 0; d < nd; d++) {
+    for (int d = d0; d < (DS ? d0 + 1 : nd); d++) {
         v4d acc[4][NTM];
 #pragma unroll
         for (int p = 0; p < 4; p++)
@@ -346,18 +361,19 @@ __global__ __launch_bounds__(kWave * kUpReduceWaves) void k_up_col(
             }
         }
         if constexpr (BR) {
-            // sum the accumulators of the run of waves that share this
-            // chunk's destination (consecutive waves of the workgroup)
-            double* mine = red + wave * NACC * kWave;
+            // the waves that continue a piece hand their sums to its first wave
+            if (valid && ch.w == 0) {
+                double* mine = red + (wave - 1) * NACC * kWave;
 #pragma unroll
-            for (int p = 0; p < 4; p++)
+                for (int p = 0; p < 4; p++)
 #pragma unroll
-                for (int t = 0; t < NTM; t++)
+                    for (int t = 0; t < NTM; t++)
 #pragma unroll
-                    for (int r = 0; r < 4; r++) mine[((p * NTM + t) * 4 + r) * kWave + lane] = acc[p][t][r];
+                        for (int r = 0; r < 4; r++) mine[((p * NTM + t) * 4 + r) * kWave + lane] = acc[p][t][r];
+            }
             __syncthreads();
-            for (int w = wave + 1; w < wave + ch.w; w++) {
-                const double* other = red + w * NACC * kWave;
+            for (int w = wave + 1; valid && w < wave + ch.w; w++) {
+                const double* other = red + (w - 1) * NACC * kWave;
 #pragma unroll
                 for (int p = 0; p < 4; p++)
 #pragma unroll
@@ -365,7 +381,7 @@ __global__ __launch_bounds__(kWave * kUpReduceWaves) void k_up_col(
 #pragma unroll
                         for (int r = 0; r < 4; r++) acc[p][t][r] += other[((p * NTM + t) * 4 + r) * kWave + lane];
             }
-            __syncthreads();
+            if constexpr (!DS) __syncthreads();  // the slots are refilled for the next density
         }
         if (!valid || ch.w == 0) continue;
         cplx* out = partial + ((size_t)ch.z * nd + d) * nexp;
@@ -562,6 +578,7 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_down_col(
 // qgroups[i] = (first quad, count, a leaf of column A, a leaf of column B or
 // -1); quads[k] = leaves (A, jyA), (B, jyA), (A, jyB), (B, jyB) or -1.  A
 // missing column or row gets zero phases; missing leaves are not stored.
+// One wave per (group, density).
 template <int NTAB, int NTP>
 __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_down_quad(
     int n, int P, int ps, int nd, int ngroups, const int4* __restrict__ qgroups, const int4* __restrict__ quads,
@@ -571,8 +588,10 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_down_quad(
     constexpr int NOUT = nchan(NTAB);
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
     const int item = __builtin_amdgcn_readfirstlane(xcd_block(blockIdx.x, gridDim.x) * kWavesPerBlock + wave);
-    if (item >= ngroups) return;  // no block-wide barriers in this kernel
-    const int4 g = qgroups[item];
+    if (item >= ngroups * nd) return;  // no block-wide barriers in this kernel
+    // one wave per (quad group, density): the densities of a group on one XCD
+    const int4 g = qgroups[item / nd];
+    const int d = item % nd;
     const int q = lane >> 4, cc = lane & 15;
     const int half = cc >> 3, ci = cc & 7;
     const int H = (P + 1) / 2, P2 = P / 2, npbox = n * n;
@@ -592,7 +611,7 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_down_quad(
     dst[0] = pot;
     if constexpr (NTAB >= 2) dst[1] = dst[2] = grad;
     if constexpr (NTAB >= 3) dst[3] = dst[4] = dst[5] = hess;
-    for (int d = 0; d < nd; d++) {
+    {
         const Fold* F = fold + ((size_t)la.anc * nd + d) * Mh4 * H16;
         for (int tg = 0; tg < nT; tg += NTP) {
             v4d FFr[NTP][NTAB], FFi[NTP][NTAB];
@@ -1392,17 +1411,24 @@ static int launch_up_mfma3(Plan& p, const UpArgs& u, int nTN) {
     // the registers leave room for it (one table; with the adjoint's two or
     // three tables the reducing form measured slower)
     const bool br = NTM <= 2 && NTAB == 1 && p.nmerge > 0;
-    const int nblk = nTN * ((p.nchunks + kUpReduceWaves - 1) / kUpReduceWaves);
+    // densities on separate workgroups where the chunks alone give few waves
+    // (one density: the same work without the density loop); with many small
+    // chunks the per-wave costs would grow instead
+    const bool ds = p.nd == 1 || (long)nTN * p.nchunks < kUpSplitWaves;
+    const int nblk = nTN * (ds ? p.nd : 1) * ((p.nchunks + kUpReduceWaves - 1) / kUpReduceWaves);
     if constexpr (NTAB == 1 && NKS <= 2) {
         // 2 x 2 leaf blocks (the forward transform, or the adjoint's one-table pass)
-        auto kq = br ? k_up_quad<NTM, NTM <= 2> : k_up_quad<NTM, false>;
+        auto kq = br ? (ds ? k_up_quad<NTM, NTM <= 2, true> : k_up_quad<NTM, NTM <= 2, false>)
+                     : (ds ? k_up_quad<NTM, false, true> : k_up_quad<NTM, false, false>);
         kq<<<nblk, kWave * kUpReduceWaves, 0, p.stream>>>(p.n, p.npw, p.pstride, p.nd, nTN, p.nchunks, u.in.pot,
                                                             p.d_pwleaf, br ? p.d_upqccb : p.d_upqcc, p.d_upsegs,
                                                             p.d_upquads, u.tab, u.lstride, u.ymid_half, p.d_phase,
                                                             p.d_levbase, p.d_mp);
     } else {
         constexpr int LPC = NTAB == 1 ? 2 : 1;
-        auto kern = br ? k_up_col<NKS, NTM, NTAB, LPC, NTM <= 2 && NTAB == 1> : k_up_col<NKS, NTM, NTAB, LPC, false>;
+        constexpr bool BRC = NTM <= 2 && NTAB == 1;
+        auto kern = br ? (ds ? k_up_col<NKS, NTM, NTAB, LPC, BRC, true> : k_up_col<NKS, NTM, NTAB, LPC, BRC, false>)
+                       : (ds ? k_up_col<NKS, NTM, NTAB, LPC, false, true> : k_up_col<NKS, NTM, NTAB, LPC, false, false>);
         kern<<<nblk, kWave * kUpReduceWaves, 0, p.stream>>>(p.n, p.npw, p.pstride, p.nd, nTN, p.nchunks, u.in,
                                                              p.d_pwleaf, br ? p.d_upccb : p.d_upcc, p.d_upcols, u.tab,
                                                              u.lstride, u.ymid_half, p.d_phase, p.d_levbase, p.d_mp);
@@ -1563,7 +1589,7 @@ static int launch_down_pair(Plan& p, const cplx* tab, int lstride, int ymid_half
 template <int NTAB, int NTP>
 static int launch_down_quad(Plan& p, const cplx* tab, int lstride, int ymid_half, double* pot, double* grad,
                             double* hess) {
-    int blocks = (p.nqgroup + kWavesPerBlock - 1) / kWavesPerBlock;
+    int blocks = (p.nqgroup * p.nd + kWavesPerBlock - 1) / kWavesPerBlock;
     k_down_quad<NTAB, NTP><<<blocks, kWave * kWavesPerBlock, 0, p.stream>>>(
         p.n, p.npw, p.pstride, p.nd, p.nqgroup, p.d_qgroups, p.d_quads, p.d_pwleaf, tab, lstride, ymid_half,
         p.d_phase, p.d_levbase, reinterpret_cast<const Fold*>(p.d_loc), pot, grad, hess, p.out_scale);
@@ -1578,7 +1604,7 @@ template <int NTAB>
 static int launch_down_mfma(Plan& p, const cplx* tab, int lstride, int ymid_half, double* pot, double* grad,
                             double* hess) {
     int nT = round_up((p.npw + 1) / 2, 16) / 16;
-    if (p.n <= 8 && p.nqgroup < 1024) {
+    if (p.n <= 8 && (long)p.nqgroup * p.nd < 1024) {
         if (nT % 2 == 0) return launch_down_pair<NTAB, 2>(p, tab, lstride, ymid_half, pot, grad, hess);
         return launch_down_pair<NTAB, 1>(p, tab, lstride, ymid_half, pot, grad, hess);
     }

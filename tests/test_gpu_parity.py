@@ -310,3 +310,20 @@ def test_stage_profiling_windows(device):
     torch.cuda.synchronize()
     out = {"pot": pot.cpu().numpy(), "grad": grad.cpu().numpy(), "hess": hess.cpu().numpy()}
     assert_close(compare(w.tree, out, ref, w.ifpgh, scale=5.0))
+
+
+def test_density_loop_with_workgroup_sums(device):
+    """Two densities on the c2 tree (19,834 leaves, n = 16): with at least 2048
+    (chunk, y tile) waves each upward wave runs the densities in turn, and with
+    merges the chunk sums of a cutoff box are added per workgroup -- the
+    density loop of the reducing upward form (its LDS slots are refilled per
+    density)."""
+    w = make_workload("c2", "full")
+    f2 = np.concatenate([w.fvals, np.random.default_rng(3).standard_normal(w.fvals.shape)], axis=2)
+    f2 = np.ascontiguousarray(f2)
+    ref = O.oracle_boxfgt(w.tree, f2, 2, w.delta, w.eps, w.ipoly, w.iperiod, w.norder, w.ifpgh)
+    got = run_gpu(w.tree, f2, 2, w.delta, w.eps, w.ipoly, w.iperiod, w.norder, w.ifpgh)
+    info = got["plan"].info
+    ntn = ((info["npw"] + 1) // 2 + 15) // 16
+    assert info["n_chunks"] * ntn >= 2048 and info["merge_writes"] > 0
+    assert_close(compare(w.tree, got, ref, w.ifpgh))
