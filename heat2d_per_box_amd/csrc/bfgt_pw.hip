@@ -1219,15 +1219,18 @@ __global__ __launch_bounds__(kMergeCols * kMergeRows) void k_merge(int len, int 
     double r0 = 0, i0 = 0, r1 = 0, i1 = 0;
     if (e < len) {
         int c = row;
-        for (; c + kMergeRows < g.z; c += 2 * kMergeRows) {
+        // four partial sums in flight per thread
+        for (; c + 3 * kMergeRows < g.z; c += 4 * kMergeRows) {
             const cplx a = mp[(size_t)(g.y + c) * len + e];
             const cplx b = mp[(size_t)(g.y + c + kMergeRows) * len + e];
-            r0 += a.x;
-            i0 += a.y;
-            r1 += b.x;
-            i1 += b.y;
+            const cplx u = mp[(size_t)(g.y + c + 2 * kMergeRows) * len + e];
+            const cplx v = mp[(size_t)(g.y + c + 3 * kMergeRows) * len + e];
+            r0 += a.x + u.x;
+            i0 += a.y + u.y;
+            r1 += b.x + v.x;
+            i1 += b.y + v.y;
         }
-        if (c < g.z) {
+        for (; c < g.z; c += kMergeRows) {
             const cplx a = mp[(size_t)(g.y + c) * len + e];
             r0 += a.x;
             i0 += a.y;
