@@ -343,17 +343,24 @@ def main():
         adj = {"ms_per_call": adj_ms, "over_forward": adj_ms / ms,
                "note": "fbar += A^T (gpot, ggrad, ghess) on the same resident plan, FP64"}
 
-    # cold call: the host-array drop-in (plan + H2D + kernels + D2H)
-    from heat2d_per_box_amd.boxfgt import boxfgt as boxfgt_host
+    # the host-array drop-in: a cold call (plan + H2D + kernels + D2H), then
+    # the same call again, which reuses the kept plan (H2D + kernels + D2H)
+    from heat2d_per_box_amd.boxfgt import boxfgt as boxfgt_host, release_plans
 
-    t2 = w.tree.copy()
-    hp = np.zeros((nb, npb, nd))
-    hg = np.zeros((nb, npb, 2, nd)) if w.ifpgh >= 2 else None
-    hh = np.zeros((nb, npb, 3, nd)) if w.ifpgh >= 3 else None
-    tc0 = time.perf_counter()
-    boxfgt_host(nd, 2, w.delta, w.eps, w.ipoly, w.iperiod, w.norder, npb, nb, t2.nlevels, t2.ltree, t2.itree,
-                t2.iptr, t2.centers, t2.boxsize, w.fvals, w.ifpgh, hp, hg, hh, precision=args.precision)
-    t_cold = time.perf_counter() - tc0
+    def host_call():
+        t2 = w.tree.copy()
+        hp = np.zeros((nb, npb, nd))
+        hg = np.zeros((nb, npb, 2, nd)) if w.ifpgh >= 2 else None
+        hh = np.zeros((nb, npb, 3, nd)) if w.ifpgh >= 3 else None
+        tc0 = time.perf_counter()
+        boxfgt_host(nd, 2, w.delta, w.eps, w.ipoly, w.iperiod, w.norder, npb, nb, t2.nlevels, t2.ltree, t2.itree,
+                    t2.iptr, t2.centers, t2.boxsize, w.fvals, w.ifpgh, hp, hg, hh, precision=args.precision)
+        return time.perf_counter() - tc0
+
+    release_plans()
+    t_cold = host_call()
+    t_kept = min(host_call() for _ in range(3))
+    release_plans()
 
     out = {
         "metric": "boxfgt leaf pts/s at eps=1e-9: split-FP32 vs FP64 GPU; adjoint/forward time",
@@ -385,6 +392,8 @@ def main():
         "stage_ms_note": "K more executions, untimed, with HIP events between every stage",
         "t_cold_s": t_cold,
         "cold_pts_per_s": npts / t_cold,
+        "t_host_kept_plan_s": t_kept,
+        "host_kept_plan_pts_per_s": npts / t_kept,
         "t_plan_s": t_plan,
         "roofline": roof,
         "stage_roofline": stages_roof,

@@ -30,6 +30,7 @@ PRECISIONS = {"fp64": 0, "split32": 1}
 
 EXPORTS = [
     "bfgt_hip_eval",
+    "bfgt_hip_release_plans",
     "bfgt_plan_create",
     "bfgt_plan_execute",
     "bfgt_plan_execute_scaled",
@@ -71,6 +72,8 @@ def lib() -> ctypes.CDLL:
             f"HIP library not built: {LIB_PATH} is missing (run `python -m heat2d_per_box_amd.build` "
             "or __graft_entry__.build())")
     L = ctypes.CDLL(LIB_PATH)
+    L.bfgt_hip_release_plans.restype = _i
+    L.bfgt_hip_release_plans.argtypes = []
     L.bfgt_hip_eval.restype = _i
     L.bfgt_hip_eval.argtypes = [_i, _i, _d, _d, _i, _i, _i, _i, _i, _i, _i, _ip, _ip, _dp, _dp, _dp, _i, _dp, _dp,
                                 _dp, _i, _i, _dp, _i, _dp, _dp, _dp, _dp, _i]

@@ -91,6 +91,11 @@ def boxfgt(nd, ndim, delta, eps, ipoly, iperiod, norder, npbox, nboxes, nlevels,
     check(rc)
 
 
+def release_plans() -> None:
+    """Free the plans that :func:`boxfgt` keeps for later calls on the same tree."""
+    check(lib().bfgt_hip_release_plans())
+
+
 class PlanInfo(dict):
     pass
 

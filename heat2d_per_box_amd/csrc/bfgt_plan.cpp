@@ -15,7 +15,9 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <cstdlib>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1043,6 +1045,80 @@ int plan_adjoint(Plan& p, const double* gpot, const double* ggrad, const double*
     return BFGT_OK;
 }
 
+// Plans of the host-array entry point.  The drivers call boxfgt on one tree
+// several times per time step and over many steps; a plan (schedules, tables,
+// device buffers) is kept while the tree arrays it was built from, the
+// parameters and the device are the same, and reused.  The key is the part
+// of the tree the plan reads (laddr, levels, parents, children of the boxes
+// in use, centres, box sizes); the colleague lists are an output of plan
+// creation and are written back into the caller's itree on every call.
+namespace {
+constexpr size_t kHostPlans = 4;  // plans kept (least recently used evicted)
+
+struct HostPlan {
+    int dev = -1, nd = 0, ipoly = 0, iperiod = 0, norder = 0, nboxes = 0, nlevels = 0, ifpgh2 = 0, precision = 0;
+    double delta = 0, eps = 0;
+    std::vector<int> tree;
+    std::vector<double> geom;
+    std::vector<int> ncoll, coll;  // colleague lists written by plan creation
+    bfgt_plan_t plan = nullptr;
+    double *dfv = nullptr, *dpot = nullptr, *dgrad = nullptr, *dhess = nullptr;
+    size_t cap = 0;  // doubles per density-value array held in dfv / dpot
+    unsigned long long used = 0;
+};
+
+std::mutex g_host_mu;
+std::vector<std::unique_ptr<HostPlan>> g_host_plans;
+unsigned long long g_host_clock = 0;
+
+void free_host_plan(HostPlan& h) {
+    hipFree(h.dfv);
+    hipFree(h.dpot);
+    hipFree(h.dgrad);
+    hipFree(h.dhess);
+    h.dfv = h.dpot = h.dgrad = h.dhess = nullptr;
+    h.cap = 0;
+    if (h.plan) bfgt_plan_destroy(h.plan);
+    h.plan = nullptr;
+}
+
+// the key fields of one call (tree parts copied from the caller's arrays,
+// whose bounds check_tree has verified)
+void host_key(HostPlan& k, const TreeView& t, int nlevels) {
+    const int nb = t.nboxes;
+    k.tree.clear();
+    k.tree.insert(k.tree.end(), t.laddr, t.laddr + 2 * (nlevels + 1));
+    k.tree.insert(k.tree.end(), t.ilevel, t.ilevel + nb);
+    k.tree.insert(k.tree.end(), t.iparent, t.iparent + nb);
+    k.tree.insert(k.tree.end(), t.nchild, t.nchild + nb);
+    k.tree.insert(k.tree.end(), t.ichild, t.ichild + 4 * (size_t)nb);
+    k.geom.assign(t.centers, t.centers + 2 * (size_t)nb);
+    k.geom.insert(k.geom.end(), t.boxsize, t.boxsize + nlevels + 1);
+}
+
+bool same_key(const HostPlan& a, const HostPlan& b) {
+    return a.dev == b.dev && a.nd == b.nd && a.ipoly == b.ipoly && a.iperiod == b.iperiod && a.norder == b.norder &&
+           a.nboxes == b.nboxes && a.nlevels == b.nlevels && a.ifpgh2 == b.ifpgh2 && a.precision == b.precision &&
+           std::memcmp(&a.delta, &b.delta, sizeof(double)) == 0 && std::memcmp(&a.eps, &b.eps, sizeof(double)) == 0 &&
+           a.tree == b.tree && a.geom.size() == b.geom.size() &&
+           std::memcmp(a.geom.data(), b.geom.data(), a.geom.size() * sizeof(double)) == 0;
+}
+
+void drop_host_plan(HostPlan* h) {
+    for (size_t i = 0; i < g_host_plans.size(); i++)
+        if (g_host_plans[i].get() == h) {
+            free_host_plan(*h);
+            g_host_plans.erase(g_host_plans.begin() + i);
+            return;
+        }
+}
+
+bool host_cache_enabled() {
+    const char* v = std::getenv("BFGT_HOST_PLAN_CACHE");
+    return !(v && v[0] == '0');
+}
+}  // namespace
+
 }  // namespace bfgt
 
 using bfgt::set_error;
@@ -1280,52 +1356,101 @@ int bfgt_hip_eval(int nd, int ndim, double delta, double eps, int ipoly, int ipe
     // with targets asking for more channels than ifpgh, the transform computes
     // them into zeroed scratch arrays the caller never sees (bfgt/boxfgt.f:207-277)
     const int ifpgh2 = with_targets ? std::max(ifpgh, ifpghtarg) : ifpgh;
-    hipStream_t stream = nullptr;
-    if (hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess)
-        return set_error(BFGT_ERR_HIP, "hipStreamCreateWithFlags failed");
-    bfgt_plan_t plan = nullptr;
-    int rc = bfgt_plan_create(&plan, nd, ndim, delta, eps, ipoly, iperiod, norder, nboxes, nlevels, ltree, itree,
-                              iptr, centers, boxsize, ifpgh2, precision, stream);
-    if (rc) {
-        hipStreamDestroy(stream);
-        return rc;
+    std::lock_guard<std::mutex> lock(bfgt::g_host_mu);
+    // the plan for this tree and these parameters: a kept one, or a new one
+    bfgt::HostPlan key;
+    if (hipGetDevice(&key.dev) != hipSuccess) return set_error(BFGT_ERR_HIP, "hipGetDevice failed");
+    key.nd = nd;
+    key.ipoly = ipoly;
+    key.iperiod = iperiod;
+    key.norder = norder;
+    key.nboxes = nboxes;
+    key.nlevels = nlevels;
+    key.ifpgh2 = ifpgh2;
+    key.precision = precision;
+    key.delta = delta;
+    key.eps = eps;
+    {
+        bfgt::TreeView tv;
+        int rc = bfgt::check_tree(nboxes, nlevels, ltree, itree, iptr, centers, boxsize, tv);
+        if (rc) return rc;
+        bfgt::host_key(key, tv, nlevels);
     }
-    plan->p.own_stream = true;  // released with the plan
-    bfgt::Plan& p = plan->p;
-    size_t nv = (size_t)nboxes * npbox * nd;
-    size_t nt = with_targets ? (size_t)ntarg : 0;
-    double *dfv = nullptr, *dpot = nullptr, *dgrad = nullptr, *dhess = nullptr;
+    const bool keep = bfgt::host_cache_enabled();
+    bfgt::HostPlan* hp = nullptr;
+    for (auto& h : bfgt::g_host_plans)
+        if (bfgt::same_key(*h, key)) hp = h.get();
+    const int nused = itree[2 * nlevels + 1];
+    if (hp) {
+        // the colleague lists plan creation wrote into itree
+        std::copy(hp->ncoll.begin(), hp->ncoll.end(), itree + iptr[5] - 1);
+        std::copy(hp->coll.begin(), hp->coll.end(), itree + iptr[6] - 1);
+    } else {
+        hipStream_t stream = nullptr;
+        if (hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess)
+            return set_error(BFGT_ERR_HIP, "hipStreamCreateWithFlags failed");
+        bfgt_plan_t plan = nullptr;
+        int rc = bfgt_plan_create(&plan, nd, ndim, delta, eps, ipoly, iperiod, norder, nboxes, nlevels, ltree, itree,
+                                  iptr, centers, boxsize, ifpgh2, precision, stream);
+        if (rc) {
+            hipStreamDestroy(stream);
+            return rc;
+        }
+        plan->p.own_stream = true;  // released with the plan
+        if (bfgt::g_host_plans.size() >= bfgt::kHostPlans) {
+            auto lru = std::min_element(bfgt::g_host_plans.begin(), bfgt::g_host_plans.end(),
+                                        [](const auto& a, const auto& b) { return a->used < b->used; });
+            bfgt::drop_host_plan(lru->get());
+        }
+        bfgt::g_host_plans.push_back(std::make_unique<bfgt::HostPlan>(std::move(key)));
+        hp = bfgt::g_host_plans.back().get();
+        hp->plan = plan;
+        hp->ncoll.assign(itree + iptr[5] - 1, itree + iptr[5] - 1 + nused);
+        hp->coll.assign(itree + iptr[6] - 1, itree + iptr[6] - 1 + 9 * (size_t)nused);
+    }
+    hp->used = ++bfgt::g_host_clock;
+    bfgt::Plan& p = hp->plan->p;
+    const size_t nv = (size_t)nboxes * npbox * nd;
+    const size_t nt = with_targets ? (size_t)ntarg : 0;
     double *dtg = nullptr, *dpe = nullptr, *dge = nullptr, *dhe = nullptr;
-    auto fail = [&](int code) {
-        hipFree(dfv);
-        hipFree(dpot);
-        hipFree(dgrad);
-        hipFree(dhess);
+    // a failed call frees its target arrays and forgets the plan; a plan is
+    // also forgotten after the call when keeping plans is switched off
+    auto finish = [&](int code) {
         hipFree(dtg);
         hipFree(dpe);
         hipFree(dge);
         hipFree(dhe);
-        bfgt_plan_destroy(plan);
+        if (code != BFGT_OK || !keep) bfgt::drop_host_plan(hp);
         return code;
     };
 #define BFGT_EVAL_CHECK(expr)                                                                        \
     do {                                                                                             \
         hipError_t _e = (expr);                                                                      \
-        if (_e != hipSuccess) return fail(set_error(BFGT_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e))); \
+        if (_e != hipSuccess) return finish(set_error(BFGT_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e))); \
     } while (0)
-    BFGT_EVAL_CHECK(hipMalloc(&dfv, nv * sizeof(double)));
-    BFGT_EVAL_CHECK(hipMalloc(&dpot, nv * sizeof(double)));
+    if (hp->cap < nv) {
+        hipFree(hp->dfv);
+        hipFree(hp->dpot);
+        hipFree(hp->dgrad);
+        hipFree(hp->dhess);
+        hp->dfv = hp->dpot = hp->dgrad = hp->dhess = nullptr;
+        hp->cap = 0;
+        BFGT_EVAL_CHECK(hipMalloc(&hp->dfv, nv * sizeof(double)));
+        BFGT_EVAL_CHECK(hipMalloc(&hp->dpot, nv * sizeof(double)));
+        if (ifpgh2 >= 2) BFGT_EVAL_CHECK(hipMalloc(&hp->dgrad, 2 * nv * sizeof(double)));
+        if (ifpgh2 >= 3) BFGT_EVAL_CHECK(hipMalloc(&hp->dhess, 3 * nv * sizeof(double)));
+        hp->cap = nv;
+    }
+    double *dfv = hp->dfv, *dpot = hp->dpot, *dgrad = hp->dgrad, *dhess = hp->dhess;
     BFGT_EVAL_CHECK(hipMemcpyAsync(dfv, fvals, nv * sizeof(double), hipMemcpyHostToDevice, p.stream));
     BFGT_EVAL_CHECK(hipMemcpyAsync(dpot, pot, nv * sizeof(double), hipMemcpyHostToDevice, p.stream));
     if (ifpgh2 >= 2) {
-        BFGT_EVAL_CHECK(hipMalloc(&dgrad, 2 * nv * sizeof(double)));
         if (ifpgh >= 2)
             BFGT_EVAL_CHECK(hipMemcpyAsync(dgrad, grad, 2 * nv * sizeof(double), hipMemcpyHostToDevice, p.stream));
         else
             BFGT_EVAL_CHECK(hipMemsetAsync(dgrad, 0, 2 * nv * sizeof(double), p.stream));
     }
     if (ifpgh2 >= 3) {
-        BFGT_EVAL_CHECK(hipMalloc(&dhess, 3 * nv * sizeof(double)));
         if (ifpgh >= 3)
             BFGT_EVAL_CHECK(hipMemcpyAsync(dhess, hess, 3 * nv * sizeof(double), hipMemcpyHostToDevice, p.stream));
         else
@@ -1340,14 +1465,14 @@ int bfgt_hip_eval(int nd, int ndim, double delta, double eps, int ipoly, int ipe
     }
     auto t1 = std::chrono::steady_clock::now();
     double ti[20];
-    rc = bfgt::plan_execute(p, dfv, dpot, dgrad, dhess, ti, 1.0);
-    if (rc) return fail(rc);
+    int rc = bfgt::plan_execute(p, dfv, dpot, dgrad, dhess, ti, 1.0);
+    if (rc) return finish(rc);
     double t_targets = 0;
     if (with_targets) {
         hipEvent_t e0 = p.ev[6], e1 = p.ev[7];
         BFGT_EVAL_CHECK(hipEventRecord(e0, p.stream));
         rc = bfgt::plan_targets(p, ntarg, dtg, ifpghtarg, dpot, dgrad, dhess, dpe, dge, dhe);
-        if (rc) return fail(rc);
+        if (rc) return finish(rc);
         BFGT_EVAL_CHECK(hipEventRecord(e1, p.stream));
         BFGT_EVAL_CHECK(hipMemcpyAsync(pote, dpe, nt * nd * sizeof(double), hipMemcpyDeviceToHost, p.stream));
         if (ifpghtarg >= 2)
@@ -1371,7 +1496,14 @@ int bfgt_hip_eval(int nd, int ndim, double delta, double eps, int ipoly, int ipe
         timeinfo[0] = std::chrono::duration<double>(t1 - t0).count();
         timeinfo[11] = t_targets;  // extra targets (slot 12 of the reference)
     }
-    return fail(BFGT_OK);
+    return finish(BFGT_OK);
+}
+
+int bfgt_hip_release_plans(void) {
+    std::lock_guard<std::mutex> lock(bfgt::g_host_mu);
+    for (auto& h : bfgt::g_host_plans) bfgt::free_host_plan(*h);
+    bfgt::g_host_plans.clear();
+    return BFGT_OK;
 }
 
 }  // extern "C"

@@ -10,11 +10,13 @@
 !
 ! `boxfgt_set_precision(iprec)` selects the arithmetic of later calls:
 ! 0 = all-FP64 (default), 1 = split-FP32 with FP64 weights and accumulation.
+! The library keeps the plan of a call for later calls on the same tree with
+! the same parameters; `boxfgt_release_plans()` frees the kept plans.
 module boxfgt_hip_mod
   use iso_c_binding
   implicit none
   private
-  public :: bfgt_hip_eval_c, bfgt_last_error_c, boxfgt_precision, c_string
+  public :: bfgt_hip_eval_c, bfgt_last_error_c, bfgt_hip_release_plans_c, boxfgt_precision, c_string
 
   integer, save :: boxfgt_precision = 0
 
@@ -39,6 +41,10 @@ module boxfgt_hip_mod
 
     type(c_ptr) function bfgt_last_error_c() bind(C, name="bfgt_last_error")
       import :: c_ptr
+    end function
+
+    integer(c_int) function bfgt_hip_release_plans_c() bind(C, name="bfgt_hip_release_plans")
+      import :: c_int
     end function
   end interface
 
@@ -66,6 +72,15 @@ subroutine boxfgt_set_precision(iprec)
   implicit none
   integer, intent(in) :: iprec
   boxfgt_precision = iprec
+end subroutine
+
+
+subroutine boxfgt_release_plans()
+  use iso_c_binding
+  use boxfgt_hip_mod, only: bfgt_hip_release_plans_c
+  implicit none
+  integer(c_int) :: ier
+  ier = bfgt_hip_release_plans_c()
 end subroutine
 
 

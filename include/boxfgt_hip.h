@@ -55,12 +55,23 @@ extern "C" {
  * bfgt/boxfgt.f:1494-1563); channels above ifpgh are computed into scratch
  * arrays and never written to grad/hess (bfgt/boxfgt.f:207-277).
  * `precision` selects BFGT_PREC_FP64 or BFGT_PREC_SPLIT32.
+ * The plan built for a call (schedules, tables, device buffers) is kept and
+ * reused by later calls with the same tree arrays (laddr, levels, parents,
+ * children, centres, box sizes), parameters and device -- the drivers call
+ * boxfgt on one tree several times per step -- so such a call costs the
+ * copies and the kernels only.  At most 4 plans are kept (least recently used
+ * dropped); the environment variable BFGT_HOST_PLAN_CACHE=0 switches keeping
+ * off.  Calls are serialised by a lock.
  */
 int bfgt_hip_eval(int nd, int ndim, double delta, double eps, int ipoly, int iperiod, int norder, int npbox,
                   int nboxes, int nlevels, int ltree, int *itree, const int *iptr, const double *centers,
                   const double *boxsize, const double *fvals, int ifpgh, double *pot, double *grad, double *hess,
                   int ifnewtree, int ntarg, const double *targs, int ifpghtarg, double *pote, double *grade,
                   double *hesse, double *timeinfo, int precision);
+
+/* Free the plans and device buffers kept by bfgt_hip_eval (no counterpart in
+ * the reference, whose boxfgt keeps nothing between calls). */
+int bfgt_hip_release_plans(void);
 
 /* Resident plan: the tree, lists and FP64 tables stay in device memory for
  * repeated calls with the same tree, delta and eps (the solvers call 2-3x per

@@ -327,3 +327,48 @@ def test_density_loop_with_workgroup_sums(device):
     ntn = ((info["npw"] + 1) // 2 + 15) // 16
     assert info["n_chunks"] * ntn >= 2048 and info["merge_writes"] > 0
     assert_close(compare(w.tree, got, ref, w.ifpgh))
+
+
+def test_host_entry_point_keeps_plans(device, monkeypatch):
+    """Repeated host-array calls on one tree reuse the kept plan: a second call
+    gives the same results and writes the colleague lists into itree again
+    (here over lists the caller scribbled on); other deltas and another tree get
+    plans of their own (more than the four kept: the least recently used is
+    dropped and rebuilt); keeping can be switched off and the plans released."""
+    from heat2d_per_box_amd.boxfgt import boxfgt, release_plans
+
+    def call(w, delta, scribble=False):
+        t = w.tree.copy()
+        nb, npb, nd = w.fvals.shape
+        if scribble:
+            nused = t.itree[2 * t.nlevels + 1]
+            t.itree[t.iptr[5] - 1:t.iptr[5] - 1 + nused] = -7
+            t.itree[t.iptr[6] - 1:t.iptr[6] - 1 + 9 * nused] = -7
+        pot = np.zeros((nb, npb, nd))
+        grad = np.zeros((nb, npb, 2, nd))
+        boxfgt(nd, 2, delta, w.eps, w.ipoly, w.iperiod, w.norder, npb, nb, t.nlevels, t.ltree, t.itree, t.iptr,
+               t.centers, t.boxsize, w.fvals, 2, pot, grad)
+        return t, pot, grad
+
+    def check_call(w, delta, scribble=False):
+        t, pot, grad = call(w, delta, scribble)
+        ref = O.oracle_boxfgt(w.tree, w.fvals, w.nd, delta, w.eps, w.ipoly, w.iperiod, w.norder, 2)
+        assert np.array_equal(t.itree, ref.itree)
+        assert leaf_rel_err(t, pot, ref.pot) < 1e-12 and leaf_rel_err(t, grad, ref.grad) < 1e-11
+        return pot, grad
+
+    release_plans()
+    w = make_workload("c3", "tiny", delta=1e-5, eps=1e-9)
+    p1, g1 = check_call(w, 1e-5)
+    p2, g2 = check_call(w, 1e-5, scribble=True)  # the kept plan
+    assert np.array_equal(p1, p2) and np.array_equal(g1, g2)
+    for delta in (2e-5, 4e-5, 8e-5, 1.6e-4):
+        check_call(w, delta)
+    check_call(w, 1e-5)  # dropped as least recently used, built again
+    check_call(make_workload("c1", "full"), make_workload("c1", "full").delta)
+    monkeypatch.setenv("BFGT_HOST_PLAN_CACHE", "0")
+    check_call(w, 1e-5)
+    monkeypatch.delenv("BFGT_HOST_PLAN_CACHE")
+    release_plans()
+    check_call(w, 1e-5)
+    release_plans()
