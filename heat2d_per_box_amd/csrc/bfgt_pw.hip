@@ -1415,9 +1415,10 @@ static int launch_up_mfma3(Plan& p, const UpArgs& u, int nTN) {
     // three tables the reducing form measured slower)
     const bool br = NTM <= 2 && NTAB == 1 && p.nmerge > 0;
     // densities on separate workgroups where the chunks alone give few waves
-    // (one density: the same work without the density loop); with many small
-    // chunks the per-wave costs would grow instead
-    const bool ds = p.nd == 1 || (long)nTN * p.nchunks < kUpSplitWaves;
+    // (one density: the same work without the density loop) and for the
+    // adjoint's two or three tables (more work per wave); with many small
+    // one-table chunks the per-wave costs would grow instead
+    const bool ds = p.nd == 1 || NTAB > 1 || (long)nTN * p.nchunks < kUpSplitWaves;
     const int nblk = nTN * (ds ? p.nd : 1) * ((p.nchunks + kUpReduceWaves - 1) / kUpReduceWaves);
     if constexpr (NTAB == 1 && NKS <= 2) {
         // 2 x 2 leaf blocks (the forward transform, or the adjoint's one-table pass)
