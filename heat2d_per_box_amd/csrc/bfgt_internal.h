@@ -77,6 +77,8 @@ constexpr int kUpReduceWaves = 4;
 constexpr int kUpSplitWaves = 2048;
 // quads (2 x 2 leaf blocks) per column-pair group of the downward pass (n <= 8)
 constexpr int kQuadsPerGroup = 4;
+// ints per translation group in patch form: 16 cells, 4 target masks
+constexpr int kPatchInts = 20;
 
 struct Plan {
     // parameters
@@ -186,6 +188,8 @@ struct Plan {
     int2* d_m2lg = nullptr;
     int* d_m2lgT_ptr = nullptr;
     int2* d_m2lgT = nullptr;
+    int* d_m2lpatch = nullptr;      // groups of four: patch form of the lists (or -2: use the entries)
+    int* d_m2lpatchT = nullptr;
     int nnearT_s = 0;
     int* d_nearT_src = nullptr;    // source boxes
     int* d_nearT_ptr = nullptr;    // CSR [nnearT_s+1]

@@ -239,6 +239,64 @@ void group_lists(int nanc, int G, const std::vector<int>& ptr, const std::vector
     }
 }
 
+// Patch form of a group of four sibling cutoff boxes (for the translation):
+// the group's sources on the 4 x 4 patch of cells around the 2 x 2 targets.
+// Record (kPatchInts ints per group): [py*4 + px] the mp slot of the source
+// at patch cell (px, py) or -1; [16 + t] for target t its cell (tx | ty << 1)
+// << 9 | a 9-bit mask of the source offsets (dx, dy) in its list, bit
+// (dx+1) + 3 (dy+1).  A source at offset d from its target is shifted by
+// conj(ex)^dx conj(ey)^dy, in the forward (code = target - source) and in
+// the transposed lists (code = source - target, conjugated shifts) alike.
+// sigma: -1 for forward lists, +1 for transposed.  A group whose entries do
+// not map one to one onto the patch (not four siblings, periodic images of a
+// coarse level, a partial last group) is marked [0] = -2: the kernel then
+// uses the entry list.
+void patch_lists(int nanc, const std::vector<int>& gptr, const std::vector<int2>& glist, const TreeView& t,
+                 int a0, int sigma, std::vector<int>& patch) {
+    const int ng = (nanc + 3) / 4;
+    patch.assign((size_t)ng * kPatchInts, -1);
+    for (int g = 0; g < ng; g++) {
+        int* rec = patch.data() + (size_t)g * kPatchInts;
+        bool ok = 4 * g + 3 < nanc;
+        int tx[4] = {0, 0, 0, 0}, ty[4] = {0, 0, 0, 0};
+        if (ok) {
+            const int b0 = a0 + 4 * g;
+            double cx = t.centers[2 * b0], cy = t.centers[2 * b0 + 1];
+            for (int q = 1; q < 4; q++) {
+                cx = std::min(cx, t.centers[2 * (b0 + q)]);
+                cy = std::min(cy, t.centers[2 * (b0 + q) + 1]);
+            }
+            int seen = 0;
+            for (int q = 0; q < 4; q++) {
+                ok = ok && t.iparent[b0 + q] == t.iparent[b0];
+                tx[q] = t.centers[2 * (b0 + q)] > cx ? 1 : 0;
+                ty[q] = t.centers[2 * (b0 + q) + 1] > cy ? 1 : 0;
+                seen |= 1 << (tx[q] + 2 * ty[q]);
+            }
+            ok = ok && seen == 15;
+        }
+        int mask[4] = {0, 0, 0, 0};
+        for (int j = gptr[g]; ok && j < gptr[g + 1]; j++) {
+            for (int q = 0; q < 4 && ok; q++) {
+                const int k = (glist[j].y >> (4 * q)) & 15;
+                if (k == 15) continue;
+                const int dx = sigma * (k % 3 - 1), dy = sigma * (k / 3 - 1);
+                const int px = tx[q] + 1 + dx, py = ty[q] + 1 + dy, bit = (dx + 1) + 3 * (dy + 1);
+                int& cell = rec[4 * py + px];
+                if ((cell >= 0 && cell != glist[j].x) || (mask[q] >> bit & 1)) ok = false;
+                cell = glist[j].x;
+                mask[q] |= 1 << bit;
+            }
+        }
+        if (!ok) {
+            std::fill(rec, rec + kPatchInts, -1);
+            rec[0] = -2;
+            continue;
+        }
+        for (int q = 0; q < 4; q++) rec[16 + q] = ((tx[q] | ty[q] << 1) << 9) | mask[q];
+    }
+}
+
 // near-field table offsets (gnd_find_loctab_ind, bfgt/bfgt_local.f:1729-1765)
 void loctab_index(int iperiod, const double* tc, const double* sc, double sbs, double bs0, int* ixy) {
     double bs = sbs / 4;
@@ -902,12 +960,21 @@ int plan_create(Plan& p, int nd, int ndim, double delta, double eps, int ipoly, 
         // of the workgroups: used only where that still fills the chip
         const int H = (p.npw + 1) / 2;
         p.m2l_group = (long)((p.nanc + 3) / 4) * ((p.nd * H * H + 255) / 256) >= 2048 ? 4 : 1;
+        std::vector<int> patch;
         group_lists(p.nanc, p.m2l_group, m2l_ptr, m2l, gptr, glist);
         if ((rc = upload(p, p.d_m2lg_ptr, gptr))) return rc;
         if ((rc = upload(p, p.d_m2lg, glist))) return rc;
+        if (p.m2l_group == 4) {
+            patch_lists(p.nanc, gptr, glist, t, t.first(p.nlevstart), -1, patch);
+            if ((rc = upload(p, p.d_m2lpatch, patch))) return rc;
+        }
         group_lists(p.nanc, p.m2l_group, m2lT_ptr, m2lT, gptr, glist);
         if ((rc = upload(p, p.d_m2lgT_ptr, gptr))) return rc;
         if ((rc = upload(p, p.d_m2lgT, glist))) return rc;
+        if (p.m2l_group == 4) {
+            patch_lists(p.nanc, gptr, glist, t, t.first(p.nlevstart), 1, patch);
+            if ((rc = upload(p, p.d_m2lpatchT, patch))) return rc;
+        }
     }
     if ((rc = upload(p, p.d_nearT_src, nsrc))) return rc;
     if ((rc = upload(p, p.d_nearT_ptr, nsptr))) return rc;

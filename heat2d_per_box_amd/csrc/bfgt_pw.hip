@@ -1283,8 +1283,9 @@ __global__ void k_m2l(int nd, int nexp, const int* __restrict__ m2l_ptr, const i
 // of neighbouring cutoff boxes are read from one L2.
 template <int G>
 __global__ void k_m2l_fold(int nd, int P, int gy, int nanc, const int* __restrict__ gptr,
-                           const int2* __restrict__ glist, const cplx* __restrict__ wshift, double csign,
-                           const cplx* __restrict__ mp, Fold* __restrict__ fold) {
+                           const int2* __restrict__ glist, const int* __restrict__ patch,
+                           const cplx* __restrict__ wshift, double csign, const cplx* __restrict__ mp,
+                           Fold* __restrict__ fold) {
     const int lb = xcd_block(blockIdx.x, gridDim.x);
     const int g = lb / gy;
     const int H = (P + 1) / 2, nexp = P * H;
@@ -1305,6 +1306,58 @@ __global__ void k_m2l_fold(int nd, int P, int gy, int nanc, const int* __restric
     cplx l1[G], l2[G];
 #pragma unroll
     for (int t = 0; t < G; t++) l1[t] = l2[t] = cplx{0.0, 0.0};
+    // groups of four in patch form: the 4 x 4 cells of sources around the
+    // 2 x 2 targets, column by column; per target the three sources of a
+    // column in its list are summed with their y shifts, then the column sum
+    // is shifted in x -- conj(ex)^dx conj(ey)^dy for a source at offset
+    // (dx, dy) from its target, in the forward and the transposed lists alike
+    const int* rec = G == 4 && patch != nullptr ? patch + (size_t)g * kPatchInts : nullptr;
+    if (rec != nullptr && rec[0] != -2) {
+        const cplx bx1 = {ex1.x, -ex1.y}, bx2 = {ex2.x, -ex2.y}, by = {ey.x, -ey.y};
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            cplx v1[4], v2[4];
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int sl = rec[4 * r + c];
+                v1[r] = v2[r] = cplx{0.0, 0.0};
+                if (sl >= 0) {
+                    v1[r] = mp[(size_t)sl * len + o1];
+                    v2[r] = mp[(size_t)sl * len + o2];
+                }
+            }
+#pragma unroll
+            for (int t = 0; t < G; t++) {
+                const int tm = rec[16 + t];
+                const int tx = (tm >> 9) & 1, ty = (tm >> 10) & 1;
+                const int dx = c - tx - 1;
+                if (dx < -1 || dx > 1) continue;
+                cplx s1 = {0.0, 0.0}, s2 = {0.0, 0.0};
+#pragma unroll
+                for (int dy = -1; dy <= 1; dy++) {
+                    if (!((tm >> ((dx + 1) + 3 * (dy + 1))) & 1)) continue;
+                    // the source row ty + 1 + dy, with ty in {0, 1}
+                    const cplx a1 = ty ? v1[dy + 2] : v1[dy + 1], a2 = ty ? v2[dy + 2] : v2[dy + 1];
+                    if (dy == 0) {
+                        s1 = cplx{s1.x + a1.x, s1.y + a1.y};
+                        s2 = cplx{s2.x + a2.x, s2.y + a2.y};
+                    } else {
+                        const cplx w = dy > 0 ? by : cplx{by.x, -by.y};
+                        s1 = cfma(w, a1, s1);
+                        s2 = cfma(w, a2, s2);
+                    }
+                }
+                if (dx == 0) {
+                    l1[t] = cplx{l1[t].x + s1.x, l1[t].y + s1.y};
+                    l2[t] = cplx{l2[t].x + s2.x, l2[t].y + s2.y};
+                } else {
+                    const cplx w1 = dx > 0 ? bx1 : cplx{bx1.x, -bx1.y}, w2 = dx > 0 ? bx2 : cplx{bx2.x, -bx2.y};
+                    l1[t] = cfma(w1, s1, l1[t]);
+                    l2[t] = cfma(w2, s2, l2[t]);
+                }
+            }
+        }
+    } else {
     // each distinct source of the group's G boxes is read once; the
     // expansions of four sources are loaded before any is used
     const int j0 = gptr[g], j1 = gptr[g + 1];
@@ -1341,6 +1394,7 @@ __global__ void k_m2l_fold(int nd, int P, int gy, int nanc, const int* __restric
     for (; j < j1; j++) {
         const int2 sj = glist[j];
         add(sj, mp[(size_t)sj.x * len + o1], mp[(size_t)sj.x * len + o2]);
+    }
     }
 #pragma unroll
     for (int t = 0; t < G; t++) {
@@ -1536,19 +1590,20 @@ int launch_reduce(Plan& p) {
     return BFGT_OK;
 }
 
-static int launch_m2l_fold(Plan& p, const int* gptr, const int2* glist, double csign) {
+static int launch_m2l_fold(Plan& p, const int* gptr, const int2* glist, const int* patch, double csign) {
     int H = (p.npw + 1) / 2;
     int gy = (p.nd * H * H + 255) / 256;
     const int G = p.m2l_group, ngroups = (p.nanc + G - 1) / G;
     auto kern = G == 4 ? k_m2l_fold<4> : k_m2l_fold<1>;
-    kern<<<ngroups * gy, 256, 0, p.stream>>>(p.nd, p.npw, gy, p.nanc, gptr, glist, p.d_wshift, csign, p.d_mp,
+    kern<<<ngroups * gy, 256, 0, p.stream>>>(p.nd, p.npw, gy, p.nanc, gptr, glist, G == 4 ? patch : nullptr,
+                                             p.d_wshift, csign, p.d_mp,
                                              reinterpret_cast<Fold*>(p.d_loc));
     BFGT_HIP_CHECK(hipGetLastError());
     return BFGT_OK;
 }
 
 int launch_m2l(Plan& p) {
-    if (use_mfma(p)) return launch_m2l_fold(p, p.d_m2lg_ptr, p.d_m2lg, 1.0);
+    if (use_mfma(p)) return launch_m2l_fold(p, p.d_m2lg_ptr, p.d_m2lg, p.d_m2lpatch, 1.0);
     int len = p.nd * p.nexp;
     int gy = (len + 255) / 256;
     k_m2l<<<dim3(p.nanc, gy), 256, 0, p.stream>>>(p.nd, p.nexp, p.d_m2l_ptr, p.d_m2l, p.d_wshift, p.d_mp, p.d_loc);
@@ -1663,7 +1718,7 @@ int launch_adj_eval(Plan& p, const double* gpot, const double* ggrad, const doub
 }
 
 // local-expansion cotangent (in d_mp) -> multipole cotangent, mirror-folded in d_loc
-int launch_adj_m2l(Plan& p) { return launch_m2l_fold(p, p.d_m2lgT_ptr, p.d_m2lgT, -1.0); }
+int launch_adj_m2l(Plan& p) { return launch_m2l_fold(p, p.d_m2lgT_ptr, p.d_m2lgT, p.d_m2lpatchT, -1.0); }
 
 // multipole cotangent -> leaf-value cotangent at PW leaves (accumulated)
 int launch_adj_transform(Plan& p, double* fbar) {

@@ -107,6 +107,15 @@ def test_dot_product_identity_full_size_c3(device, delta):
     assert err < 1e-13, (err, info)
 
 
+def test_dot_product_identity_full_size_c3_free_space(device):
+    """The transposed translation groups of four in free space (source patches
+    cut at the domain edge)."""
+    w = make_workload("c3", "full", delta=1e-5, eps=1e-9)
+    err, info = _dot_identity(w.tree, w.nd, w.delta, w.eps, w.ipoly, 0, w.norder, 3)
+    assert info["translation_group"] == 4
+    assert err < 1e-13, (err, info)
+
+
 def test_dot_product_identity_full_size_c5(device):
     """16-leaf upward chunks in the adjoint's chunk sums (104,317 leaves)."""
     w = make_workload("c5", "full")

@@ -99,6 +99,17 @@ def test_full_size_c3_matches_oracle(device, delta):
     assert_close(compare(w.tree, got, ref, w.ifpgh))
 
 
+def test_full_size_c3_free_space_matches_oracle(device):
+    """The c3 tree in free space (iperiod = 0) at delta = 1e-5: translation
+    groups of four whose 4 x 4 source patches are cut at the domain edge
+    (empty cells, partial target lists)."""
+    w = make_workload("c3", "full", delta=1e-5, eps=1e-9)
+    ref = O.oracle_boxfgt(w.tree, w.fvals, w.nd, w.delta, w.eps, w.ipoly, 0, w.norder, w.ifpgh)
+    got = run_gpu(w.tree, w.fvals, w.nd, w.delta, w.eps, w.ipoly, 0, w.norder, w.ifpgh)
+    assert got["plan"].info["translation_group"] == 4
+    assert_close(compare(w.tree, got, ref, w.ifpgh))
+
+
 @pytest.mark.parametrize("iperiod,n,delta,ifpgh", [(0, 12, 1e-3, 3), (1, 16, 1e-3, 2), (1, 12, 3e-4, 1),
                                                     (0, 16, 1e-2, 3)])
 def test_column_passes_match_oracle(device, iperiod, n, delta, ifpgh):
