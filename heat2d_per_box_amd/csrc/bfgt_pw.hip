@@ -611,7 +611,7 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_down_quad(
     dst[0] = pot;
     if constexpr (NTAB >= 2) dst[1] = dst[2] = grad;
     if constexpr (NTAB >= 3) dst[3] = dst[4] = dst[5] = hess;
-    {
+    {  // this wave's density
         const Fold* F = fold + ((size_t)la.anc * nd + d) * Mh4 * H16;
         for (int tg = 0; tg < nT; tg += NTP) {
             v4d FFr[NTP][NTAB], FFi[NTP][NTAB];
