@@ -18,6 +18,7 @@
 #include <cstdlib>
 #include <memory>
 #include <mutex>
+#include <thread>
 #include <string>
 #include <vector>
 
@@ -1184,6 +1185,90 @@ bool host_cache_enabled() {
     const char* v = std::getenv("BFGT_HOST_PLAN_CACHE");
     return !(v && v[0] == '0');
 }
+
+// Staging of the host-array entry's copies.  Copies from pageable host
+// memory run far below the link rate; the arrays go instead in chunks
+// through two pinned buffers, each chunk's host copy split over several host
+// threads and overlapping the previous chunk's DMA.  Guarded by g_host_mu.
+constexpr size_t kStageBytes = size_t(32) << 20;
+constexpr size_t kStageMin = size_t(4) << 20;  // smaller arrays are copied directly
+struct Stage {
+    void* buf[2] = {nullptr, nullptr};
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    int dev = -1;
+};
+Stage g_stage;
+
+void stage_free() {
+    for (int i = 0; i < 2; i++) {
+        if (g_stage.buf[i]) (void)hipHostFree(g_stage.buf[i]);
+        if (g_stage.ev[i]) (void)hipEventDestroy(g_stage.ev[i]);
+        g_stage.buf[i] = nullptr;
+        g_stage.ev[i] = nullptr;
+    }
+    g_stage.dev = -1;
+}
+
+bool stage_ready(int dev) {
+    if (g_stage.dev == dev) return true;
+    stage_free();
+    for (int i = 0; i < 2; i++)
+        if (hipHostMalloc(&g_stage.buf[i], kStageBytes, hipHostMallocDefault) != hipSuccess ||
+            hipEventCreateWithFlags(&g_stage.ev[i], hipEventDisableTiming) != hipSuccess) {
+            stage_free();
+            return false;
+        }
+    g_stage.dev = dev;
+    return true;
+}
+
+void par_copy(void* dst, const void* src, size_t n) {
+    const unsigned nt = std::min(8u, std::max(1u, std::thread::hardware_concurrency()));
+    const size_t part = ((n + nt - 1) / nt + 4095) & ~size_t(4095);
+    std::vector<std::thread> th;
+    for (size_t off = part; off < n; off += part)
+        th.emplace_back([=] { std::memcpy((char*)dst + off, (const char*)src + off, std::min(part, n - off)); });
+    std::memcpy(dst, src, std::min(part, n));
+    for (auto& x : th) x.join();
+}
+
+// host -> device; k counts the chunks of one call (the buffers alternate)
+hipError_t stage_upload(hipStream_t s, void* dev, const void* host, size_t n, int& k) {
+    if (n < kStageMin || g_stage.dev < 0) return hipMemcpyAsync(dev, host, n, hipMemcpyHostToDevice, s);
+    for (size_t off = 0; off < n; off += kStageBytes, k++) {
+        const size_t len = std::min(kStageBytes, n - off);
+        const int b = k & 1;
+        hipError_t e;
+        if (k >= 2 && (e = hipEventSynchronize(g_stage.ev[b])) != hipSuccess) return e;  // its last DMA is done
+        par_copy(g_stage.buf[b], (const char*)host + off, len);
+        if ((e = hipMemcpyAsync((char*)dev + off, g_stage.buf[b], len, hipMemcpyHostToDevice, s)) != hipSuccess) return e;
+        if ((e = hipEventRecord(g_stage.ev[b], s)) != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+// device -> host after the stream's work; chunk i + 1 is in flight while
+// chunk i is copied out of its buffer
+hipError_t stage_download(hipStream_t s, void* host, const void* dev, size_t n) {
+    if (n < kStageMin || g_stage.dev < 0) {
+        hipError_t e = hipMemcpyAsync(host, dev, n, hipMemcpyDeviceToHost, s);
+        return e != hipSuccess ? e : hipStreamSynchronize(s);
+    }
+    const size_t nc = (n + kStageBytes - 1) / kStageBytes;
+    auto issue = [&](size_t i) -> hipError_t {
+        const size_t off = i * kStageBytes, len = std::min(kStageBytes, n - off);
+        hipError_t e = hipMemcpyAsync(g_stage.buf[i & 1], (const char*)dev + off, len, hipMemcpyDeviceToHost, s);
+        return e != hipSuccess ? e : hipEventRecord(g_stage.ev[i & 1], s);
+    };
+    hipError_t e = issue(0);
+    for (size_t i = 0; e == hipSuccess && i < nc; i++) {
+        if (i + 1 < nc && (e = issue(i + 1)) != hipSuccess) break;
+        if ((e = hipEventSynchronize(g_stage.ev[i & 1])) != hipSuccess) break;
+        const size_t off = i * kStageBytes;
+        par_copy((char*)host + off, g_stage.buf[i & 1], std::min(kStageBytes, n - off));
+    }
+    return e;
+}
 }  // namespace
 
 }  // namespace bfgt
@@ -1448,6 +1533,7 @@ int bfgt_hip_eval(int nd, int ndim, double delta, double eps, int ipoly, int ipe
     for (auto& h : bfgt::g_host_plans)
         if (bfgt::same_key(*h, key)) hp = h.get();
     const int nused = itree[2 * nlevels + 1];
+    const bool reused = hp != nullptr;
     if (hp) {
         // the colleague lists plan creation wrote into itree
         std::copy(hp->ncoll.begin(), hp->ncoll.end(), itree + iptr[5] - 1);
@@ -1509,17 +1595,22 @@ int bfgt_hip_eval(int nd, int ndim, double delta, double eps, int ipoly, int ipe
         hp->cap = nv;
     }
     double *dfv = hp->dfv, *dpot = hp->dpot, *dgrad = hp->dgrad, *dhess = hp->dhess;
-    BFGT_EVAL_CHECK(hipMemcpyAsync(dfv, fvals, nv * sizeof(double), hipMemcpyHostToDevice, p.stream));
-    BFGT_EVAL_CHECK(hipMemcpyAsync(dpot, pot, nv * sizeof(double), hipMemcpyHostToDevice, p.stream));
+    // the pinned staging buffers are set up when a plan is first reused (a
+    // single call gains nothing from pinning them); without them, and for a
+    // first call before that, the copies go direct
+    if (reused) (void)bfgt::stage_ready(hp->dev);
+    int chunk = 0;
+    BFGT_EVAL_CHECK(bfgt::stage_upload(p.stream, dfv, fvals, nv * sizeof(double), chunk));
+    BFGT_EVAL_CHECK(bfgt::stage_upload(p.stream, dpot, pot, nv * sizeof(double), chunk));
     if (ifpgh2 >= 2) {
         if (ifpgh >= 2)
-            BFGT_EVAL_CHECK(hipMemcpyAsync(dgrad, grad, 2 * nv * sizeof(double), hipMemcpyHostToDevice, p.stream));
+            BFGT_EVAL_CHECK(bfgt::stage_upload(p.stream, dgrad, grad, 2 * nv * sizeof(double), chunk));
         else
             BFGT_EVAL_CHECK(hipMemsetAsync(dgrad, 0, 2 * nv * sizeof(double), p.stream));
     }
     if (ifpgh2 >= 3) {
         if (ifpgh >= 3)
-            BFGT_EVAL_CHECK(hipMemcpyAsync(dhess, hess, 3 * nv * sizeof(double), hipMemcpyHostToDevice, p.stream));
+            BFGT_EVAL_CHECK(bfgt::stage_upload(p.stream, dhess, hess, 3 * nv * sizeof(double), chunk));
         else
             BFGT_EVAL_CHECK(hipMemsetAsync(dhess, 0, 3 * nv * sizeof(double), p.stream));
     }
@@ -1551,11 +1642,9 @@ int bfgt_hip_eval(int nd, int ndim, double delta, double eps, int ipoly, int ipe
         BFGT_EVAL_CHECK(hipEventElapsedTime(&ms, e0, e1));
         t_targets = ms * 1e-3;
     }
-    BFGT_EVAL_CHECK(hipMemcpyAsync(pot, dpot, nv * sizeof(double), hipMemcpyDeviceToHost, p.stream));
-    if (ifpgh >= 2)
-        BFGT_EVAL_CHECK(hipMemcpyAsync(grad, dgrad, 2 * nv * sizeof(double), hipMemcpyDeviceToHost, p.stream));
-    if (ifpgh >= 3)
-        BFGT_EVAL_CHECK(hipMemcpyAsync(hess, dhess, 3 * nv * sizeof(double), hipMemcpyDeviceToHost, p.stream));
+    BFGT_EVAL_CHECK(bfgt::stage_download(p.stream, pot, dpot, nv * sizeof(double)));
+    if (ifpgh >= 2) BFGT_EVAL_CHECK(bfgt::stage_download(p.stream, grad, dgrad, 2 * nv * sizeof(double)));
+    if (ifpgh >= 3) BFGT_EVAL_CHECK(bfgt::stage_download(p.stream, hess, dhess, 3 * nv * sizeof(double)));
     BFGT_EVAL_CHECK(hipStreamSynchronize(p.stream));
 #undef BFGT_EVAL_CHECK
     if (timeinfo) {
@@ -1570,6 +1659,7 @@ int bfgt_hip_release_plans(void) {
     std::lock_guard<std::mutex> lock(bfgt::g_host_mu);
     for (auto& h : bfgt::g_host_plans) bfgt::free_host_plan(*h);
     bfgt::g_host_plans.clear();
+    bfgt::stage_free();
     return BFGT_OK;
 }
 

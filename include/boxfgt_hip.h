@@ -59,7 +59,8 @@ extern "C" {
  * reused by later calls with the same tree arrays (laddr, levels, parents,
  * children, centres, box sizes), parameters and device -- the drivers call
  * boxfgt on one tree several times per step -- so such a call costs the
- * copies and the kernels only.  At most 4 plans are kept (least recently used
+ * copies and the kernels only; from the first reuse on, the host arrays are
+ * copied through pinned staging buffers by several host threads.  At most 4 plans are kept (least recently used
  * dropped); the environment variable BFGT_HOST_PLAN_CACHE=0 switches keeping
  * off.  Calls are serialised by a lock.
  */
