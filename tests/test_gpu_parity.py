@@ -383,3 +383,35 @@ def test_host_entry_point_keeps_plans(device, monkeypatch):
     release_plans()
     check_call(w, 1e-5)
     release_plans()
+
+
+def test_host_entry_staged_copies(device):
+    """Host arrays above the staging threshold (c2: 40 MB pot, 80 MB grad): the
+    first call copies directly, the call on the kept plan goes through the
+    pinned buffers in 32 MB chunks (several per array, both buffers); both add
+    the same transform to nonzero starting arrays, bit for bit, and match the
+    oracle."""
+    from heat2d_per_box_amd.boxfgt import boxfgt, release_plans
+
+    w = make_workload("c2", "full")
+    nb, npb, nd = w.fvals.shape
+    rng = np.random.default_rng(11)
+    pot0 = 1e-6 * rng.standard_normal((nb, npb, nd))
+    grad0 = 1e-4 * rng.standard_normal((nb, npb, 2, nd))
+    assert pot0.nbytes >= 32 << 20 and grad0.nbytes >= 64 << 20
+
+    def call():
+        t = w.tree.copy()
+        pot, grad = pot0.copy(), grad0.copy()
+        boxfgt(nd, 2, w.delta, w.eps, w.ipoly, w.iperiod, w.norder, npb, nb, t.nlevels, t.ltree, t.itree, t.iptr,
+               t.centers, t.boxsize, w.fvals, 2, pot, grad)
+        return t, pot, grad
+
+    release_plans()
+    t1, p1, g1 = call()
+    t2, p2, g2 = call()
+    release_plans()
+    assert np.array_equal(p1, p2) and np.array_equal(g1, g2) and np.array_equal(t1.itree, t2.itree)
+    ref = O.oracle_boxfgt(w.tree, w.fvals, nd, w.delta, w.eps, w.ipoly, w.iperiod, w.norder, 2)
+    assert leaf_rel_err(t1, p1, pot0 + ref.pot) < 1e-12
+    assert leaf_rel_err(t1, g1, grad0 + ref.grad) < 1e-11
