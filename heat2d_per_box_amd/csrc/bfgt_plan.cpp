@@ -1225,10 +1225,16 @@ bool stage_ready(int dev) {
 void par_copy(void* dst, const void* src, size_t n) {
     const unsigned nt = std::min(8u, std::max(1u, std::thread::hardware_concurrency()));
     const size_t part = ((n + nt - 1) / nt + 4095) & ~size_t(4095);
+    auto copy = [=](size_t off) { std::memcpy((char*)dst + off, (const char*)src + off, std::min(part, n - off)); };
     std::vector<std::thread> th;
-    for (size_t off = part; off < n; off += part)
-        th.emplace_back([=] { std::memcpy((char*)dst + off, (const char*)src + off, std::min(part, n - off)); });
-    std::memcpy(dst, src, std::min(part, n));
+    size_t off = part;
+    try {
+        for (; off < n; off += part) th.emplace_back(copy, off);
+    } catch (...) {
+        // no thread to be had: this thread copies what is left
+        for (; off < n; off += part) copy(off);
+    }
+    copy(0);
     for (auto& x : th) x.join();
 }
 
@@ -1290,11 +1296,17 @@ int bfgt_plan_create(bfgt_plan_t* out, int nd, int ndim, double delta, double ep
                      const double* centers, const double* boxsize, int ifpgh, int precision, void* stream) {
     if (!out) return set_error(BFGT_ERR_ARG, "null plan pointer");
     *out = nullptr;
-    std::unique_ptr<bfgt_plan> h(new bfgt_plan());
-    int rc = bfgt::plan_create(h->p, nd, ndim, delta, eps, ipoly, iperiod, norder, nboxes, nlevels, ltree, itree,
+    std::unique_ptr<bfgt_plan> h;
+    int rc;
+    try {
+        h.reset(new bfgt_plan());
+        rc = bfgt::plan_create(h->p, nd, ndim, delta, eps, ipoly, iperiod, norder, nboxes, nlevels, ltree, itree,
                                iptr, centers, boxsize, ifpgh, precision, stream);
+    } catch (const std::exception& e) {  // host allocation: no C++ exception crosses the C ABI
+        rc = set_error(BFGT_ERR_RESOURCE, std::string("plan set-up: ") + e.what());
+    }
     if (rc) {
-        bfgt::release(h->p);
+        if (h) bfgt::release(h->p);
         return rc;
     }
     *out = h.release();
@@ -1484,11 +1496,11 @@ int bfgt_plan_get_table(bfgt_plan_t plan, int which, int level, void* host_out, 
     return BFGT_OK;
 }
 
-int bfgt_hip_eval(int nd, int ndim, double delta, double eps, int ipoly, int iperiod, int norder, int npbox,
-                  int nboxes, int nlevels, int ltree, int* itree, const int* iptr, const double* centers,
-                  const double* boxsize, const double* fvals, int ifpgh, double* pot, double* grad, double* hess,
-                  int ifnewtree, int ntarg, const double* targs, int ifpghtarg, double* pote, double* grade,
-                  double* hesse, double* timeinfo, int precision) {
+static int hip_eval(int nd, int ndim, double delta, double eps, int ipoly, int iperiod, int norder, int npbox,
+                    int nboxes, int nlevels, int ltree, int* itree, const int* iptr, const double* centers,
+                    const double* boxsize, const double* fvals, int ifpgh, double* pot, double* grad, double* hess,
+                    int ifnewtree, int ntarg, const double* targs, int ifpghtarg, double* pote, double* grade,
+                    double* hesse, double* timeinfo, int precision) {
     auto t0 = std::chrono::steady_clock::now();
     if (ifnewtree != 0) return set_error(BFGT_ERR_IFNEWTREE, "ifnewtree=1 is not supported");
     if (ifpghtarg < 0 || ifpghtarg > 3) return set_error(BFGT_ERR_TARGETS, "ifpghtarg must be 0..3");
@@ -1653,6 +1665,20 @@ int bfgt_hip_eval(int nd, int ndim, double delta, double eps, int ipoly, int ipe
         timeinfo[11] = t_targets;  // extra targets (slot 12 of the reference)
     }
     return finish(BFGT_OK);
+}
+
+int bfgt_hip_eval(int nd, int ndim, double delta, double eps, int ipoly, int iperiod, int norder, int npbox,
+                  int nboxes, int nlevels, int ltree, int* itree, const int* iptr, const double* centers,
+                  const double* boxsize, const double* fvals, int ifpgh, double* pot, double* grad, double* hess,
+                  int ifnewtree, int ntarg, const double* targs, int ifpghtarg, double* pote, double* grade,
+                  double* hesse, double* timeinfo, int precision) {
+    try {
+        return hip_eval(nd, ndim, delta, eps, ipoly, iperiod, norder, npbox, nboxes, nlevels, ltree, itree, iptr,
+                        centers, boxsize, fvals, ifpgh, pot, grad, hess, ifnewtree, ntarg, targs, ifpghtarg, pote,
+                        grade, hesse, timeinfo, precision);
+    } catch (const std::exception& e) {  // host allocation: no C++ exception crosses the C ABI
+        return set_error(BFGT_ERR_RESOURCE, std::string("boxfgt: ") + e.what());
+    }
 }
 
 int bfgt_hip_release_plans(void) {
