@@ -130,3 +130,23 @@ def test_device_targets_polynomial_exact_and_match_oracle(device, ipoly, n):
     rp, rg, rh = O.oracle_eval_targets(t, t.itree, nd, ipoly, n, tg, 3, vals, grad, hess)
     rel = lambda a, b: np.linalg.norm(a - b) / np.linalg.norm(b)  # noqa: E731
     assert rel(pe, rp) < 1e-13 and rel(ge, rg) < 1e-13 and rel(he, rh) < 1e-13
+
+
+@pytest.mark.parametrize("case", [1, 2])
+def test_host_targets_on_kept_plan(device, case):
+    """The same host call with extra targets twice: the second runs on the kept
+    plan (target arrays of its own; with ifpghtarg > ifpgh the scratch channels
+    zeroed again) and gives the first call's outputs bit for bit."""
+    from heat2d_per_box_amd.boxfgt import release_plans
+
+    name, scale, kw, ifpgh, ifpt = CASES[case]
+    w = make_workload(name, scale, **kw)
+    rng = np.random.default_rng(9)
+    tg = _targets(rng, 500)
+    pot0 = 0.01 * rng.standard_normal(w.fvals.shape)
+    release_plans()
+    first = _host(w, ifpgh, ifpt, tg, pot0)
+    second = _host(w, ifpgh, ifpt, tg, pot0)
+    release_plans()
+    for a, b in zip(first[:6], second[:6]):
+        assert np.array_equal(a, b)
