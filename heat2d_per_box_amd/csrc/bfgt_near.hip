@@ -320,6 +320,143 @@ __global__ __launch_bounds__(64 * kNearWaves) void k_near_mfma(
     }
 }
 
+// n <= 8 and two or more densities: one wave per (target, pair of densities
+// d0, d0 + 1).  At n <= 8 a 16x16 tile of one density and one table is a
+// quarter full, so the x contraction holds the two densities in its row
+// halves (h) and two x tables in its column halves (g):
+//   FF_X[(h, j2)][(g, k1)] = sum_j1 F_{d0+h}[j1][j2] Tx_{tx(X,g)}[k1][j1]
+// and the y contraction of density h takes the K-steps of its row half and
+// holds two y tables in its row halves (v):
+//   Q_{h,X}[(v, k2)][(g, k1)] = sum_j2 Ty_{ty(X,v)}[k2][j2] FF_X[(h, j2)][(g, k1)]
+// Tiles X0: x tables (0 | 1), y tables (0 | 1): pot, gx / gy, hxy;
+//       X1: x tables (2 | 0), y tables (0 | 2): hxx, - / -, hyy.
+// Three tables take 4 + 8 products for two densities where one density per
+// wave and one table per tile take 2 x 18.  Quadrants marked "-", those of
+// a missing density d0 + 1 = nd and, for pot only, all but (0, 0) hold
+// finite values that are never stored.
+template <int NTAB>
+__device__ constexpr int pack_tx(int X, int g) { return X == 0 ? (NTAB == 1 ? 0 : g) : (g ? 0 : 2); }
+template <int NTAB>
+__device__ constexpr int pack_ty(int X, int v) { return X == 0 ? (NTAB == 1 ? 0 : v) : (v ? 2 : 0); }
+// output channel of quadrant (v, g) of tile X: 0 pot, 1 gx, 2 gy, 3 hxx, 4 hxy, 5 hyy; -1 none
+template <int NTAB>
+__device__ constexpr int pack_chan(int X, int g, int v) {
+    if (X == 1) return g != v ? -1 : (g == 0 ? 3 : 5);
+    if (NTAB == 1) return g == 0 && v == 0 ? 0 : -1;
+    return g + v >= NTAB ? -1 : (g + v) * (g + v + 1) / 2 + v;
+}
+
+template <int NKS, int NTAB>
+__global__ __launch_bounds__(64 * kNearWaves) void k_near_pack(
+    int n, int nd, int nitems, const int* __restrict__ tgt, const int* __restrict__ tlev,
+    const int* __restrict__ ptr, const NearPair* __restrict__ pairs, const double* __restrict__ tnearT,
+    const double* __restrict__ fvals, double* __restrict__ pot, double* __restrict__ grad,
+    double* __restrict__ hess, double oscale) {
+    static_assert(NKS <= 2, "two densities per tile need n <= 8");
+    constexpr int NX = NTAB == 3 ? 2 : 1;
+    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+    const int item = __builtin_amdgcn_readfirstlane(blockIdx.x * kNearWaves + wave);
+    if (item >= nitems) return;  // no block-wide barriers in this kernel
+    const int ndp = (nd + 1) / 2;
+    const int it = item / ndp, d0 = 2 * (item % ndp);
+    const int q = lane >> 4, c = lane & 15, h = c >> 3, cc = c & 7;
+    const int b = tgt[it];
+    const int p0 = ptr[it], p1 = ptr[it + 1];
+    const double* T = tnearT + (size_t)tlev[it] * 13 * 3 * 256;
+    const int nn = n * n;
+    const int cl = min(cc, n - 1);
+    // A operand of the x contraction: row c = (density h, source y index cc)
+    const int dh = min(d0 + h, nd - 1);
+    // per lane: x table of column half h, y table of row half h, per tile
+    int tx[NX], ty[NX];
+#pragma unroll
+    for (int X = 0; X < NX; X++) {
+        tx[X] = h ? pack_tx<NTAB>(X, 1) : pack_tx<NTAB>(X, 0);
+        ty[X] = h ? pack_ty<NTAB>(X, 1) : pack_ty<NTAB>(X, 0);
+    }
+    v4d out[2][NX];
+#pragma unroll
+    for (int hh = 0; hh < 2; hh++)
+#pragma unroll
+        for (int X = 0; X < NX; X++) out[hh][X] = v4d{0, 0, 0, 0};
+    for (int p = p0; p < p1; p += 2) {
+        const bool two = p + 1 < p1;
+        const NearPair pr[2] = {pairs[p], pairs[two ? p + 1 : p]};
+        v4d FF[2][NX];
+#pragma unroll
+        for (int s2 = 0; s2 < 2; s2++) {
+            const double* fs = fvals + (size_t)pr[s2].src * nn * nd + dh;
+            const double* Tx = T + (size_t)(pr[s2].ix + 6) * 3 * 256;
+            const double wgt = (s2 == 0 || two) ? 1.0 : 0.0;
+#pragma unroll
+            for (int X = 0; X < NX; X++) FF[s2][X] = v4d{0, 0, 0, 0};
+#pragma unroll
+            for (int s = 0; s < NKS; s++) {
+                const int j1 = 4 * s + q;
+                const double a = wgt * fs[(size_t)(min(j1, n - 1) + n * cl) * nd];
+#pragma unroll
+                for (int X = 0; X < NX; X++) FF[s2][X] = mfma(a, Tx[tx[X] * 256 + j1 * 16 + cc], FF[s2][X]);
+            }
+        }
+        // element s of FF_X on lane (q, c) = FF_X[4s + q][c]: rows of density
+        // half s / 2, source y indices 4 (s % 2) + q
+#pragma unroll
+        for (int s2 = 0; s2 < 2; s2++) {
+            const double* Ty = T + (size_t)(pr[s2].iy + 6) * 3 * 256;
+#pragma unroll
+            for (int ss = 0; ss < NKS; ss++) {
+                const int j2 = 4 * ss + q;
+                double ya[NX];
+#pragma unroll
+                for (int X = 0; X < NX; X++) ya[X] = Ty[ty[X] * 256 + j2 * 16 + cc];  // A[(v, k2 = cc)][j2]
+#pragma unroll
+                for (int hh = 0; hh < 2; hh++)
+#pragma unroll
+                    for (int X = 0; X < NX; X++) out[hh][X] = mfma(ya[X], FF[s2][X][2 * hh + ss], out[hh][X]);
+            }
+        }
+    }
+    // element r of Q_{hh,X} on lane (q, c): density d0 + hh, y-table half
+    // v = r / 2, target y index k2 = 4 (r % 2) + q, x-table half h, target x
+    // index k1 = cc
+    if (p1 > p0) {
+        double old[2][NX][4];
+#pragma unroll
+        for (int hh = 0; hh < 2; hh++) {
+            const int dd = min(d0 + hh, nd - 1);
+#pragma unroll
+            for (int X = 0; X < NX; X++) {
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const int o = h ? pack_chan<NTAB>(X, 1, r / 2) : pack_chan<NTAB>(X, 0, r / 2);
+                    double* dst = o <= 0 ? pot : (o <= 2 ? grad : hess);
+                    const int str = o <= 0 ? 1 : (o <= 2 ? 2 : 3), cmp = o <= 0 ? 0 : (o <= 2 ? o - 1 : o - 3);
+                    const size_t pb = (size_t)b * nn + cl + (size_t)n * min(4 * (r % 2) + q, n - 1);
+                    old[hh][X][r] = o >= 0 ? dst[(pb * str + cmp) * nd + dd] : 0.0;
+                }
+            }
+        }
+#pragma unroll
+        for (int hh = 0; hh < 2; hh++) {
+            const int dd = d0 + hh;
+#pragma unroll
+            for (int X = 0; X < NX; X++) {
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const int o = h ? pack_chan<NTAB>(X, 1, r / 2) : pack_chan<NTAB>(X, 0, r / 2);
+                    double* dst = o <= 0 ? pot : (o <= 2 ? grad : hess);
+                    const int str = o <= 0 ? 1 : (o <= 2 ? 2 : 3), cmp = o <= 0 ? 0 : (o <= 2 ? o - 1 : o - 3);
+                    const int k2 = 4 * (r % 2) + q;
+                    if (o >= 0 && dd < nd && k2 < n && cc < n) {
+                        const size_t pb = (size_t)b * nn + cc + (size_t)n * k2;
+                        dst[(pb * str + cmp) * nd + dd] = old[hh][X][r] + oscale * out[hh][X][r];
+                    }
+                }
+            }
+        }
+    }
+}
+
 // Adjoint, one wave per (source, density) over the (target, offsets) pairs
 // that read it:
 //   W_v   = sum_{o: x(o) = v} G_o Ty_{y(o)}   (K = target y index)
@@ -424,6 +561,24 @@ int launch_near_mfma1(Plan& p, const double* fvals, double* pot, double* grad, d
 }
 
 template <int NKS, int NTAB>
+int launch_near_pack2(Plan& p, const double* fvals, double* pot, double* grad, double* hess) {
+    int nitems = p.nnear_t * ((p.nd + 1) / 2);
+    int blocks = (nitems + kNearWaves - 1) / kNearWaves;
+    k_near_pack<NKS, NTAB><<<blocks, 64 * kNearWaves, 0, p.stream>>>(p.n, p.nd, nitems, p.d_near_tgt, p.d_near_lev,
+                                                                   p.d_near_ptr, p.d_near, p.d_tnearT, fvals, pot,
+                                                                   grad, hess, p.out_scale);
+    BFGT_HIP_CHECK(hipGetLastError());
+    return BFGT_OK;
+}
+
+template <int NKS>
+int launch_near_pack1(Plan& p, const double* fvals, double* pot, double* grad, double* hess) {
+    if (p.ifpgh == 1) return launch_near_pack2<NKS, 1>(p, fvals, pot, grad, hess);
+    if (p.ifpgh == 2) return launch_near_pack2<NKS, 2>(p, fvals, pot, grad, hess);
+    return launch_near_pack2<NKS, 3>(p, fvals, pot, grad, hess);
+}
+
+template <int NKS, int NTAB>
 int launch_near_adj_mfma2(Plan& p, const double* gpot, const double* ggrad, const double* ghess, double* fbar) {
     int nitems = p.nnearT_s * p.nd;
     int blocks = (nitems + kNearWaves - 1) / kNearWaves;
@@ -459,6 +614,8 @@ size_t near_smem_bytes(int n, int ntab) { return sizeof(double) * (size_t)n * n 
 int launch_near(Plan& p, const double* fvals, double* pot, double* grad, double* hess) {
     if (p.nnear_t == 0) return BFGT_OK;
     if (near_use_mfma(p)) {
+        if (p.n <= 4 && p.nd >= 2) return launch_near_pack1<1>(p, fvals, pot, grad, hess);
+        if (p.n <= 8 && p.nd >= 2) return launch_near_pack1<2>(p, fvals, pot, grad, hess);
         switch ((p.n + 3) / 4) {
             case 1: return launch_near_mfma1<1>(p, fvals, pot, grad, hess);
             case 2: return launch_near_mfma1<2>(p, fvals, pot, grad, hess);

@@ -173,6 +173,27 @@ def test_random_trees_match_oracle(device, seed):
     assert_close(compare(t, got, ref, ifpgh))
 
 
+@pytest.mark.parametrize("n,nd,ifpgh,iperiod,ipoly", [(8, 3, 3, 1, 0), (8, 2, 2, 0, 1), (6, 3, 1, 1, 0), (4, 2, 3, 0, 0),
+                                                      (4, 3, 2, 1, 1), (8, 4, 3, 0, 0), (6, 2, 3, 1, 1)])
+def test_near_field_density_pairs(device, n, nd, ifpgh, iperiod, ipoly):
+    """n <= 8 with two or more densities: the near field holds two densities and
+    two tables per matrix tile (an odd nd leaves the last tile's second density
+    empty).  Small delta, so that most leaves are near-field targets."""
+    rng = np.random.default_rng(300 + 10 * n + nd)
+    g = GaussianSum(rng.uniform(-0.5, 0.5, (3, 2)), rng.uniform(0.02, 0.1, 3), rng.uniform(-1, 1, 3))
+    comps = lambda p: [g(p), np.sin(3 * p[..., 0]) * g(p), g(p) ** 2, np.cos(2 * p[..., 1]) + 0.1 * g(p)]  # noqa: E731
+    fun = lambda p: np.stack(comps(p)[:nd], axis=-1)  # noqa: E731
+    t = build_adaptive(fun, nd, n, ipoly, 1e-6, 6, iperiod=iperiod, min_level=1, eta=0.0)
+    fv = t.sample(fun, n, ipoly, nd)
+    delta, eps = 1e-5, 1e-9
+    print(f"n={n} nd={nd} ifpgh={ifpgh} iperiod={iperiod} ipoly={ipoly} leaves={t.nleaves}", flush=True)
+    ref = O.oracle_boxfgt(t, fv, nd, delta, eps, ipoly, iperiod, n, ifpgh)
+    got = run_gpu(t, fv, nd, delta, eps, ipoly, iperiod, n, ifpgh)
+    assert got["plan"].info["n_near_pairs"] > 0
+    assert np.array_equal(got["itree"], ref.itree)
+    assert_close(compare(t, got, ref, ifpgh))
+
+
 def test_fourier_mode_closed_form(device):
     """Heat semigroup on a Fourier mode: pot = pi delta exp(-delta|k|^2/4) f."""
     t = build_uniform(4)
