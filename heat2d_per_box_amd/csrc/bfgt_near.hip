@@ -321,7 +321,7 @@ __global__ __launch_bounds__(64 * kNearWaves) void k_near_mfma(
 }
 
 // n <= 8 and two or more densities: one wave per (target, pair of densities
-// d0, d0 + 1).  At n <= 8 a 16x16 tile of one density and one table is a
+// d0, d0 + 1, tile).  At n <= 8 a 16x16 tile of one density and one table is a
 // quarter full, so the x contraction holds the two densities in its row
 // halves (h) and two x tables in its column halves (g):
 //   FF_X[(h, j2)][(g, k1)] = sum_j1 F_{d0+h}[j1][j2] Tx_{tx(X,g)}[k1][j1]
@@ -353,12 +353,12 @@ __global__ __launch_bounds__(64 * kNearWaves) void k_near_pack(
     const double* __restrict__ fvals, double* __restrict__ pot, double* __restrict__ grad,
     double* __restrict__ hess, double oscale) {
     static_assert(NKS <= 2, "two densities per tile need n <= 8");
-    constexpr int NX = NTAB == 3 ? 2 : 1;
+    constexpr int NX = NTAB == 3 ? 2 : 1;  // tiles, one wave each
     const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
     const int item = __builtin_amdgcn_readfirstlane(blockIdx.x * kNearWaves + wave);
     if (item >= nitems) return;  // no block-wide barriers in this kernel
     const int ndp = (nd + 1) / 2;
-    const int it = item / ndp, d0 = 2 * (item % ndp);
+    const int X = item % NX, it = item / NX / ndp, d0 = 2 * (item / NX % ndp);
     const int q = lane >> 4, c = lane & 15, h = c >> 3, cc = c & 7;
     const int b = tgt[it];
     const int p0 = ptr[it], p1 = ptr[it + 1];
@@ -367,38 +367,28 @@ __global__ __launch_bounds__(64 * kNearWaves) void k_near_pack(
     const int cl = min(cc, n - 1);
     // A operand of the x contraction: row c = (density h, source y index cc)
     const int dh = min(d0 + h, nd - 1);
-    // per lane: x table of column half h, y table of row half h, per tile
-    int tx[NX], ty[NX];
-#pragma unroll
-    for (int X = 0; X < NX; X++) {
-        tx[X] = h ? pack_tx<NTAB>(X, 1) : pack_tx<NTAB>(X, 0);
-        ty[X] = h ? pack_ty<NTAB>(X, 1) : pack_ty<NTAB>(X, 0);
-    }
-    v4d out[2][NX];
-#pragma unroll
-    for (int hh = 0; hh < 2; hh++)
-#pragma unroll
-        for (int X = 0; X < NX; X++) out[hh][X] = v4d{0, 0, 0, 0};
+    // per lane: x table of column half h, y table of row half h
+    const int tx = h ? pack_tx<NTAB>(X, 1) : pack_tx<NTAB>(X, 0);
+    const int ty = h ? pack_ty<NTAB>(X, 1) : pack_ty<NTAB>(X, 0);
+    v4d out[2] = {v4d{0, 0, 0, 0}, v4d{0, 0, 0, 0}};
     for (int p = p0; p < p1; p += 2) {
         const bool two = p + 1 < p1;
         const NearPair pr[2] = {pairs[p], pairs[two ? p + 1 : p]};
-        v4d FF[2][NX];
+        v4d FF[2];
 #pragma unroll
         for (int s2 = 0; s2 < 2; s2++) {
             const double* fs = fvals + (size_t)pr[s2].src * nn * nd + dh;
             const double* Tx = T + (size_t)(pr[s2].ix + 6) * 3 * 256;
             const double wgt = (s2 == 0 || two) ? 1.0 : 0.0;
-#pragma unroll
-            for (int X = 0; X < NX; X++) FF[s2][X] = v4d{0, 0, 0, 0};
+            FF[s2] = v4d{0, 0, 0, 0};
 #pragma unroll
             for (int s = 0; s < NKS; s++) {
                 const int j1 = 4 * s + q;
                 const double a = wgt * fs[(size_t)(min(j1, n - 1) + n * cl) * nd];
-#pragma unroll
-                for (int X = 0; X < NX; X++) FF[s2][X] = mfma(a, Tx[tx[X] * 256 + j1 * 16 + cc], FF[s2][X]);
+                FF[s2] = mfma(a, Tx[tx * 256 + j1 * 16 + cc], FF[s2]);
             }
         }
-        // element s of FF_X on lane (q, c) = FF_X[4s + q][c]: rows of density
+        // element s of FF on lane (q, c) = FF[4s + q][c]: rows of density
         // half s / 2, source y indices 4 (s % 2) + q
 #pragma unroll
         for (int s2 = 0; s2 < 2; s2++) {
@@ -406,13 +396,9 @@ __global__ __launch_bounds__(64 * kNearWaves) void k_near_pack(
 #pragma unroll
             for (int ss = 0; ss < NKS; ss++) {
                 const int j2 = 4 * ss + q;
-                double ya[NX];
+                const double ya = Ty[ty * 256 + j2 * 16 + cc];  // A[(v, k2 = cc)][j2]
 #pragma unroll
-                for (int X = 0; X < NX; X++) ya[X] = Ty[ty[X] * 256 + j2 * 16 + cc];  // A[(v, k2 = cc)][j2]
-#pragma unroll
-                for (int hh = 0; hh < 2; hh++)
-#pragma unroll
-                    for (int X = 0; X < NX; X++) out[hh][X] = mfma(ya[X], FF[s2][X][2 * hh + ss], out[hh][X]);
+                for (int hh = 0; hh < 2; hh++) out[hh] = mfma(ya, FF[s2][2 * hh + ss], out[hh]);
             }
         }
     }
@@ -420,37 +406,32 @@ __global__ __launch_bounds__(64 * kNearWaves) void k_near_pack(
     // v = r / 2, target y index k2 = 4 (r % 2) + q, x-table half h, target x
     // index k1 = cc
     if (p1 > p0) {
-        double old[2][NX][4];
+        int o[4];
+#pragma unroll
+        for (int r = 0; r < 4; r++) o[r] = pack_chan<NTAB>(X, h, r / 2);
+        double old[2][4];
 #pragma unroll
         for (int hh = 0; hh < 2; hh++) {
             const int dd = min(d0 + hh, nd - 1);
 #pragma unroll
-            for (int X = 0; X < NX; X++) {
-#pragma unroll
-                for (int r = 0; r < 4; r++) {
-                    const int o = h ? pack_chan<NTAB>(X, 1, r / 2) : pack_chan<NTAB>(X, 0, r / 2);
-                    double* dst = o <= 0 ? pot : (o <= 2 ? grad : hess);
-                    const int str = o <= 0 ? 1 : (o <= 2 ? 2 : 3), cmp = o <= 0 ? 0 : (o <= 2 ? o - 1 : o - 3);
-                    const size_t pb = (size_t)b * nn + cl + (size_t)n * min(4 * (r % 2) + q, n - 1);
-                    old[hh][X][r] = o >= 0 ? dst[(pb * str + cmp) * nd + dd] : 0.0;
-                }
+            for (int r = 0; r < 4; r++) {
+                double* dst = o[r] <= 0 ? pot : (o[r] <= 2 ? grad : hess);
+                const int str = o[r] <= 0 ? 1 : (o[r] <= 2 ? 2 : 3), cmp = o[r] <= 0 ? 0 : (o[r] <= 2 ? o[r] - 1 : o[r] - 3);
+                const size_t pb = (size_t)b * nn + cl + (size_t)n * min(4 * (r % 2) + q, n - 1);
+                old[hh][r] = o[r] >= 0 ? dst[(pb * str + cmp) * nd + dd] : 0.0;
             }
         }
 #pragma unroll
         for (int hh = 0; hh < 2; hh++) {
             const int dd = d0 + hh;
 #pragma unroll
-            for (int X = 0; X < NX; X++) {
-#pragma unroll
-                for (int r = 0; r < 4; r++) {
-                    const int o = h ? pack_chan<NTAB>(X, 1, r / 2) : pack_chan<NTAB>(X, 0, r / 2);
-                    double* dst = o <= 0 ? pot : (o <= 2 ? grad : hess);
-                    const int str = o <= 0 ? 1 : (o <= 2 ? 2 : 3), cmp = o <= 0 ? 0 : (o <= 2 ? o - 1 : o - 3);
-                    const int k2 = 4 * (r % 2) + q;
-                    if (o >= 0 && dd < nd && k2 < n && cc < n) {
-                        const size_t pb = (size_t)b * nn + cc + (size_t)n * k2;
-                        dst[(pb * str + cmp) * nd + dd] = old[hh][X][r] + oscale * out[hh][X][r];
-                    }
+            for (int r = 0; r < 4; r++) {
+                double* dst = o[r] <= 0 ? pot : (o[r] <= 2 ? grad : hess);
+                const int str = o[r] <= 0 ? 1 : (o[r] <= 2 ? 2 : 3), cmp = o[r] <= 0 ? 0 : (o[r] <= 2 ? o[r] - 1 : o[r] - 3);
+                const int k2 = 4 * (r % 2) + q;
+                if (o[r] >= 0 && dd < nd && k2 < n && cc < n) {
+                    const size_t pb = (size_t)b * nn + cc + (size_t)n * k2;
+                    dst[(pb * str + cmp) * nd + dd] = old[hh][r] + oscale * out[hh][r];
                 }
             }
         }
@@ -562,7 +543,7 @@ int launch_near_mfma1(Plan& p, const double* fvals, double* pot, double* grad, d
 
 template <int NKS, int NTAB>
 int launch_near_pack2(Plan& p, const double* fvals, double* pot, double* grad, double* hess) {
-    int nitems = p.nnear_t * ((p.nd + 1) / 2);
+    int nitems = p.nnear_t * ((p.nd + 1) / 2) * (NTAB == 3 ? 2 : 1);
     int blocks = (nitems + kNearWaves - 1) / kNearWaves;
     k_near_pack<NKS, NTAB><<<blocks, 64 * kNearWaves, 0, p.stream>>>(p.n, p.nd, nitems, p.d_near_tgt, p.d_near_lev,
                                                                    p.d_near_ptr, p.d_near, p.d_tnearT, fvals, pot,
