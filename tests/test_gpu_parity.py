@@ -174,11 +174,14 @@ def test_random_trees_match_oracle(device, seed):
 
 
 @pytest.mark.parametrize("n,nd,ifpgh,iperiod,ipoly", [(8, 3, 3, 1, 0), (8, 2, 2, 0, 1), (6, 3, 1, 1, 0), (4, 2, 3, 0, 0),
-                                                      (4, 3, 2, 1, 1), (8, 4, 3, 0, 0), (6, 2, 3, 1, 1)])
-def test_near_field_density_pairs(device, n, nd, ifpgh, iperiod, ipoly):
-    """n <= 8 with two or more densities: the near field holds two densities and
-    two tables per matrix tile (an odd nd leaves the last tile's second density
-    empty).  Small delta, so that most leaves are near-field targets."""
+                                                      (4, 3, 2, 1, 1), (8, 4, 3, 0, 0), (6, 2, 3, 1, 1),
+                                                      (8, 1, 3, 1, 0), (8, 1, 2, 0, 1), (6, 1, 1, 1, 1), (4, 1, 3, 0, 0),
+                                                      (4, 1, 2, 1, 0)])
+def test_packed_near_field(device, n, nd, ifpgh, iperiod, ipoly):
+    """n <= 8: with nd >= 2 the near field holds two densities and two tables
+    per matrix tile (an odd nd leaves the last tile's second density empty);
+    nd = 1 runs the one-density kernel.  Small delta, so that most leaves are
+    near-field targets."""
     rng = np.random.default_rng(300 + 10 * n + nd)
     g = GaussianSum(rng.uniform(-0.5, 0.5, (3, 2)), rng.uniform(0.02, 0.1, 3), rng.uniform(-1, 1, 3))
     comps = lambda p: [g(p), np.sin(3 * p[..., 0]) * g(p), g(p) ** 2, np.cos(2 * p[..., 1]) + 0.1 * g(p)]  # noqa: E731
