@@ -11,6 +11,8 @@
 // read-modify-write of pot/grad/hess: no atomics.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "bfgt_internal.h"
 
 namespace bfgt {
@@ -231,7 +233,7 @@ constexpr int kNearWaves = 4;
 // where element s of FF_t^T's accumulator is the B operand of K-step s of
 // the second product.  Channels o: pot (0,0), gx (1,0), gy (0,1), hxx (2,0),
 // hxy (1,1), hyy (0,2) as (x table, y table).  NKS = ceil(n/4).
-template <int NKS, int NTAB>
+template <int NKS, int NTAB, int NSPLIT>
 __global__ __launch_bounds__(64 * kNearWaves) void k_near_mfma(
     int n, int nd, int nitems, const int* __restrict__ tgt, const int* __restrict__ tlev,
     const int* __restrict__ ptr, const NearPair* __restrict__ pairs, const double* __restrict__ tnearT,
@@ -242,13 +244,23 @@ __global__ __launch_bounds__(64 * kNearWaves) void k_near_mfma(
     // wave-uniform: schedule entries are then read with scalar loads
     const int item = __builtin_amdgcn_readfirstlane(blockIdx.x * kNearWaves + wave);
     if (item >= nitems) return;  // no block-wide barriers in this kernel
-    const int it = item / nd, d = item % nd;
+    const int tw = item % NSPLIT, it = item / NSPLIT / nd, d = item / NSPLIT % nd;
     const int q = lane >> 4, c = lane & 15;
     const int b = tgt[it];
     const int p0 = ptr[it], p1 = ptr[it + 1];
     const double* T = tnearT + (size_t)tlev[it] * 13 * 3 * 256;
     const int nn = n * n;
     const int cl = min(c, n - 1);
+    // TW: this wave's x tables, 0 = table 0, 1 = tables 1 and 2 (their x
+    // contractions and the outputs o whose near_x(o) is one of them), or -1
+    // for all
+    auto body = [&](auto twc) {
+    constexpr int TW = decltype(twc)::value;
+    constexpr auto mine_t = [](int t) { return TW < 0 || (TW == 0 ? t == 0 : t >= 1); };
+    constexpr auto mine = [](int o) {
+        const int t = near_x(o);
+        return TW < 0 || (TW == 0 ? t == 0 : t >= 1);
+    };
     v4d out[NOUT];
 #pragma unroll
     for (int o = 0; o < NOUT; o++) out[o] = v4d{0, 0, 0, 0};
@@ -271,7 +283,8 @@ __global__ __launch_bounds__(64 * kNearWaves) void k_near_mfma(
                 // A = F^T: A[j2 = c][j1]; B = Tx_t^T: B[j1][k1 = c] = Tx_t[c][j1] (transposed table)
                 const double a = wgt * fs[(size_t)(min(j1, n - 1) + n * cl) * nd];
 #pragma unroll
-                for (int t = 0; t < NTAB; t++) FF[h][t] = mfma(a, Tx[t * 256 + j1 * 16 + c], FF[h][t]);
+                for (int t = 0; t < NTAB; t++)
+                    if (mine_t(t)) FF[h][t] = mfma(a, Tx[t * 256 + j1 * 16 + c], FF[h][t]);
             }
         }
         // FF_t^T: element s of lane (q, c) = FF_t^T[j2 = 4s + q][k1 = c]
@@ -283,9 +296,11 @@ __global__ __launch_bounds__(64 * kNearWaves) void k_near_mfma(
                 const int j2 = 4 * s + q;
                 double ty[NTAB];
 #pragma unroll
-                for (int u = 0; u < NTAB; u++) ty[u] = Ty[u * 256 + j2 * 16 + c];  // A[k2 = c][j2] = Ty[c][j2]
+                for (int u = 0; u < NTAB; u++)
+                    if (TW <= 0 || u + 1 < NTAB) ty[u] = Ty[u * 256 + j2 * 16 + c];  // A[k2 = c][j2] = Ty[c][j2]
 #pragma unroll
-                for (int o = 0; o < NOUT; o++) out[o] = mfma(ty[near_y(o)], FF[h][near_x(o)][s], out[o]);
+                for (int o = 0; o < NOUT; o++)
+                    if (mine(o)) out[o] = mfma(ty[near_y(o)], FF[h][near_x(o)][s], out[o]);
             }
         }
     }
@@ -303,7 +318,8 @@ __global__ __launch_bounds__(64 * kNearWaves) void k_near_mfma(
         for (int r = 0; r < 4; r++) {
             const size_t pb = (size_t)b * nn + min(c, n - 1) + (size_t)n * min(4 * r + q, n - 1);
 #pragma unroll
-            for (int o = 0; o < NOUT; o++) old[r][o] = dst[o][(pb * stride_o[o] + comp_o[o]) * nd + d];
+            for (int o = 0; o < NOUT; o++)
+                if (mine(o)) old[r][o] = dst[o][(pb * stride_o[o] + comp_o[o]) * nd + d];
         }
         if (c < n) {
 #pragma unroll
@@ -313,10 +329,18 @@ __global__ __launch_bounds__(64 * kNearWaves) void k_near_mfma(
                     const size_t pb = (size_t)b * nn + c + (size_t)n * k2;
 #pragma unroll
                     for (int o = 0; o < NOUT; o++)
-                        dst[o][(pb * stride_o[o] + comp_o[o]) * nd + d] = old[r][o] + oscale * out[o][r];
+                        if (mine(o)) dst[o][(pb * stride_o[o] + comp_o[o]) * nd + d] = old[r][o] + oscale * out[o][r];
                 }
             }
         }
+    }
+    };
+    if constexpr (NSPLIT == 1) {
+        body(std::integral_constant<int, -1>{});
+    } else {
+        static_assert(NSPLIT == 2 && NTAB >= 2, "x table 0 | the others");
+        if (tw == 0) body(std::integral_constant<int, 0>{});
+        else body(std::integral_constant<int, 1>{});
     }
 }
 
@@ -523,15 +547,28 @@ __global__ void k_pad_near(int n, int nlev, const double* __restrict__ tloc, dou
     }
 }
 
-template <int NKS, int NTAB>
-int launch_near_mfma2(Plan& p, const double* fvals, double* pot, double* grad, double* hess) {
-    int nitems = p.nnear_t * p.nd;
+template <int NKS, int NTAB, int NSPLIT>
+int launch_near_mfma3(Plan& p, const double* fvals, double* pot, double* grad, double* hess) {
+    int nitems = p.nnear_t * p.nd * NSPLIT;
     int blocks = (nitems + kNearWaves - 1) / kNearWaves;
-    k_near_mfma<NKS, NTAB><<<blocks, 64 * kNearWaves, 0, p.stream>>>(p.n, p.nd, nitems, p.d_near_tgt, p.d_near_lev,
+    k_near_mfma<NKS, NTAB, NSPLIT><<<blocks, 64 * kNearWaves, 0, p.stream>>>(p.n, p.nd, nitems, p.d_near_tgt, p.d_near_lev,
                                                                    p.d_near_ptr, p.d_near, p.d_tnearT, fvals, pot,
                                                                    grad, hess, p.out_scale);
     BFGT_HIP_CHECK(hipGetLastError());
     return BFGT_OK;
+}
+
+// One density gives one wave per target, too few to fill the chip: its
+// tables are then split over two waves (x table 0 | the others; c3 tree,
+// n = 8, pot + grad 16.3 -> 13.3 us; n = 16, three tables 22.6 -> 17.7 us).
+// With more densities the split's second read of the leaf values costs more
+// than it gains (n = 16, two densities: 24.9 -> 28.7 us).
+template <int NKS, int NTAB>
+int launch_near_mfma2(Plan& p, const double* fvals, double* pot, double* grad, double* hess) {
+    if constexpr (NTAB >= 2) {
+        if (p.nd == 1) return launch_near_mfma3<NKS, NTAB, 2>(p, fvals, pot, grad, hess);
+    }
+    return launch_near_mfma3<NKS, NTAB, 1>(p, fvals, pot, grad, hess);
 }
 
 template <int NKS>

@@ -197,6 +197,19 @@ def test_packed_near_field(device, n, nd, ifpgh, iperiod, ipoly):
     assert_close(compare(t, got, ref, ifpgh))
 
 
+@pytest.mark.parametrize("nd,ifpgh", [(1, 3), (1, 2), (2, 3)])
+def test_near_field_n16(device, nd, ifpgh):
+    """n = 16 on a near-field-dominated tree (the 406-leaf c2 tree at
+    delta = 1e-6): with one density each target's tables are split over two
+    waves (x table 0 | the others), with two one wave per (target, density)."""
+    w = make_workload("c2", "small")
+    fv = np.ascontiguousarray(np.concatenate([w.fvals, 0.5 * w.fvals], axis=-1)[..., :nd])
+    ref = O.oracle_boxfgt(w.tree, fv, nd, 1e-6, 1e-9, w.ipoly, w.iperiod, w.norder, ifpgh)
+    got = run_gpu(w.tree, fv, nd, 1e-6, 1e-9, w.ipoly, w.iperiod, w.norder, ifpgh)
+    assert got["plan"].info["n_near_pairs"] > 0
+    assert_close(compare(w.tree, got, ref, ifpgh))
+
+
 def test_fourier_mode_closed_form(device):
     """Heat semigroup on a Fourier mode: pot = pi delta exp(-delta|k|^2/4) f."""
     t = build_uniform(4)
