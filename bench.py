@@ -17,6 +17,7 @@ host cores, on the same tree and density.
 from __future__ import annotations
 
 import argparse
+import dataclasses
 import json
 import os
 import sys
@@ -149,6 +150,8 @@ def main():
     ap.add_argument("--precision", default="fp64", choices=["fp64", "split32"])
     ap.add_argument("--delta", type=float, default=None, help="override delta (config c3 only)")
     ap.add_argument("--eps", type=float, default=None, help="override eps (config c3 only)")
+    ap.add_argument("--densities", type=int, default=None,
+                    help="keep the first K densities of the workload (e.g. c3 with one density)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-split", action="store_true", help="skip timing the other precision")
     ap.add_argument("--cpu-threads", type=int, default=16)
@@ -173,6 +176,10 @@ def main():
         if args.eps is not None:
             kw["eps"] = args.eps
     w = make_workload(args.config, args.scale, **kw)
+    if args.densities is not None and args.densities < w.nd:
+        k = max(args.densities, 1)
+        w = dataclasses.replace(w, fvals=np.ascontiguousarray(w.fvals[..., :k]), nd=k,
+                                description=f"{w.description}; the first {k} of its {w.nd} densities")
     t_workload = time.time() - t0
     print(f"[bench] workload {args.config}/{args.scale}: {w.tree.nleaves} leaves, {w.npts} pts, built in "
           f"{t_workload:.1f}s", file=sys.stderr, flush=True)
