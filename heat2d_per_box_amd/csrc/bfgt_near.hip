@@ -226,6 +226,9 @@ constexpr int near_chan(int ntab) { return ntab == 1 ? 1 : (ntab == 2 ? 3 : 6); 
 constexpr int near_x(int o) { return o == 1 || o == 4 ? 1 : (o == 3 ? 2 : 0); }
 constexpr int near_y(int o) { return o == 2 || o == 4 ? 1 : (o == 5 ? 2 : 0); }
 constexpr int kNearWaves = 4;
+// The part of the x tables a wave of a split target handles: 0 = table 0,
+// 1 = tables 1 and 2, -1 = all (no split).
+constexpr bool near_part_has(int tw, int t) { return tw < 0 || (tw == 0 ? t == 0 : t >= 1); }
 
 // One wave per (target, density).  For each (source, offsets) pair:
 //   FF_t^T = F^T Tx_t^T           (x contraction, K = source x index)
@@ -251,89 +254,88 @@ __global__ __launch_bounds__(64 * kNearWaves) void k_near_mfma(
     const double* T = tnearT + (size_t)tlev[it] * 13 * 3 * 256;
     const int nn = n * n;
     const int cl = min(c, n - 1);
-    // TW: this wave's x tables, 0 = table 0, 1 = tables 1 and 2 (their x
-    // contractions and the outputs o whose near_x(o) is one of them), or -1
-    // for all
+    // TW: this wave's part of the x tables (near_part_has): their x
+    // contractions and the outputs o whose x table near_x(o) is one of them
     auto body = [&](auto twc) {
-    constexpr int TW = decltype(twc)::value;
-    constexpr auto mine_t = [](int t) { return TW < 0 || (TW == 0 ? t == 0 : t >= 1); };
-    constexpr auto mine = [](int o) {
-        const int t = near_x(o);
-        return TW < 0 || (TW == 0 ? t == 0 : t >= 1);
-    };
-    v4d out[NOUT];
+        constexpr int TW = decltype(twc)::value;
+        v4d out[NOUT];
 #pragma unroll
-    for (int o = 0; o < NOUT; o++) out[o] = v4d{0, 0, 0, 0};
-    // two pairs per iteration so that their loads overlap (a repeated last
-    // pair of an odd count contributes with weight zero)
-    for (int p = p0; p < p1; p += 2) {
-        const bool two = p + 1 < p1;
-        const NearPair pr[2] = {pairs[p], pairs[two ? p + 1 : p]};
-        v4d FF[2][NTAB];
+        for (int o = 0; o < NOUT; o++) out[o] = v4d{0, 0, 0, 0};
+        // two pairs per iteration so that their loads overlap (a repeated last
+        // pair of an odd count contributes with weight zero)
+        for (int p = p0; p < p1; p += 2) {
+            const bool two = p + 1 < p1;
+            const NearPair pr[2] = {pairs[p], pairs[two ? p + 1 : p]};
+            v4d FF[2][NTAB];
 #pragma unroll
-        for (int h = 0; h < 2; h++) {
-            const double* fs = fvals + (size_t)pr[h].src * nn * nd + d;
-            const double* Tx = T + (size_t)(pr[h].ix + 6) * 3 * 256;
-            const double wgt = (h == 0 || two) ? 1.0 : 0.0;
+            for (int h = 0; h < 2; h++) {
+                const double* fs = fvals + (size_t)pr[h].src * nn * nd + d;
+                const double* Tx = T + (size_t)(pr[h].ix + 6) * 3 * 256;
+                const double wgt = (h == 0 || two) ? 1.0 : 0.0;
 #pragma unroll
-            for (int t = 0; t < NTAB; t++) FF[h][t] = v4d{0, 0, 0, 0};
+                for (int t = 0; t < NTAB; t++) FF[h][t] = v4d{0, 0, 0, 0};
 #pragma unroll
-            for (int s = 0; s < NKS; s++) {
-                const int j1 = 4 * s + q;
-                // A = F^T: A[j2 = c][j1]; B = Tx_t^T: B[j1][k1 = c] = Tx_t[c][j1] (transposed table)
-                const double a = wgt * fs[(size_t)(min(j1, n - 1) + n * cl) * nd];
+                for (int s = 0; s < NKS; s++) {
+                    const int j1 = 4 * s + q;
+                    // A = F^T: A[j2 = c][j1]; B = Tx_t^T: B[j1][k1 = c] = Tx_t[c][j1] (transposed table)
+                    const double a = wgt * fs[(size_t)(min(j1, n - 1) + n * cl) * nd];
 #pragma unroll
-                for (int t = 0; t < NTAB; t++)
-                    if (mine_t(t)) FF[h][t] = mfma(a, Tx[t * 256 + j1 * 16 + c], FF[h][t]);
+                    for (int t = 0; t < NTAB; t++)
+                        if (near_part_has(TW, t)) FF[h][t] = mfma(a, Tx[t * 256 + j1 * 16 + c], FF[h][t]);
+                }
             }
-        }
-        // FF_t^T: element s of lane (q, c) = FF_t^T[j2 = 4s + q][k1 = c]
+            // FF_t^T: element s of lane (q, c) = FF_t^T[j2 = 4s + q][k1 = c]
 #pragma unroll
-        for (int h = 0; h < 2; h++) {
-            const double* Ty = T + (size_t)(pr[h].iy + 6) * 3 * 256;
+            for (int h = 0; h < 2; h++) {
+                const double* Ty = T + (size_t)(pr[h].iy + 6) * 3 * 256;
 #pragma unroll
-            for (int s = 0; s < NKS; s++) {
-                const int j2 = 4 * s + q;
-                double ty[NTAB];
+                for (int s = 0; s < NKS; s++) {
+                    const int j2 = 4 * s + q;
+                    double ty[NTAB];
+                    // A[k2 = c][j2] = Ty[c][j2]; x tables 1 and 2 meet only the y
+                    // tables u with 1 + u < NTAB
 #pragma unroll
-                for (int u = 0; u < NTAB; u++)
-                    if (TW <= 0 || u + 1 < NTAB) ty[u] = Ty[u * 256 + j2 * 16 + c];  // A[k2 = c][j2] = Ty[c][j2]
-#pragma unroll
-                for (int o = 0; o < NOUT; o++)
-                    if (mine(o)) out[o] = mfma(ty[near_y(o)], FF[h][near_x(o)][s], out[o]);
-            }
-        }
-    }
-    // element r of lane (q, c) = out[k1 = c][k2 = 4r + q].  The outputs are
-    // read at clamped points for all r first (one memory round trip), then
-    // added and stored where valid.
-    if (p1 > p0) {
-        double* dst[NOUT];
-        dst[0] = pot;
-        if constexpr (NTAB >= 2) dst[1] = dst[2] = grad;
-        if constexpr (NTAB >= 3) dst[3] = dst[4] = dst[5] = hess;
-        constexpr int stride_o[6] = {1, 2, 2, 3, 3, 3}, comp_o[6] = {0, 0, 1, 0, 1, 2};
-        double old[4][NOUT];
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-            const size_t pb = (size_t)b * nn + min(c, n - 1) + (size_t)n * min(4 * r + q, n - 1);
-#pragma unroll
-            for (int o = 0; o < NOUT; o++)
-                if (mine(o)) old[r][o] = dst[o][(pb * stride_o[o] + comp_o[o]) * nd + d];
-        }
-        if (c < n) {
-#pragma unroll
-            for (int r = 0; r < 4; r++) {
-                const int k2 = 4 * r + q;
-                if (k2 < n) {
-                    const size_t pb = (size_t)b * nn + c + (size_t)n * k2;
+                    for (int u = 0; u < NTAB; u++)
+                        if (TW <= 0 || u + 1 < NTAB) ty[u] = Ty[u * 256 + j2 * 16 + c];
 #pragma unroll
                     for (int o = 0; o < NOUT; o++)
-                        if (mine(o)) dst[o][(pb * stride_o[o] + comp_o[o]) * nd + d] = old[r][o] + oscale * out[o][r];
+                        if (near_part_has(TW, near_x(o)))
+                            out[o] = mfma(ty[near_y(o)], FF[h][near_x(o)][s], out[o]);
                 }
             }
         }
-    }
+        // element r of lane (q, c) = out[k1 = c][k2 = 4r + q].  The outputs are
+        // read at clamped points for all r first (one memory round trip), then
+        // added and stored where valid.
+        if (p1 > p0) {
+            double* dst[NOUT];
+            dst[0] = pot;
+            if constexpr (NTAB >= 2) dst[1] = dst[2] = grad;
+            if constexpr (NTAB >= 3) dst[3] = dst[4] = dst[5] = hess;
+            constexpr int stride_o[6] = {1, 2, 2, 3, 3, 3}, comp_o[6] = {0, 0, 1, 0, 1, 2};
+            double old[4][NOUT];
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const size_t pb = (size_t)b * nn + min(c, n - 1) + (size_t)n * min(4 * r + q, n - 1);
+#pragma unroll
+                for (int o = 0; o < NOUT; o++)
+                    if (near_part_has(TW, near_x(o)))
+                        old[r][o] = dst[o][(pb * stride_o[o] + comp_o[o]) * nd + d];
+            }
+            if (c < n) {
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const int k2 = 4 * r + q;
+                    if (k2 < n) {
+                        const size_t pb = (size_t)b * nn + c + (size_t)n * k2;
+#pragma unroll
+                        for (int o = 0; o < NOUT; o++)
+                            if (near_part_has(TW, near_x(o)))
+                                dst[o][(pb * stride_o[o] + comp_o[o]) * nd + d] = old[r][o] + oscale * out[o][r];
+                    }
+                }
+            }
+        }
     };
     if constexpr (NSPLIT == 1) {
         body(std::integral_constant<int, -1>{});
@@ -440,7 +442,8 @@ __global__ __launch_bounds__(64 * kNearWaves) void k_near_pack(
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 double* dst = o[r] <= 0 ? pot : (o[r] <= 2 ? grad : hess);
-                const int str = o[r] <= 0 ? 1 : (o[r] <= 2 ? 2 : 3), cmp = o[r] <= 0 ? 0 : (o[r] <= 2 ? o[r] - 1 : o[r] - 3);
+                const int str = o[r] <= 0 ? 1 : (o[r] <= 2 ? 2 : 3);
+                const int cmp = o[r] <= 0 ? 0 : (o[r] <= 2 ? o[r] - 1 : o[r] - 3);
                 const size_t pb = (size_t)b * nn + cl + (size_t)n * min(4 * (r % 2) + q, n - 1);
                 old[hh][r] = o[r] >= 0 ? dst[(pb * str + cmp) * nd + dd] : 0.0;
             }
@@ -451,7 +454,8 @@ __global__ __launch_bounds__(64 * kNearWaves) void k_near_pack(
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 double* dst = o[r] <= 0 ? pot : (o[r] <= 2 ? grad : hess);
-                const int str = o[r] <= 0 ? 1 : (o[r] <= 2 ? 2 : 3), cmp = o[r] <= 0 ? 0 : (o[r] <= 2 ? o[r] - 1 : o[r] - 3);
+                const int str = o[r] <= 0 ? 1 : (o[r] <= 2 ? 2 : 3);
+                const int cmp = o[r] <= 0 ? 0 : (o[r] <= 2 ? o[r] - 1 : o[r] - 3);
                 const int k2 = 4 * (r % 2) + q;
                 if (o[r] >= 0 && dd < nd && k2 < n && cc < n) {
                     const size_t pb = (size_t)b * nn + cc + (size_t)n * k2;
@@ -551,9 +555,9 @@ template <int NKS, int NTAB, int NSPLIT>
 int launch_near_mfma3(Plan& p, const double* fvals, double* pot, double* grad, double* hess) {
     int nitems = p.nnear_t * p.nd * NSPLIT;
     int blocks = (nitems + kNearWaves - 1) / kNearWaves;
-    k_near_mfma<NKS, NTAB, NSPLIT><<<blocks, 64 * kNearWaves, 0, p.stream>>>(p.n, p.nd, nitems, p.d_near_tgt, p.d_near_lev,
-                                                                   p.d_near_ptr, p.d_near, p.d_tnearT, fvals, pot,
-                                                                   grad, hess, p.out_scale);
+    k_near_mfma<NKS, NTAB, NSPLIT><<<blocks, 64 * kNearWaves, 0, p.stream>>>(
+        p.n, p.nd, nitems, p.d_near_tgt, p.d_near_lev, p.d_near_ptr, p.d_near, p.d_tnearT, fvals, pot, grad, hess,
+        p.out_scale);
     BFGT_HIP_CHECK(hipGetLastError());
     return BFGT_OK;
 }
