@@ -99,6 +99,19 @@ def test_full_size_c3_matches_oracle(device, delta):
     assert_close(compare(w.tree, got, ref, w.ifpgh))
 
 
+@pytest.mark.parametrize("delta", [1e-5, 1e-3])
+def test_full_size_c3_one_density(device, delta):
+    """The c3 tree with its first density only: one wave per leaf group in the
+    n <= 8 downward pass, and the near field's tables split over two waves
+    per target (x table 0 | the others)."""
+    w = make_workload("c3", "full", delta=delta, eps=1e-9)
+    fv = np.ascontiguousarray(w.fvals[..., :1])
+    ref = O.oracle_boxfgt(w.tree, fv, 1, w.delta, w.eps, w.ipoly, w.iperiod, w.norder, w.ifpgh)
+    got = run_gpu(w.tree, fv, 1, w.delta, w.eps, w.ipoly, w.iperiod, w.norder, w.ifpgh)
+    assert got["plan"].info["xc_down"] < got["plan"].info["n_pw_leaves"]
+    assert_close(compare(w.tree, got, ref, w.ifpgh))
+
+
 def test_full_size_c3_free_space_matches_oracle(device):
     """The c3 tree in free space (iperiod = 0) at delta = 1e-5: translation
     groups of four whose 4 x 4 source patches are cut at the domain edge
