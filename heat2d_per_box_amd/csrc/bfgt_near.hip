@@ -471,7 +471,7 @@ __global__ __launch_bounds__(64 * kNearWaves) void k_near_pack(
 //   W_v   = sum_{o: x(o) = v} G_o Ty_{y(o)}   (K = target y index)
 //   fbar += sum_v Tx_v^T W_v                (K = target x index)
 // with element s of W_v's accumulator the B operand of K-step s.
-template <int NKS, int NTAB>
+template <int NKS, int NTAB, int NPP>
 __global__ __launch_bounds__(64 * kNearWaves) void k_near_adj_mfma(
     int n, int nd, int nitems, const int* __restrict__ src, const int* __restrict__ ptr,
     const NearPairT* __restrict__ pairs, const double* __restrict__ tnear, const double* __restrict__ gpot,
@@ -487,46 +487,65 @@ __global__ __launch_bounds__(64 * kNearWaves) void k_near_adj_mfma(
     const int p0 = ptr[is], p1 = ptr[is + 1];
     const int nn = n * n;
     const int cl = min(c, n - 1);
-    v4d acc = {0, 0, 0, 0};
-    for (int p = p0; p < p1; p++) {
-        const NearPairT pr = pairs[p];
-        const double* T = tnear + (size_t)pr.lev * 13 * 3 * 256;
-        const double* Tx = T + (size_t)(pr.ix + 6) * 3 * 256;
-        const double* Ty = T + (size_t)(pr.iy + 6) * 3 * 256;
-        const size_t tb = (size_t)pr.tgt * nn;
-        v4d W[NTAB];
+    // NPP pairs per pass so that their loads overlap (a pass past the end of
+    // the list repeats its last pair with weight zero), one accumulator each
+    v4d acc2[NPP];
 #pragma unroll
-        for (int v = 0; v < NTAB; v++) W[v] = v4d{0, 0, 0, 0};
+    for (int h = 0; h < NPP; h++) acc2[h] = v4d{0, 0, 0, 0};
+    for (int p = p0; p < p1; p += NPP) {
+        NearPairT prs[NPP];
 #pragma unroll
-        for (int s = 0; s < NKS; s++) {
-            const int k2 = 4 * s + q;
-            // A = G_o: A[k1 = c][k2]; B = Ty_u: B[k2][j2 = c] = Ty_u[k2][c]
-            const size_t pb = tb + cl + (size_t)n * min(k2, n - 1);
-            double ty[NTAB];
+        for (int h = 0; h < NPP; h++) prs[h] = pairs[min(p + h, p1 - 1)];
+        v4d W[NPP][NTAB];
 #pragma unroll
-            for (int u = 0; u < NTAB; u++) ty[u] = Ty[u * 256 + k2 * 16 + c];
+        for (int h = 0; h < NPP; h++) {
+            const NearPairT pr = prs[h];
+            const double wgt = p + h < p1 ? 1.0 : 0.0;
+            const double* T = tnear + (size_t)pr.lev * 13 * 3 * 256;
+            const double* Ty = T + (size_t)(pr.iy + 6) * 3 * 256;
+            const size_t tb = (size_t)pr.tgt * nn;
 #pragma unroll
-            for (int o = 0; o < NOUT; o++) {
-                double g;
-                switch (o) {
-                    case 0: g = gpot[pb * nd + d]; break;
-                    case 1: g = ggrad[(pb * 2 + 0) * nd + d]; break;
-                    case 2: g = ggrad[(pb * 2 + 1) * nd + d]; break;
-                    case 3: g = ghess[(pb * 3 + 0) * nd + d]; break;
-                    case 4: g = ghess[(pb * 3 + 1) * nd + d]; break;
-                    default: g = ghess[(pb * 3 + 2) * nd + d]; break;
+            for (int v = 0; v < NTAB; v++) W[h][v] = v4d{0, 0, 0, 0};
+#pragma unroll
+            for (int s = 0; s < NKS; s++) {
+                const int k2 = 4 * s + q;
+                // A = G_o: A[k1 = c][k2]; B = Ty_u: B[k2][j2 = c] = Ty_u[k2][c]
+                const size_t pb = tb + cl + (size_t)n * min(k2, n - 1);
+                double ty[NTAB];
+#pragma unroll
+                for (int u = 0; u < NTAB; u++) ty[u] = Ty[u * 256 + k2 * 16 + c];
+#pragma unroll
+                for (int o = 0; o < NOUT; o++) {
+                    double g;
+                    switch (o) {
+                        case 0: g = gpot[pb * nd + d]; break;
+                        case 1: g = ggrad[(pb * 2 + 0) * nd + d]; break;
+                        case 2: g = ggrad[(pb * 2 + 1) * nd + d]; break;
+                        case 3: g = ghess[(pb * 3 + 0) * nd + d]; break;
+                        case 4: g = ghess[(pb * 3 + 1) * nd + d]; break;
+                        default: g = ghess[(pb * 3 + 2) * nd + d]; break;
+                    }
+                    W[h][near_x(o)] = mfma(wgt * g, ty[near_y(o)], W[h][near_x(o)]);
                 }
-                W[near_x(o)] = mfma(g, ty[near_y(o)], W[near_x(o)]);
             }
         }
         // W_v: element s of lane (q, c) = W_v[k1 = 4s + q][j2 = c]
 #pragma unroll
-        for (int s = 0; s < NKS; s++) {
-            const int k1 = 4 * s + q;
+        for (int h = 0; h < NPP; h++) {
+            const NearPairT pr = prs[h];
+            const double* Tx = tnear + (size_t)pr.lev * 13 * 3 * 256 + (size_t)(pr.ix + 6) * 3 * 256;
 #pragma unroll
-            for (int v = 0; v < NTAB; v++) acc = mfma(Tx[v * 256 + k1 * 16 + c], W[v][s], acc);  // A[j1 = c][k1]
+            for (int s = 0; s < NKS; s++) {
+                const int k1 = 4 * s + q;
+#pragma unroll
+                for (int v = 0; v < NTAB; v++)
+                    acc2[h] = mfma(Tx[v * 256 + k1 * 16 + c], W[h][v][s], acc2[h]);  // A[j1 = c][k1]
+            }
         }
     }
+    v4d acc = acc2[0];
+#pragma unroll
+    for (int h = 1; h < NPP; h++) acc += acc2[h];
     // element r of lane (q, c) = fbar[j1 = 4r + q][j2 = c]
     if (c < n && p1 > p0) {
 #pragma unroll
@@ -600,11 +619,21 @@ int launch_near_pack1(Plan& p, const double* fvals, double* pot, double* grad, d
     return launch_near_pack2<NKS, 3>(p, fvals, pot, grad, hess);
 }
 
+// One density gives one wave per source, too few to hide the loads of a
+// pair behind the products of another: two pairs per pass then (c3 tree with
+// one density 33.2 -> 28.8 us); with three densities one (53.0 -> 54.7 us
+// with two).
 template <int NKS, int NTAB>
 int launch_near_adj_mfma2(Plan& p, const double* gpot, const double* ggrad, const double* ghess, double* fbar) {
     int nitems = p.nnearT_s * p.nd;
     int blocks = (nitems + kNearWaves - 1) / kNearWaves;
-    k_near_adj_mfma<NKS, NTAB><<<blocks, 64 * kNearWaves, 0, p.stream>>>(
+    if (p.nd == 1) {
+        k_near_adj_mfma<NKS, NTAB, 2><<<blocks, 64 * kNearWaves, 0, p.stream>>>(
+            p.n, p.nd, nitems, p.d_nearT_src, p.d_nearT_ptr, p.d_nearT, p.d_tnear, gpot, ggrad, ghess, fbar);
+        BFGT_HIP_CHECK(hipGetLastError());
+        return BFGT_OK;
+    }
+    k_near_adj_mfma<NKS, NTAB, 1><<<blocks, 64 * kNearWaves, 0, p.stream>>>(
         p.n, p.nd, nitems, p.d_nearT_src, p.d_nearT_ptr, p.d_nearT, p.d_tnear, gpot, ggrad, ghess, fbar);
     BFGT_HIP_CHECK(hipGetLastError());
     return BFGT_OK;
