@@ -13,11 +13,15 @@ carry a delta cotangent (its delta derivative needs no third derivatives).
 
 Plans are resident per (delta, ifpgh) on one tree, because the solvers call
 the transform 2-3 times per time step on one tree with one delta
-(hpots/hpots2dadap.f:291-381).
+(hpots/hpots2dadap.f:291-381).  At most ``max_plans`` of them are kept (least
+recently used closed first), so that a loop over many deltas -- an
+optimisation in delta, finite-difference probes -- holds a bounded amount of
+device memory.
 """
 from __future__ import annotations
 
-from typing import Dict, Optional, Tuple
+from collections import OrderedDict
+from typing import Optional, Tuple
 
 import torch
 
@@ -34,21 +38,36 @@ class GaussTransform:
     """
 
     def __init__(self, tree, nd: int, eps: float, ipoly: int, norder: int, ifpgh: int = 1,
-                 iperiod: Optional[int] = None):
+                 iperiod: Optional[int] = None, max_plans: int = 4):
         if ifpgh not in (1, 2, 3):
             raise ValueError("ifpgh must be 1, 2 or 3")
+        if max_plans < 2:
+            raise ValueError("max_plans must be at least 2 (a forward and a backward plan of one delta)")
         self.tree, self.nd, self.eps, self.ipoly, self.norder, self.ifpgh = tree, nd, eps, ipoly, norder, ifpgh
         self.iperiod = tree.iperiod if iperiod is None else iperiod
-        self._plans: Dict[Tuple[float, int], BoxFGT] = {}
+        self.max_plans = int(max_plans)
+        self._plans: "OrderedDict[Tuple[float, int], BoxFGT]" = OrderedDict()
 
     def plan(self, delta: float, ifpgh: int) -> BoxFGT:
         key = (float(delta), int(ifpgh))
         p = self._plans.get(key)
-        if p is None:
-            p = BoxFGT(self.tree.copy(), self.nd, float(delta), self.eps, self.ipoly, self.norder, ifpgh,
-                       iperiod=self.iperiod)
-            self._plans[key] = p
+        if p is not None:
+            self._plans.move_to_end(key)
+            return p
+        while len(self._plans) >= self.max_plans:
+            _, old = self._plans.popitem(last=False)
+            # the plan's stream may still run work on device arrays the
+            # caller owns; closing frees only the plan's own buffers
+            old.stream.synchronize()
+            old.close()
+        p = BoxFGT(self.tree.copy(), self.nd, float(delta), self.eps, self.ipoly, self.norder, ifpgh,
+                   iperiod=self.iperiod)
+        self._plans[key] = p
         return p
+
+    @property
+    def n_plans(self) -> int:
+        return len(self._plans)
 
     def clear(self) -> None:
         for p in self._plans.values():

@@ -4,9 +4,9 @@
 scalars and numpy arrays, with the same meaning, in-place side effects
 (``itree``'s colleague lists are recomputed, ``pot/grad/hess`` are
 accumulated on leaf boxes, ``timeinfo`` is zeroed and filled) and the same
-restrictions the GPU library states (2-D, ``ifnewtree=0``, no extra
-targets), raising :class:`BoxFGTError` where the reference would silently
-misbehave.
+restrictions the GPU library states (2-D, ``ifnewtree=0``), extra
+targets included (``ifpghtarg``, ``targs``, ``pote/grade/hesse``), raising
+:class:`BoxFGTError` where the reference would silently misbehave.
 
 ``BoxFGT`` is the resident form: the plan (lists, FP64 tables, schedules)
 lives on the GPU and each call runs on device tensors already in HBM.
@@ -100,6 +100,26 @@ class PlanInfo(dict):
     pass
 
 
+class _StreamOrder:
+    def __init__(self, cur, plan_stream):
+        self.cur, self.ps = cur, plan_stream
+
+    def __enter__(self):
+        self.ps.wait_stream(self.cur)
+
+    def __exit__(self, *exc):
+        self.cur.wait_stream(self.ps)
+        return False
+
+
+class _NoOrder:
+    def __enter__(self):
+        return None
+
+    def __exit__(self, *exc):
+        return False
+
+
 class BoxFGT:
     """Resident box FGT plan on the current GPU.
 
@@ -145,6 +165,16 @@ class BoxFGT:
                           "xc_adj_transform": int(wk[4]), "merge_reads": int(wk[5]),
                           "merge_writes": int(wk[6])})
 
+    def _ordered(self):
+        """Order the plan's stream after the caller's current stream before a
+        launch, and the current stream after the plan's stream afterwards, so
+        that tensors produced and consumed on whatever stream is current at the
+        call see the library's work in order."""
+        import torch
+
+        cur = torch.cuda.current_stream()
+        return _StreamOrder(cur, self.stream) if cur != self.stream else _NoOrder()
+
     def _check_tensor(self, t, shape, name):
         import torch
 
@@ -165,10 +195,11 @@ class BoxFGT:
         h = self._check_tensor(hess, (nb, npb, 3, nd), "hess")
         ti = np.zeros(20) if timeinfo else None
         tip = _ptr(ti) if ti is not None else None
-        if scale == 1.0:
-            check(lib().bfgt_plan_execute(self._h, f, p, g, h, tip))
-        else:
-            check(lib().bfgt_plan_execute_scaled(self._h, float(scale), f, p, g, h, tip))
+        with self._ordered():
+            if scale == 1.0:
+                check(lib().bfgt_plan_execute(self._h, f, p, g, h, tip))
+            else:
+                check(lib().bfgt_plan_execute_scaled(self._h, float(scale), f, p, g, h, tip))
         return ti
 
     def leaf_points(self, xy=None):
@@ -178,7 +209,8 @@ class BoxFGT:
         if xy is None:
             xy = torch.zeros((self.nboxes, self.npbox, 2), dtype=torch.float64, device="cuda")
         x = self._check_tensor(xy, (self.nboxes, self.npbox, 2), "xy")
-        check(lib().bfgt_plan_leaf_points(self._h, x))
+        with self._ordered():
+            check(lib().bfgt_plan_leaf_points(self._h, x))
         return xy
 
     def eval_targets(self, targs, ifpghtarg: int, pot, pote, grad=None, hess=None, grade=None, hesse=None):
@@ -193,7 +225,8 @@ class BoxFGT:
         pe = self._check_tensor(pote, (nt, nd), "pote")
         ge = self._check_tensor(grade, (nt, 2, nd), "grade")
         he = self._check_tensor(hesse, (nt, 3, nd), "hesse")
-        check(lib().bfgt_plan_eval_targets(self._h, nt, tg, int(ifpghtarg), p, g, h, pe, ge, he))
+        with self._ordered():
+            check(lib().bfgt_plan_eval_targets(self._h, nt, tg, int(ifpghtarg), p, g, h, pe, ge, he))
 
     def adjoint(self, gpot, fbar, ggrad=None, ghess=None, timeinfo: bool = False):
         """Transposed operator: ``fbar += A^T (gpot, ggrad, ghess)`` on device
@@ -204,7 +237,8 @@ class BoxFGT:
         gg = self._check_tensor(ggrad, (nb, npb, 2, nd), "ggrad")
         gh = self._check_tensor(ghess, (nb, npb, 3, nd), "ghess")
         ti = np.zeros(20) if timeinfo else None
-        check(lib().bfgt_plan_adjoint(self._h, gp, gg, gh, fb, _ptr(ti) if ti is not None else None))
+        with self._ordered():
+            check(lib().bfgt_plan_adjoint(self._h, gp, gg, gh, fb, _ptr(ti) if ti is not None else None))
         return ti
 
     def profile_begin(self, max_steps: int, stage: int = -1) -> None:

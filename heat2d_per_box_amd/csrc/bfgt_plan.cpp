@@ -41,7 +41,6 @@ size_t node_buffer_doubles(int n);
 size_t up_smem_bytes(int n, int npw);
 size_t down_smem_bytes(int n, int npw, int ntab);
 size_t near_smem_bytes(int n, int ntab);
-int up_max_out();
 size_t loc_buffer_cplx(const Plan& p);
 int build_pw_nodes(Plan& p);
 
@@ -447,8 +446,7 @@ int plan_create(Plan& p, int nd, int ndim, double delta, double eps, int ipoly, 
     hipDeviceProp_t prop;
     BFGT_HIP_CHECK(hipGetDeviceProperties(&prop, p.device));
     size_t lds_max = prop.sharedMemPerBlock;
-    if (p.has_pw && (up_smem_bytes(p.n, npw) > lds_max || down_smem_bytes(p.n, npw, ifpgh) > lds_max ||
-                     p.nexp > up_max_out()))
+    if (p.has_pw && (up_smem_bytes(p.n, npw) > lds_max || down_smem_bytes(p.n, npw, ifpgh) > lds_max))
         return set_error(BFGT_ERR_RESOURCE, "plane-wave expansion too large for the leaf kernels");
     if (near_smem_bytes(p.n, ifpgh) > lds_max || 9 * p.n * p.n * sizeof(double) > lds_max)
         return set_error(BFGT_ERR_RESOURCE, "leaf order too large for the near-field kernels");
@@ -981,6 +979,9 @@ int plan_create(Plan& p, int nd, int ndim, double delta, double eps, int ipoly, 
     if ((rc = upload(p, p.d_nearT_ptr, nsptr))) return rc;
     if ((rc = upload(p, p.d_nearT, pairsT))) return rc;
     p.has_adjoint = !p.has_pw || use_mfma(p);
+    // split-FP32 exists for the matrix-core leaf transforms only; a plan on the
+    // vector kernels runs (and reports) FP64
+    if (p.has_pw && !use_mfma(p)) p.precision = BFGT_PREC_FP64;
     if (p.has_pw) {
         size_t len = (size_t)nd * p.nexp;
         if ((rc = dalloc(p, p.d_mp, (size_t)p.nmpslot * len))) return rc;

@@ -1045,6 +1045,9 @@ __global__ __launch_bounds__(kVecThreads) void k_up_vec(int n, int npw, int nd, 
     cplx* spy = spx + npw;                           // H
     const UpChunk ch = chunks[blockIdx.x];
     const int tid = threadIdx.x;
+    // blockIdx.y selects the segment of kUpMaxE * kVecThreads expansion
+    // entries this block accumulates (long expansions take several)
+    const int e0 = blockIdx.y * kUpMaxE * kVecThreads;
     for (int d = 0; d < nd; d++) {
         cplx acc[kUpMaxE];
 #pragma unroll
@@ -1077,7 +1080,7 @@ __global__ __launch_bounds__(kVecThreads) void k_up_vec(int n, int npw, int nd, 
             __syncthreads();
 #pragma unroll
             for (int r = 0; r < kUpMaxE; r++) {
-                int e = tid + r * kVecThreads;
+                int e = e0 + tid + r * kVecThreads;
                 if (e < nexp) {
                     int k1 = e % npw, k2 = e / npw;
                     const cplx* fcol = sff + n * k2;
@@ -1090,7 +1093,7 @@ __global__ __launch_bounds__(kVecThreads) void k_up_vec(int n, int npw, int nd, 
         cplx* out = partial + ((size_t)ch.slot * nd + d) * nexp;
 #pragma unroll
         for (int r = 0; r < kUpMaxE; r++) {
-            int e = tid + r * kVecThreads;
+            int e = e0 + tid + r * kVecThreads;
             if (e < nexp) out[(size_t)(e % npw) * H + e / npw] = acc[r];
         }
     }
@@ -1437,14 +1440,13 @@ size_t down_smem_bytes(int n, int npw, int ntab) {
     return sizeof(cplx) * ((size_t)ntab * npw * n + (size_t)npw * kDownChunk + (size_t)ntab * kDownChunk * n + npw + H);
 }
 
-int up_max_out() { return 32 * kVecThreads; }
-
 template <int E>
 static int launch_up_vec(Plan& p, const double* fvals, size_t sh) {
     if (sh > 65536)
         BFGT_HIP_CHECK(hipFuncSetAttribute((const void*)k_up_vec<E>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
-    k_up_vec<E><<<p.nchunks, kVecThreads, sh, p.stream>>>(p.n, p.npw, p.nd, fvals, p.d_pwleaf, p.d_chunks, p.d_tpoly,
-                                                          p.d_ts, p.d_mp);
+    const int nseg = (p.nexp + E * kVecThreads - 1) / (E * kVecThreads);
+    k_up_vec<E><<<dim3(p.nchunks, nseg), kVecThreads, sh, p.stream>>>(p.n, p.npw, p.nd, fvals, p.d_pwleaf,
+                                                                      p.d_chunks, p.d_tpoly, p.d_ts, p.d_mp);
     BFGT_HIP_CHECK(hipGetLastError());
     return BFGT_OK;
 }
@@ -1571,9 +1573,7 @@ int launch_upward(Plan& p, const double* fvals) {
     if (e <= 2) return launch_up_vec<2>(p, fvals, sh);
     if (e <= 4) return launch_up_vec<4>(p, fvals, sh);
     if (e <= 8) return launch_up_vec<8>(p, fvals, sh);
-    if (e <= 16) return launch_up_vec<16>(p, fvals, sh);
-    if (e <= 32) return launch_up_vec<32>(p, fvals, sh);
-    return set_error(BFGT_ERR_RESOURCE, "plane-wave expansion too long for the leaf kernels");
+    return launch_up_vec<16>(p, fvals, sh);  // segments of 4096 entries
 }
 
 // the merge schedule of the upward pass that ran: per chunk, or per

@@ -54,7 +54,9 @@ extern "C" {
  * hesse(nd,3,ntarg), which are overwritten (bfgtmain STEP 11,
  * bfgt/boxfgt.f:1494-1563); channels above ifpgh are computed into scratch
  * arrays and never written to grad/hess (bfgt/boxfgt.f:207-277).
- * `precision` selects BFGT_PREC_FP64 or BFGT_PREC_SPLIT32.
+ * `precision` selects BFGT_PREC_FP64 or BFGT_PREC_SPLIT32.  Split-FP32 applies
+ * to the matrix-core leaf transforms (norder <= 16, npw <= 128); a plan
+ * outside those sizes runs FP64 and reports precision 0 (bfgt_plan_query).
  * The plan built for a call (schedules, tables, device buffers) is kept and
  * reused by later calls with the same tree arrays (laddr, levels, parents,
  * children, centres, box sizes), parameters and device -- the drivers call
@@ -120,7 +122,8 @@ int bfgt_plan_eval_targets(bfgt_plan_t plan, int ntarg, const double *d_targs, i
  * conjugated, transposed PW tables; the translation over the transposed
  * lists with conjugated shifts; near-field tables read transposed per
  * (target, source) pair.  Arithmetic is FP64 for every plan precision.
- * Needs norder <= 16 and npw <= 128 (BFGT_ERR_RESOURCE otherwise).
+ * A plan with PW leaves needs norder <= 16 and npw <= 128 (the matrix-core
+ * kernels; BFGT_ERR_RESOURCE otherwise); near-field-only plans have no limit.
  * timeinfo, when non-NULL, is filled like bfgt_plan_execute's, per
  * transposed stage. */
 int bfgt_plan_adjoint(bfgt_plan_t plan, const double *d_gpot, const double *d_ggrad, const double *d_ghess,

@@ -273,3 +273,57 @@ def test_heat_solve_gradients_fd(device):
     assert abs(fd_d.item() - delta.grad.item()) <= 1e-6 * abs(delta.grad.item())
     assert math.isfinite(loss.item())
     gt.clear()
+
+
+@pytest.mark.parametrize("n,iperiod", [(20, 1), (24, 0)])
+def test_dot_product_identity_vector_near_field(device, n, iperiod):
+    """Leaf orders above 16: a plan whose leaves are all near-field targets
+    (delta = 1e-6) has the adjoint of the vector near-field kernel (one block
+    per source, tables read transposed); nd = 2, pot + grad + hess."""
+    from tests.test_gpu_parity import _vector_tree
+
+    t, _ = _vector_tree(n, iperiod, 2)
+    err, info = _dot_identity(t, 2, 1e-6, 1e-9, 0, iperiod, n, 3, seed=n)
+    assert not info["has_pw"] and info["n_near_pairs"] > 0
+    assert err < 1e-13, (err, info)
+
+
+def test_adjoint_refused_on_vector_pw_kernels(device):
+    """A plan with PW leaves at leaf order 20 runs the vector kernels, which
+    have no transpose: the adjoint returns BFGT_ERR_RESOURCE and writes nothing."""
+    import torch
+    from heat2d_per_box_amd.boxfgt import BoxFGT, BoxFGTError
+    from tests.test_gpu_parity import _vector_tree
+
+    t, fv = _vector_tree(20, 1, 1)
+    plan = BoxFGT(t.copy(), 1, 5e-4, 1e-9, 0, 20, 1)
+    assert plan.info["has_pw"] and not plan.info["mfma"]
+    g = torch.ones((t.centers.shape[0], 400, 1), dtype=torch.float64, device="cuda")
+    fbar = torch.zeros_like(g)
+    with pytest.raises(BoxFGTError) as ei:
+        plan.adjoint(g, fbar)
+    assert ei.value.code == 12
+    assert float(fbar.abs().max()) == 0.0
+
+
+def test_autograd_plan_cache_is_bounded(device):
+    """A loop over many deltas keeps at most max_plans resident plans (least
+    recently used closed first); a delta whose plan was closed is rebuilt and
+    gives the same result bit for bit."""
+    import torch
+    from heat2d_per_box_amd.autograd import GaussTransform
+
+    w = make_workload("c1", "full")
+    gt = GaussTransform(w.tree, w.nd, w.eps, w.ipoly, w.norder, ifpgh=1, max_plans=3)
+    f = torch.from_numpy(w.fvals).cuda()
+    first = gt(f, 1e-3).cpu().numpy()
+    for k in range(6):
+        gt(f, 1e-3 * (1.1 + 0.1 * k))
+        assert gt.n_plans <= 3
+    again = gt(f, 1e-3).cpu().numpy()
+    assert np.array_equal(first, again)
+    ref = O.oracle_boxfgt(w.tree, w.fvals, w.nd, 1e-3, w.eps, w.ipoly, w.iperiod, w.norder, 1)
+    lv = w.tree.nchild == 0
+    assert np.linalg.norm((again - ref.pot)[lv]) / np.linalg.norm(ref.pot[lv]) < 1e-12
+    gt.clear()
+    assert gt.n_plans == 0

@@ -465,3 +465,68 @@ def test_host_entry_staged_copies(device):
     ref = O.oracle_boxfgt(w.tree, w.fvals, nd, w.delta, w.eps, w.ipoly, w.iperiod, w.norder, 2)
     assert leaf_rel_err(t1, p1, pot0 + ref.pot) < 1e-12
     assert leaf_rel_err(t1, g1, grad0 + ref.grad) < 1e-11
+
+
+@pytest.mark.parametrize("delta,eps", [(1e-1, 1e-6), (1e-1, 1e-9), (1e-5, 1e-6), (1e-3, 1e-6)])
+def test_full_size_c3_regimes(device, delta, eps):
+    """The remaining legs of the delta x eps sweep on the full 6,709-leaf c3
+    tree.  delta = 1e-1 is the Fourier-series regime at full size: one cutoff
+    box (the root) whose expansion is the sum of every upward chunk (hundreds
+    of chunk sums merged into one), no translation partners and no near
+    field; delta = 1e-5 and 1e-3 at eps = 1e-6 give the shorter expansions
+    (npw = 34, 28) with the quad passes, and at 1e-5 the grouped translation
+    and the packed near field."""
+    w = make_workload("c3", "full", delta=delta, eps=eps)
+    print(f"case c3/full delta={delta} eps={eps}: leaves={w.tree.nleaves}", flush=True)
+    ref = O.oracle_boxfgt(w.tree, w.fvals, w.nd, w.delta, w.eps, w.ipoly, w.iperiod, w.norder, w.ifpgh)
+    got = run_gpu(w.tree, w.fvals, w.nd, w.delta, w.eps, w.ipoly, w.iperiod, w.norder, w.ifpgh)
+    info = got["plan"].info
+    assert info["npwlevel"] == ref.info["npwlevel"] and info["npw"] == ref.info["npw"]
+    assert np.array_equal(got["itree"], ref.itree)
+    if delta == 1e-1:
+        assert info["n_ancestors"] == 1 and info["n_near_pairs"] == 0
+        assert info["n_pw_leaves"] == w.tree.nleaves and info["merge_reads"] > 100
+    else:
+        assert info["xc_down"] < info["n_pw_leaves"]
+        if delta == 1e-5:
+            assert info["n_near_pairs"] > 0 and info["translation_group"] == 4
+    assert_close(compare(w.tree, got, ref, w.ifpgh))
+
+
+def _vector_tree(n, iperiod, nd):
+    g = GaussianSum(np.array([[0.12, -0.2], [-0.3, 0.25]]), np.array([0.03, 0.07]), np.array([1.0, -0.6]))
+    fun = lambda p: np.stack([g(p), np.cos(3 * p[..., 1]) * g(p)][:nd], axis=-1)  # noqa: E731
+    t = build_adaptive(fun, nd, n, 0, 1e-8, 5, iperiod=iperiod, min_level=2, eta=0.0)
+    return t, t.sample(fun, n, 0, nd)
+
+
+@pytest.mark.parametrize("n", [20, 24])
+@pytest.mark.parametrize("iperiod", [0, 1])
+def test_vector_fallback_matches_oracle(device, n, iperiod):
+    """Leaf orders 17..32 run the vector-FMA kernels (upward, downward and
+    near field): nd = 2, pot + grad + hess, a tree with PW leaves and near
+    pairs (delta = 5e-4 puts the cutoff at level 3 of a level 2-4 tree)."""
+    t, fv = _vector_tree(n, iperiod, 2)
+    ref = O.oracle_boxfgt(t, fv, 2, 5e-4, 1e-9, 0, iperiod, n, 3)
+    got = run_gpu(t, fv, 2, 5e-4, 1e-9, 0, iperiod, n, 3)
+    info = got["plan"].info
+    print(f"n={n} iperiod={iperiod}: leaves={t.nleaves} npw={info['npw']} pw leaves={info['n_pw_leaves']} "
+          f"near pairs={info['n_near_pairs']}", flush=True)
+    assert not info["mfma"] and info["n_pw_leaves"] > 0 and info["n_near_pairs"] > 0
+    assert np.array_equal(got["itree"], ref.itree)
+    assert_close(compare(t, got, ref, 3))
+
+
+def test_npw_above_128_matches_oracle(device):
+    """An expansion longer than the matrix-core kernels take (npw = 142 > 128:
+    eps = 1e-20 in free space with the cutoff box just under twice the cutoff
+    distance) runs the vector-FMA kernels at n = 8."""
+    t = build_uniform(5, iperiod=0)
+    g = GaussianSum(np.array([[0.1, 0.05]]), np.array([0.08]), np.array([1.0]))
+    fv = t.sample(lambda p: g(p)[..., None], 8, 0, 1)
+    delta, eps = 9.399e-05, 1e-20
+    ref = O.oracle_boxfgt(t, fv, 1, delta, eps, 0, 0, 8, 2)
+    got = run_gpu(t, fv, 1, delta, eps, 0, 0, 8, 2)
+    info = got["plan"].info
+    assert info["npw"] == ref.info["npw"] and info["npw"] > 128 and not info["mfma"]
+    assert_close(compare(t, got, ref, 2))

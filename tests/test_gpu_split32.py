@@ -64,3 +64,48 @@ def test_split32_differs_from_fp64_only_in_pw_stages(device):
     b = run_gpu(t, fv, 1, 1e-6, 1e-9, 0, 1, n, 2, precision="split32")
     assert a["plan"].info["n_pw_leaves"] == 0
     assert np.array_equal(a["pot"], b["pot"]) and np.array_equal(a["grad"], b["grad"])
+
+
+FULL = [
+    ("c2", {}),
+    ("c3", {"delta": 1e-5, "eps": 1e-6}),
+    ("c3", {"delta": 1e-3, "eps": 1e-9}),
+    ("c3", {"delta": 1e-1, "eps": 1e-9}),
+    ("c5", {}),
+]
+
+
+@pytest.mark.parametrize("name,kw", FULL, ids=lambda v: str(v))
+def test_split32_full_size(device, name, kw):
+    """The split kernels at the workload sizes the bench runs: c2 (19,834
+    leaves, n = 16), the c3 sweep legs on the 6,709-leaf tree (n = 8, nd = 3,
+    pot + grad + hess; delta = 1e-1 is the Fourier regime with one cutoff box)
+    and c5 (104,317 leaves, 16-leaf chunks).  Same bars as above."""
+    w = make_workload(name, "full", **kw)
+    ref = O.oracle_boxfgt(w.tree, w.fvals, w.nd, w.delta, w.eps, w.ipoly, w.iperiod, w.norder, w.ifpgh)
+    got = run_gpu(w.tree, w.fvals, w.nd, w.delta, w.eps, w.ipoly, w.iperiod, w.norder, w.ifpgh,
+                  precision="split32")
+    assert got["plan"].info["precision"] == 1
+    errs = {"pot": leaf_rel_err(w.tree, got["pot"], ref.pot)}
+    if w.ifpgh >= 2:
+        errs["grad"] = leaf_rel_err(w.tree, got["grad"], ref.grad)
+    if w.ifpgh >= 3:
+        errs["hess"] = leaf_rel_err(w.tree, got["hess"], ref.hess)
+    print(f"split32 {name}/full/{kw}: leaves={w.tree.nleaves} npw={got['plan'].info['npw']} errors {errs}",
+          flush=True)
+    assert errs["pot"] < 2e-6, errs
+    for k in ("grad", "hess"):
+        if k in errs:
+            assert errs[k] < 1e-5, errs
+
+
+def test_split32_on_vector_kernels_runs_fp64(device):
+    """Leaf orders above 16 have no split kernels: the plan runs FP64, says so
+    in its precision field, and matches the FP64 plan bit for bit."""
+    from tests.test_gpu_parity import _vector_tree
+
+    t, fv = _vector_tree(20, 1, 2)
+    a = run_gpu(t, fv, 2, 5e-4, 1e-9, 0, 1, 20, 2, precision="fp64")
+    b = run_gpu(t, fv, 2, 5e-4, 1e-9, 0, 1, 20, 2, precision="split32")
+    assert b["plan"].info["precision"] == 0 and not b["plan"].info["mfma"]
+    assert np.array_equal(a["pot"], b["pot"]) and np.array_equal(a["grad"], b["grad"])
