@@ -34,6 +34,7 @@ EXPORTS = [
     "bfgt_plan_create",
     "bfgt_plan_execute",
     "bfgt_plan_execute_scaled",
+    "bfgt_plan_execute_noalloc",
     "bfgt_plan_leaf_points",
     "bfgt_plan_adjoint",
     "bfgt_plan_eval_targets",
@@ -86,6 +87,8 @@ def lib() -> ctypes.CDLL:
     L.bfgt_plan_eval_targets.argtypes = [_vp, _i, _vp, _i, _vp, _vp, _vp, _vp, _vp, _vp]
     L.bfgt_plan_adjoint.restype = _i
     L.bfgt_plan_adjoint.argtypes = [_vp, _vp, _vp, _vp, _vp, _dp]
+    L.bfgt_plan_execute_noalloc.restype = _i
+    L.bfgt_plan_execute_noalloc.argtypes = [_vp, _vp, _vp, _vp, _vp, _dp]
     L.bfgt_plan_execute_scaled.restype = _i
     L.bfgt_plan_execute_scaled.argtypes = [_vp, _d, _vp, _vp, _vp, _vp, _dp]
     L.bfgt_plan_leaf_points.restype = _i

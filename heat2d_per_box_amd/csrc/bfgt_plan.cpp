@@ -1278,6 +1278,88 @@ hipError_t stage_download(hipStream_t s, void* host, const void* dev, size_t n) 
 }
 }  // namespace
 
+// Static workspace of bfgt_plan_execute_noalloc: the cutoff-level mp slots
+// and chunk partial sums, then the local expansions, of the plan being
+// evaluated, in one file-scope device array instead of the plan's own
+// buffers.  Every GPU that runs the library holds its own copy until the
+// process ends; its device address is looked up once per device
+// (hipGetSymbolAddress) and that address, never the array's host-side name,
+// is what the kernels receive.  All calls on one GPU share the array, so
+// calls that overlap on one GPU (several host threads, several streams)
+// race; the contents left by the previous call are cleared where the
+// evaluation needs zeros.
+constexpr size_t kScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) double scratch[kScratchBytes / sizeof(double)];
+
+namespace {
+
+std::mutex g_scratch_mu;
+std::vector<void*> g_scratch_ptr;  // per device, null until looked up
+
+int scratch_address(int dev, void** out) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if ((int)g_scratch_ptr.size() <= dev) g_scratch_ptr.resize(dev + 1, nullptr);
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return set_error(BFGT_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[dev] = ptr;
+    }
+    *out = g_scratch_ptr[dev];
+    return BFGT_OK;
+}
+
+// zero fill by a kernel (not a memset node), so that a caller may capture it
+__global__ void k_zero_cplx(cplx* __restrict__ a, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        a[i] = cplx{0.0, 0.0};
+}
+
+int zero_cplx(hipStream_t s, cplx* a, size_t n) {
+    if (n == 0) return BFGT_OK;
+    const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 4096);
+    k_zero_cplx<<<blocks, 256, 0, s>>>(a, n);
+    BFGT_HIP_CHECK(hipGetLastError());
+    return BFGT_OK;
+}
+
+}  // namespace
+
+// plan_execute with the plan's workspace (d_mp, d_loc) replaced, for this
+// call only, by the static workspace
+int plan_execute_noalloc(Plan& p, const double* fvals, double* pot, double* grad, double* hess, double* timeinfo) {
+    if (!fvals || !pot || (p.ifpgh >= 2 && !grad) || (p.ifpgh >= 3 && !hess))
+        return set_error(BFGT_ERR_ARG, "null device array");
+    if (!p.has_pw) return plan_execute(p, fvals, pot, grad, hess, timeinfo, 1.0);  // no workspace used
+    const size_t mp_cplx = (size_t)p.nmpslot * p.nd * p.nexp;
+    const size_t loc_cplx = (size_t)p.nanc * std::max(loc_buffer_cplx(p), 2 * fold_len(p));
+    const size_t mp_bytes = (mp_cplx * sizeof(cplx) + 255) / 256 * 256;
+    const size_t need = mp_bytes + loc_cplx * sizeof(cplx);
+    if (need > kScratchBytes)
+        return set_error(BFGT_ERR_RESOURCE, "the plan's workspace (" + std::to_string(need >> 20) +
+                                                " MiB) exceeds the static workspace of " +
+                                                std::to_string(kScratchBytes >> 20) + " MiB");
+    BFGT_HIP_CHECK(hipSetDevice(p.device));
+    void* base = nullptr;
+    int rc = scratch_address(p.device, &base);
+    if (rc) return rc;
+    cplx* mp = static_cast<cplx*>(base);
+    cplx* loc = reinterpret_cast<cplx*>(static_cast<char*>(base) + mp_bytes);
+    // the mp slots of cutoff boxes without PW leaves are read as zeros, and
+    // the padding of the folded local expansions is never written
+    if ((rc = zero_cplx(p.stream, mp, (size_t)p.nanc * p.nd * p.nexp))) return rc;
+    if ((rc = zero_cplx(p.stream, loc, loc_cplx))) return rc;
+    cplx* own_mp = p.d_mp;
+    cplx* own_loc = p.d_loc;
+    p.d_mp = mp;
+    p.d_loc = loc;
+    rc = plan_execute(p, fvals, pot, grad, hess, timeinfo, 1.0);
+    p.d_mp = own_mp;
+    p.d_loc = own_loc;
+    return rc;
+}
+
 }  // namespace bfgt
 
 using bfgt::set_error;
@@ -1318,6 +1400,12 @@ int bfgt_plan_execute(bfgt_plan_t plan, const double* d_fvals, double* d_pot, do
                       double* timeinfo) {
     if (!plan) return set_error(BFGT_ERR_ARG, "null plan");
     return bfgt::plan_execute(plan->p, d_fvals, d_pot, d_grad, d_hess, timeinfo, 1.0);
+}
+
+int bfgt_plan_execute_noalloc(bfgt_plan_t plan, const double* d_fvals, double* d_pot, double* d_grad,
+                              double* d_hess, double* timeinfo) {
+    if (!plan) return set_error(BFGT_ERR_ARG, "null plan");
+    return bfgt::plan_execute_noalloc(plan->p, d_fvals, d_pot, d_grad, d_hess, timeinfo);
 }
 
 int bfgt_plan_execute_scaled(bfgt_plan_t plan, double scale, const double* d_fvals, double* d_pot, double* d_grad,

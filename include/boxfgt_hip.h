@@ -97,6 +97,20 @@ int bfgt_plan_create(bfgt_plan_t *plan, int nd, int ndim, double delta, double e
 int bfgt_plan_execute(bfgt_plan_t plan, const double *d_fvals, double *d_pot, double *d_grad, double *d_hess,
                       double *timeinfo);
 
+/* bfgt_plan_execute with the evaluation's workspace (the cutoff-level mp
+ * expansions and chunk partial sums, and the local expansions) taken from a
+ * static device array of the library instead of the plan's own buffers, and
+ * no allocation of any kind.  No counterpart in the reference: it is for
+ * callers that cannot allocate (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  The array holds 64 MiB per GPU,
+ * kept until the process ends; a plan whose workspace is larger is refused
+ * with BFGT_ERR_RESOURCE before any device work.  All calls on one GPU share
+ * the array, so calls that overlap on one GPU (several host threads or
+ * streams) race; each call clears what it needs zeroed.  Same arguments,
+ * outputs and stream as bfgt_plan_execute, bit for bit the same results. */
+int bfgt_plan_execute_noalloc(bfgt_plan_t plan, const double *d_fvals, double *d_pot, double *d_grad,
+                              double *d_hess, double *timeinfo);
+
 /* Extra targets on device arrays: interpolate the leaf arrays pot (and grad,
  * hess for ifpghtarg >= 2, 3) -- normally the outputs of bfgt_plan_execute --
  * at d_targs(2,ntarg) into d_pote(nd,ntarg), d_grade(nd,2,ntarg),
