@@ -412,6 +412,30 @@ struct Fold {
 
 __device__ __host__ inline int round_up(int a, int b) { return (a + b - 1) / b * b; }
 
+// Buffer accesses: a wave-uniform base in a buffer resource (scalar
+// registers), a 32-bit lane offset in one vector register and a uniform byte
+// offset in a scalar register.  The per-step and per-table parts of an
+// address then cost no vector registers, and the compiler cannot fold them
+// into 64-bit per-lane addresses that stay live across a loop.  Every base
+// here is the start of one table, phase level, folded expansion or leaf, so
+// the offsets stay far below the 4 GB range of the resource.
+// The resource is raw (stride 0), 4 GB - 1 bytes long, with the 32-bit
+// data format in its last word (the format untyped loads and stores expect).
+typedef __amdgpu_buffer_rsrc_t rsrc_t;
+__device__ inline rsrc_t rsrc(const void* base) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, -1, 0x00020000);
+}
+__device__ inline cplx bload_c(rsrc_t r, unsigned voff, int soff) {
+    return __builtin_bit_cast(cplx, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0));
+}
+__device__ inline double bload_d(rsrc_t r, unsigned voff, int soff) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+__device__ inline void bstore_d(double v, rsrc_t r, unsigned voff, int soff) {
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(__attribute__((ext_vector_type(2))) unsigned, v), r,
+                                          voff, soff, 0);
+}
+
 // Downward pass.  y frequencies are processed in tiles of 16:
 //   FF_t^T = S^T a_t + i D^T b_t      a_t + i b_t = E_t diag(px), the x
 //                                     contraction folded with
@@ -431,9 +455,12 @@ __device__ __host__ inline int round_up(int a, int b) { return (a + b - 1) / b *
 // and then runs the y contraction and the output update for the group's
 // leaves, LG at a time (the y-table loads of a step serve all LG leaves; a
 // missing leaf is a zero-phase copy that is not stored).  groups[i] =
-// (first leaf, count) over the leaf array sorted by (box, level, jx, jy).  Per leaf the operands and products are those of the one-leaf
-// formulation: FF_t^T = S^T a_t + i D^T b_t, out_o^T += Re[(E_y diag(py w))^T
-// FF_x(o)^T].
+// (first leaf, count) over the leaf array sorted by (box, level, jx, jy).
+// Per leaf the operands and products are those of the one-leaf formulation:
+// FF_t^T = S^T a_t + i D^T b_t, out_o^T += Re[(E_y diag(py w))^T FF_x(o)^T].
+// Both contractions load the operands of the next K-step before issuing the
+// products of the current one, and all addresses are buffer offsets, so that
+// the two-table kernel fits three waves per SIMD (165 registers).
 template <int NTAB, int NTP, int LG>
 __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_down_col(
     int n, int P, int ps, int nd, int ngroups, const int2* __restrict__ groups, const PwLeaf* __restrict__ leaves,
@@ -449,122 +476,166 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_down_col(
     const int H = (P + 1) / 2, P2 = P / 2, npbox = n * n;
     const int Mh4 = round_up(H, 4), H16 = round_up(H, 16);
     const int nks1 = Mh4 / 4, nT = H16 / 16;
-    const int tstride = 16 * ps;
+    const int tbytes = 16 * ps * (int)sizeof(cplx);  // one table of a level
     const PwLeaf l0 = leaves[grp.x];
-    const cplx* E = tab + (size_t)l0.level * lstride;  // [t][m][i]
-    const cplx* ph = phase + levbase[l0.level];
-    const cplx* phx = ph + l0.jx * ps;
+    const rsrc_t rE = rsrc(tab + (size_t)l0.level * lstride);  // [t][m][i]
+    const rsrc_t rP = rsrc(phase + levbase[l0.level]);         // [cell][k]
+    const int sPx = l0.jx * ps * (int)sizeof(cplx);
     const double w2 = 2.0 * oscale;
+    // lane offsets (bytes): table row q, point cc; phase row q; folded
+    // expansion row q, column cc of a y tile
+    const unsigned oE = (unsigned)(q * 16 + cc) * sizeof(cplx);
+    const unsigned oP = (unsigned)q * sizeof(cplx);
+    const unsigned oF = (unsigned)(q * H16 + cc) * sizeof(Fold);
     for (int d = 0; d < nd; d++) {
-        const Fold* F = fold + ((size_t)l0.anc * nd + d) * Mh4 * H16;
+        const rsrc_t rF = rsrc(fold + ((size_t)l0.anc * nd + d) * Mh4 * H16);
         for (int tg = 0; tg < nT; tg += NTP) {
             v4d FFr[NTP][NTAB], FFi[NTP][NTAB];
 #pragma unroll
             for (int p = 0; p < NTP; p++)
 #pragma unroll
                 for (int t = 0; t < NTAB; t++) FFr[p][t] = FFi[p][t] = v4d{0, 0, 0, 0};
-            // two K-steps per pass (a step past the end reads the last rows
-            // with weight zero)
-            for (int s0 = 0; s0 < nks1; s0 += 2) {
+            // x contraction, one K-step (rows 4s..4s+3 of the folded
+            // expansion) per iteration; the operands of step s + 1 are loaded
+            // before the products of step s are issued (the scheduling
+            // barrier keeps that order), so that their latency overlaps them
+            struct XOp {
+                cplx S[NTP], D[NTP];
+                cplx e[NTAB];
+                cplx px;
+            };
+            auto xload = [&](int s) {
+                XOp o;
+                const int sF = (4 * s * H16 + 16 * tg) * (int)sizeof(Fold);
 #pragma unroll
-              for (int s1 = 0; s1 < 2; s1++) {
-                const int s = min(s0 + s1, nks1 - 1);
-                const double ws = s0 + s1 < nks1 ? 1.0 : 0.0;
-                const int m = 4 * s + q;
-                Fold sd[NTP];
+                for (int p = 0; p < NTP; p++) {
+                    o.S[p] = bload_c(rF, oF, sF + 16 * p * (int)sizeof(Fold));
+                    o.D[p] = bload_c(rF, oF, sF + 16 * p * (int)sizeof(Fold) + (int)sizeof(cplx));
+                }
 #pragma unroll
-                for (int p = 0; p < NTP; p++) sd[p] = F[m * H16 + 16 * (tg + p) + cc];
-                cplx px = phx[m];
-                px.x *= ws;
-                px.y *= ws;
+                for (int t = 0; t < NTAB; t++) o.e[t] = bload_c(rE, oE, t * tbytes + s * 64 * (int)sizeof(cplx));
+                o.px = bload_c(rP, oP, sPx + 4 * s * (int)sizeof(cplx));
+                return o;
+            };
+            XOp xc = xload(0);
+            for (int s = 0; s < nks1; s++) {
+                const XOp xn = xload(min(s + 1, nks1 - 1));
+                __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
                 for (int t = 0; t < NTAB; t++) {
-                    cplx e = cmul(E[t * tstride + m * 16 + cc], px);
+                    const cplx e = cmul(xc.e[t], xc.px);
 #pragma unroll
                     for (int p = 0; p < NTP; p++) {
-                        FFr[p][t] = mfma(sd[p].S.x, e.x, FFr[p][t]);
-                        FFi[p][t] = mfma(sd[p].S.y, e.x, FFi[p][t]);
-                        FFr[p][t] = mfma(-sd[p].D.y, e.y, FFr[p][t]);
-                        FFi[p][t] = mfma(sd[p].D.x, e.y, FFi[p][t]);
+                        FFr[p][t] = mfma(xc.S[p].x, e.x, FFr[p][t]);
+                        FFi[p][t] = mfma(xc.S[p].y, e.x, FFi[p][t]);
+                        FFr[p][t] = mfma(-xc.D[p].y, e.y, FFr[p][t]);
+                        FFi[p][t] = mfma(xc.D[p].x, e.y, FFi[p][t]);
                     }
                 }
-              }
+                xc = xn;
             }
+            // y contraction: K-step k covers rows 16 tg + 4k + q of FF^T,
+            // element k % 4 of FF[k / 4]; steps whose rows are all padding
+            // are skipped (uniform).  Operands of step k + 1 (clamped to a
+            // valid step) are loaded before the products of step k.
+            const int nyk = min(4 * NTP, (H - 16 * tg + 3) / 4);
             for (int b = 0; b < grp.y; b += LG) {
-            const cplx* phy[LG];
-            int box[LG];
-            double wg[LG];
-#pragma unroll
-            for (int g = 0; g < LG; g++) {
-                const PwLeaf lf = leaves[grp.x + min(b + g, grp.y - 1)];
-                phy[g] = ph + lf.jy * ps;
-                box[g] = lf.box;
-                wg[g] = b + g < grp.y ? 1.0 : 0.0;
-            }
-            v4d out[LG][NOUT];
-#pragma unroll
-            for (int g = 0; g < LG; g++)
-#pragma unroll
-                for (int o = 0; o < NOUT; o++) out[g][o] = v4d{0, 0, 0, 0};
-            // element s of FF[p] holds FF^T[16 (tg+p) + 4s + q][cc]: the B operand of K-step s
-#pragma unroll
-            for (int p = 0; p < NTP; p++) {
-#pragma unroll
-                for (int s = 0; s < 4; s++) {
-                    const int m2s = 16 * (tg + p) + 4 * s + q;
-                    if (16 * (tg + p) + 4 * s >= H) break;  // rows of this step are all padding (uniform)
-                    const double w = (ymid_half && m2s >= P2) ? 0.5 : 1.0;
-                    cplx ey[NTAB];
-#pragma unroll
-                    for (int u = 0; u < NTAB; u++) ey[u] = E[u * tstride + m2s * 16 + cc];
-#pragma unroll
-                    for (int g = 0; g < LG; g++) {
-                        cplx pyw = phy[g][m2s];
-                        pyw.x *= w * wg[g];
-                        pyw.y *= w * wg[g];
-                        cplx eu[NTAB];
-#pragma unroll
-                        for (int u = 0; u < NTAB; u++) eu[u] = cmul(ey[u], pyw);
-#pragma unroll
-                        for (int o = 0; o < NOUT; o++)
-                            out[g][o] = mfma(eu[chan_y(o)].x, FFr[p][chan_x(o)][s], out[g][o]);
-#pragma unroll
-                        for (int o = 0; o < NOUT; o++)
-                            out[g][o] = mfma(-eu[chan_y(o)].y, FFi[p][chan_x(o)][s], out[g][o]);
-                    }
-                }
-            }
-            // out^T holds [k2 = drow(q, r)][k1 = cc].  The outputs are read for
-            // all (leaf, r) at clamped points first, so that one memory round
-            // trip serves the whole batch, then added and stored where valid.
-            double* dst[NOUT];
-            dst[0] = pot;
-            if constexpr (NTAB >= 2) dst[1] = dst[2] = grad;
-            if constexpr (NTAB >= 3) dst[3] = dst[4] = dst[5] = hess;
-            constexpr int stride_o[6] = {1, 2, 2, 3, 3, 3}, comp_o[6] = {0, 0, 1, 0, 1, 2};
-            double old[LG][4][NOUT];
-#pragma unroll
-            for (int g = 0; g < LG; g++)
-#pragma unroll
-                for (int r = 0; r < 4; r++) {
-                    const size_t pb = (size_t)box[g] * npbox + min(cc, n - 1) + (size_t)n * min(drow(q, r), n - 1);
-#pragma unroll
-                    for (int o = 0; o < NOUT; o++) old[g][r][o] = dst[o][(pb * stride_o[o] + comp_o[o]) * nd + d];
-                }
-            if (cc < n) {
+                // the table rows are read again by every pass (from L2): kept
+                // live across the passes they would hold 8 NTAB NTP registers
+                asm volatile("" ::: "memory");
+                int sPy[LG], box[LG];
+                double wg[LG];
 #pragma unroll
                 for (int g = 0; g < LG; g++) {
-                    if (b + g >= grp.y) break;
+                    const PwLeaf lf = leaves[grp.x + min(b + g, grp.y - 1)];
+                    sPy[g] = (lf.jy * ps + 16 * tg) * (int)sizeof(cplx);
+                    box[g] = lf.box;
+                    wg[g] = b + g < grp.y ? 1.0 : 0.0;
+                }
+                struct YOp {
+                    cplx ey[NTAB];
+                    cplx py[LG];
+                };
+                auto yload = [&](int k) {
+                    YOp o;
+                    const int kk = min(k, nyk - 1);
 #pragma unroll
-                    for (int r = 0; r < 4; r++)
-                        if (drow(q, r) < n) {
-                            const size_t pb = (size_t)box[g] * npbox + cc + (size_t)n * drow(q, r);
+                    for (int u = 0; u < NTAB; u++)
+                        o.ey[u] = bload_c(rE, oE, u * tbytes + (16 * tg + 4 * kk) * 16 * (int)sizeof(cplx));
+#pragma unroll
+                    for (int g = 0; g < LG; g++) o.py[g] = bload_c(rP, oP, sPy[g] + 4 * kk * (int)sizeof(cplx));
+                    return o;
+                };
+                v4d out[LG][NOUT];
+#pragma unroll
+                for (int g = 0; g < LG; g++)
+#pragma unroll
+                    for (int o = 0; o < NOUT; o++) out[g][o] = v4d{0, 0, 0, 0};
+                YOp yc = yload(0);
+#pragma unroll
+                for (int k = 0; k < 4 * NTP; k++) {
+                    YOp yn = yc;
+                    if (k + 1 < 4 * NTP) yn = yload(k + 1);
+                    __builtin_amdgcn_sched_barrier(0);
+                    if (k < nyk) {
+                        const int p = k / 4, s = k % 4;
+                        const double w = (ymid_half && 16 * tg + 4 * k + q >= P2) ? 0.5 : 1.0;
+#pragma unroll
+                        for (int g = 0; g < LG; g++) {
+                            cplx pyw = yc.py[g];
+                            pyw.x *= w * wg[g];
+                            pyw.y *= w * wg[g];
+                            cplx eu[NTAB];
+#pragma unroll
+                            for (int u = 0; u < NTAB; u++) eu[u] = cmul(yc.ey[u], pyw);
 #pragma unroll
                             for (int o = 0; o < NOUT; o++)
-                                dst[o][(pb * stride_o[o] + comp_o[o]) * nd + d] = old[g][r][o] + w2 * out[g][o][r];
+                                out[g][o] = mfma(eu[chan_y(o)].x, FFr[p][chan_x(o)][s], out[g][o]);
+#pragma unroll
+                            for (int o = 0; o < NOUT; o++)
+                                out[g][o] = mfma(-eu[chan_y(o)].y, FFi[p][chan_x(o)][s], out[g][o]);
                         }
+                    }
+                    yc = yn;
                 }
-            }
+                // out^T holds [k2 = drow(q, r)][k1 = cc].  The outputs are
+                // read for all (leaf, r) at clamped points first, so that one
+                // memory round trip serves the whole batch, then added and
+                // stored where valid.  A buffer resource per (leaf, channel),
+                // the row in the lane offset.
+                constexpr int stride_o[6] = {1, 2, 2, 3, 3, 3}, comp_o[6] = {0, 0, 1, 0, 1, 2};
+                rsrc_t rO[LG][NOUT];
+#pragma unroll
+                for (int g = 0; g < LG; g++)
+#pragma unroll
+                    for (int o = 0; o < NOUT; o++) {
+                        double* a = o == 0 ? pot : (o < 3 ? grad : hess);
+                        rO[g][o] = rsrc(a + ((size_t)box[g] * npbox * stride_o[o] + comp_o[o]) * nd + d);
+                    }
+                unsigned orow[4];
+#pragma unroll
+                for (int r = 0; r < 4; r++)
+                    orow[r] = (unsigned)(min(cc, n - 1) + n * min(drow(q, r), n - 1)) * (unsigned)(nd * sizeof(double));
+                double old[LG][4][NOUT];
+#pragma unroll
+                for (int g = 0; g < LG; g++)
+#pragma unroll
+                    for (int r = 0; r < 4; r++)
+#pragma unroll
+                        for (int o = 0; o < NOUT; o++) old[g][r][o] = bload_d(rO[g][o], orow[r] * stride_o[o], 0);
+                if (cc < n) {
+#pragma unroll
+                    for (int g = 0; g < LG; g++) {
+                        if (b + g >= grp.y) break;
+#pragma unroll
+                        for (int r = 0; r < 4; r++)
+                            if (drow(q, r) < n) {
+#pragma unroll
+                                for (int o = 0; o < NOUT; o++)
+                                    bstore_d(old[g][r][o] + w2 * out[g][o][r], rO[g][o], orow[r] * stride_o[o], 0);
+                            }
+                    }
+                }
             }
         }
     }
@@ -1625,8 +1696,11 @@ template <int NTAB, int NTP>
 static int launch_down_col(Plan& p, const cplx* tab, int lstride, int ymid_half, double* pot, double* grad,
                            double* hess) {
     int blocks = (p.ncolgrp + kWavesPerBlock - 1) / kWavesPerBlock;
-    // leaves per y-contraction pass: as many as the registers hold
-    constexpr int LG = NTAB == 1 ? 4 : (NTAB == 2 ? 2 : 1);
+    // leaves per y-contraction pass: four for one table; one for two or three
+    // tables, which keeps the two-table kernel at three waves per SIMD (two
+    // leaves per pass need two y-operand sets and two output batches: two
+    // waves per SIMD, 10 % slower on the c2 tree)
+    constexpr int LG = NTAB == 1 ? 4 : 1;
     k_down_col<NTAB, NTP, LG><<<blocks, kWave * kWavesPerBlock, 0, p.stream>>>(
         p.n, p.npw, p.pstride, p.nd, p.ncolgrp, p.d_colgrp, p.d_pwleaf, tab, lstride, ymid_half, p.d_phase,
         p.d_levbase, reinterpret_cast<const Fold*>(p.d_loc), pot, grad, hess, p.out_scale);
