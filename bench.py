@@ -351,8 +351,11 @@ def main():
         adj = {"ms_per_call": adj_ms, "over_forward": adj_ms / ms,
                "note": "fbar += A^T (gpot, ggrad, ghess) on the same resident plan, FP64"}
 
-    # the host-array drop-in: a cold call (plan + H2D + kernels + D2H), then
-    # the same call again, which reuses the kept plan (H2D + kernels + D2H)
+    # the host-array drop-in: a cold call (plan + H2D + kernels + D2H, and the
+    # pinned staging buffers set up), then the same call again, which reuses
+    # the kept plan (H2D + kernels + D2H); then calls that build a new plan
+    # every time with the staging buffers in place (plan keeping switched
+    # off), as a driver that re-meshes every step makes them
     from heat2d_per_box_amd.boxfgt import boxfgt as boxfgt_host, release_plans
 
     def host_call():
@@ -360,14 +363,22 @@ def main():
         hp = np.zeros((nb, npb, nd))
         hg = np.zeros((nb, npb, 2, nd)) if w.ifpgh >= 2 else None
         hh = np.zeros((nb, npb, 3, nd)) if w.ifpgh >= 3 else None
+        ti = np.zeros(20)
         tc0 = time.perf_counter()
         boxfgt_host(nd, 2, w.delta, w.eps, w.ipoly, w.iperiod, w.norder, npb, nb, t2.nlevels, t2.ltree, t2.itree,
-                    t2.iptr, t2.centers, t2.boxsize, w.fvals, w.ifpgh, hp, hg, hh, precision=args.precision)
-        return time.perf_counter() - tc0
+                    t2.iptr, t2.centers, t2.boxsize, w.fvals, w.ifpgh, hp, hg, hh, timeinfo=ti,
+                    precision=args.precision)
+        wall = time.perf_counter() - tc0
+        parts = {"plan_host_s": ti[13], "tables_s": ti[14], "copies_in_s": ti[15], "kernels_s": float(ti[1:7].sum()),
+                 "copies_out_s": ti[16]}
+        return wall, parts
 
     release_plans()
-    t_cold = host_call()
-    t_kept = min(host_call() for _ in range(3))
+    t_cold, cold_parts = host_call()
+    t_kept, kept_parts = min((host_call() for _ in range(3)), key=lambda r: r[0])
+    os.environ["BFGT_HOST_PLAN_CACHE"] = "0"
+    t_remesh, remesh_parts = min((host_call() for _ in range(3)), key=lambda r: r[0])
+    del os.environ["BFGT_HOST_PLAN_CACHE"]
     release_plans()
 
     out = {
@@ -402,6 +413,10 @@ def main():
         "cold_pts_per_s": npts / t_cold,
         "t_host_kept_plan_s": t_kept,
         "host_kept_plan_pts_per_s": npts / t_kept,
+        "t_host_new_plan_s": t_remesh,
+        "host_call_parts": {"cold": cold_parts, "kept_plan": kept_parts, "new_plan": remesh_parts,
+                            "note": "host-array calls (timeinfo slots 14-17 and 2-7): a new plan's input copies "
+                                    "run beside its build, so copies_in_s is the wait for them after it"},
         "t_plan_s": t_plan,
         "roofline": roof,
         "stage_roofline": stages_roof,

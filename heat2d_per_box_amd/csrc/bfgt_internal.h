@@ -229,8 +229,9 @@ __device__ inline int xcd_block(int b, int nb) {
 // status helpers
 int set_error(int code, const std::string& msg);
 
-// tables (bfgt_tables.hip)
+// tables (bfgt_tables.hip): launched asynchronously, then waited for
 int build_tables(Plan& p);
+int finish_tables(Plan& p);
 
 // hot path (bfgt_pw.hip, bfgt_near.hip)
 int launch_upward(Plan& p, const double* fvals);

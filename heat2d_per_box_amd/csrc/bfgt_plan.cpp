@@ -789,7 +789,10 @@ int plan_create(Plan& p, int nd, int ndim, double delta, double eps, int ipoly, 
     p.merge_reads_b = 0;
     for (const int4& r : mergeb) p.merge_reads_b += r.z;
 
-    // near-field schedule (bfgt/boxfgt.f:923-1071), pairs with empty tables dropped
+    // near-field schedule (bfgt/boxfgt.f:923-1071), pairs with empty tables
+    // dropped: needs the sparsity bounds of the tables, which were generated
+    // on the device while the PW schedule was built
+    if ((rc = finish_tables(p))) return rc;
     std::vector<int> ntgt, nlev, nptr;
     std::vector<NearPair> pairs;
     const int ni = 2 * (p.n + 1);
@@ -1196,17 +1199,24 @@ constexpr size_t kStageMin = size_t(4) << 20;  // smaller arrays are copied dire
 struct Stage {
     void* buf[2] = {nullptr, nullptr};
     hipEvent_t ev[2] = {nullptr, nullptr};
+    hipStream_t up = nullptr;   // the input copies of a call that builds a plan
+    hipEvent_t up_done = nullptr;
     int dev = -1;
 };
 Stage g_stage;
 
 void stage_free() {
+    if (g_stage.up) (void)hipStreamSynchronize(g_stage.up);
     for (int i = 0; i < 2; i++) {
         if (g_stage.buf[i]) (void)hipHostFree(g_stage.buf[i]);
         if (g_stage.ev[i]) (void)hipEventDestroy(g_stage.ev[i]);
         g_stage.buf[i] = nullptr;
         g_stage.ev[i] = nullptr;
     }
+    if (g_stage.up_done) (void)hipEventDestroy(g_stage.up_done);
+    if (g_stage.up) (void)hipStreamDestroy(g_stage.up);
+    g_stage.up_done = nullptr;
+    g_stage.up = nullptr;
     g_stage.dev = -1;
 }
 
@@ -1219,6 +1229,11 @@ bool stage_ready(int dev) {
             stage_free();
             return false;
         }
+    if (hipStreamCreateWithFlags(&g_stage.up, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&g_stage.up_done, hipEventDisableTiming) != hipSuccess) {
+        stage_free();
+        return false;
+    }
     g_stage.dev = dev;
     return true;
 }
@@ -1276,6 +1291,40 @@ hipError_t stage_download(hipStream_t s, void* host, const void* dev, size_t n) 
     }
     return e;
 }
+// The inputs of a host-array call on stream s: leaf values and the outputs
+// that are accumulated into (channels above ifpgh start from zero).
+hipError_t upload_inputs(hipStream_t s, size_t nv, int ifpgh, int ifpgh2, const double* fvals, const double* pot,
+                         const double* grad, const double* hess, double* dfv, double* dpot, double* dgrad,
+                         double* dhess) {
+    int chunk = 0;
+    hipError_t e = stage_upload(s, dfv, fvals, nv * sizeof(double), chunk);
+    if (e == hipSuccess) e = stage_upload(s, dpot, pot, nv * sizeof(double), chunk);
+    if (e == hipSuccess && ifpgh2 >= 2)
+        e = ifpgh >= 2 ? stage_upload(s, dgrad, grad, 2 * nv * sizeof(double), chunk)
+                       : hipMemsetAsync(dgrad, 0, 2 * nv * sizeof(double), s);
+    if (e == hipSuccess && ifpgh2 >= 3)
+        e = ifpgh >= 3 ? stage_upload(s, dhess, hess, 3 * nv * sizeof(double), chunk)
+                       : hipMemsetAsync(dhess, 0, 3 * nv * sizeof(double), s);
+    return e;
+}
+
+hipError_t alloc_io(HostPlan& h, size_t nv, int ifpgh2) {
+    hipError_t e = hipSuccess;
+    if (h.cap >= nv) return e;
+    (void)hipFree(h.dfv);
+    (void)hipFree(h.dpot);
+    (void)hipFree(h.dgrad);
+    (void)hipFree(h.dhess);
+    h.dfv = h.dpot = h.dgrad = h.dhess = nullptr;
+    h.cap = 0;
+    if ((e = hipMalloc(&h.dfv, nv * sizeof(double))) != hipSuccess) return e;
+    if ((e = hipMalloc(&h.dpot, nv * sizeof(double))) != hipSuccess) return e;
+    if (ifpgh2 >= 2 && (e = hipMalloc(&h.dgrad, 2 * nv * sizeof(double))) != hipSuccess) return e;
+    if (ifpgh2 >= 3 && (e = hipMalloc(&h.dhess, 3 * nv * sizeof(double))) != hipSuccess) return e;
+    h.cap = nv;
+    return e;
+}
+
 }  // namespace
 
 // Static workspace of bfgt_plan_execute_noalloc: the cutoff-level mp slots
@@ -1635,28 +1684,68 @@ static int hip_eval(int nd, int ndim, double delta, double eps, int ipoly, int i
         if (bfgt::same_key(*h, key)) hp = h.get();
     const int nused = itree[2 * nlevels + 1];
     const bool reused = hp != nullptr;
+    const size_t nv = (size_t)nboxes * npbox * nd;
+    double t_plan_host = 0, t_tables = 0;  // of a plan built by this call
+    // the pinned staging buffers, set up on the first call on a device
+    // (without them the copies go direct)
+    const bool staged = bfgt::stage_ready(key.dev);
     if (hp) {
         // the colleague lists plan creation wrote into itree
         std::copy(hp->ncoll.begin(), hp->ncoll.end(), itree + iptr[5] - 1);
         std::copy(hp->coll.begin(), hp->coll.end(), itree + iptr[6] - 1);
     } else {
+        // A new plan: the inputs are copied by a second host thread on the
+        // staging stream while this one builds the plan (host lists and
+        // schedules, device tables); the plan's stream waits for the copies.
+        auto io = std::make_unique<bfgt::HostPlan>(std::move(key));
+        hipError_t ea = bfgt::alloc_io(*io, nv, ifpgh2);
+        if (ea != hipSuccess) {
+            bfgt::free_host_plan(*io);
+            return set_error(BFGT_ERR_HIP, std::string("device arrays: ") + hipGetErrorString(ea));
+        }
+        hipError_t eu = hipSuccess;
+        std::thread copier;
+        auto copy_in = [&]() {
+            eu = bfgt::upload_inputs(bfgt::g_stage.up, nv, ifpgh, ifpgh2, fvals, pot, grad, hess, io->dfv, io->dpot,
+                                     io->dgrad, io->dhess);
+            if (eu == hipSuccess) eu = hipEventRecord(bfgt::g_stage.up_done, bfgt::g_stage.up);
+        };
+        if (staged) {
+            try {
+                copier = std::thread(copy_in);
+            } catch (...) {
+                copy_in();  // no thread to be had: copy first
+            }
+        }
         hipStream_t stream = nullptr;
-        if (hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess)
-            return set_error(BFGT_ERR_HIP, "hipStreamCreateWithFlags failed");
         bfgt_plan_t plan = nullptr;
-        int rc = bfgt_plan_create(&plan, nd, ndim, delta, eps, ipoly, iperiod, norder, nboxes, nlevels, ltree, itree,
+        int rc = BFGT_OK;
+        if (hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess)
+            rc = set_error(BFGT_ERR_HIP, "hipStreamCreateWithFlags failed");
+        else
+            rc = bfgt_plan_create(&plan, nd, ndim, delta, eps, ipoly, iperiod, norder, nboxes, nlevels, ltree, itree,
                                   iptr, centers, boxsize, ifpgh2, precision, stream);
+        if (copier.joinable()) copier.join();
+        if (rc == BFGT_OK && staged && eu != hipSuccess)
+            rc = set_error(BFGT_ERR_HIP, std::string("copies in: ") + hipGetErrorString(eu));
+        if (rc == BFGT_OK && staged && (eu = hipStreamWaitEvent(stream, bfgt::g_stage.up_done, 0)) != hipSuccess)
+            rc = set_error(BFGT_ERR_HIP, std::string("hipStreamWaitEvent: ") + hipGetErrorString(eu));
         if (rc) {
-            hipStreamDestroy(stream);
+            if (staged) (void)hipStreamSynchronize(bfgt::g_stage.up);
+            if (plan) bfgt_plan_destroy(plan);
+            else if (stream) (void)hipStreamDestroy(stream);
+            bfgt::free_host_plan(*io);
             return rc;
         }
         plan->p.own_stream = true;  // released with the plan
+        t_plan_host = plan->p.t_plan_host;
+        t_tables = plan->p.t_tables;
         if (bfgt::g_host_plans.size() >= bfgt::kHostPlans) {
             auto lru = std::min_element(bfgt::g_host_plans.begin(), bfgt::g_host_plans.end(),
                                         [](const auto& a, const auto& b) { return a->used < b->used; });
             bfgt::drop_host_plan(lru->get());
         }
-        bfgt::g_host_plans.push_back(std::make_unique<bfgt::HostPlan>(std::move(key)));
+        bfgt::g_host_plans.push_back(std::move(io));
         hp = bfgt::g_host_plans.back().get();
         hp->plan = plan;
         hp->ncoll.assign(itree + iptr[5] - 1, itree + iptr[5] - 1 + nused);
@@ -1664,7 +1753,6 @@ static int hip_eval(int nd, int ndim, double delta, double eps, int ipoly, int i
     }
     hp->used = ++bfgt::g_host_clock;
     bfgt::Plan& p = hp->plan->p;
-    const size_t nv = (size_t)nboxes * npbox * nd;
     const size_t nt = with_targets ? (size_t)ntarg : 0;
     double *dtg = nullptr, *dpe = nullptr, *dge = nullptr, *dhe = nullptr;
     // a failed call frees its target arrays and forgets the plan; a plan is
@@ -1682,39 +1770,13 @@ static int hip_eval(int nd, int ndim, double delta, double eps, int ipoly, int i
         hipError_t _e = (expr);                                                                      \
         if (_e != hipSuccess) return finish(set_error(BFGT_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e))); \
     } while (0)
-    if (hp->cap < nv) {
-        hipFree(hp->dfv);
-        hipFree(hp->dpot);
-        hipFree(hp->dgrad);
-        hipFree(hp->dhess);
-        hp->dfv = hp->dpot = hp->dgrad = hp->dhess = nullptr;
-        hp->cap = 0;
-        BFGT_EVAL_CHECK(hipMalloc(&hp->dfv, nv * sizeof(double)));
-        BFGT_EVAL_CHECK(hipMalloc(&hp->dpot, nv * sizeof(double)));
-        if (ifpgh2 >= 2) BFGT_EVAL_CHECK(hipMalloc(&hp->dgrad, 2 * nv * sizeof(double)));
-        if (ifpgh2 >= 3) BFGT_EVAL_CHECK(hipMalloc(&hp->dhess, 3 * nv * sizeof(double)));
-        hp->cap = nv;
-    }
+    BFGT_EVAL_CHECK(bfgt::alloc_io(*hp, nv, ifpgh2));
     double *dfv = hp->dfv, *dpot = hp->dpot, *dgrad = hp->dgrad, *dhess = hp->dhess;
-    // the pinned staging buffers are set up when a plan is first reused (a
-    // single call gains nothing from pinning them); without them, and for a
-    // first call before that, the copies go direct
-    if (reused) (void)bfgt::stage_ready(hp->dev);
-    int chunk = 0;
-    BFGT_EVAL_CHECK(bfgt::stage_upload(p.stream, dfv, fvals, nv * sizeof(double), chunk));
-    BFGT_EVAL_CHECK(bfgt::stage_upload(p.stream, dpot, pot, nv * sizeof(double), chunk));
-    if (ifpgh2 >= 2) {
-        if (ifpgh >= 2)
-            BFGT_EVAL_CHECK(bfgt::stage_upload(p.stream, dgrad, grad, 2 * nv * sizeof(double), chunk));
-        else
-            BFGT_EVAL_CHECK(hipMemsetAsync(dgrad, 0, 2 * nv * sizeof(double), p.stream));
-    }
-    if (ifpgh2 >= 3) {
-        if (ifpgh >= 3)
-            BFGT_EVAL_CHECK(bfgt::stage_upload(p.stream, dhess, hess, 3 * nv * sizeof(double), chunk));
-        else
-            BFGT_EVAL_CHECK(hipMemsetAsync(dhess, 0, 3 * nv * sizeof(double), p.stream));
-    }
+    // the outputs travel both ways: the accumulation (pot += ...) happens on
+    // the device.  A new plan's inputs went in while it was built.
+    if (reused || !staged)
+        BFGT_EVAL_CHECK(bfgt::upload_inputs(p.stream, nv, ifpgh, ifpgh2, fvals, pot, grad, hess, dfv, dpot, dgrad,
+                                            dhess));
     if (with_targets) {
         BFGT_EVAL_CHECK(hipMalloc(&dtg, 2 * nt * sizeof(double)));
         BFGT_EVAL_CHECK(hipMalloc(&dpe, nt * nd * sizeof(double)));
@@ -1722,6 +1784,8 @@ static int hip_eval(int nd, int ndim, double delta, double eps, int ipoly, int i
         if (ifpghtarg >= 3) BFGT_EVAL_CHECK(hipMalloc(&dhe, 3 * nt * nd * sizeof(double)));
         BFGT_EVAL_CHECK(hipMemcpyAsync(dtg, targs, 2 * nt * sizeof(double), hipMemcpyHostToDevice, p.stream));
     }
+    // the copies are timed on their own: wait for them before the kernels
+    if (timeinfo) BFGT_EVAL_CHECK(hipStreamSynchronize(p.stream));
     auto t1 = std::chrono::steady_clock::now();
     double ti[20];
     int rc = bfgt::plan_execute(p, dfv, dpot, dgrad, dhess, ti, 1.0);
@@ -1743,15 +1807,24 @@ static int hip_eval(int nd, int ndim, double delta, double eps, int ipoly, int i
         BFGT_EVAL_CHECK(hipEventElapsedTime(&ms, e0, e1));
         t_targets = ms * 1e-3;
     }
+    auto t2 = std::chrono::steady_clock::now();
     BFGT_EVAL_CHECK(bfgt::stage_download(p.stream, pot, dpot, nv * sizeof(double)));
     if (ifpgh >= 2) BFGT_EVAL_CHECK(bfgt::stage_download(p.stream, grad, dgrad, 2 * nv * sizeof(double)));
     if (ifpgh >= 3) BFGT_EVAL_CHECK(bfgt::stage_download(p.stream, hess, dhess, 3 * nv * sizeof(double)));
     BFGT_EVAL_CHECK(hipStreamSynchronize(p.stream));
 #undef BFGT_EVAL_CHECK
     if (timeinfo) {
+        auto t3 = std::chrono::steady_clock::now();
         for (int i = 0; i < 20; i++) timeinfo[i] = ti[i];
         timeinfo[0] = std::chrono::duration<double>(t1 - t0).count();
         timeinfo[11] = t_targets;  // extra targets (slot 12 of the reference)
+        // slots the reference leaves zero: the set-up split into its parts,
+        // and the copies back
+        timeinfo[13] = t_plan_host;  // host lists and schedules of a new plan
+        timeinfo[14] = t_tables;     // its tables (device)
+        timeinfo[15] = std::chrono::duration<double>(t1 - t0).count() - t_plan_host - t_tables;  // copies in, checks
+        timeinfo[16] = std::chrono::duration<double>(t3 - t2).count();  // copies out
+        timeinfo[17] = reused ? 1.0 : 0.0;  // the call ran on a kept plan
     }
     return finish(BFGT_OK);
 }

@@ -582,9 +582,8 @@ __global__ void k_m2l_shifts(int npw, double xmin, const double* ts, cplx* w) {
 int build_phase_tables(Plan& p);
 
 int build_tables(Plan& p) {
-    hipEvent_t e0, e1;
-    BFGT_HIP_CHECK(hipEventCreate(&e0));
-    BFGT_HIP_CHECK(hipEventCreate(&e1));
+    hipEvent_t& e0 = p.ev[8];
+    hipEvent_t& e1 = p.ev[9];
     BFGT_HIP_CHECK(hipEventRecord(e0, p.stream));
     const int n = p.n;
     k_node_rules<<<1, 64, 0, p.stream>>>(n, p.ipoly, p.d_nodes);
@@ -614,12 +613,16 @@ int build_tables(Plan& p) {
     BFGT_HIP_CHECK(hipMemcpyAsync(p.h_indloc.data(), p.d_indloc, nind * sizeof(int), hipMemcpyDeviceToHost,
                                   p.stream));
     BFGT_HIP_CHECK(hipEventRecord(e1, p.stream));
-    BFGT_HIP_CHECK(hipStreamSynchronize(p.stream));
+    return BFGT_OK;
+}
+
+// waits for the tables launched by build_tables (the host copy of the
+// sparsity bounds included) and records their device time
+int finish_tables(Plan& p) {
+    BFGT_HIP_CHECK(hipEventSynchronize(p.ev[9]));
     float ms = 0;
-    BFGT_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+    BFGT_HIP_CHECK(hipEventElapsedTime(&ms, p.ev[8], p.ev[9]));
     p.t_tables = ms * 1e-3;
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
     return BFGT_OK;
 }
 

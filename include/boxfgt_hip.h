@@ -48,7 +48,10 @@ extern "C" {
  * Side effects as in the reference: the colleague lists in itree are
  * recomputed (bfgt/boxfgt.f:521-528); timeinfo(1..20) is zeroed and filled
  * (slot 1 set-up, 2 leaf->PW, 3 merge, 4 translation, 5 split (fused into 6),
- * 6 PW->pot/grad/hess, 7 near field, 12 extra targets).  ifnewtree must be
+ * 6 PW->pot/grad/hess, 7 near field, 12 extra targets; in slots the reference
+ * leaves zero: 14 the host part of a new plan (lists, schedules), 15 its
+ * table generation, 16 the rest of slot 1 (copies in, checks), 17 the copies
+ * out, 18 = 1 if the call reused a kept plan).  ifnewtree must be
  * 0.  With ifpghtarg = 1..3 and ntarg > 0 the leaf pot (and grad, hess) are
  * interpolated at targs(2,ntarg) into pote(nd,ntarg), grade(nd,2,ntarg),
  * hesse(nd,3,ntarg), which are overwritten (bfgtmain STEP 11,
@@ -61,8 +64,10 @@ extern "C" {
  * reused by later calls with the same tree arrays (laddr, levels, parents,
  * children, centres, box sizes), parameters and device -- the drivers call
  * boxfgt on one tree several times per step -- so such a call costs the
- * copies and the kernels only; from the first reuse on, the host arrays are
- * copied through pinned staging buffers by several host threads.  At most 4 plans are kept (least recently used
+ * copies and the kernels only.  The host arrays are copied through pinned
+ * staging buffers (set up on the first call on a device) by several host
+ * threads; a call that builds a plan copies its inputs on a second thread
+ * while it builds the plan.  At most 4 plans are kept (least recently used
  * dropped); the environment variable BFGT_HOST_PLAN_CACHE=0 switches keeping
  * off.  Calls are serialised by a lock.
  */
@@ -72,8 +77,9 @@ int bfgt_hip_eval(int nd, int ndim, double delta, double eps, int ipoly, int ipe
                   int ifnewtree, int ntarg, const double *targs, int ifpghtarg, double *pote, double *grade,
                   double *hesse, double *timeinfo, int precision);
 
-/* Free the plans and device buffers kept by bfgt_hip_eval (no counterpart in
- * the reference, whose boxfgt keeps nothing between calls). */
+/* Free the plans, device buffers and pinned staging buffers kept by
+ * bfgt_hip_eval (no counterpart in the reference, whose boxfgt keeps nothing
+ * between calls). */
 int bfgt_hip_release_plans(void);
 
 /* Resident plan: the tree, lists and FP64 tables stay in device memory for

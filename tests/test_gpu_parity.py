@@ -436,9 +436,10 @@ def test_host_entry_point_keeps_plans(device, monkeypatch):
 
 
 def test_host_entry_staged_copies(device):
-    """Host arrays above the staging threshold (c2: 40 MB pot, 80 MB grad): the
-    first call copies directly, the call on the kept plan goes through the
-    pinned buffers in 32 MB chunks (several per array, both buffers); both add
+    """Host arrays above the staging threshold (c2: 40 MB pot, 80 MB grad) go
+    through the pinned buffers in 32 MB chunks (several per array, both
+    buffers): the first call copies its inputs on a second host thread while
+    it builds the plan, the call on the kept plan copies them itself; both add
     the same transform to nonzero starting arrays, bit for bit, and match the
     oracle."""
     from heat2d_per_box_amd.boxfgt import boxfgt, release_plans
