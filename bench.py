@@ -89,10 +89,16 @@ def measured_traffic(workload, precision, kernel):
 
 def cpu_baseline(w, gpu_out, max_threads, extra=None):
     """Time the reference Fortran boxfgt (or the C oracle) on the host cores, and
-    report the relative L2 difference (leaf points) of the GPU outputs from it."""
+    report the relative L2 difference (leaf points) of the GPU outputs from it.
+    The reference runs on min(max_threads, CPUs this process may use) OpenMP
+    threads bound close (OMP_PROC_BIND=close, OMP_PLACES=cores, set before its
+    OpenMP runtime starts); its own stage times (timeinfo(1..12)) are kept."""
     from oracle import oracle as O
 
-    nthreads = max(1, min(os.cpu_count() or 1, max_threads))
+    affinity = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    nthreads = max(1, min(affinity, max_threads))
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
     if O.reference_available():
         kind = "reference"
         run = lambda: O.reference_boxfgt(w.tree, w.fvals, w.nd, w.delta, w.eps, w.ipoly, w.iperiod, w.norder,  # noqa: E731
@@ -104,14 +110,16 @@ def cpu_baseline(w, gpu_out, max_threads, extra=None):
                                       w.ifpgh)
     r = run()  # warm-up
     # full calls for about 10 s of host work (at least 5), median reported
-    times = []
+    times, tinfo = [], []
     t_start = time.time()
     while len(times) < 5 or (time.time() - t_start < 10.0 and len(times) < 1000):
         t0 = time.perf_counter()
         r = run()
         times.append(time.perf_counter() - t0)
+        tinfo.append(np.asarray(r.timeinfo[:12], dtype=float))
     t_total = time.time() - t_start
     tmed = float(np.median(times))
+    ti_med = np.median(np.array(tinfo), axis=0)
     lv = w.tree.nchild == 0
 
     def rel(outs):
@@ -135,6 +143,14 @@ def cpu_baseline(w, gpu_out, max_threads, extra=None):
                   f"median of {len(times)} calls over {t_total:.1f} s after one warm-up; "
                   f"{'amdflang -O3 -fopenmp reference' if kind == 'reference' else 'C oracle, 1 thread'}",
         "seconds_per_call": tmed,
+        "host_cpus": os.cpu_count(),
+        "affinity_cpus": affinity,
+        "omp_proc_bind": os.environ.get("OMP_PROC_BIND"),
+        "timeinfo_median_s": [float(x) for x in ti_med] if kind == "reference" else None,
+        "steps_1_to_6_s": float(ti_med[1:7].sum()) if kind == "reference" else None,
+        "timeinfo_note": "reference timeinfo(1..12): 1 set-up (tables), 2-6 PW steps 1-5, 7 near field, "
+                         "8 asymptotic, 9-11 tree changes, 12 targets; steps_1_to_6_s = sum of slots 2-7, "
+                         "the part a kept GPU plan runs (compare with ms_per_step)",
         "gpu_vs_cpu_rel_err": err,
         **{f"gpu_{k}_vs_cpu_rel_err": v for k, v in err_extra.items()},
     }
@@ -146,7 +162,8 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5"])
-    ap.add_argument("--scale", default="full", choices=["full", "small", "tiny"])
+    ap.add_argument("--scale", default="full", choices=["full", "mixed", "small", "tiny"],
+                    help="mixed: c2 or c5 at full size with near-field leaves (floor below the cutoff level)")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "split32"])
     ap.add_argument("--delta", type=float, default=None, help="override delta (config c3 only)")
     ap.add_argument("--eps", type=float, default=None, help="override eps (config c3 only)")
@@ -377,6 +394,7 @@ def main():
     t_cold, cold_parts = host_call()
     t_kept, kept_parts = min((host_call() for _ in range(3)), key=lambda r: r[0])
     os.environ["BFGT_HOST_PLAN_CACHE"] = "0"
+    host_call()  # runs on the kept plan one last time and drops it
     t_remesh, remesh_parts = min((host_call() for _ in range(3)), key=lambda r: r[0])
     del os.environ["BFGT_HOST_PLAN_CACHE"]
     release_plans()

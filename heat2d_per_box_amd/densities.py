@@ -142,9 +142,12 @@ def _c2(scale: str):
     w = np.r_[0.005, rng.uniform(0.05, 0.1, size=4)]
     a = rng.uniform(-1, 1, size=5)
     g = GaussianSum(c, w, a)
-    floor = {"full": 7, "small": 4, "tiny": 3}[scale]
-    maxl = {"full": 10, "small": 7, "tiny": 5}[scale]
-    rad = {"full": 0.03, "small": 0.03, "tiny": 0.02}[scale]
+    # "mixed": a level-4 floor (the cutoff level of delta = 1e-4 at eps = 1e-9),
+    # so that the leaves away from the sharp Gaussian are near-field targets,
+    # and a wider disc of level-10 leaves around it (~20k leaves in all)
+    floor = {"full": 7, "mixed": 4, "small": 4, "tiny": 3}[scale]
+    maxl = {"full": 10, "mixed": 10, "small": 7, "tiny": 5}[scale]
+    rad = {"full": 0.03, "mixed": 0.075, "small": 0.03, "tiny": 0.02}[scale]
     # the sharp Gaussian is refined to the deepest level within 6 widths of its centre
     t = build_adaptive(g, 1, 16, 0, 1e-9, maxl, min_level=floor, discs=[(c[0, 0], c[0, 1], rad, maxl)])
     return t, g, 1, 16, 1e-4, 1e-9, 2, (
@@ -181,14 +184,19 @@ def _c4(scale: str):
 def _c5(scale: str):
     rng = np.random.default_rng(5)
     g = gaussian_sum(rng, 40, 0.002, 0.1, logw=True)
-    floor = {"full": 8, "small": 4, "tiny": 3}[scale]
-    maxl = {"full": 12, "small": 7, "tiny": 5}[scale]
-    # Gaussians narrower than 0.01 are refined to the deepest level within one width
-    discs = [(cx, cy, 1.0 * wi, maxl) for (cx, cy), wi in zip(g.centers, g.widths) if wi < 0.01]
+    # "mixed": a level-5 floor, below the cutoff level 6 of delta = 1e-5 at
+    # eps = 1e-10, so that leaves at levels 5 and 6 are near-field targets,
+    # and wider discs (1.7 widths) at the deepest level (~100k leaves in all)
+    floor = {"full": 8, "mixed": 5, "small": 4, "tiny": 3}[scale]
+    maxl = {"full": 12, "mixed": 12, "small": 7, "tiny": 5}[scale]
+    reach = 1.7 if scale == "mixed" else 1.0
+    # Gaussians narrower than 0.01 are refined to the deepest level within `reach` widths
+    discs = [(cx, cy, reach * wi, maxl) for (cx, cy), wi in zip(g.centers, g.widths) if wi < 0.01]
     t = build_adaptive(g, 1, 16, 0, 1e-10, maxl, min_level=floor, discs=discs)
     return t, g, 1, 16, 1e-5, 1e-10, 2, (
         f"adaptive periodic tree: tail rule (tol 1e-10), uniform level-{floor} floor, levels up to {maxl} "
-        "within one width of the Gaussians narrower than 0.01; n=16 Legendre, nd=1, delta=1e-5, eps=1e-10, pot+grad")
+        f"within {reach:g} width(s) of the Gaussians narrower than 0.01; n=16 Legendre, nd=1, delta=1e-5, "
+        "eps=1e-10, pot+grad")
 
 
 WORKLOADS = {"c1": _c1, "c2": _c2, "c3": _c3, "c4": _c4, "c5": _c5}
@@ -196,7 +204,11 @@ WORKLOADS = {"c1": _c1, "c2": _c2, "c3": _c3, "c4": _c4, "c5": _c5}
 
 def make_workload(name: str, scale: str = "full", **kw) -> Workload:
     """Build one of the seeded workloads.  ``scale`` in {"full", "small", "tiny"} shrinks
-    the tree depth for CPU-sized parity tests; densities and FGT parameters are kept."""
+    the tree depth for CPU-sized parity tests; densities and FGT parameters are kept.
+    ``"mixed"`` (c2 and c5 only) keeps the full size but lowers the uniform floor
+    below the cutoff level, so that the near field runs beside the PW stages."""
+    if scale == "mixed" and name not in ("c2", "c5"):
+        raise ValueError("the mixed scale exists for c2 and c5 only")
     t, dens, nd, n, delta, eps, ifpgh, desc = WORKLOADS[name](scale, **kw)
     fv = t.sample(dens, n, 0, nd)
     return Workload(name, t, fv, nd, n, 0, delta, eps, ifpgh, 1, dens, desc)

@@ -531,3 +531,21 @@ def test_npw_above_128_matches_oracle(device):
     info = got["plan"].info
     assert info["npw"] == ref.info["npw"] and info["npw"] > 128 and not info["mfma"]
     assert_close(compare(t, got, ref, 2))
+
+
+@pytest.mark.parametrize("name", ["c2", "c5"])
+def test_mixed_workloads_match_oracle(device, name):
+    """The headline shapes with a near field: c2/mixed (20,353 leaves, 5.2 M
+    points, 231 near-field target leaves at the cutoff level 4) and c5/mixed
+    (106,246 leaves, 27.2 M points, leaves at levels 5 and 6 against the
+    cutoff level 6): every stage runs at full size, the near field of n = 16
+    beside 16-leaf upward chunks and grouped translation."""
+    w = make_workload(name, "mixed")
+    print(f"case {name}/mixed: leaves={w.tree.nleaves}", flush=True)
+    ref = O.oracle_boxfgt(w.tree, w.fvals, w.nd, w.delta, w.eps, w.ipoly, w.iperiod, w.norder, w.ifpgh)
+    got = run_gpu(w.tree, w.fvals, w.nd, w.delta, w.eps, w.ipoly, w.iperiod, w.norder, w.ifpgh)
+    info = got["plan"].info
+    assert info["n_near_pairs"] == ref.info["npairs"] and info["n_near_pairs"] > 1000
+    assert info["n_pw_leaves"] > 0.9 * w.tree.nleaves
+    assert np.array_equal(got["itree"], ref.itree)
+    assert_close(compare(w.tree, got, ref, w.ifpgh))
