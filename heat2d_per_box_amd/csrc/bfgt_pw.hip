@@ -93,6 +93,30 @@ __device__ inline double leaf_in(const LeafIn& in, int o, size_t pb, int nd, int
 // every load is unconditional and in bounds, and padding rows or columns
 // either contribute exact zeros or land in outputs that are not stored.
 
+// Buffer accesses: a wave-uniform base in a buffer resource (scalar
+// registers), a 32-bit lane offset in one vector register and a uniform byte
+// offset in a scalar register.  The per-step and per-table parts of an
+// address then cost no vector registers, and the compiler cannot fold them
+// into 64-bit per-lane addresses that stay live across a loop.  Every base
+// here is the start of one table, phase level, folded expansion or leaf, so
+// the offsets stay far below the 4 GB range of the resource.
+// The resource is raw (stride 0), 4 GB - 1 bytes long, with the 32-bit
+// data format in its last word (the format untyped loads and stores expect).
+typedef __amdgpu_buffer_rsrc_t rsrc_t;
+__device__ inline rsrc_t rsrc(const void* base) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, -1, 0x00020000);
+}
+__device__ inline cplx bload_c(rsrc_t r, unsigned voff, int soff) {
+    return __builtin_bit_cast(cplx, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0));
+}
+__device__ inline double bload_d(rsrc_t r, unsigned voff, int soff) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+__device__ inline void bstore_d(double v, rsrc_t r, unsigned voff, int soff) {
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(__attribute__((ext_vector_type(2))) unsigned, v), r,
+                                          voff, soff, 0);
+}
+
 // Upward pass.  One wave per (chunk, y-frequency tile tN): for the leaves of
 // the chunk
 //   G_v  = sum_o F_o (T_y(o) diag py w)    (rows m1 < 16, columns k2 of the tile)
@@ -285,13 +309,23 @@ __global__ __launch_bounds__(kWave * kUpReduceWaves) void k_up_col(
     const bool valid = ci < nchunks;  // waves past the end still meet the barriers
     const int q = lane >> 4, cc = lane & 15;
     const int H = (P + 1) / 2, Mlo = (P + 1) / 2, nexp = P * H, npbox = n * n;
-    const int tstride = 16 * ps;
+    const int tbytes = 16 * ps * (int)sizeof(cplx);  // one table of a level
+    const int rowb = ps * (int)sizeof(cplx);         // one point row of a table
     const int k2 = 16 * tN + cc;
     const double wy = (ymid_half && k2 >= P / 2) ? 0.5 : 1.0;
     const int ccl = min(cc, n - 1);
-    int pidx[NKS];  // clamped point index of (cc, m2 = 4s + q)
+    // lane offsets (bytes): table row q, frequency k2 (y) or cc (x tile);
+    // phase frequency k2 or cc; leaf values at the clamped point (cc,
+    // m2 = 4s + q) of a density-value array with one component
+    const unsigned oTy = (unsigned)(q * ps + k2) * sizeof(cplx);
+    const unsigned oTx = (unsigned)(q * ps + cc) * sizeof(cplx);
+    const unsigned oPy = (unsigned)k2 * sizeof(cplx);
+    const unsigned oPx = (unsigned)cc * sizeof(cplx);
+    unsigned oF[NKS];
 #pragma unroll
-    for (int s = 0; s < NKS; s++) pidx[s] = ccl + n * min(4 * s + q, n - 1);
+    for (int s = 0; s < NKS; s++) oF[s] = (unsigned)(ccl + n * min(4 * s + q, n - 1)) * (unsigned)(nd * sizeof(double));
+    constexpr int stride_o[6] = {1, 2, 2, 3, 3, 3}, comp_o[6] = {0, 0, 1, 0, 1, 2};
+    constexpr int NARR = NTAB;  // leaf arrays read: pot, grad, hess
     const int4 ch = chunks[valid ? ci : 0];
     for (int d = d0; d < (DS ? d0 + 1 : nd); d++) {
         v4d acc[4][NTM];
@@ -302,56 +336,86 @@ __global__ __launch_bounds__(kWave * kUpReduceWaves) void k_up_col(
         for (int c = ch.x; valid && c < ch.x + ch.y; c++) {
             const int2 col = cols[c];
             const PwLeaf l0 = leaves[col.x];
-            const cplx* T = tab + (size_t)l0.level * lstride;
-            const cplx* ph = phase + levbase[l0.level];
+            const rsrc_t rT = rsrc(tab + (size_t)l0.level * lstride);  // [t][m][k]
+            const rsrc_t rP = rsrc(phase + levbase[l0.level]);         // [cell][k]
             v4d Gr[NTAB], Gi[NTAB];
 #pragma unroll
             for (int v = 0; v < NTAB; v++) Gr[v] = Gi[v] = v4d{0, 0, 0, 0};
-            for (int li = 0; li < col.y; li += LPC) {
-                size_t pb0[LPC];
+            // y contraction, LPC leaves per pass; the operands of the next
+            // K-step (of this pass or of the next pass's first) are loaded
+            // before the products of the current one
+            struct Pass {
+                rsrc_t r[LPC][NARR];
                 cplx py[LPC];
-                double wl[LPC];
+            };
+            auto pass_at = [&](int li) {
+                Pass o;
 #pragma unroll
                 for (int h = 0; h < LPC; h++) {
                     const PwLeaf lf = leaves[col.x + min(li + h, col.y - 1)];
-                    pb0[h] = (size_t)lf.box * npbox;
-                    wl[h] = li + h < col.y ? 1.0 : 0.0;
+                    const double wl = li + h < col.y ? 1.0 : 0.0;
+#pragma unroll
+                    for (int a = 0; a < NARR; a++) {
+                        const double* arr = a == 0 ? in.pot : (a == 1 ? in.grad : in.hess);
+                        o.r[h][a] = rsrc(arr + (size_t)lf.box * npbox * (a + 1) * nd + d);
+                    }
                     // y phase of this lane's column (upward shift: conjugate of the table)
-                    py[h] = ph[lf.jy * ps + k2];
-                    py[h].x *= wy * wl[h];
-                    py[h].y *= -wy * wl[h];
+                    o.py[h] = bload_c(rP, oPy, lf.jy * rowb);
+                    o.py[h].x *= wy * wl;
+                    o.py[h].y *= -wy * wl;
                 }
+                return o;
+            };
+            struct YOp {
+                cplx tt[NTAB];
+                double a[LPC][NOUT];
+            };
+            auto yload = [&](const Pass& pa, int s) {
+                YOp o;
+#pragma unroll
+                for (int u = 0; u < NTAB; u++) o.tt[u] = bload_c(rT, oTy, u * tbytes + 4 * s * rowb);
+#pragma unroll
+                for (int h = 0; h < LPC; h++)
+#pragma unroll
+                    for (int oo = 0; oo < NOUT; oo++)
+                        o.a[h][oo] = bload_d(pa.r[h][stride_o[oo] - 1], oF[s] * stride_o[oo],
+                                             comp_o[oo] * nd * (int)sizeof(double));
+                return o;
+            };
+            Pass pc = pass_at(0);
+            YOp yc = yload(pc, 0);
+            for (int li = 0; li < col.y; li += LPC) {
+                // the table rows are read again by every pass (from L2)
+                asm volatile("" ::: "memory");
+                const Pass pn = pass_at(min(li + LPC, col.y - 1));
 #pragma unroll
                 for (int s = 0; s < NKS; s++) {
-                    const int m2 = 4 * s + q;
-                    cplx tt[NTAB];
-#pragma unroll
-                    for (int u = 0; u < NTAB; u++) tt[u] = T[u * tstride + m2 * ps + k2];
+                    const YOp yn = s + 1 < NKS ? yload(pc, s + 1) : yload(pn, 0);
+                    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
                     for (int h = 0; h < LPC; h++) {
                         cplx ty[NTAB];
 #pragma unroll
-                        for (int u = 0; u < NTAB; u++) ty[u] = cmul(tt[u], py[h]);
+                        for (int u = 0; u < NTAB; u++) ty[u] = cmul(yc.tt[u], pc.py[h]);
 #pragma unroll
                         for (int o = 0; o < NOUT; o++) {
-                            const double a = leaf_in(in, o, pb0[h] + pidx[s], nd, d);
-                            Gr[chan_x(o)] = mfma(a, ty[chan_y(o)].x, Gr[chan_x(o)]);
-                            Gi[chan_x(o)] = mfma(a, ty[chan_y(o)].y, Gi[chan_x(o)]);
+                            Gr[chan_x(o)] = mfma(yc.a[h][o], ty[chan_y(o)].x, Gr[chan_x(o)]);
+                            Gi[chan_x(o)] = mfma(yc.a[h][o], ty[chan_y(o)].y, Gi[chan_x(o)]);
                         }
                     }
+                    yc = yn;
                 }
+                pc = pn;
             }
 #pragma unroll
             for (int t = 0; t < NTM; t++) {
-                const int k1 = 16 * t + cc;
-                cplx px = ph[l0.jx * ps + k1];
+                cplx px = bload_c(rP, oPx, (l0.jx * ps + 16 * t) * (int)sizeof(cplx));
                 px.y = -px.y;
 #pragma unroll
                 for (int v = 0; v < NTAB; v++) {
 #pragma unroll
                     for (int s = 0; s < NKS; s++) {
-                        const int m1 = 4 * s + q;
-                        cplx tx = cmul(T[v * tstride + m1 * ps + k1], px);
+                        cplx tx = cmul(bload_c(rT, oTx, v * tbytes + 4 * s * rowb + 16 * t * (int)sizeof(cplx)), px);
                         acc[0][t] = mfma(tx.x, Gr[v][s], acc[0][t]);
                         acc[1][t] = mfma(tx.y, Gi[v][s], acc[1][t]);
                         acc[2][t] = mfma(tx.x, Gi[v][s], acc[2][t]);
@@ -411,30 +475,6 @@ struct Fold {
 };
 
 __device__ __host__ inline int round_up(int a, int b) { return (a + b - 1) / b * b; }
-
-// Buffer accesses: a wave-uniform base in a buffer resource (scalar
-// registers), a 32-bit lane offset in one vector register and a uniform byte
-// offset in a scalar register.  The per-step and per-table parts of an
-// address then cost no vector registers, and the compiler cannot fold them
-// into 64-bit per-lane addresses that stay live across a loop.  Every base
-// here is the start of one table, phase level, folded expansion or leaf, so
-// the offsets stay far below the 4 GB range of the resource.
-// The resource is raw (stride 0), 4 GB - 1 bytes long, with the 32-bit
-// data format in its last word (the format untyped loads and stores expect).
-typedef __amdgpu_buffer_rsrc_t rsrc_t;
-__device__ inline rsrc_t rsrc(const void* base) {
-    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, -1, 0x00020000);
-}
-__device__ inline cplx bload_c(rsrc_t r, unsigned voff, int soff) {
-    return __builtin_bit_cast(cplx, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0));
-}
-__device__ inline double bload_d(rsrc_t r, unsigned voff, int soff) {
-    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
-}
-__device__ inline void bstore_d(double v, rsrc_t r, unsigned voff, int soff) {
-    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(__attribute__((ext_vector_type(2))) unsigned, v), r,
-                                          voff, soff, 0);
-}
 
 // Downward pass.  y frequencies are processed in tiles of 16:
 //   FF_t^T = S^T a_t + i D^T b_t      a_t + i b_t = E_t diag(px), the x
