@@ -226,6 +226,39 @@ __device__ inline int xcd_block(int b, int nb) {
     return x * q + (x < r ? x : r) + b / kXcds;
 }
 
+// Split-FP32 products (BFGT_PREC_SPLIT32): an FP64 operand x is split into
+// hi = fl32(x) and lo = fl32(x - hi), and a*b is formed as hi*hi + hi*lo +
+// lo*hi on the FP32 matrix cores, accumulated in FP32.
+// Lane layout of v_mfma_f32_16x16x4f32 on gfx950 (measured): A holds A[c][q],
+// B holds B[q][c], element r of the accumulator holds D[4q + r][c] -- the
+// transpose of the FP64 instruction's row order.  Register r of lane (q, c)
+// therefore serves as the B operand of a K-step whose K index is 4q + r, and
+// the A operand of that step is loaded at K index 4q + r.
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+__device__ inline v4f mfma32(float a, float b, v4f c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+
+struct Split {
+    float h, l;
+};
+__device__ inline Split split(double x) {
+    float h = (float)x;
+    return Split{h, (float)(x - (double)h)};
+}
+__device__ inline Split neg(Split s) { return Split{-s.h, -s.l}; }
+// c += a*b, both factors split
+__device__ inline v4f mfma3(Split a, Split b, v4f c) {
+    c = mfma32(a.l, b.h, c);
+    c = mfma32(a.h, b.l, c);
+    return mfma32(a.h, b.h, c);
+}
+// c += a*b, a split, b an FP32 accumulator value
+__device__ inline v4f mfma2(Split a, float b, v4f c) {
+    c = mfma32(a.l, b, c);
+    return mfma32(a.h, b, c);
+}
+__device__ inline int drow32(int q, int r) { return 4 * q + r; }
+
 // status helpers
 int set_error(int code, const std::string& msg);
 

@@ -346,6 +346,83 @@ __global__ __launch_bounds__(64 * kNearWaves) void k_near_mfma(
     }
 }
 
+// Split-FP32 near field (BFGT_PREC_SPLIT32, n <= 16): one wave per
+// (target, density), the products of k_near_mfma on the FP32 matrix cores
+// with hi/lo operands (three products per table product, two where the
+// second factor is the FP32 accumulator of the first), accumulated in FP32
+// per target and added to the FP64 outputs.  v_mfma_f32_16x16x4f32 holds
+// accumulator element r of lane (q, c) at row 4q + r: the x contraction takes
+// source x index j1 = 4s + q in K-step s, the y contraction source y index
+// j2 = 4q + r in K-step r (all four: rows past n meet zero table columns).
+template <int NKS, int NTAB>
+__global__ __launch_bounds__(64 * kNearWaves) void k_near_split(
+    int n, int nd, int nitems, const int* __restrict__ tgt, const int* __restrict__ tlev,
+    const int* __restrict__ ptr, const NearPair* __restrict__ pairs, const double* __restrict__ tnearT,
+    const double* __restrict__ fvals, double* __restrict__ pot, double* __restrict__ grad,
+    double* __restrict__ hess, double oscale) {
+    constexpr int NOUT = near_chan(NTAB);
+    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+    const int item = __builtin_amdgcn_readfirstlane(blockIdx.x * kNearWaves + wave);
+    if (item >= nitems) return;  // no block-wide barriers in this kernel
+    const int it = item / nd, d = item % nd;
+    const int q = lane >> 4, c = lane & 15;
+    const int b = tgt[it];
+    const int p0 = ptr[it], p1 = ptr[it + 1];
+    const double* T = tnearT + (size_t)tlev[it] * 13 * 3 * 256;
+    const int nn = n * n;
+    const int cl = min(c, n - 1);
+    v4f out[NOUT];
+#pragma unroll
+    for (int o = 0; o < NOUT; o++) out[o] = v4f{0, 0, 0, 0};
+    for (int p = p0; p < p1; p++) {
+        const NearPair pr = pairs[p];
+        const double* fs = fvals + (size_t)pr.src * nn * nd + d;
+        const double* Tx = T + (size_t)(pr.ix + 6) * 3 * 256;
+        const double* Ty = T + (size_t)(pr.iy + 6) * 3 * 256;
+        // FF_t^T[j2 = row][k1 = col] = sum_j1 F[j1][j2] Tx_t[k1][j1]
+        v4f FF[NTAB];
+#pragma unroll
+        for (int t = 0; t < NTAB; t++) FF[t] = v4f{0, 0, 0, 0};
+#pragma unroll
+        for (int s = 0; s < NKS; s++) {
+            const int j1 = 4 * s + q;
+            const Split a = split(fs[(size_t)(min(j1, n - 1) + n * cl) * nd]);
+#pragma unroll
+            for (int t = 0; t < NTAB; t++) FF[t] = mfma3(a, split(Tx[t * 256 + j1 * 16 + c]), FF[t]);
+        }
+        // element r of FF_t on lane (q, c) = FF_t^T[j2 = 4q + r][k1 = c]
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int j2 = drow32(q, r);
+            Split ty[NTAB];
+#pragma unroll
+            for (int u = 0; u < NTAB; u++) ty[u] = split(Ty[u * 256 + j2 * 16 + c]);
+#pragma unroll
+            for (int o = 0; o < NOUT; o++) out[o] = mfma2(ty[near_y(o)], FF[near_x(o)][r], out[o]);
+        }
+    }
+    // element r of out_o on lane (q, c): target point k1 = c, k2 = 4q + r
+    if (p1 > p0 && c < n) {
+        double* dst[NOUT];
+        dst[0] = pot;
+        if constexpr (NTAB >= 2) dst[1] = dst[2] = grad;
+        if constexpr (NTAB >= 3) dst[3] = dst[4] = dst[5] = hess;
+        constexpr int stride_o[6] = {1, 2, 2, 3, 3, 3}, comp_o[6] = {0, 0, 1, 0, 1, 2};
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int k2 = drow32(q, r);
+            if (k2 < n) {
+                const size_t pb = (size_t)b * nn + c + (size_t)n * k2;
+#pragma unroll
+                for (int o = 0; o < NOUT; o++) {
+                    double& v = dst[o][(pb * stride_o[o] + comp_o[o]) * nd + d];
+                    v += oscale * (double)out[o][r];
+                }
+            }
+        }
+    }
+}
+
 // n <= 8 and two or more densities: one wave per (target, pair of densities
 // d0, d0 + 1, tile).  At n <= 8 a 16x16 tile of one density and one table is a
 // quarter full, so the x contraction holds the two densities in its row
@@ -612,6 +689,24 @@ int launch_near_pack2(Plan& p, const double* fvals, double* pot, double* grad, d
     return BFGT_OK;
 }
 
+template <int NKS, int NTAB>
+int launch_near_split2(Plan& p, const double* fvals, double* pot, double* grad, double* hess) {
+    int nitems = p.nnear_t * p.nd;
+    int blocks = (nitems + kNearWaves - 1) / kNearWaves;
+    k_near_split<NKS, NTAB><<<blocks, 64 * kNearWaves, 0, p.stream>>>(p.n, p.nd, nitems, p.d_near_tgt, p.d_near_lev,
+                                                                    p.d_near_ptr, p.d_near, p.d_tnearT, fvals, pot,
+                                                                    grad, hess, p.out_scale);
+    BFGT_HIP_CHECK(hipGetLastError());
+    return BFGT_OK;
+}
+
+template <int NKS>
+int launch_near_split1(Plan& p, const double* fvals, double* pot, double* grad, double* hess) {
+    if (p.ifpgh == 1) return launch_near_split2<NKS, 1>(p, fvals, pot, grad, hess);
+    if (p.ifpgh == 2) return launch_near_split2<NKS, 2>(p, fvals, pot, grad, hess);
+    return launch_near_split2<NKS, 3>(p, fvals, pot, grad, hess);
+}
+
 template <int NKS>
 int launch_near_pack1(Plan& p, const double* fvals, double* pot, double* grad, double* hess) {
     if (p.ifpgh == 1) return launch_near_pack2<NKS, 1>(p, fvals, pot, grad, hess);
@@ -664,6 +759,14 @@ size_t near_smem_bytes(int n, int ntab) { return sizeof(double) * (size_t)n * n 
 
 int launch_near(Plan& p, const double* fvals, double* pot, double* grad, double* hess) {
     if (p.nnear_t == 0) return BFGT_OK;
+    if (near_use_mfma(p) && p.precision == BFGT_PREC_SPLIT32) {
+        switch ((p.n + 3) / 4) {
+            case 1: return launch_near_split1<1>(p, fvals, pot, grad, hess);
+            case 2: return launch_near_split1<2>(p, fvals, pot, grad, hess);
+            case 3: return launch_near_split1<3>(p, fvals, pot, grad, hess);
+            default: return launch_near_split1<4>(p, fvals, pot, grad, hess);
+        }
+    }
     if (near_use_mfma(p)) {
         if (p.n <= 4 && p.nd >= 2) return launch_near_pack1<1>(p, fvals, pot, grad, hess);
         if (p.n <= 8 && p.nd >= 2) return launch_near_pack1<2>(p, fvals, pot, grad, hess);

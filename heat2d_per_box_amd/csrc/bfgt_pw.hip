@@ -927,35 +927,6 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_down_pair(
 // are converted to FP64 where they leave the kernel: the PW expansions (the
 // merge and translation stay FP64) and the pot/grad/hess accumulation.
 //
-// Lane layout of v_mfma_f32_16x16x4f32 on gfx950 (measured): A holds A[c][q],
-// B holds B[q][c], element r of the accumulator holds D[4q + r][c] -- the
-// transpose of the FP64 instruction's row order.  Register r of lane (q, c)
-// therefore serves as the B operand of a K-step whose K index is 4q + r, and
-// the A operand of that step is loaded at K index 4q + r.
-typedef float v4f __attribute__((ext_vector_type(4)));
-
-__device__ inline v4f mfma32(float a, float b, v4f c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
-
-struct Split {
-    float h, l;
-};
-__device__ inline Split split(double x) {
-    float h = (float)x;
-    return Split{h, (float)(x - (double)h)};
-}
-__device__ inline Split neg(Split s) { return Split{-s.h, -s.l}; }
-// c += a*b, both factors split
-__device__ inline v4f mfma3(Split a, Split b, v4f c) {
-    c = mfma32(a.l, b.h, c);
-    c = mfma32(a.h, b.l, c);
-    return mfma32(a.h, b.h, c);
-}
-// c += a*b, a split, b an FP32 accumulator value
-__device__ inline v4f mfma2(Split a, float b, v4f c) {
-    c = mfma32(a.l, b, c);
-    return mfma32(a.h, b, c);
-}
-__device__ inline int drow32(int q, int r) { return 4 * q + r; }
 
 // Forward upward pass, split-FP32 (one wave per (chunk, y tile), one leaf at
 // a time, the same two products).

@@ -1,12 +1,13 @@
-"""Split-FP32 leaf transforms: error against the FP64 oracle.
+"""Split-FP32 leaf transforms and near field: error against the FP64 oracle.
 
 The split-FP32 path (BFGT_PREC_SPLIT32) runs the leaf-to-PW and PW-to-leaf
-contractions as hi/lo FP32 MFMA products accumulated in FP32, with FP64
-phases, tables, translation, near field and output accumulation.  FP32
-accumulation bounds its error near the FP32 unit roundoff (6e-8) times the
-conditioning of the sums, independent of eps, so the bar here is 2e-6
-relative L2 over leaf points for pot and 1e-5 for grad/hess.  The measured
-figures are printed so the per-config error can be read off the log.
+contractions and the near-field table contractions as hi/lo FP32 MFMA
+products accumulated in FP32, with FP64 phases, tables, translation and
+output accumulation.  FP32 accumulation bounds its error near the FP32 unit
+roundoff (6e-8) times the conditioning of the sums, independent of eps, so
+the bar here is 2e-6 relative L2 over leaf points for pot and 1e-5 for
+grad/hess.  The measured figures are printed so the per-config error can be
+read off the log.
 """
 import numpy as np
 import pytest
@@ -48,22 +49,31 @@ def test_split32_error_vs_oracle(device, name, scale, kw):
             assert errs[k] < 1e-5, errs
 
 
-def test_split32_differs_from_fp64_only_in_pw_stages(device):
-    """In the Fourier-series regime with a single PW level the split path
-    changes only the leaf transforms: the near field and translation stay
-    FP64, so on a tree without PW leaves both precisions agree exactly."""
+@pytest.mark.parametrize("n,nd,ifpgh", [(8, 1, 2), (8, 3, 3), (16, 2, 3), (12, 1, 1)])
+def test_split32_near_field_only(device, n, nd, ifpgh):
+    """A tree whose leaves are all near-field targets (delta = 1e-6): the
+    split path's near field alone, against the oracle, with the bars above
+    (n = 8 with three densities is the case the FP64 path packs two densities
+    per tile for; the split near field runs one wave per target and density)."""
     from heat2d_per_box_amd.tree import build_uniform
 
     t = build_uniform(2)
-    rng = np.random.default_rng(3)
-    n = 8
-    fv = rng.standard_normal((t.centers.shape[0], n * n, 1))
+    rng = np.random.default_rng(3 + n + nd)
+    fv = rng.standard_normal((t.centers.shape[0], n * n, nd))
     fv[t.nchild != 0] = 0.0
-    # delta small enough that every leaf is at or above the cutoff: near field only
-    a = run_gpu(t, fv, 1, 1e-6, 1e-9, 0, 1, n, 2, precision="fp64")
-    b = run_gpu(t, fv, 1, 1e-6, 1e-9, 0, 1, n, 2, precision="split32")
-    assert a["plan"].info["n_pw_leaves"] == 0
-    assert np.array_equal(a["pot"], b["pot"]) and np.array_equal(a["grad"], b["grad"])
+    ref = O.oracle_boxfgt(t, fv, nd, 1e-6, 1e-9, 0, 1, n, ifpgh)
+    got = run_gpu(t, fv, nd, 1e-6, 1e-9, 0, 1, n, ifpgh, precision="split32")
+    assert got["plan"].info["n_pw_leaves"] == 0 and got["plan"].info["n_near_pairs"] > 0
+    errs = {"pot": leaf_rel_err(t, got["pot"], ref.pot)}
+    if ifpgh >= 2:
+        errs["grad"] = leaf_rel_err(t, got["grad"], ref.grad)
+    if ifpgh >= 3:
+        errs["hess"] = leaf_rel_err(t, got["hess"], ref.hess)
+    print(f"split32 near field n={n} nd={nd} ifpgh={ifpgh}: errors {errs}", flush=True)
+    assert 0 < errs["pot"] < 2e-6, errs
+    for k in ("grad", "hess"):
+        if k in errs:
+            assert errs[k] < 1e-5, errs
 
 
 FULL = [
