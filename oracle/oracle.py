@@ -42,7 +42,7 @@ def build(with_reference: Optional[bool] = None) -> None:
     if with_reference is None:
         with_reference = os.path.isdir(os.environ.get("REF_DIR", "/root/reference"))
     if with_reference:
-        targets.append("ref")
+        targets += ["ref", "initpot"]
     subprocess.run(["make", "-s", "-C", HERE, "-j8"] + targets, check=True)
 
 

@@ -5,10 +5,15 @@ library, reproduces the periodic heat semigroup on a Fourier mode
 import os
 import subprocess
 
+import numpy as np
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 EXE = os.path.join(ROOT, "heat2d_per_box_amd", "_build", "test_boxfgt_dropin")
+# the reference's own driver hpots/test_initpot.f linked with the reference
+# boxfgt and with the drop-in (oracle/Makefile target initpot)
+REF_DRV = os.path.join(ROOT, "oracle", "_ref", "test_initpot_ref")
+HIP_DRV = os.path.join(ROOT, "oracle", "_ref", "test_initpot_hip")
 
 
 @pytest.mark.gpu
@@ -35,3 +40,37 @@ def test_fortran_shim_links_without_gpu():
     r = subprocess.run([EXE], capture_output=True, text=True, timeout=60)
     assert r.returncode != 0
     assert "status 11" in r.stdout
+
+
+def _run_initpot(exe, where):
+    """Run one build of hpots/test_initpot.f in its own directory; return its
+    fort.134 records (x, y, exact pot, computed pot, |difference|) per leaf
+    point (hpots/test_initpot.f:346-377) and its printout."""
+    where.mkdir()
+    r = subprocess.run([exe], cwd=where, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    vals = np.array((where / "fort.134").read_text().split(), dtype=float)
+    return vals.reshape(-1, 5), r.stdout
+
+
+@pytest.mark.gpu
+def test_reference_initpot_driver(device, tmp_path):
+    """The reference's known-answer driver, unchanged, with the drop-in linked
+    in place of bfgt/boxfgt.o: a Chebyshev (ipoly = 1) n = 8 periodic tree
+    resolving sin(4 pi x) cos(4 pi y), initpot's boxfgt call at delta = 4e-5,
+    eps = 1e-9, pot divided by 4 pi dt and compared with the heat semigroup
+    on that mode.  Its largest error matches the reference's up to rounding
+    (1e-12 of the largest |pot|), and its potential agrees with the reference
+    build's to 1e-12."""
+    if not (os.path.exists(REF_DRV) and os.path.exists(HIP_DRV)):
+        pytest.skip("reference driver not built (reference sources absent at build time)")
+    ref, _ = _run_initpot(REF_DRV, tmp_path / "ref")
+    got, out = _run_initpot(HIP_DRV, tmp_path / "hip")
+    assert ref.shape == got.shape and ref.shape[0] > 1000
+    assert np.array_equal(ref[:, :3], got[:, :3])  # same points, same exact values
+    err_ref, err_gpu = ref[:, 4].max(), got[:, 4].max()
+    diff = np.linalg.norm(got[:, 3] - ref[:, 3]) / np.linalg.norm(ref[:, 3])
+    print(f"initpot: {ref.shape[0]} leaf points; max error vs the semigroup: reference {err_ref:.3e}, "
+          f"drop-in {err_gpu:.3e}; drop-in vs reference pot {diff:.3e}", flush=True)
+    assert err_gpu <= err_ref + 1e-12 * np.abs(ref[:, 3]).max()
+    assert diff < 1e-12
