@@ -58,8 +58,9 @@ extern "C" {
  * bfgt/boxfgt.f:1494-1563); channels above ifpgh are computed into scratch
  * arrays and never written to grad/hess (bfgt/boxfgt.f:207-277).
  * `precision` selects BFGT_PREC_FP64 or BFGT_PREC_SPLIT32.  Split-FP32 applies
- * to the matrix-core leaf transforms (norder <= 16, npw <= 128); a plan
- * outside those sizes runs FP64 and reports precision 0 (bfgt_plan_query).
+ * to the matrix-core leaf transforms and near field (norder <= 16, npw <=
+ * 128); a plan outside those sizes runs FP64 and reports precision 0
+ * (bfgt_plan_query).
  * The plan built for a call (schedules, tables, device buffers) is kept and
  * reused by later calls with the same tree arrays (laddr, levels, parents,
  * children, centres, box sizes), parameters and device -- the drivers call
