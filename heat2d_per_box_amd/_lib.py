@@ -25,6 +25,7 @@ STATUS = {
     10: "BFGT_ERR_HIP",
     11: "BFGT_ERR_NODEVICE",
     12: "BFGT_ERR_RESOURCE",
+    13: "BFGT_ERR_CALLBACK",
 }
 PRECISIONS = {"fp64": 0, "split32": 1}
 
@@ -45,9 +46,19 @@ EXPORTS = [
     "bfgt_plan_profile_begin",
     "bfgt_plan_profile_stage",
     "bfgt_plan_profile_end",
+    "bfgt_tree_build",
+    "bfgt_tree_adapt",
+    "bfgt_tree_query",
+    "bfgt_tree_export",
+    "bfgt_tree_plan_create",
+    "bfgt_tree_destroy",
     "bfgt_last_error",
     "bfgt_device_count",
 ]
+
+# int fun(void* ctx, int nd, int npts, const double* d_xy, double* d_vals, void* stream)
+TREE_FUN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                            ctypes.c_void_p, ctypes.c_void_p)
 
 
 class BoxFGTError(RuntimeError):
@@ -107,6 +118,19 @@ def lib() -> ctypes.CDLL:
     L.bfgt_plan_profile_stage.argtypes = [_vp, _i, _i]
     L.bfgt_plan_profile_end.restype = _i
     L.bfgt_plan_profile_end.argtypes = [_vp, _dp, _ip]
+    L.bfgt_tree_build.restype = _i
+    L.bfgt_tree_build.argtypes = [ctypes.POINTER(_vp), _i, _i, _i, _d, _d, _i, _i, _d, TREE_FUN, _vp, _dp, _i, _i,
+                                  _dp, _vp]
+    L.bfgt_tree_adapt.restype = _i
+    L.bfgt_tree_adapt.argtypes = [_vp, _d, _dp, TREE_FUN, _vp, _i]
+    L.bfgt_tree_query.restype = _i
+    L.bfgt_tree_query.argtypes = [_vp, _ip, ctypes.POINTER(_vp)]
+    L.bfgt_tree_export.restype = _i
+    L.bfgt_tree_export.argtypes = [_vp, _ip, _ip, _dp, _dp, _dp]
+    L.bfgt_tree_plan_create.restype = _i
+    L.bfgt_tree_plan_create.argtypes = [ctypes.POINTER(_vp), _vp, _d, _d, _i, _i, _vp]
+    L.bfgt_tree_destroy.restype = _i
+    L.bfgt_tree_destroy.argtypes = [_vp]
     L.bfgt_last_error.restype = ctypes.c_char_p
     L.bfgt_last_error.argtypes = []
     L.bfgt_device_count.restype = _i

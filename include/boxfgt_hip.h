@@ -33,7 +33,8 @@ extern "C" {
 #define BFGT_ERR_ARG 9       /* null pointer or invalid scalar argument */
 #define BFGT_ERR_HIP 10      /* HIP runtime failure */
 #define BFGT_ERR_NODEVICE 11 /* no GPU visible */
-#define BFGT_ERR_RESOURCE 12 /* kernel resource limits (LDS) exceeded for this size */
+#define BFGT_ERR_RESOURCE 12 /* kernel resource limits (LDS) exceeded for this size, or a tree outgrew its capacity */
+#define BFGT_ERR_CALLBACK 13 /* a user function callback returned non-zero */
 
 /* arithmetic of the leaf transforms and near-field contractions */
 #define BFGT_PREC_FP64 0    /* all-FP64 (the drop-in) */
@@ -218,6 +219,85 @@ int bfgt_plan_work(bfgt_plan_t plan, long long *counts, int ncounts);
  * otherwise n_ancestors x nd x npw x H complex; 8 the near-field schedule's
  * CSR pointer over its target leaves (n_near_targets + 1 ints). */
 int bfgt_plan_get_table(bfgt_plan_t plan, int which, int level, void *host_out, size_t nbytes);
+
+/*
+ * Adaptive quadtrees resident in device memory: the tree builder and the
+ * per-step re-meshing of the solvers, on the GPU.
+ *
+ * A tree holds the reference's tree arrays (levels, parents, children,
+ * colleagues, centres, box sizes) and one array of values on the leaf nodes,
+ * vals(nd,npbox,nboxes), in device memory with room for max_boxes boxes.  Its
+ * boxes are ordered by level, and within a level the children of a level's
+ * boxes follow their parents' order (child order mm, pm, mp, pp); the
+ * reference orders the boxes that its level-restriction and re-meshing steps
+ * add differently, so box numbers may differ from its trees while the boxes,
+ * their links and their values agree.
+ *
+ * The user function is a callback fun(ctx, nd, npts, d_xy, d_vals, stream)
+ * that writes f at npts points: d_xy(2,npts) and d_vals(nd,npts) are device
+ * arrays; d_xy is complete when the callback is called, and the callback
+ * enqueues its work on `stream` (the tree's stream) or completes it before
+ * returning, and returns 0 (non-zero stops the call with BFGT_ERR_CALLBACK).  It replaces the reference's `external fun`,
+ * fun(nd,xy,dpars,zpars,ipars,f), which is called one point at a time; the
+ * parameters travel in ctx.
+ */
+typedef struct bfgt_tree *bfgt_tree_t;
+typedef int (*bfgt_tree_fun_t)(void *ctx, int nd, int npts, const double *d_xy, double *d_vals, void *stream);
+
+/* Build a tree resolving fun on [-boxlen/2, boxlen/2]^2 and keep fun's values
+ * on its leaf nodes as the tree's values.
+ * Replaces: vol_tree_mem2 + vol_tree_build2 (common/tree_ext.f:996, :1363;
+ * refinement rule vol_tree_find_box_refine, fun_err,
+ * common/tree_vol_coeffs_nd.f:689-943; level restriction vol_tree_fix_lr,
+ * :1240).  A box at level l is split when the tail of its tensor-product
+ * coefficients (iptype 0/1/2: max, l1, l2 norm over the tail mask of
+ * tens_prod_get_rmask, common/polytens.f:1579) times (boxsize(l)/2)**eta over
+ * the mask norm exceeds eps * rsc * rintl(l) (rsc = 1, 1/boxlen**2,
+ * 1/boxlen for iptype 0, 1, 2).  With rintl_in = NULL the norms rintl(l) are
+ * computed while the tree grows, as vol_tree_mem2 does; otherwise
+ * rintl_in(0..max_levels) are used, as vol_tree_build2 does.  rintl_out
+ * (max_levels+1 entries, may be NULL) receives the norms used, the last
+ * computed one repeated past the deepest level.  The tree is then made
+ * level-restricted (2:1 across edges and corners, periodic wrap for
+ * iperiod = 1), fun sampled on the boxes this adds, and the colleague lists
+ * computed.  stream = NULL uses a stream of the tree's own. */
+int bfgt_tree_build(bfgt_tree_t *out, int nd, int ipoly, int iperiod, double eps, double boxlen, int norder,
+                    int iptype, double eta, bfgt_tree_fun_t fun, void *ctx, const double *rintl_in,
+                    int max_levels, int max_boxes, double *rintl_out, void *stream);
+
+/* Re-mesh the tree for a new fun and carry the tree's values along.
+ * Replaces: vol_tree_adap_fv (common/tree_ext.f:2337) with the tree's ipoly,
+ * iptype and iperiod: leaves whose fun is not resolved (rule as above with
+ * eta = 1 and eps * rsc * rintl(l)) are split, down to max_levels, and the
+ * values interpolated onto the children; boxes whose four children are
+ * leaves are merged (from the finest level up) when fun and the values,
+ * interpolated from the children onto the parent's nodes, both have an l2
+ * tail at most eps * rsc (vol_tree_coarsen_f2, common/tree_ext.f:2062); then
+ * the tree is level-restricted with the values interpolated onto the added
+ * boxes (vol_tree_fix_lr_interp, common/tree_vol_coeffs_nd.f:2571).
+ * rintl holds max_levels+1 norms.  norder must be even.  On an error other
+ * than BFGT_ERR_ARG the tree is left in an unspecified state. */
+int bfgt_tree_adapt(bfgt_tree_t tree, double eps, const double *rintl, bfgt_tree_fun_t fun, void *ctx,
+                    int max_levels);
+
+/* Tree facts: ivals[0..9] = {nboxes, nlevels, ltree (= 2*(nlevels+1) +
+ * 17*nboxes), n_leaves, max_boxes, nd, norder, ipoly, iperiod, iptype};
+ * *d_vals (may be NULL) receives the device address of vals(nd,npbox,nboxes),
+ * valid until the next build or adapt call on the tree. */
+int bfgt_tree_query(bfgt_tree_t tree, int *ivals, double **d_vals);
+
+/* Copy the tree to host memory in the reference layout: itree(ltree) with
+ * laddr, ilevel, iparent, nchild, ichild, ncoll, coll in that order, iptr(8),
+ * centers(2,nboxes), boxsize(0:nlevels), and (when vals is not NULL)
+ * vals(nd,npbox,nboxes). */
+int bfgt_tree_export(bfgt_tree_t tree, int *itree, int *iptr, double *centers, double *boxsize, double *vals);
+
+/* A resident FGT plan on the tree as it is now (bfgt_plan_create on its
+ * arrays); execute it with the tree's values from bfgt_tree_query. */
+int bfgt_tree_plan_create(bfgt_plan_t *out, bfgt_tree_t tree, double delta, double eps, int ifpgh, int precision,
+                          void *stream);
+
+int bfgt_tree_destroy(bfgt_tree_t tree);
 
 const char *bfgt_last_error(void);
 int bfgt_device_count(void);

@@ -74,6 +74,7 @@ def load_oracle() -> ctypes.CDLL:
                                        ctypes.c_double, _dp]
         lib.oracle_pw2pgh.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, ctypes.c_double, ctypes.c_double,
                                       _dp, _dp, _dp]
+        lib.oracle_tree_fun_eval.restype = None
         lib.oracle_loctab_level.argtypes = [ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                             ctypes.c_double, _dp, _dp, _dp, _ip]
         _oracle = lib
@@ -304,3 +305,158 @@ def loctab_level(eps, ipoly, n, delta, boxdim):
     lib.oracle_loctab_level(eps, ipoly, n, delta, boxdim, *[_p(a) for a in t], _p(ind, _ip))
     # stored column-major per offset: element (m, j) at m + n*j, i.e. C array [k][j][m]
     return t + [ind]
+
+
+# -- the reference's tree builder and adapter (common/tree_ext.f) ------------
+
+def tree_fun_dpars(bumps, t=0.0):
+    """dpars of the test function in tree_funcs.c: bumps = [(a, cx, cy, vx, vy, s), ...]."""
+    return np.array([float(t), float(len(bumps))] + [float(v) for b in bumps for v in b], dtype=np.float64)
+
+
+def tree_fun_eval(nd, xy, dpars):
+    """The test function of tree_funcs.c at points xy (npts, 2): (npts, nd)."""
+    lib = load_oracle()
+    xy = np.ascontiguousarray(xy, dtype=np.float64)
+    out = np.zeros((xy.shape[0], nd))
+    lib.oracle_tree_fun_eval(ctypes.c_int(nd), ctypes.c_int(xy.shape[0]), _p(xy),
+                             _p(np.ascontiguousarray(dpars, dtype=np.float64)), _p(out))
+    return out
+
+
+@dataclass
+class RefTree:
+    """A tree as the reference's builders leave it: itree laid out for
+    (mlevels, mboxes) capacities (iptr gives the offsets), the first nboxes
+    boxes and nlevels+1 levels in use."""
+    nboxes: int
+    nlevels: int
+    itree: np.ndarray
+    iptr: np.ndarray
+    centers: np.ndarray   # (nboxes, 2)
+    boxsize: np.ndarray   # (nlevels + 1,)
+    vals: np.ndarray      # (nboxes, npbox, nd)
+
+    def _a(self, k, width=1):
+        o = int(self.iptr[k]) - 1
+        a = self.itree[o : o + width * self.nboxes]
+        return a.reshape(self.nboxes, width) if width > 1 else a
+
+    @property
+    def laddr(self):
+        return self.itree[: 2 * (self.nlevels + 1)].reshape(-1, 2)
+
+    @property
+    def ilevel(self):
+        return self._a(1)
+
+    @property
+    def iparent(self):
+        return self._a(2)
+
+    @property
+    def nchild(self):
+        return self._a(3)
+
+    @property
+    def ichild(self):
+        return self._a(4, 4)
+
+    @property
+    def ncoll(self):
+        return self._a(5)
+
+    @property
+    def coll(self):
+        return self._a(6, 9)
+
+
+def _tree_fun_ptr():
+    lib = load_oracle()
+    return ctypes.c_void_p(ctypes.cast(lib.oracle_tree_fun, ctypes.c_void_p).value)
+
+
+def reference_tree_mem(nd, ipoly, iperiod, eps, boxlen, norder, iptype, eta, dpars, ifnewtree=1):
+    """vol_tree_mem2 (common/tree_ext.f:996): (mboxes, mlevels, ltree, rintl(0:200))."""
+    lib = load_reference()
+    c_int, c_dbl = ctypes.c_int, ctypes.c_double
+    zk = np.array([1.0, 0.0])
+    zpars = np.zeros(2)
+    ipars = np.zeros(1, dtype=np.int32)
+    dp = np.ascontiguousarray(dpars, dtype=np.float64).copy()
+    mboxes, mlevels, ltree = c_int(0), c_int(0), c_int(0)
+    rintl = np.zeros(201)
+    args = [c_int(2), c_int(ipoly), c_int(iperiod), c_dbl(eps)]
+    refs = [ctypes.byref(a) for a in args] + [_p(zk)]
+    a2 = [c_dbl(boxlen), c_int(norder), c_int(iptype), c_dbl(eta)]
+    refs += [ctypes.byref(a) for a in a2] + [_tree_fun_ptr()]
+    ndc, ifn = c_int(nd), c_int(ifnewtree)
+    refs += [ctypes.byref(ndc), _p(dp), _p(zpars), _p(ipars, _ip), ctypes.byref(ifn), ctypes.byref(mboxes),
+             ctypes.byref(mlevels), ctypes.byref(ltree), _p(rintl)]
+    lib.vol_tree_mem2_(*refs)
+    return mboxes.value, mlevels.value, ltree.value, rintl
+
+
+def reference_tree_build(nd, ipoly, iperiod, eps, boxlen, norder, iptype, eta, dpars, rintl, mboxes, mlevels,
+                         ltree) -> RefTree:
+    """vol_tree_build2 (common/tree_ext.f:1363) with capacities from vol_tree_mem2."""
+    lib = load_reference()
+    c_int, c_dbl = ctypes.c_int, ctypes.c_double
+    npbox = norder * norder
+    zk = np.array([1.0, 0.0])
+    zpars = np.zeros(2)
+    ipars = np.zeros(1, dtype=np.int32)
+    dp = np.ascontiguousarray(dpars, dtype=np.float64).copy()
+    rl = np.zeros(max(mlevels + 1, 201))
+    rl[: len(rintl)] = rintl[: len(rl)]
+    itree = np.zeros(ltree, dtype=np.int32)
+    iptr = np.zeros(8, dtype=np.int32)
+    centers = np.zeros((mboxes, 2))
+    boxsize = np.zeros(mlevels + 1)
+    fvals = np.zeros((mboxes, npbox, nd))
+    nboxes, nlevels = c_int(0), c_int(0)
+    args = [c_int(2), c_int(ipoly), c_int(iperiod), c_dbl(eps)]
+    refs = [ctypes.byref(a) for a in args] + [_p(zk)]
+    a2 = [c_dbl(boxlen), c_int(norder), c_int(iptype), c_dbl(eta)]
+    refs += [ctypes.byref(a) for a in a2] + [_tree_fun_ptr()]
+    ndc, mb, ml, lt = c_int(nd), c_int(mboxes), c_int(mlevels), c_int(ltree)
+    refs += [ctypes.byref(ndc), _p(dp), _p(zpars), _p(ipars, _ip), _p(rl), ctypes.byref(nboxes), ctypes.byref(mb),
+             ctypes.byref(nlevels), ctypes.byref(ml), ctypes.byref(lt), _p(itree, _ip), _p(iptr, _ip), _p(centers),
+             _p(boxsize), _p(fvals)]
+    lib.vol_tree_build2_(*refs)
+    nb, nl = nboxes.value, nlevels.value
+    return RefTree(nb, nl, itree, iptr, centers[:nb].copy(), boxsize[: nl + 1].copy(), fvals[:nb].copy())
+
+
+def reference_tree_adapt(tree: RefTree, uvals, nd, ipoly, iperiod, eps, norder, iptype, rintl, dpars, mboxes,
+                         mlevels, ltree) -> RefTree:
+    """vol_tree_adap_fv (common/tree_ext.f:2337) on copies of a tree built by
+    reference_tree_build with the same capacities; returns the adapted tree
+    with uvals interpolated onto it (in RefTree.vals)."""
+    lib = load_reference()
+    c_int, c_dbl = ctypes.c_int, ctypes.c_double
+    npbox = norder * norder
+    zpars = np.zeros(2)
+    ipars = np.zeros(1, dtype=np.int32)
+    dp = np.ascontiguousarray(dpars, dtype=np.float64).copy()
+    rl = np.zeros(max(mlevels + 1, 201))
+    rl[: len(rintl)] = rintl[: len(rl)]
+    itree = tree.itree.copy()
+    iptr = tree.iptr.copy()
+    centers = np.zeros((mboxes, 2))
+    centers[: tree.nboxes] = tree.centers
+    boxsize = np.zeros(mlevels + 1)
+    boxsize[: tree.nlevels + 1] = tree.boxsize
+    uv = np.zeros((mboxes, npbox, nd))
+    uv[: tree.nboxes] = uvals
+    nboxes, nlevels = c_int(tree.nboxes), c_int(tree.nlevels)
+    args = [c_int(nd), c_int(2), c_dbl(eps), c_int(ipoly), c_int(iptype), c_int(norder), c_int(npbox)]
+    refs = [ctypes.byref(a) for a in args]
+    mb, ml, lt = c_int(mboxes), c_int(mlevels), c_int(ltree)
+    refs += [ctypes.byref(nboxes), ctypes.byref(mb), ctypes.byref(nlevels), ctypes.byref(ml), ctypes.byref(lt),
+             _p(itree, _ip), _p(iptr, _ip), _p(centers), _p(boxsize), _p(rl), _p(dp), _p(zpars), _p(ipars, _ip)]
+    ip = c_int(iperiod)
+    refs += [ctypes.byref(ip), _tree_fun_ptr(), _p(uv)]
+    lib.vol_tree_adap_fv_(*refs)
+    nb, nl = nboxes.value, nlevels.value
+    return RefTree(nb, nl, itree, iptr, centers[:nb].copy(), boxsize[: nl + 1].copy(), uv[:nb].copy())
