@@ -87,6 +87,83 @@ def measured_traffic(workload, precision, kernel):
     return None
 
 
+TREE_WORKLOAD = ("64 moving Gaussians a*exp(-|x-c(t)|^2/1e-5) (seed 7), n=16, eps=1e-9, periodic, l2 tail rule: "
+                 "tree build at t=0, then one re-mesh with the values carried after the Gaussians move to t=0.002")
+
+
+def tree_bumps():
+    """(a, cx, cy, vx, vy, s) of the tree workload's Gaussians."""
+    rng = np.random.default_rng(7)
+    return [(1.0 + rng.random(), rng.uniform(-.45, .45), rng.uniform(-.45, .45), rng.uniform(-1, 1),
+             rng.uniform(-1, 1), 1e-5) for _ in range(64)]
+
+
+def tree_fun_gpu(bumps, t):
+    import torch
+
+    col = lambda j: torch.tensor([b[j] for b in bumps], dtype=torch.float64, device="cuda")  # noqa: E731
+    a, cx, cy, vx, vy, s = (col(j) for j in range(6))
+    cx, cy = cx + vx * t, cy + vy * t
+
+    def f(xy):
+        dx = xy[:, 0:1] - cx[None, :]
+        dy = xy[:, 1:2] - cy[None, :]
+        return (a[None, :] * torch.exp(-(dx * dx + dy * dy) / s[None, :])).sum(dim=1)
+
+    return f
+
+
+def tree_leg():
+    """The adaptive tree on the GPU (bfgt_tree_build + bfgt_tree_adapt) with
+    the user function evaluated on the GPU: best of 3 after a warm-up."""
+    import torch
+
+    from heat2d_per_box_amd.devtree import DeviceTree
+
+    b = tree_bumps()
+    f0, f1 = tree_fun_gpu(b, 0.0), tree_fun_gpu(b, 0.002)
+    builds, adapts, sizes = [], [], None
+    for rep in range(4):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        D = DeviceTree.build(f0, 1, 16, 1e-9, 0, 1, max_levels=20, max_boxes=1 << 17)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        nb0 = D.info["nboxes"]
+        D.adapt(f1, 1e-9, max_levels=20)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        sizes = (nb0, D.info["nboxes"], D.info["nleaves"])
+        D.close()
+        if rep:
+            builds.append(t1 - t0)
+            adapts.append(t2 - t1)
+    return {"workload": TREE_WORKLOAD, "boxes_built": sizes[0], "boxes_after_remesh": sizes[1],
+            "leaves_after_remesh": sizes[2], "gpu_build_s": min(builds), "gpu_remesh_s": min(adapts),
+            "user_function": "torch on the GPU (the reference calls its Fortran function per point on the host)"}
+
+
+def tree_reference(nthreads):
+    """The reference's vol_tree_mem2 + vol_tree_build2 and vol_tree_adap_fv on
+    the tree workload (one call each, C test function per point)."""
+    from oracle import oracle as O
+
+    if not O.reference_available():
+        return None
+    O.set_reference_threads(nthreads)
+    b = tree_bumps()
+    dp0, dp1 = O.tree_fun_dpars(b, 0.0), O.tree_fun_dpars(b, 0.002)
+    t0 = time.perf_counter()
+    mb, ml, lt, rintl = O.reference_tree_mem(1, 0, 1, 1e-9, 1.0, 16, 2, 1.0, dp0)
+    caps = (2 * mb, ml + 5, 2 * lt)
+    T = O.reference_tree_build(1, 0, 1, 1e-9, 1.0, 16, 2, 1.0, dp0, rintl, *caps)
+    t1 = time.perf_counter()
+    A = O.reference_tree_adapt(T, T.vals.copy(), 1, 0, 1, 1e-9, 16, 2, rintl, dp1, *caps)
+    t2 = time.perf_counter()
+    return {"build_s": t1 - t0, "remesh_s": t2 - t1, "boxes_built": T.nboxes, "boxes_after_remesh": A.nboxes,
+            "threads": nthreads}
+
+
 def cpu_baseline(w, gpu_out, max_threads, extra=None):
     """Time the reference Fortran boxfgt (or the C oracle) on the host cores, and
     report the relative L2 difference (leaf points) of the GPU outputs from it.
@@ -134,6 +211,7 @@ def cpu_baseline(w, gpu_out, max_threads, extra=None):
 
     err = rel(gpu_out)
     err_extra = {k: rel(v) for k, v in (extra or {}).items()}
+    tree_ref = tree_reference(nthreads) if kind == "reference" else None
     return {
         "value": w.npts * w.nd / tmed,
         "unit": "leaf pts/s",
@@ -153,6 +231,7 @@ def cpu_baseline(w, gpu_out, max_threads, extra=None):
                          "the part a kept GPU plan runs (compare with ms_per_step)",
         "gpu_vs_cpu_rel_err": err,
         **{f"gpu_{k}_vs_cpu_rel_err": v for k, v in err_extra.items()},
+        "tree_remesh": tree_ref,
     }
 
 
@@ -170,6 +249,7 @@ def main():
     ap.add_argument("--densities", type=int, default=None,
                     help="keep the first K densities of the workload (e.g. c3 with one density)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-tree", action="store_true", help="skip timing the adaptive tree build and re-mesh")
     ap.add_argument("--no-split", action="store_true", help="skip timing the other precision")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -440,6 +520,9 @@ def main():
         "stage_roofline": stages_roof,
         "adjoint": adj,
     }
+    if not args.no_tree:
+        print("[bench] timing the adaptive tree", file=sys.stderr, flush=True)
+        out["tree_remesh"] = tree_leg()
     if other is not None:
         out2 = other.pop("_out")
         out[other["precision"]] = other

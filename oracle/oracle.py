@@ -376,6 +376,14 @@ def _tree_fun_ptr():
     return ctypes.c_void_p(ctypes.cast(lib.oracle_tree_fun, ctypes.c_void_p).value)
 
 
+def set_reference_threads(nthreads: int) -> None:
+    """OpenMP threads of the compiled reference (its runtime is libomp)."""
+    try:
+        ctypes.CDLL("libomp.so").omp_set_num_threads(int(nthreads))
+    except OSError:
+        pass
+
+
 def reference_tree_mem(nd, ipoly, iperiod, eps, boxlen, norder, iptype, eta, dpars, ifnewtree=1):
     """vol_tree_mem2 (common/tree_ext.f:996): (mboxes, mlevels, ltree, rintl(0:200))."""
     lib = load_reference()
