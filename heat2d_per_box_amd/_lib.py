@@ -48,6 +48,7 @@ EXPORTS = [
     "bfgt_plan_profile_end",
     "bfgt_tree_build",
     "bfgt_tree_adapt",
+    "bfgt_tree_import",
     "bfgt_tree_query",
     "bfgt_tree_export",
     "bfgt_tree_plan_create",
@@ -121,6 +122,9 @@ def lib() -> ctypes.CDLL:
     L.bfgt_tree_build.restype = _i
     L.bfgt_tree_build.argtypes = [ctypes.POINTER(_vp), _i, _i, _i, _d, _d, _i, _i, _d, TREE_FUN, _vp, _dp, _i, _i,
                                   _dp, _vp]
+    L.bfgt_tree_import.restype = _i
+    L.bfgt_tree_import.argtypes = [ctypes.POINTER(_vp), _i, _i, _i, _i, _i, _d, _i, _i, _ip, _ip, _dp, _dp, _dp, _i,
+                                   _vp]
     L.bfgt_tree_adapt.restype = _i
     L.bfgt_tree_adapt.argtypes = [_vp, _d, _dp, TREE_FUN, _vp, _i]
     L.bfgt_tree_query.restype = _i

@@ -982,7 +982,6 @@ int range_integrals(DeviceTree& t, Rng r, double rsc, std::vector<double>& out) 
 
 int tree_build(DeviceTree& t, double eps, bfgt_tree_fun_t fun, void* ctx, const double* rintl_in, int max_levels,
                double* rintl_out) {
-    if (int rc = tree_alloc(t)) return rc;
     // root
     int zi = 0, mi = -1, m4[4] = {-1, -1, -1, -1};
     double z2[2] = {0.0, 0.0};
@@ -1049,6 +1048,69 @@ int tree_build(DeviceTree& t, double eps, bfgt_tree_fun_t fun, void* ctx, const 
         for (int l = 0; l <= max_levels; l++) rintl_out[l] = rintl[l];
     if (int rc = renumber(t, false)) return rc;
     if (int rc = level_restrict(t, fun, ctx)) return rc;
+    return colleagues(t);
+}
+
+// A tree given in the reference layout (1-based links, boxes in any order
+// with the root first), with values vals(nd,npbox,nboxes) or zeros; checked
+// for consistent links, levels and cells, then renumbered.
+int tree_import(DeviceTree& t, int nboxes, int nlevels, const int* itree, const int* iptr, const double* centers,
+                const double* boxsize, const double* vals) {
+    if (nboxes < 1 || nboxes > t.cap) return set_error(BFGT_ERR_ARG, "nboxes must be in 1..max_boxes");
+    if (nlevels < 0 || nlevels > kTreeMaxLevel) return set_error(BFGT_ERR_TREE, "nlevels outside 0..28");
+    for (int l = 0; l <= nlevels; l++)
+        if (boxsize[l] != t.boxsize[l]) return set_error(BFGT_ERR_TREE, "box sizes are not boxlen / 2^level");
+    const int* ilevel = itree + iptr[1] - 1;
+    const int* iparent = itree + iptr[2] - 1;
+    const int* nchild = itree + iptr[3] - 1;
+    const int* ichild = itree + iptr[4] - 1;
+    std::vector<int> lev(nboxes), par(nboxes), nch(nboxes), ch(4 * (size_t)nboxes), cx(nboxes), cy(nboxes);
+    std::vector<double> ctr(2 * (size_t)nboxes);
+    for (int b = 0; b < nboxes; b++) {
+        const int l = ilevel[b];
+        if (l < 0 || l > nlevels) return set_error(BFGT_ERR_TREE, "box level outside 0..nlevels");
+        const double bs = t.boxsize[l];
+        const double fx = (centers[2 * b] + t.boxlen / 2) / bs - 0.5, fy = (centers[2 * b + 1] + t.boxlen / 2) / bs - 0.5;
+        const long ix = std::lround(fx), iy = std::lround(fy);
+        if (ix < 0 || iy < 0 || ix >= (1L << l) || iy >= (1L << l) || std::fabs(fx - ix) > 1e-6 ||
+            std::fabs(fy - iy) > 1e-6)
+            return set_error(BFGT_ERR_TREE, "box centre is not a cell centre of its level");
+        lev[b] = l;
+        cx[b] = (int)ix;
+        cy[b] = (int)iy;
+        ctr[2 * b] = centers[2 * b];
+        ctr[2 * b + 1] = centers[2 * b + 1];
+        par[b] = iparent[b] > 0 ? iparent[b] - 1 : -1;
+        nch[b] = nchild[b];
+        if (nch[b] != 0 && nch[b] != 4) return set_error(BFGT_ERR_TREE, "nchild must be 0 or 4");
+        for (int j = 0; j < 4; j++) ch[4 * b + j] = nch[b] ? ichild[4 * (size_t)b + j] - 1 : -1;
+    }
+    if (lev[0] != 0 || par[0] != -1) return set_error(BFGT_ERR_TREE, "box 1 must be the root");
+    for (int b = 0; b < nboxes; b++)
+        for (int j = 0; nch[b] && j < 4; j++) {
+            const int c = ch[4 * b + j];
+            if (c < 0 || c >= nboxes || par[c] != b || lev[c] != lev[b] + 1 || cx[c] != 2 * cx[b] + (j & 1) ||
+                cy[c] != 2 * cy[b] + (j >> 1))
+                return set_error(BFGT_ERR_TREE, "inconsistent child links");
+        }
+    const size_t n = (size_t)nboxes;
+    BFGT_HIP_CHECK(hipMemcpyAsync(t.A.lev, lev.data(), sizeof(int) * n, hipMemcpyHostToDevice, t.stream));
+    BFGT_HIP_CHECK(hipMemcpyAsync(t.A.par, par.data(), sizeof(int) * n, hipMemcpyHostToDevice, t.stream));
+    BFGT_HIP_CHECK(hipMemcpyAsync(t.A.nch, nch.data(), sizeof(int) * n, hipMemcpyHostToDevice, t.stream));
+    BFGT_HIP_CHECK(hipMemcpyAsync(t.A.ch, ch.data(), sizeof(int) * 4 * n, hipMemcpyHostToDevice, t.stream));
+    BFGT_HIP_CHECK(hipMemcpyAsync(t.A.cx, cx.data(), sizeof(int) * n, hipMemcpyHostToDevice, t.stream));
+    BFGT_HIP_CHECK(hipMemcpyAsync(t.A.cy, cy.data(), sizeof(int) * n, hipMemcpyHostToDevice, t.stream));
+    BFGT_HIP_CHECK(hipMemcpyAsync(t.A.ctr, ctr.data(), sizeof(double) * 2 * n, hipMemcpyHostToDevice, t.stream));
+    const size_t per = (size_t)t.npbox * t.nd;
+    if (vals)
+        BFGT_HIP_CHECK(hipMemcpyAsync(t.A.vals, vals, sizeof(double) * per * n, hipMemcpyHostToDevice, t.stream));
+    else
+        BFGT_HIP_CHECK(hipMemsetAsync(t.A.vals, 0, sizeof(double) * per * n, t.stream));
+    t.nboxes = nboxes;
+    // the host vectors above must outlive the copies
+    if (int rc = sync(t)) return rc;
+    if (int rc = renumber(t, false)) return rc;
+    if (t.nboxes != nboxes) return set_error(BFGT_ERR_TREE, "boxes not reachable from the root");
     return colleagues(t);
 }
 
@@ -1119,6 +1181,54 @@ int tree_adapt(DeviceTree& t, double eps, const double* rintl, bfgt_tree_fun_t f
 }
 
 }  // namespace
+
+struct TreeParams {
+    int nd, ipoly, iperiod, norder, iptype;
+    double eta, boxlen;
+    int max_boxes;
+    void* stream;
+};
+
+// A new tree object set up from the parameters and filled by `fill`; the
+// handle is returned only when filling succeeded.
+template <typename Fill>
+int tree_new(bfgt_tree_t* out, const TreeParams& tp, Fill fill) {
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return set_error(BFGT_ERR_NODEVICE, "no GPU visible");
+    std::unique_ptr<bfgt_tree> h;
+    int rc;
+    try {
+        h.reset(new bfgt_tree());
+        DeviceTree& t = h->t;
+        BFGT_HIP_CHECK(hipGetDevice(&t.device));
+        if (tp.stream) {
+            t.stream = (hipStream_t)tp.stream;
+        } else {
+            BFGT_HIP_CHECK(hipStreamCreateWithFlags(&t.stream, hipStreamNonBlocking));
+            t.own_stream = true;
+        }
+        t.nd = tp.nd;
+        t.n = tp.norder;
+        t.npbox = tp.norder * tp.norder;
+        t.ipoly = tp.ipoly;
+        t.iperiod = tp.iperiod ? 1 : 0;
+        t.iptype = tp.iptype;
+        t.eta = tp.eta;
+        t.boxlen = tp.boxlen;
+        t.cap = tp.max_boxes;
+        t.boxsize.assign(kTreeMaxLevel + 1, 0.0);
+        t.boxsize[0] = tp.boxlen;
+        for (int l = 1; l <= kTreeMaxLevel; l++) t.boxsize[l] = t.boxsize[l - 1] / 2;
+        t.rules = make_rules(tp.norder, tp.ipoly, tp.iptype);
+        rc = fill(t);
+    } catch (const std::exception& e) {
+        rc = set_error(BFGT_ERR_RESOURCE, std::string("tree set-up: ") + e.what());
+    }
+    if (rc) return rc;
+    *out = h.release();
+    return BFGT_OK;
+}
+
 }  // namespace bfgt
 
 // --------------------------------------------------------------------------
@@ -1142,40 +1252,30 @@ int bfgt_tree_build(bfgt_tree_t* out, int nd, int ipoly, int iperiod, double eps
     if (max_levels < 0 || max_levels > bfgt::kTreeMaxLevel)
         return set_error(BFGT_ERR_ARG, "max_levels must be in 0..28");
     if (max_boxes < 1) return set_error(BFGT_ERR_ARG, "max_boxes must be positive");
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return set_error(BFGT_ERR_NODEVICE, "no GPU visible");
-    std::unique_ptr<bfgt_tree> h;
-    int rc;
-    try {
-        h.reset(new bfgt_tree());
-        bfgt::DeviceTree& t = h->t;
-        BFGT_HIP_CHECK(hipGetDevice(&t.device));
-        if (stream) {
-            t.stream = (hipStream_t)stream;
-        } else {
-            BFGT_HIP_CHECK(hipStreamCreateWithFlags(&t.stream, hipStreamNonBlocking));
-            t.own_stream = true;
-        }
-        t.nd = nd;
-        t.n = norder;
-        t.npbox = norder * norder;
-        t.ipoly = ipoly;
-        t.iperiod = iperiod ? 1 : 0;
-        t.iptype = iptype;
-        t.eta = eta;
-        t.boxlen = boxlen;
-        t.cap = max_boxes;
-        t.boxsize.assign(bfgt::kTreeMaxLevel + 1, 0.0);
-        t.boxsize[0] = boxlen;
-        for (int l = 1; l <= bfgt::kTreeMaxLevel; l++) t.boxsize[l] = t.boxsize[l - 1] / 2;
-        t.rules = bfgt::make_rules(norder, ipoly, iptype);
-        rc = bfgt::tree_build(t, eps, fun, ctx, rintl_in, max_levels, rintl_out);
-    } catch (const std::exception& e) {
-        rc = set_error(BFGT_ERR_RESOURCE, std::string("tree set-up: ") + e.what());
-    }
-    if (rc) return rc;
-    *out = h.release();
-    return BFGT_OK;
+    bfgt::TreeParams tp{nd, ipoly, iperiod, norder, iptype, eta, boxlen, max_boxes, stream};
+    return bfgt::tree_new(out, tp, [&](bfgt::DeviceTree& t) {
+        if (int rc = bfgt::tree_alloc(t)) return rc;
+        return bfgt::tree_build(t, eps, fun, ctx, rintl_in, max_levels, rintl_out);
+    });
+}
+
+int bfgt_tree_import(bfgt_tree_t* out, int nd, int ipoly, int iperiod, int norder, int iptype, double eta,
+                     int nboxes, int nlevels, const int* itree, const int* iptr, const double* centers,
+                     const double* boxsize, const double* vals, int max_boxes, void* stream) {
+    if (!out) return set_error(BFGT_ERR_ARG, "null tree pointer");
+    *out = nullptr;
+    if (!itree || !iptr || !centers || !boxsize) return set_error(BFGT_ERR_ARG, "null tree array");
+    if (nd < 1) return set_error(BFGT_ERR_ARG, "nd must be >= 1");
+    if (ipoly != 0 && ipoly != 1) return set_error(BFGT_ERR_IPOLY, "ipoly must be 0 or 1");
+    if (norder < 2 || norder > bfgt::kMaxOrder) return set_error(BFGT_ERR_NORDER, "norder must be in 2..32");
+    if (iptype < 0 || iptype > 2) return set_error(BFGT_ERR_ARG, "iptype must be 0, 1 or 2");
+    if (!(boxsize[0] > 0)) return set_error(BFGT_ERR_TREE, "boxsize(0) must be positive");
+    if (max_boxes < nboxes) return set_error(BFGT_ERR_ARG, "max_boxes below nboxes");
+    bfgt::TreeParams tp{nd, ipoly, iperiod, norder, iptype, eta, boxsize[0], max_boxes, stream};
+    return bfgt::tree_new(out, tp, [&](bfgt::DeviceTree& t) {
+        if (int rc = bfgt::tree_alloc(t)) return rc;
+        return bfgt::tree_import(t, nboxes, nlevels, itree, iptr, centers, boxsize, vals);
+    });
 }
 
 int bfgt_tree_adapt(bfgt_tree_t tree, double eps, const double* rintl, bfgt_tree_fun_t fun, void* ctx,

@@ -107,11 +107,47 @@ class DeviceTree:
         self.rintl = rout
         return self
 
+    @classmethod
+    def from_tree(cls, tree, norder: int, vals=None, nd: int = 1, ipoly: int = 0, iptype: int = 2,
+                  eta: float = 1.0, rintl=None, max_boxes: Optional[int] = None, stream=None) -> "DeviceTree":
+        """A device tree from a :class:`heat2d_per_box_amd.tree.QuadTree` (or any
+        tree in the reference layout with ``nlevels, itree, iptr, centers,
+        boxsize, iperiod``) and values ``(nboxes, npbox, nd)``; ``rintl`` are
+        the norms :meth:`adapt` uses by default."""
+        import torch
+
+        self = object.__new__(cls)
+        self._h = ctypes.c_void_p()
+        self.stream = torch.cuda.current_stream() if stream is None else stream
+        nb = tree.centers.shape[0]
+        self.nd, self.norder, self.ipoly, self.iperiod = nd, norder, ipoly, tree.iperiod
+        self.boxlen, self.iptype, self.eta = float(tree.boxsize[0]), iptype, eta
+        itree = np.ascontiguousarray(tree.itree, dtype=np.int32)
+        iptr = np.ascontiguousarray(tree.iptr, dtype=np.int32)
+        centers = np.ascontiguousarray(tree.centers, dtype=np.float64)
+        boxsize = np.ascontiguousarray(tree.boxsize, dtype=np.float64)
+        v = None
+        if vals is not None:
+            v = np.ascontiguousarray(vals, dtype=np.float64)
+            if v.shape != (nb, norder * norder, nd):
+                raise ValueError(f"vals must have shape {(nb, norder * norder, nd)}")
+        rc = lib().bfgt_tree_import(ctypes.byref(self._h), nd, ipoly, tree.iperiod, norder, iptype, eta, nb,
+                                    tree.nlevels, itree.ctypes.data_as(_ip), iptr.ctypes.data_as(_ip),
+                                    centers.ctypes.data_as(_dp), boxsize.ctypes.data_as(_dp),
+                                    v.ctypes.data_as(_dp) if v is not None else None,
+                                    max_boxes if max_boxes is not None else 4 * nb,
+                                    ctypes.c_void_p(self.stream.cuda_stream))
+        check(rc)
+        self.rintl = None if rintl is None else np.ascontiguousarray(rintl, dtype=np.float64)
+        return self
+
     def adapt(self, fun: Callable, eps: float, rintl=None, max_levels: int = 20, host: bool = False) -> None:
         """Re-mesh for a new ``fun`` and interpolate the tree's values onto the
         new tree (``vol_tree_adap_fv``).  ``rintl`` defaults to the norms of the
         build."""
         r = self.rintl if rintl is None else np.ascontiguousarray(rintl, dtype=np.float64)
+        if r is None:
+            raise ValueError("no norms: pass rintl")
         if r.shape[0] < max_levels + 1:
             raise ValueError(f"rintl needs {max_levels + 1} entries")
         cb = _Callback(fun, host)

@@ -280,6 +280,16 @@ int bfgt_tree_build(bfgt_tree_t *out, int nd, int ipoly, int iperiod, double eps
 int bfgt_tree_adapt(bfgt_tree_t tree, double eps, const double *rintl, bfgt_tree_fun_t fun, void *ctx,
                     int max_levels);
 
+/* A device tree from a tree in the reference layout (itree/iptr as the
+ * reference's builders lay it out, boxes in any order with the root first,
+ * any capacity layout) and values vals(nd,npbox,nboxes) (NULL: zeros), with
+ * room for max_boxes boxes; for re-meshing a tree built elsewhere.  The boxes
+ * are renumbered (breadth first, children in parent order) and the colleague
+ * lists recomputed; links, levels and cells are checked (BFGT_ERR_TREE). */
+int bfgt_tree_import(bfgt_tree_t *out, int nd, int ipoly, int iperiod, int norder, int iptype, double eta,
+                     int nboxes, int nlevels, const int *itree, const int *iptr, const double *centers,
+                     const double *boxsize, const double *vals, int max_boxes, void *stream);
+
 /* Tree facts: ivals[0..9] = {nboxes, nlevels, ltree (= 2*(nlevels+1) +
  * 17*nboxes), n_leaves, max_boxes, nd, norder, ipoly, iperiod, iptype};
  * *d_vals (may be NULL) receives the device address of vals(nd,npbox,nboxes),

@@ -181,3 +181,72 @@ def test_capacity_and_callback_errors(device):
 
     with pytest.raises(Boom):
         DeviceTree.build(bad, 1, 8, 1e-6, 0, 1, max_levels=12, max_boxes=20000, host=True)
+
+
+def capacity_layout(q, vals, mboxes, mlevels):
+    """A QuadTree in the reference builders' capacity layout (laddr for
+    mlevels + 1 levels, arrays for mboxes boxes), as an oracle.RefTree."""
+    nb, nl = q.nboxes, q.nlevels
+    ltree = 2 * (mlevels + 1) + 17 * mboxes
+    itree = np.zeros(ltree, dtype=np.int32)
+    lad = 2 * (mlevels + 1)
+    iptr = np.array([1, lad + 1, lad + 1 + mboxes, lad + 1 + 2 * mboxes, lad + 1 + 3 * mboxes,
+                     lad + 1 + 7 * mboxes, lad + 1 + 8 * mboxes, lad + 1 + 17 * mboxes], dtype=np.int32)
+    itree[: 2 * (nl + 1)] = q.laddr.reshape(-1)
+    for k, arr in ((1, q.ilevel), (2, q.iparent), (3, q.nchild), (4, q.ichild.reshape(-1)), (5, q.ncoll),
+                   (6, q.coll.reshape(-1))):
+        o = iptr[k] - 1
+        itree[o : o + arr.shape[0]] = arr
+    return O.RefTree(nb, nl, itree, iptr, q.centers.copy(), q.boxsize.copy(), np.ascontiguousarray(vals))
+
+
+def test_import_round_trip(device):
+    """A tree built elsewhere (the 19,834-leaf benchmark tree, and a tree of
+    the reference's builder in its capacity layout and box order) comes back
+    from the device with the same boxes, links, colleagues and values."""
+    from heat2d_per_box_amd.densities import make_workload
+    from heat2d_per_box_amd.devtree import DeviceTree
+
+    w = make_workload("c2", "full")
+    D = DeviceTree.from_tree(w.tree, w.norder, w.fvals, nd=w.nd)
+    q, v = D.to_host()
+    assert q.nboxes == w.tree.nboxes
+    compare_trees(q, w.tree)
+    assert compare_values(q, v, w.tree, w.fvals, leaves_only=False) == 0.0
+    D.close()
+    T, rintl, caps, dp = ref_build(2, 0, 1, 1e-6, 8, 2)
+    R = as_quadtree(T, 1)
+    D = DeviceTree.from_tree(R, 8, T.vals, nd=2)
+    q, v = D.to_host()
+    compare_trees(q, R)
+    assert compare_values(q, v, R, T.vals, leaves_only=False) == 0.0
+    D.close()
+
+
+def test_adapt_benchmark_tree_matches_reference(device):
+    """Re-meshing the 19,834-leaf benchmark tree (n = 16, 26,445 boxes) for
+    two moving Gaussians, the benchmark density carried along: the device and
+    the reference's vol_tree_adap_fv (on the same tree in its capacity
+    layout) give the same tree and the carried values to rounding."""
+    from heat2d_per_box_amd.densities import make_workload
+    from heat2d_per_box_amd.devtree import DeviceTree
+
+    w = make_workload("c2", "full")
+    nd, n = w.nd, w.norder
+    bumps = [(1.0, 0.1, -0.05, 1.0, 0.5, 2e-4), (0.7, -0.3, 0.25, -0.5, 0.3, 5e-4)]
+    dp = O.tree_fun_dpars(bumps, 0.02)
+    mlev = w.tree.nlevels + 6
+    rintl = np.full(201, 0.05)
+    mboxes = 3 * w.tree.nboxes
+    R0 = capacity_layout(w.tree, w.fvals, mboxes, mlev)
+    A = O.reference_tree_adapt(R0, w.fvals, nd, 0, 1, 1e-9, n, 2, rintl, dp, mboxes, mlev, R0.itree.shape[0])
+    D = DeviceTree.from_tree(w.tree, n, w.fvals, nd=nd, max_boxes=mboxes)
+    D.adapt(host_fun(nd, dp), 1e-9, rintl=rintl[: mlev + 1], max_levels=mlev, host=True)
+    q, v = D.to_host()
+    R = as_quadtree(A, 1)
+    print(f"benchmark tree re-meshed: {w.tree.nboxes} -> reference {A.nboxes}, device {q.nboxes} boxes", flush=True)
+    compare_trees(q, R)
+    err = compare_values(q, v, R, A.vals, leaves_only=True)
+    print(f"  carried values: max rel diff {err:.2e}", flush=True)
+    assert err <= 1e-13, err
+    D.close()
