@@ -312,6 +312,30 @@ def compute_colleagues(t: QuadTree, iperiod: int) -> None:
         ncoll[B] = valid.sum(axis=1)
 
 
+def with_capacity(t: QuadTree, nbmax: int) -> QuadTree:
+    """The same tree laid out for ``nbmax`` boxes (itree and centers sized for
+    them, iptr offsets accordingly), the way callers allocate room for a tree
+    that may grow (vol_tree_mem sizes, common/tree_vol_coeffs_nd.f:542-549)."""
+    nb, nl = t.nboxes, t.nlevels
+    if nbmax < nb:
+        raise ValueError("capacity below the box count")
+    ltree = 2 * (nl + 1) + 17 * nbmax
+    itree = np.full(ltree, -1, dtype=np.int32)
+    iptr = np.zeros(8, dtype=np.int32)
+    iptr[0] = 1
+    iptr[1] = 2 * (nl + 1) + 1
+    widths = (1, 1, 1, 4, 1, 9)  # ilevel, iparent, nchild, ichild, ncoll, coll
+    for k in range(2, 8):
+        iptr[k] = iptr[k - 1] + widths[k - 2] * nbmax
+    itree[: 2 * (nl + 1)] = t.itree[: 2 * (nl + 1)]
+    for k, w in zip(range(1, 7), widths):
+        s0, d0 = int(t.iptr[k]) - 1, int(iptr[k]) - 1
+        itree[d0 : d0 + w * nb] = t.itree[s0 : s0 + w * nb]
+    centers = np.zeros((nbmax, 2))
+    centers[:nb] = t.centers[:nb]
+    return QuadTree(nl, itree, iptr, centers, t.boxsize.copy(), t.iperiod)
+
+
 def build_tree(refined: Iterable[Tuple[int, int, int]], boxlen: float = 1.0, iperiod: int = 1) -> QuadTree:
     """Tree from an explicit set of refined boxes ``(level, ix, iy)`` (root ``(0,0,0)``)."""
     by: Dict[int, list] = {}

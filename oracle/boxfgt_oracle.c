@@ -498,15 +498,23 @@ static void merge_patterns(int n, const int *ia, const int *ib, const int *ic, i
     }
 }
 
+/* half-width nloctab = 2^(mrefinelev+1) (mrefinelev+3) of the offset range
+ * of the near-field tables (bfgt/boxfgt.f:625): 6 for mrefinelev=0, 96 for
+ * the refining call's mrefinelev=3. */
+static int loctab_half(int mref) { return (1 << (mref + 1)) * (mref + 3); }
+
 /*
- * All near-field tables of one target level, mk_loctab_all with
- * mrefinelev=0 (bfgt/bfgt_volrouts.f:407-660).  Offsets k=-6..6 (index
- * k+6); tables column-major n x n per offset: tab[(k+6)*n*n + m + n*j]
- * (m = source node, j = target node); ind[(k+6)*2*(n+1) + 2*j + {0,1}].
+ * All near-field tables of one target level, mk_loctab_all
+ * (bfgt/bfgt_volrouts.f:407-660) for targets refined up to mref times.
+ * Offsets k=-nloctab..nloctab (index k+nloctab, nloctab = loctab_half);
+ * tables column-major n x n per offset: tab[(k+nloctab)*n*n + m + n*j]
+ * (m = source node, j = target node); ind[(k+nloctab)*2*(n+1) + 2*j + {0,1}].
+ * Offsets the enumeration never reaches stay zero with empty patterns.
  */
-void oracle_loctab_level(double eps, int ipoly, int n, double delta, double boxdim, double *tab,
-                         double *tabx, double *tabxx, int *ind) {
-    const int nloctab = 6, m0 = 1, nquad = 50;
+void oracle_loctab_level_m(double eps, int ipoly, int n, double delta, double boxdim, int mref, double *tab,
+                           double *tabx, double *tabxx, int *ind) {
+    const int nloctab = loctab_half(mref), m0 = mref + 1, nquad = 50;
+    const int nslot = 2 * nloctab + 1;
     double tnodes[MAXN], ws[MAXN], umat[MAXN * MAXN];
     if (ipoly == 0) oracle_legeexps(n, tnodes, ws, umat);
     else oracle_chebexps(n, tnodes, ws, umat);
@@ -520,10 +528,10 @@ void oracle_loctab_level(double eps, int ipoly, int n, double delta, double boxd
         for (int j = 0; j < n; j++) polyv[i + nquad * j] = pt[j];
     }
     size_t nn = (size_t)n * n;
-    memset(tab, 0, sizeof(double) * nn * 13);
-    memset(tabx, 0, sizeof(double) * nn * 13);
-    memset(tabxx, 0, sizeof(double) * nn * 13);
-    for (int i = 0; i < 13 * 2 * (n + 1); i += 2) { ind[i] = 0; ind[i + 1] = -1; }
+    memset(tab, 0, sizeof(double) * nn * nslot);
+    memset(tabx, 0, sizeof(double) * nn * nslot);
+    memset(tabxx, 0, sizeof(double) * nn * nslot);
+    for (int i = 0; i < nslot * 2 * (n + 1); i += 2) { ind[i] = 0; ind[i + 1] = -1; }
     double t0[MAXN * MAXN], t1[MAXN * MAXN], t2[MAXN * MAXN];
     int ia[2 * (MAXN + 1)], ib[2 * (MAXN + 1)], ic[2 * (MAXN + 1)];
     for (int i0 = m0; i0 >= -1; i0--) {
@@ -536,7 +544,7 @@ void oracle_loctab_level(double eps, int ipoly, int n, double delta, double boxd
             double scale = pow(2.0, i0);
             double cen = k / (double)(1 << m0);
             mk_loctab(ipoly, n, delta, tnodes, boxdim, cen, scale, t0, t1, t2, nquad, xn, wn, polyv);
-            double *A = tab + (k + 6) * nn, *Ax = tabx + (k + 6) * nn, *Axx = tabxx + (k + 6) * nn;
+            double *A = tab + (k + nloctab) * nn, *Ax = tabx + (k + nloctab) * nn, *Axx = tabxx + (k + nloctab) * nn;
             /* fold U: A = U^T * t (dgemm 't','n') */
             for (int j = 0; j < n; j++)
                 for (int i = 0; i < n; i++) {
@@ -553,7 +561,8 @@ void oracle_loctab_level(double eps, int ipoly, int n, double delta, double boxd
                 }
             /* mirror to the targets on the other side; done in place, in the
              * reference's loop order, so k=0 is symmetrised the same way */
-            double *B = tab + (-k + 6) * nn, *Bx = tabx + (-k + 6) * nn, *Bxx = tabxx + (-k + 6) * nn;
+            double *B = tab + (-k + nloctab) * nn, *Bx = tabx + (-k + nloctab) * nn,
+                   *Bxx = tabxx + (-k + nloctab) * nn;
             for (int j = 0; j < n; j++)
                 for (int m = 0; m < n; m++) {
                     B[m + n * j] = A[(n - 1 - m) + n * (n - 1 - j)];
@@ -562,7 +571,7 @@ void oracle_loctab_level(double eps, int ipoly, int n, double delta, double boxd
                 }
             int kk[2] = {-k, k};
             for (int s = 0; s < 2; s++) {
-                int q = kk[s] + 6;
+                int q = kk[s] + nloctab;
                 sparse_pattern(eps, n, tab + q * nn, ia);
                 sparse_pattern(eps, n, tabx + q * nn, ib);
                 sparse_pattern(eps, n, tabxx + q * nn, ic);
@@ -571,6 +580,12 @@ void oracle_loctab_level(double eps, int ipoly, int n, double delta, double boxd
         }
     }
     free(polyv);
+}
+
+/* the tables of the plain call (mrefinelev=0: offsets -6..6) */
+void oracle_loctab_level(double eps, int ipoly, int n, double delta, double boxdim, double *tab,
+                         double *tabx, double *tabxx, int *ind) {
+    oracle_loctab_level_m(eps, ipoly, n, delta, boxdim, 0, tab, tabx, tabxx, ind);
 }
 
 /* ------------------------------------------------------------------ */
@@ -790,10 +805,11 @@ static int pwshift_ind(int iperiod, const double *tc, const double *sc, double b
     return jx[0] + 3 * jx[1];
 }
 
-/* gnd_find_loctab_ind (bfgt/bfgt_local.f:1729-1765), mrefinelev=0:
- * truncation of (t-s)/(b_src/4) toward zero, nearest periodic image. */
-static void loctab_ind(int iperiod, const double *tc, const double *sc, double sbs, double bs0, int *ixy) {
-    double bs = sbs / 4;
+/* gnd_find_loctab_ind (bfgt/bfgt_local.f:1729-1765): truncation of
+ * (t-s)/(b_src/2^(2+mrefinelev)) toward zero, nearest periodic image. */
+static void loctab_ind(int iperiod, const double *tc, const double *sc, double sbs, double bs0, int mref,
+                       int *ixy) {
+    double bs = sbs / (double)(1 << (2 + mref));
     for (int i = 0; i < 2; i++) {
         double dx = (tc[i] - sc[i]) / bs;
         if (iperiod == 1) {
@@ -898,9 +914,9 @@ static void pw2pgh(int nd, int n, int npw, const cplx *pw, const cplx *tp, const
  * table offsets (ix,iy) of one target level. */
 static void tens_prod_to_pghloc(int nd, int n, const double *f, int ifpgh, double *pot, double *grad,
                                 double *hess, const double *tab, const double *tabx, const double *tabxx,
-                                const int *ind, const int *ixy) {
+                                const int *ind, int nloctab, const int *ixy) {
     size_t nn = (size_t)n * n;
-    int qx = ixy[0] + 6, qy = ixy[1] + 6;
+    int qx = ixy[0] + nloctab, qy = ixy[1] + nloctab;
     const int *ix = ind + qx * 2 * (n + 1), *iy = ind + qy * 2 * (n + 1);
     const double *Tx = tab + qx * nn, *Txd = tabx + qx * nn, *Txdd = tabxx + qx * nn;
     const double *Ty = tab + qy * nn, *Tyd = tabx + qy * nn, *Tydd = tabxx + qy * nn;
@@ -957,20 +973,48 @@ static void tens_prod_to_pghloc(int nd, int n, const double *f, int ifpgh, doubl
 /* driver                                                               */
 /* ------------------------------------------------------------------ */
 
+/* state handed from the transform to the refinement steps */
+typedef struct newtree_in {
+    int nd, n, ipoly, ifpgh, iperiod, npw, npwlevel, nlevstart, mref, nbmax;
+    int *nlevels;
+    tree_t *T;
+    double *centers;
+    const double *boxsize, *fvals;
+    double *pot, *grad, *hess;
+    const double *xq, *wq, *umat;
+    double eps;
+    cplx *exps;
+    const int64_t *aloc;
+    size_t blk;
+    const cplx *wms, *tp, *tx, *txx;
+    const int *ifpwexp, *nlist1, *list1;
+    int mnl;
+    const double *tab, *tabx, *tabxx;
+    const int *ind;
+    double *timeinfo;
+    int *info;
+} newtree_in;
+static int newtree_steps(const newtree_in *S);
+
 /*
- * oracle_boxfgt: boxfgt + bfgtmain (bfgt/boxfgt.f:5-286, 291-1115,
- * 1494-1591) for ndim=2, ifnewtree=0, ifpghtarg=0.  Same side effects as
- * the reference: itree's colleague lists are recomputed, pot/grad/hess are
+ * oracle_boxfgt_nt: boxfgt + bfgtmain (bfgt/boxfgt.f:5-286, 291-1115,
+ * 1172-1483) for ndim=2 and ifpghtarg=0.  Same side effects as the
+ * reference: itree's colleague lists are recomputed, pot/grad/hess are
  * accumulated (only leaf boxes are written), timeinfo(1..20) is zeroed and
- * slots 1-7 are filled.  info (may be NULL) receives
- * {npwlevel, npw, nlevstart, nlevend, nleafbox, ndirect_pairs}.
- * Returns 0, or a nonzero code for unsupported requests.
+ * slots 1-7 (9-11 with ifnewtree=1) are filled.  With ifnewtree=1 the tree
+ * (itree, *nlevels, centers) and pot/grad/hess are then refined, coarsened
+ * and 2:1-rebalanced to resolve pot (STEPS 8-10); nboxes is the capacity of
+ * the box arrays.  info (may be NULL) receives {npwlevel, npw, nlevstart,
+ * nlevend, nleafbox, ndirect_pairs, boxes created by STEP 8, boxes whose
+ * refinement stopped at the finest level}.  Returns 0, or a nonzero code
+ * for unsupported requests (7: the refined tree outgrows nboxes).
  */
-int oracle_boxfgt(int nd, double delta, double eps, int ipoly, int iperiod, int norder, int npbox,
-                  int nboxes, int nlevels, int *itree, const int *iptr, const double *centers,
-                  const double *boxsize, const double *fvals, int ifpgh, double *pot, double *grad,
-                  double *hess, double *timeinfo, int *info) {
-    (void)nboxes;
+int oracle_boxfgt_nt(int nd, double delta, double eps, int ipoly, int iperiod, int norder, int npbox,
+                     int nboxes, int *nlevels_io, int *itree, const int *iptr, double *centers_io,
+                     const double *boxsize, const double *fvals, int ifpgh, double *pot, double *grad,
+                     double *hess, int ifnewtree, double *timeinfo, int *info) {
+    const int nlevels = *nlevels_io;
+    const double *centers = centers_io;
     if (norder < 1 || norder > MAXN || npbox != norder * norder) return 3;
     if (ipoly != 0 && ipoly != 1) return 4;
     if (ifpgh < 1 || ifpgh > 3) return 5;
@@ -1032,45 +1076,56 @@ int oracle_boxfgt(int nd, double delta, double eps, int ipoly, int iperiod, int 
     int *list1 = (int *)malloc(sizeof(int) * nb * mnl);
     modified_list1(&T, npwlevel, ifpwexp, iperiod, mnl, nlist1, list1);
 
+    /* near-field tables: with ifnewtree=1 every target may be refined
+     * mrefinelev=3 times (bfgt/boxfgt.f:619-636) */
+    const int mref = ifnewtree == 1 ? 3 : 0;
+    const int nloc = loctab_half(mref), nslot = 2 * nloc + 1;
     size_t nn = (size_t)n * n;
-    double *tab = (double *)malloc(sizeof(double) * nn * 13 * (nlevels + 1));
-    double *tabx = (double *)malloc(sizeof(double) * nn * 13 * (nlevels + 1));
-    double *tabxx = (double *)malloc(sizeof(double) * nn * 13 * (nlevels + 1));
-    int *ind = (int *)malloc(sizeof(int) * 2 * (n + 1) * 13 * (nlevels + 1));
+    double *tab = (double *)malloc(sizeof(double) * nn * nslot * (nlevels + 1));
+    double *tabx = (double *)malloc(sizeof(double) * nn * nslot * (nlevels + 1));
+    double *tabxx = (double *)malloc(sizeof(double) * nn * nslot * (nlevels + 1));
+    int *ind = (int *)malloc(sizeof(int) * 2 * (n + 1) * nslot * (nlevels + 1));
     for (int l = 0; l <= nlevels; l++)
-        oracle_loctab_level(eps, ipoly, n, delta, boxsize[l], tab + nn * 13 * l, tabx + nn * 13 * l,
-                            tabxx + nn * 13 * l, ind + 2 * (n + 1) * 13 * l);
+        oracle_loctab_level_m(eps, ipoly, n, delta, boxsize[l], mref, tab + nn * nslot * l,
+                              tabx + nn * nslot * l, tabxx + nn * nslot * l, ind + 2 * (n + 1) * nslot * l);
 
     cplx *exps = NULL;
     double t2;
+    /* kept for the refinement step (STEP 8 shifts and evaluates the local
+     * expansions of refined leaves) */
+    cplx *tpoly = NULL, *tp = NULL, *tx = NULL, *txx = NULL, *wms = NULL, *wsh = NULL;
+    int64_t *amp = NULL, *aloc = NULL;
+    int *nlistpw = NULL, *listpw = NULL;
+    int has_pw = 0;
     if (!(npwlevel >= nlevels && npwlevel > 0)) {
+        has_pw = 1;
         int nlevstart = npwlevel > 0 ? npwlevel : 0;
         int H = (npw + 1) / 2, nexp = npw * H;
-        cplx *tpoly = (cplx *)malloc(sizeof(cplx) * n * npw * (nlevels + 1));
-        cplx *tp = (cplx *)malloc(sizeof(cplx) * n * npw * (nlevels + 1));
-        cplx *tx = (cplx *)malloc(sizeof(cplx) * n * npw * (nlevels + 1));
-        cplx *txx = (cplx *)malloc(sizeof(cplx) * n * npw * (nlevels + 1));
+        tpoly = (cplx *)malloc(sizeof(cplx) * n * npw * (nlevels + 1));
+        tp = (cplx *)malloc(sizeof(cplx) * n * npw * (nlevels + 1));
+        tx = (cplx *)malloc(sizeof(cplx) * n * npw * (nlevels + 1));
+        txx = (cplx *)malloc(sizeof(cplx) * n * npw * (nlevels + 1));
         for (int l = nlevstart; l <= nlevels; l++) {
             oracle_poly2pw(n, ipoly, npw, ws, ts, delta0, boxsize[l], tpoly + (size_t)n * npw * l);
             oracle_pw2pgh(n, npw, ts, xq, delta0, boxsize[l], tp + (size_t)n * npw * l,
                           tx + (size_t)n * npw * l, txx + (size_t)n * npw * l);
         }
         int nmax = nlevels - (npwlevel > 0 ? npwlevel : 0);
-        cplx *wms = (cplx *)malloc(sizeof(cplx) * nexp * 4 * (nmax > 0 ? nmax : 1));
+        wms = (cplx *)malloc(sizeof(cplx) * nexp * 4 * (nmax > 0 ? nmax : 1));
         mk_merge_split(boxsize[nlevels] / sqrt(delta0) / 2, npw, ts, nmax, wms);
-        cplx *wsh = (cplx *)malloc(sizeof(cplx) * nexp * 9);
+        wsh = (cplx *)malloc(sizeof(cplx) * nexp * 9);
         mk_translation(boxsize[nlevstart] / sqrt(delta0), npw, ts, wsh);
         double xmin = boxsize[nlevstart];
-        int *nlistpw = (int *)malloc(sizeof(int) * nb);
-        int *listpw = (int *)malloc(sizeof(int) * nb * 8);
+        nlistpw = (int *)malloc(sizeof(int) * nb);
+        listpw = (int *)malloc(sizeof(int) * nb * 8);
         compute_listpw(&T, npwlevel, nlistpw, listpw);
 
         /* expansion storage (gnd_mpalloc, boxfgt.f:1598-1702): mp for
          * every box at levels >= nlevstart, loc separately at nlevstart,
          * loc aliased to mp below it */
         size_t blk = (size_t)nd * nexp;
-        int64_t *amp = (int64_t *)malloc(sizeof(int64_t) * nb);
-        int64_t *aloc = (int64_t *)malloc(sizeof(int64_t) * nb);
+        amp = (int64_t *)malloc(sizeof(int64_t) * nb);
+        aloc = (int64_t *)malloc(sizeof(int64_t) * nb);
         int64_t off = 0;
         for (int l = nlevstart; l <= nlevels; l++)
             for (int b = LFIRST(&T, l); b <= LLAST(&T, l); b++) amp[b] = off, off += blk;
@@ -1155,8 +1210,6 @@ int oracle_boxfgt(int nd, double delta, double eps, int ipoly, int iperiod, int 
                            grad + (size_t)b * npbox * nd * 2, hess + (size_t)b * npbox * nd * 3);
         t2 = wtime();
         timeinfo[5] += t2 - t1;
-        free(tpoly); free(tp); free(tx); free(txx); free(wms); free(wsh);
-        free(nlistpw); free(listpw); free(amp); free(aloc);
         if (info) { info[2] = nlevstart; }
     } else {
         t2 = wtime();
@@ -1169,8 +1222,9 @@ int oracle_boxfgt(int nd, double delta, double eps, int ipoly, int iperiod, int 
     long npairs = 0;
     if (!(npwlevel == 0 && iperiod == 0)) {
         for (int l = 0; l <= nlevend; l++) {
-            const double *Tl = tab + nn * 13 * l, *Txl = tabx + nn * 13 * l, *Txxl = tabxx + nn * 13 * l;
-            const int *Il = ind + 2 * (n + 1) * 13 * l;
+            const double *Tl = tab + nn * nslot * l, *Txl = tabx + nn * nslot * l,
+                         *Txxl = tabxx + nn * nslot * l;
+            const int *Il = ind + 2 * (n + 1) * nslot * l;
             for (int b = LFIRST(&T, l); b <= LLAST(&T, l); b++) {
                 double *pb = pot + (size_t)b * npbox * nd;
                 double *gb = grad ? grad + (size_t)b * npbox * nd * 2 : NULL;
@@ -1182,8 +1236,8 @@ int oracle_boxfgt(int nd, double delta, double eps, int ipoly, int iperiod, int 
                     const double *fj = fvals + (size_t)jb * npbox * nd;
                     int ixy[2];
                     if (iperiod == 1 && jl == 1) {
-                        loctab_ind(0, centers + 2 * b, centers + 2 * jb, bs, bs0, ixy);
-                        tens_prod_to_pghloc(nd, n, fj, ifpgh, pb, gb, hb, Tl, Txl, Txxl, Il, ixy);
+                        loctab_ind(0, centers + 2 * b, centers + 2 * jb, bs, bs0, mref, ixy);
+                        tens_prod_to_pghloc(nd, n, fj, ifpgh, pb, gb, hb, Tl, Txl, Txxl, Il, nloc, ixy);
                         npairs++;
                         /* periodic images of a level-1 source (boxfgt.f:951-1053) */
                         int im[2], nim = 0;
@@ -1207,14 +1261,15 @@ int oracle_boxfgt(int nd, double delta, double eps, int ipoly, int iperiod, int 
                                 ct[2][1] = centers[2 * jb + 1];
                             }
                             for (int k = 0; k < (1 << nim) - 1; k++) {
-                                loctab_ind(0, centers + 2 * b, ct[k], bs, bs0, ixy);
-                                tens_prod_to_pghloc(nd, n, fj, ifpgh, pb, gb, hb, Tl, Txl, Txxl, Il, ixy);
+                                loctab_ind(0, centers + 2 * b, ct[k], bs, bs0, mref, ixy);
+                                tens_prod_to_pghloc(nd, n, fj, ifpgh, pb, gb, hb, Tl, Txl, Txxl, Il, nloc,
+                                                    ixy);
                                 npairs++;
                             }
                         }
                     } else {
-                        loctab_ind(iperiod, centers + 2 * b, centers + 2 * jb, bs, bs0, ixy);
-                        tens_prod_to_pghloc(nd, n, fj, ifpgh, pb, gb, hb, Tl, Txl, Txxl, Il, ixy);
+                        loctab_ind(iperiod, centers + 2 * b, centers + 2 * jb, bs, bs0, mref, ixy);
+                        tens_prod_to_pghloc(nd, n, fj, ifpgh, pb, gb, hb, Tl, Txl, Txxl, Il, nloc, ixy);
                         npairs++;
                     }
                 }
@@ -1229,11 +1284,631 @@ int oracle_boxfgt(int nd, double delta, double eps, int ipoly, int iperiod, int 
         info[3] = nlevend;
         info[4] = nleafbox;
         info[5] = (int)npairs;
+        info[6] = info[7] = 0;
+    }
+    int rc = 0;
+    if (ifnewtree == 1) {
+        newtree_in S = {nd, n, ipoly, ifpgh, iperiod, npw, npwlevel, has_pw ? (npwlevel > 0 ? npwlevel : 0) : -1,
+                        mref, nboxes, nlevels_io, &T, centers_io, boxsize, fvals, pot, grad, hess,
+                        xq, wq, umat, eps, exps, aloc, (size_t)nd * npw * ((npw + 1) / 2), wms, tp, tx, txx,
+                        ifpwexp, nlist1, list1, mnl, tab, tabx, tabxx, ind, timeinfo, info};
+        rc = newtree_steps(&S);
     }
     free(exps);
+    free(tpoly); free(tp); free(tx); free(txx); free(wms); free(wsh);
+    free(nlistpw); free(listpw); free(amp); free(aloc);
     free(ifpwexp); free(nlist1); free(list1);
     free(tab); free(tabx); free(tabxx); free(ind);
+    return rc;
+}
+
+/* the plain call (ifnewtree=0): the tree is only read (colleague lists
+ * rewritten) */
+int oracle_boxfgt(int nd, double delta, double eps, int ipoly, int iperiod, int norder, int npbox,
+                  int nboxes, int nlevels, int *itree, const int *iptr, const double *centers,
+                  const double *boxsize, const double *fvals, int ifpgh, double *pot, double *grad,
+                  double *hess, double *timeinfo, int *info) {
+    int nl = nlevels;
+    return oracle_boxfgt_nt(nd, delta, eps, ipoly, iperiod, norder, npbox, nboxes, &nl, itree, iptr,
+                            (double *)centers, boxsize, fvals, ifpgh, pot, grad, hess, 0, timeinfo, info);
+}
+
+/* ------------------------------------------------------------------ */
+/* ifnewtree=1: refine, coarsen and rebalance the tree to resolve pot    */
+/* (bfgtmain STEPS 8-10, bfgt/boxfgt.f:1172-1483)                        */
+/* ------------------------------------------------------------------ */
+
+
+/* ortho_trans_2d with itype=0 (common/tensor_prod_routs_nd.f:21-130): the
+ * tensor-product coefficients c[(k1 + n k2) nf + f] of nf functions per
+ * point, u[k + n j] = U(coefficient k, value j). */
+static void nt_trans(int nf, int n, const double *u, const double *v, double *c) {
+    double w[MAXN * MAXN];
+    for (int f = 0; f < nf; f++) {
+        for (int j2 = 0; j2 < n; j2++)
+            for (int k1 = 0; k1 < n; k1++) {
+                double s = 0;
+                for (int j1 = 0; j1 < n; j1++) s += u[k1 + n * j1] * v[(j1 + n * j2) * nf + f];
+                w[k1 + n * j2] = s;
+            }
+        for (int k2 = 0; k2 < n; k2++)
+            for (int k1 = 0; k1 < n; k1++) {
+                double s = 0;
+                for (int j2 = 0; j2 < n; j2++) s += u[k2 + n * j2] * w[k1 + n * j2];
+                c[(k1 + n * k2) * nf + f] = s;
+            }
+    }
+}
+
+/* ortho_eval_2d (common/tensor_prod_routs_nd.f:527-562): the expansion at
+ * m x m points, px[k + n a] = p_k(x_a), py likewise; out[(a + m b) nf + f]. */
+static void nt_eval(int nf, int n, const double *c, int m, const double *px, const double *py, double *out) {
+    double w[MAXN * MAXN];
+    for (int f = 0; f < nf; f++) {
+        for (int k2 = 0; k2 < n; k2++)
+            for (int a = 0; a < m; a++) {
+                double s = 0;
+                for (int k1 = 0; k1 < n; k1++) s += px[k1 + n * a] * c[(k1 + n * k2) * nf + f];
+                w[a + m * k2] = s;
+            }
+        for (int b = 0; b < m; b++)
+            for (int a = 0; a < m; a++) {
+                double s = 0;
+                for (int k2 = 0; k2 < n; k2++) s += py[k2 + n * b] * w[a + m * k2];
+                out[(a + m * b) * nf + f] = s;
+            }
+    }
+}
+
+/* fun_err with iptype=2 (common/tree_vol_coeffs_nd.f:855-935): the largest
+ * l2 norm over the nf functions of the masked coefficients, times rscale */
+static double nt_fun_err(int nf, int npbox, const double *c, const double *rmask, double rscale) {
+    double e = 0;
+    for (int f = 0; f < nf; f++) {
+        double s = 0;
+        for (int i = 0; i < npbox; i++) s += rmask[i] * c[i * nf + f] * c[i * nf + f];
+        s = sqrt(s);
+        if (s > e) e = s;
+    }
+    return e * rscale;
+}
+
+static double nt_dist(double d, int iperiod, double bs0) {
+    if (iperiod == 1) {
+        double dp = d + bs0, dm = d - bs0;
+        if (fabs(d) > fabs(dp)) d = dp;
+        if (fabs(d) > fabs(dm)) d = dm;
+    }
+    return fabs(d);
+}
+
+/* box arrays moved together by the reorganisations */
+typedef struct {
+    tree_t *T;
+    double *ctr, *pot, *coefs, *grad, *hess;
+    int *iflag; /* may be NULL */
+    int nf[4];  /* doubles per box of pot, coefs, grad, hess */
+    int nbmax;
+} nt_boxes;
+
+/* Renumber: new position p takes old box order[p] (0-based); parents and
+ * children are mapped through the inverse, -1 kept (vol_tree_reorg and its
+ * variants, common/tree_vol_coeffs_nd.f:1973-2157, 2384-2570, 2916-3126). */
+static void nt_permute(nt_boxes *B, const int *order, int cnt) {
+    tree_t *T = B->T;
+    int nbmax = B->nbmax;
+    int *inv = (int *)malloc(sizeof(int) * nbmax);
+    for (int i = 0; i < nbmax; i++) inv[i] = -1;
+    for (int p = 0; p < cnt; p++) inv[order[p]] = p;
+    int *il = (int *)malloc(sizeof(int) * nbmax), *ip = (int *)malloc(sizeof(int) * nbmax);
+    int *nc = (int *)malloc(sizeof(int) * nbmax), *ic = (int *)malloc(sizeof(int) * 4 * nbmax);
+    int *fl = (int *)malloc(sizeof(int) * nbmax);
+    double *c2 = (double *)malloc(sizeof(double) * 2 * nbmax);
+    memcpy(il, T->ilevel, sizeof(int) * nbmax);
+    memcpy(ip, T->iparent, sizeof(int) * nbmax);
+    memcpy(nc, T->nchild, sizeof(int) * nbmax);
+    memcpy(ic, T->ichild, sizeof(int) * 4 * nbmax);
+    memcpy(c2, B->ctr, sizeof(double) * 2 * nbmax);
+    if (B->iflag) memcpy(fl, B->iflag, sizeof(int) * nbmax);
+    for (int p = 0; p < cnt; p++) {
+        int o = order[p];
+        T->ilevel[p] = il[o];
+        T->nchild[p] = nc[o];
+        T->iparent[p] = ip[o] > 0 ? inv[ip[o] - 1] + 1 : -1;
+        for (int j = 0; j < 4; j++) T->ichild[4 * p + j] = ic[4 * o + j] > 0 ? inv[ic[4 * o + j] - 1] + 1 : -1;
+        B->ctr[2 * p] = c2[2 * o];
+        B->ctr[2 * p + 1] = c2[2 * o + 1];
+        if (B->iflag) B->iflag[p] = fl[o];
+    }
+    double *arr[4] = {B->pot, B->coefs, B->grad, B->hess};
+    for (int a = 0; a < 4; a++) {
+        if (!arr[a] || B->nf[a] == 0) continue;
+        size_t w = (size_t)B->nf[a];
+        double *cp = (double *)malloc(sizeof(double) * w * nbmax);
+        memcpy(cp, arr[a], sizeof(double) * w * nbmax);
+        for (int p = 0; p < cnt; p++) memcpy(arr[a] + w * p, cp + w * order[p], sizeof(double) * w);
+        free(cp);
+    }
+    free(inv); free(il); free(ip); free(nc); free(ic); free(fl); free(c2);
+}
+
+/* vol_tree_refine_boxes_flag_interp (common/tree_vol_coeffs_nd.f:3127-3223):
+ * the flagged boxes of [first, last] (0-based) are split in order; the
+ * children's pot comes from the parent's coefficients, grad/hess from
+ * theirs; children of flag-1 boxes become flag++ (3). */
+static int nt_refine_flagged(nt_boxes *B, int first, int last, int ilev, int *nbctr, int nd, int n,
+                             int ifpgh, const double *u, const double *p2c, const double *bsz) {
+    tree_t *T = B->T;
+    int npbox = n * n;
+    double *gbuf = (double *)malloc(sizeof(double) * 5 * npbox * nd), *hbuf = gbuf + 2 * npbox * nd;
+    for (int ib = first; ib <= last; ib++) {
+        if (B->iflag[ib] <= 0) continue;
+        if (*nbctr + 4 > B->nbmax) { free(gbuf); return 7; }
+        double *gcoef = NULL, *hcoef = NULL;
+        if (ifpgh >= 2) { nt_trans(2 * nd, n, u, B->grad + (size_t)ib * 2 * npbox * nd, gbuf); gcoef = gbuf; }
+        if (ifpgh >= 3) { nt_trans(3 * nd, n, u, B->hess + (size_t)ib * 3 * npbox * nd, hbuf); hcoef = hbuf; }
+        T->nchild[ib] = 4;
+        double bsh = bsz[ilev + 1] / 2;
+        for (int j = 0; j < 4; j++) {
+            int jb = *nbctr + j;
+            int sx = j & 1, sy = j >> 1;
+            B->ctr[2 * jb] = B->ctr[2 * ib] + (sx ? bsh : -bsh);
+            B->ctr[2 * jb + 1] = B->ctr[2 * ib + 1] + (sy ? bsh : -bsh);
+            const double *px = p2c + sx * n * n, *py = p2c + sy * n * n;
+            nt_eval(nd, n, B->coefs + (size_t)ib * npbox * nd, n, px, py, B->pot + (size_t)jb * npbox * nd);
+            nt_trans(nd, n, u, B->pot + (size_t)jb * npbox * nd, B->coefs + (size_t)jb * npbox * nd);
+            if (gcoef) nt_eval(2 * nd, n, gcoef, n, px, py, B->grad + (size_t)jb * 2 * npbox * nd);
+            if (hcoef) nt_eval(3 * nd, n, hcoef, n, px, py, B->hess + (size_t)jb * 3 * npbox * nd);
+            T->iparent[jb] = ib + 1;
+            T->nchild[jb] = 0;
+            for (int l = 0; l < 4; l++) T->ichild[4 * jb + l] = -1;
+            T->ichild[4 * ib + j] = jb + 1;
+            T->ilevel[jb] = ilev + 1;
+            if (B->iflag[ib] == 1) B->iflag[jb] = 3;
+            if (B->iflag[ib] == 2) B->iflag[jb] = 0;
+        }
+        *nbctr += 4;
+    }
+    free(gbuf);
     return 0;
+}
+
+/* vol_updateflags (common/tree_vol_coeffs_nd.f:1733-1850) on [first, last] */
+static void nt_updateflags(nt_boxes *B, int first, int last, int curlev, int iperiod, const double *bsz) {
+    tree_t *T = B->T;
+    double distest = 1.05 * (bsz[curlev] + bsz[curlev + 1]) / 2.0, bs0 = bsz[0];
+    for (int ib = first; ib <= last; ib++) {
+        if (B->iflag[ib] != 3) continue;
+        B->iflag[ib] = 0;
+        for (int i = 0; i < T->ncoll[ib] && B->iflag[ib] == 0; i++) {
+            int jb = T->coll[9 * ib + i] - 1;
+            for (int j = 0; j < 4; j++) {
+                int kb = T->ichild[4 * jb + j];
+                if (kb <= 0) continue;
+                kb -= 1;
+                if (T->nchild[kb] <= 0) continue;
+                int ict = 0;
+                for (int k = 0; k < 2; k++)
+                    if (nt_dist(B->ctr[2 * kb + k] - B->ctr[2 * ib + k], iperiod, bs0) <= distest) ict++;
+                if (ict == 2) { B->iflag[ib] = 1; break; }
+            }
+        }
+    }
+}
+
+/* vol_tree_reorg_pgh (common/tree_vol_coeffs_nd.f:2916-3126): levels 0 and
+ * 1 stay, every deeper level lists its laddr range then its tail */
+static void nt_reorg_tail(nt_boxes *B, int nlevels, const int *tail, int cnt) {
+    tree_t *T = B->T;
+    int *order = (int *)malloc(sizeof(int) * (cnt > 0 ? cnt : 1));
+    int p = 0;
+    for (int l = 0; l <= (nlevels < 1 ? nlevels : 1); l++)
+        for (int b = LFIRST(T, l); b <= LLAST(T, l); b++) order[p++] = b;
+    int *newl = (int *)malloc(sizeof(int) * 2 * (nlevels + 1));
+    for (int l = 0; l <= nlevels; l++) { newl[2 * l] = T->laddr[2 * l]; newl[2 * l + 1] = T->laddr[2 * l + 1]; }
+    for (int l = 2; l <= nlevels; l++) {
+        newl[2 * l] = p + 1;
+        for (int b = LFIRST(T, l); b <= LLAST(T, l); b++) order[p++] = b;
+        for (int b = tail[2 * l] - 1; b <= tail[2 * l + 1] - 1; b++) order[p++] = b;
+        newl[2 * l + 1] = p;
+    }
+    nt_permute(B, order, p);
+    for (int l = 0; l <= nlevels; l++) { T->laddr[2 * l] = newl[2 * l]; T->laddr[2 * l + 1] = newl[2 * l + 1]; }
+    free(order);
+    free(newl);
+}
+
+static void nt_colleagues(tree_t *T, int nlevels, int iperiod) {
+    T->nlevels = nlevels;
+    T->nboxes = T->laddr[2 * nlevels + 1];
+    oracle_computecoll(T, iperiod);
+}
+
+/* vol_tree_fix_lr_interp (common/tree_vol_coeffs_nd.f:2571-2915) */
+static int nt_fix_lr(nt_boxes *B, int nlevels, int iperiod, int nd, int n, int ifpgh, const double *u,
+                     const double *p2c, const double *bsz) {
+    tree_t *T = B->T;
+    double bs0 = bsz[0];
+    int nboxes = T->laddr[2 * nlevels + 1];
+    for (int i = 0; i < B->nbmax; i++) B->iflag[i] = 0;
+    /* flag: leaves touching a box two levels finer (via the grandparent's colleagues) */
+    for (int ilev = nlevels; ilev >= 2; ilev--) {
+        double distest = 1.05 * (bsz[ilev - 1] + bsz[ilev - 2]) / 2.0;
+        for (int ib = LFIRST(T, ilev); ib <= LLAST(T, ilev); ib++) {
+            int dad = T->iparent[ib] - 1, gd = T->iparent[dad] - 1;
+            for (int i = 0; i < T->ncoll[gd]; i++) {
+                int jb = T->coll[9 * gd + i] - 1;
+                if (T->nchild[jb] != 0 || B->iflag[jb] != 0) continue;
+                int ict = 0;
+                for (int k = 0; k < 2; k++)
+                    if (nt_dist(B->ctr[2 * jb + k] - B->ctr[2 * dad + k], iperiod, bs0) <= distest) ict++;
+                if (ict == 2) B->iflag[jb] = 1;
+            }
+        }
+    }
+    /* flag+: leaves bigger than and touching a flagged or flag+ box */
+    for (int ilev = nlevels; ilev >= 1; ilev--) {
+        double distest = 1.05 * (bsz[ilev] + bsz[ilev - 1]) / 2.0;
+        for (int ib = LFIRST(T, ilev); ib <= LLAST(T, ilev); ib++) {
+            if (B->iflag[ib] != 1 && B->iflag[ib] != 2) continue;
+            int dad = T->iparent[ib] - 1;
+            for (int i = 0; i < T->ncoll[dad]; i++) {
+                int jb = T->coll[9 * dad + i] - 1;
+                if (T->nchild[jb] != 0 || B->iflag[jb] != 0) continue;
+                int ict = 0;
+                for (int k = 0; k < 2; k++)
+                    if (nt_dist(B->ctr[2 * jb + k] - B->ctr[2 * ib + k], iperiod, bs0) <= distest) ict++;
+                if (ict == 2) B->iflag[jb] = 2;
+            }
+        }
+    }
+    int *tail = (int *)malloc(sizeof(int) * 2 * (nlevels + 1));
+    for (int l = 0; l <= nlevels; l++) { tail[2 * l] = 0; tail[2 * l + 1] = -1; }
+    int rc = 0;
+    for (int ilev = 1; ilev <= nlevels - 2 && !rc; ilev++) {
+        tail[2 * (ilev + 1)] = nboxes + 1;
+        rc = nt_refine_flagged(B, LFIRST(T, ilev), LLAST(T, ilev), ilev, &nboxes, nd, n, ifpgh, u, p2c, bsz);
+        tail[2 * (ilev + 1) + 1] = nboxes;
+    }
+    if (rc) { free(tail); return rc; }
+    nt_reorg_tail(B, nlevels, tail, nboxes);
+    nt_colleagues(T, nlevels, iperiod);
+    for (int i = 0; i < nboxes; i++)
+        if (B->iflag[i] != 3) B->iflag[i] = 0;
+    for (int l = 0; l <= nlevels; l++) { tail[2 * l] = 0; tail[2 * l + 1] = -1; }
+    for (int ilev = 2; ilev <= nlevels - 2 && !rc; ilev++) {
+        nt_updateflags(B, LFIRST(T, ilev), LLAST(T, ilev), ilev, iperiod, bsz);
+        nt_updateflags(B, tail[2 * ilev] - 1, tail[2 * ilev + 1] - 1, ilev, iperiod, bsz);
+        tail[2 * (ilev + 1)] = nboxes + 1;
+        rc = nt_refine_flagged(B, LFIRST(T, ilev), LLAST(T, ilev), ilev, &nboxes, nd, n, ifpgh, u, p2c, bsz);
+        if (!rc)
+            rc = nt_refine_flagged(B, tail[2 * ilev] - 1, tail[2 * ilev + 1] - 1, ilev, &nboxes, nd, n, ifpgh, u,
+                                   p2c, bsz);
+        tail[2 * (ilev + 1) + 1] = nboxes;
+        /* colleagues of the boxes just created: children of the parent's
+         * colleagues that touch */
+        for (int ib = tail[2 * (ilev + 1)] - 1; ib <= tail[2 * (ilev + 1) + 1] - 1; ib++) {
+            T->ncoll[ib] = 0;
+            int dad = T->iparent[ib] - 1;
+            for (int i = 0; i < T->ncoll[dad]; i++) {
+                int jb = T->coll[9 * dad + i] - 1;
+                for (int j = 0; j < 4; j++) {
+                    int kb = T->ichild[4 * jb + j];
+                    if (kb <= 0) continue;
+                    kb -= 1;
+                    int ok = 1;
+                    for (int k = 0; k < 2; k++)
+                        if (nt_dist(B->ctr[2 * kb + k] - B->ctr[2 * ib + k], iperiod, bs0) > 1.05 * bsz[ilev + 1]) {
+                            ok = 0;
+                            break;
+                        }
+                    if (ok) T->coll[9 * ib + T->ncoll[ib]++] = kb + 1;
+                }
+            }
+        }
+    }
+    if (!rc) {
+        nt_reorg_tail(B, nlevels, tail, nboxes);
+        nt_colleagues(T, nlevels, iperiod);
+    }
+    free(tail);
+    return rc;
+}
+
+static int newtree_steps(const newtree_in *S) {
+    tree_t *T = S->T;
+    const int nd = S->nd, n = S->n, npbox = n * n, nbmax = S->nbmax, ifpgh = S->ifpgh, iperiod = S->iperiod;
+    const int mnl = S->mnl, nloc = loctab_half(S->mref), nslot = 2 * nloc + 1;
+    const size_t nn = (size_t)n * n;
+    int nlevels = *S->nlevels;
+    const double *bsz = S->boxsize;
+    const double bs0 = bsz[0];
+    double *ctr = S->centers;
+    double *pot = S->pot, *grad = S->grad, *hess = S->hess;
+    const int H = (S->npw + 1) / 2, nexp = S->npw * H;
+    int rc = 0;
+
+    /* ===== STEP 8 (bfgt/boxfgt.f:1172-1412) ===== */
+    double t1 = wtime();
+    /* tens_prod_get_rmask, iptype=2 (common/polytens.f:1579-1621) */
+    double rmask[MAXN * MAXN], rsum = 0;
+    for (int k2 = 0; k2 < n; k2++)
+        for (int k1 = 0; k1 < n; k1++) {
+            int on = (k1 + 1) * (k1 + 1) + (k2 + 1) * (k2 + 1) >= n * n;
+            rmask[k1 + n * k2] = on;
+            rsum += on;
+        }
+    rsum = sqrt(rsum);
+    /* treedata_lpnorm, iptype=2 (common/tree_data_routs_nd.f:1131-1225) */
+    double rnorm = 0;
+    for (int l = 0; l <= nlevels; l++) {
+        double sc = (bsz[l] / 2) * (bsz[l] / 2);
+        for (int b = LFIRST(T, l); b <= LLAST(T, l); b++) {
+            if (T->nchild[b] != 0) continue;
+            for (int i = 0; i < npbox; i++)
+                for (int d = 0; d < nd; d++) {
+                    double v = pot[((size_t)b * npbox + i) * nd + d];
+                    rnorm += v * v * S->wq[i % n] * S->wq[i / n] * sc;
+                }
+        }
+    }
+    rnorm = sqrt(rnorm);
+    const double reps = S->eps * sqrt(1.0 / (bs0 * bs0)) * rnorm;
+    const int nboxes0 = T->nboxes;
+
+    int *ifrefine = (int *)calloc(nbmax, sizeof(int));
+    int *irefinelev = (int *)calloc(nbmax, sizeof(int));
+    int *imaxrefinelev = (int *)calloc(nbmax, sizeof(int));
+    int *iflag = (int *)calloc(nbmax, sizeof(int));
+    int *pwexp = (int *)calloc(nbmax, sizeof(int));
+    int *nl1 = (int *)calloc(nbmax, sizeof(int));
+    int *l1 = (int *)calloc((size_t)nbmax * mnl, sizeof(int));
+    double *coefsp = (double *)calloc((size_t)nbmax * npbox * nd, sizeof(double));
+    cplx **locp = (cplx **)calloc(nbmax, sizeof(cplx *));
+    cplx *locnew = S->exps ? (cplx *)calloc((size_t)(nbmax - nboxes0) * S->blk + 1, sizeof(cplx)) : NULL;
+    int *nblock = (int *)calloc(nlevels + 2, sizeof(int));
+    for (int b = 0; b < nboxes0; b++) {
+        pwexp[b] = S->ifpwexp[b];
+        nl1[b] = S->nlist1[b];
+        memcpy(l1 + (size_t)mnl * b, S->list1 + (size_t)mnl * b, sizeof(int) * mnl);
+        if (S->exps && S->nlevstart >= 0 && T->ilevel[b] >= S->nlevstart) locp[b] = S->exps + S->aloc[b];
+    }
+    /* coefficients of pot on every leaf (treedata_trans_nd) */
+    for (int b = 0; b < nboxes0; b++)
+        if (T->nchild[b] == 0) nt_trans(nd, n, S->umat, pot + (size_t)b * npbox * nd, coefsp + (size_t)b * npbox * nd);
+    /* leaves above the finest level whose pot is not resolved */
+    int maxrefinelev = 0;
+    for (int l = 0; l <= nlevels - 1; l++) {
+        double rscale = bsz[l] / 2;
+        for (int b = LFIRST(T, l); b <= LLAST(T, l); b++) {
+            if (T->nchild[b] != 0) continue;
+            double erra = nt_fun_err(nd, npbox, coefsp + (size_t)b * npbox * nd, rmask, rscale) / rsum;
+            if (erra > reps) {
+                ifrefine[b] = 1;
+                imaxrefinelev[b] = (int)ceil(log(erra / reps) / log(2.0) / n);
+            }
+            if (imaxrefinelev[b] > maxrefinelev) maxrefinelev = imaxrefinelev[b];
+        }
+    }
+    /* refine; children are appended and refined in turn when the loop
+     * reaches them.  Deviation: a box at the finest level is not split (the
+     * reference would write a level its boxsize, laddr and tables do not
+     * hold). */
+    int nnew = 0, ncapped = 0;
+    for (int ib = 0; ib < nbmax && !rc; ib++) {
+        if (!(ifrefine[ib] == 1 && irefinelev[ib] <= maxrefinelev)) continue;
+        int ilev = T->ilevel[ib];
+        if (ilev >= nlevels) {
+            ncapped++;
+            continue;
+        }
+        irefinelev[ib]++;
+        double bsh = bsz[ilev] / 4, rscale = bsh;
+        T->nchild[ib] = 4;
+        for (int j = 0; j < 4 && !rc; j++) {
+            nnew++;
+            int jb = nboxes0 + nnew - 1;
+            if (jb >= nbmax) { rc = 7; break; }
+            pwexp[jb] = pwexp[ib];
+            nl1[jb] = nl1[ib];
+            memcpy(l1 + (size_t)mnl * jb, l1 + (size_t)mnl * ib, sizeof(int) * mnl);
+            irefinelev[jb] = irefinelev[ib] + 1;
+            ctr[2 * jb] = ctr[2 * ib] + ((j & 1) ? bsh : -bsh);
+            ctr[2 * jb + 1] = ctr[2 * ib + 1] + ((j >> 1) ? bsh : -bsh);
+            T->iparent[jb] = ib + 1;
+            T->nchild[jb] = 0;
+            for (int l = 0; l < 4; l++) T->ichild[4 * jb + l] = -1;
+            T->ichild[4 * ib + j] = jb + 1;
+            int jlev = ilev + 1;
+            T->ilevel[jb] = jlev;
+            nblock[jlev]++;
+            double *pj = pot + (size_t)jb * npbox * nd;
+            double *gj = grad ? grad + (size_t)jb * npbox * nd * 2 : NULL;
+            double *hj = hess ? hess + (size_t)jb * npbox * nd * 3 : NULL;
+            if (pwexp[jb] == 1) {
+                /* the parent's local expansion shifted to the child
+                 * (gnd_shiftpw_loc), evaluated on its grid */
+                int k = msshift_ind(ctr + 2 * jb, ctr + 2 * ib);
+                int klev = nlevels - ilev;
+                const cplx *w = S->wms + (size_t)(klev - 1) * 4 * nexp + (size_t)k * nexp;
+                locp[jb] = locnew + (size_t)(jb - nboxes0) * S->blk;
+                for (int d = 0; d < nd; d++)
+                    for (int q = 0; q < nexp; q++)
+                        locp[jb][(size_t)d * nexp + q] = locp[ib][(size_t)d * nexp + q] * w[q];
+                pw2pgh(nd, n, S->npw, locp[jb], S->tp + (size_t)n * S->npw * jlev, S->tx + (size_t)n * S->npw * jlev,
+                       S->txx + (size_t)n * S->npw * jlev, ifpgh, pj, gj, hj);
+            }
+            /* the parent's near-field sources (list 1 is symmetric) with the
+             * child's level tables */
+            for (int k = 0; k < nl1[jb]; k++) {
+                int kb = l1[(size_t)mnl * jb + k];
+                int ixy[2];
+                loctab_ind(iperiod, ctr + 2 * jb, ctr + 2 * kb, bsz[T->ilevel[kb]], bs0, S->mref, ixy);
+                if (abs(ixy[0]) > nloc || abs(ixy[1]) > nloc) { rc = 8; break; }
+                tens_prod_to_pghloc(nd, n, S->fvals + (size_t)kb * npbox * nd, ifpgh, pj, gj, hj,
+                                    S->tab + nn * nslot * jlev, S->tabx + nn * nslot * jlev,
+                                    S->tabxx + nn * nslot * jlev, S->ind + 2 * (n + 1) * nslot * jlev, nloc, ixy);
+            }
+            nt_trans(nd, n, S->umat, pj, coefsp + (size_t)jb * npbox * nd);
+            double erra = nt_fun_err(nd, npbox, coefsp + (size_t)jb * npbox * nd, rmask, rscale) / rsum;
+            if (erra > reps) {
+                ifrefine[jb] = 1;
+                /* generations below the original leaf, against its estimate */
+                int kb = jb, k, kdad = -1;
+                for (k = 1; k <= maxrefinelev; k++) {
+                    kdad = T->iparent[kb] - 1;
+                    if (kdad < nboxes0) break;
+                    kb = kdad;
+                }
+                if (kdad < 0 || k > imaxrefinelev[kdad]) ifrefine[jb] = 0;
+            }
+        }
+    }
+    if (S->info) {
+        S->info[6] = nnew;
+        S->info[7] = ncapped;
+    }
+    nt_boxes B = {T, ctr, pot, coefsp, ifpgh >= 2 ? grad : NULL, ifpgh >= 3 ? hess : NULL, iflag,
+                  {npbox * nd, npbox * nd, ifpgh >= 2 ? 2 * npbox * nd : 0, ifpgh >= 3 ? 3 * npbox * nd : 0}, nbmax};
+    if (!rc && nnew > 0) {
+        /* fgt_vol_tree_reorg (common/tree_vol_coeffs_nd.f:1973-2157): per
+         * level the old boxes, then the new ones in creation order */
+        int cnt = nboxes0 + nnew;
+        int *order = (int *)malloc(sizeof(int) * cnt);
+        int p = 0;
+        order[p++] = 0;
+        for (int l = 1; l <= nlevels; l++) {
+            int first = p + 1;
+            for (int b = LFIRST(T, l); b <= LLAST(T, l); b++) order[p++] = b;
+            for (int b = nboxes0; b < cnt; b++)
+                if (T->ilevel[b] == l) order[p++] = b;
+            T->laddr[2 * l] = first;
+            T->laddr[2 * l + 1] = p;
+        }
+        nt_permute(&B, order, p);
+        free(order);
+        T->nboxes = p;
+    }
+    double t2 = wtime();
+    S->timeinfo[8] += t2 - t1;
+
+    if (!rc && nnew > 0 && !(nnew < 0.1 * nboxes0)) {
+        /* ===== STEP 9: coarsen (vol_tree_coarsen, common/tree_vol_coeffs_nd.f:2158-2383) ===== */
+        t1 = wtime();
+        int n2 = n / 2;
+        double c2p[2 * MAXN * MAXN], pv[MAXN + 2];
+        for (int i = 0; i < n2; i++) {
+            if (S->ipoly == 0) oracle_legepols(S->xq[i] * 2 + 1.0, n - 1, pv);
+            else oracle_chebpols(S->xq[i] * 2 + 1.0, n - 1, pv);
+            for (int k = 0; k < n; k++) c2p[k + n * i] = pv[k];
+            if (S->ipoly == 0) oracle_legepols(S->xq[n2 + i] * 2 - 1.0, n - 1, pv);
+            else oracle_chebpols(S->xq[n2 + i] * 2 - 1.0, n - 1, pv);
+            for (int k = 0; k < n; k++) c2p[n * n + k + n * i] = pv[k];
+        }
+        int *ifdelete = (int *)calloc(nbmax, sizeof(int));
+        int ndel = 0;
+        double *v2 = (double *)malloc(sizeof(double) * 6 * npbox * nd), *cc = v2 + 3 * npbox * nd;
+        for (int ilev = nlevels - 1; ilev >= 0; ilev--) {
+            for (int ib = LFIRST(T, ilev); ib <= LLAST(T, ilev); ib++) {
+                if (T->nchild[ib] != 4) continue;
+                int nleaf = 0;
+                for (int j = 0; j < 4; j++)
+                    if (T->nchild[T->ichild[4 * ib + j] - 1] == 0) nleaf++;
+                if (nleaf != 4) continue;
+                double *pb = pot + (size_t)ib * npbox * nd;
+                for (int j = 0; j < 4; j++) {
+                    int jb = T->ichild[4 * ib + j] - 1;
+                    const double *px = c2p + (j & 1) * n * n, *py = c2p + (j >> 1) * n * n;
+                    nt_eval(nd, n, coefsp + (size_t)jb * npbox * nd, n2, px, py, v2);
+                    int ox = (j & 1) * n2, oy = (j >> 1) * n2;
+                    for (int b = 0; b < n2; b++)
+                        for (int a = 0; a < n2; a++)
+                            for (int d = 0; d < nd; d++)
+                                pb[((ox + a) + n * (oy + b)) * nd + d] = v2[(a + n2 * b) * nd + d];
+                }
+                nt_trans(nd, n, S->umat, pb, coefsp + (size_t)ib * npbox * nd);
+                double erra = nt_fun_err(nd, npbox, coefsp + (size_t)ib * npbox * nd, rmask, 1.0) / rsum;
+                if (erra > reps) continue;
+                for (int a = 0; a < 2; a++) {
+                    int nf = (a == 0 ? 2 : 3) * nd;
+                    double *arr = a == 0 ? (ifpgh >= 2 ? grad : NULL) : (ifpgh >= 3 ? hess : NULL);
+                    if (!arr) continue;
+                    for (int j = 0; j < 4; j++) {
+                        int jb = T->ichild[4 * ib + j] - 1;
+                        const double *px = c2p + (j & 1) * n * n, *py = c2p + (j >> 1) * n * n;
+                        nt_trans(nf, n, S->umat, arr + (size_t)jb * npbox * nf, cc);
+                        nt_eval(nf, n, cc, n2, px, py, v2);
+                        int ox = (j & 1) * n2, oy = (j >> 1) * n2;
+                        for (int b = 0; b < n2; b++)
+                            for (int a2 = 0; a2 < n2; a2++)
+                                for (int f = 0; f < nf; f++)
+                                    arr[((size_t)ib * npbox + (ox + a2) + n * (oy + b)) * nf + f] =
+                                        v2[(a2 + n2 * b) * nf + f];
+                    }
+                }
+                for (int j = 0; j < 4; j++) {
+                    int jb = T->ichild[4 * ib + j] - 1;
+                    ndel++;
+                    T->iparent[jb] = -1;
+                    T->nchild[jb] = 0;
+                    for (int l = 0; l < 4; l++) T->ichild[4 * jb + l] = -1;
+                    ifdelete[jb] = 1;
+                }
+                T->nchild[ib] = 0;
+                for (int j = 0; j < 4; j++) T->ichild[4 * ib + j] = -1;
+            }
+        }
+        if (ndel > 0) {
+            /* fgt_vol_tree_reorg_after_coarsen (common/tree_vol_coeffs_nd.f:2384-2570) */
+            int cnt0 = T->laddr[2 * nlevels + 1];
+            int *order = (int *)malloc(sizeof(int) * cnt0);
+            int p = 0;
+            order[p++] = 0;
+            for (int l = 1; l <= nlevels; l++) {
+                int first = p + 1;
+                for (int b = LFIRST(T, l); b <= LLAST(T, l); b++)
+                    if (!ifdelete[b]) order[p++] = b;
+                T->laddr[2 * l] = first;
+                T->laddr[2 * l + 1] = p;
+            }
+            nt_permute(&B, order, p);
+            free(order);
+            int nnl = nlevels;
+            for (int l = nlevels; l >= 0; l--)
+                if (T->laddr[2 * l + 1] < T->laddr[2 * l]) nnl--;
+            nlevels = nnl;
+            T->nlevels = nlevels;
+            T->nboxes = T->laddr[2 * nlevels + 1];
+        }
+        free(ifdelete);
+        free(v2);
+        t2 = wtime();
+        S->timeinfo[9] += t2 - t1;
+    }
+
+    if (!rc && nnew > 0) {
+        /* ===== STEP 10: 2:1 rebalance (bfgt/boxfgt.f:1459-1483) ===== */
+        t1 = wtime();
+        nt_colleagues(T, nlevels, iperiod);
+        double p2c[2 * MAXN * MAXN], pv[MAXN + 2];
+        for (int i = 0; i < n; i++) {
+            if (S->ipoly == 0) oracle_legepols(S->xq[i] / 2 - 0.5, n - 1, pv);
+            else oracle_chebpols(S->xq[i] / 2 - 0.5, n - 1, pv);
+            for (int k = 0; k < n; k++) p2c[k + n * i] = pv[k];
+            if (S->ipoly == 0) oracle_legepols(S->xq[i] / 2 + 0.5, n - 1, pv);
+            else oracle_chebpols(S->xq[i] / 2 + 0.5, n - 1, pv);
+            for (int k = 0; k < n; k++) p2c[n * n + k + n * i] = pv[k];
+        }
+        rc = nt_fix_lr(&B, nlevels, iperiod, nd, n, ifpgh, S->umat, p2c, bsz);
+        t2 = wtime();
+        S->timeinfo[10] += t2 - t1;
+    }
+    T->nlevels = nlevels;
+    T->nboxes = T->laddr[2 * nlevels + 1];
+    *S->nlevels = nlevels;
+    free(ifrefine); free(irefinelev); free(imaxrefinelev); free(iflag); free(pwexp); free(nl1); free(l1);
+    free(coefsp); free(locp); free(locnew); free(nblock);
+    return rc;
 }
 
 /* ------------------------------------------------------------------ */

@@ -100,3 +100,29 @@ def level_restricted(t, iperiod, boxlen=1.0):
                 if a.size and np.abs(a - bb).max() > 1:
                     return False
     return True
+
+
+def same_tree_arrays(a, b):
+    """Box-for-box comparison in the given numbering: nlevels, laddr, levels,
+    parents, children, colleague lists (the first ncoll entries) and centres
+    (bit for bit).  Returns a list of differences, empty when equal."""
+    msgs = []
+    if a.nlevels != b.nlevels:
+        return [f"nlevels {a.nlevels} != {b.nlevels}"]
+    nb = a.nboxes
+    if nb != b.nboxes:
+        return [f"box counts {nb} != {b.nboxes}"]
+    if not np.array_equal(a.laddr, b.laddr):
+        msgs.append("laddr differs")
+    for name in ("ilevel", "iparent", "nchild", "ichild", "ncoll"):
+        x, y = np.asarray(getattr(a, name)), np.asarray(getattr(b, name))
+        if not np.array_equal(x, y):
+            bad = np.nonzero((x != y).reshape(nb, -1).any(axis=1))[0]
+            msgs.append(f"{name} differs at boxes {bad[:8].tolist()}")
+    if not msgs:
+        m = np.arange(9)[None, :] < np.asarray(a.ncoll)[:, None]
+        if not np.array_equal(np.where(m, a.coll, 0), np.where(m, b.coll, 0)):
+            msgs.append("colleague lists differ")
+    if not np.array_equal(np.asarray(a.centers)[:nb], np.asarray(b.centers)[:nb]):
+        msgs.append("centres differ")
+    return msgs
