@@ -31,6 +31,7 @@ PRECISIONS = {"fp64": 0, "split32": 1}
 
 EXPORTS = [
     "bfgt_hip_eval",
+    "bfgt_hip_boxfgt",
     "bfgt_hip_release_plans",
     "bfgt_plan_create",
     "bfgt_plan_execute",
@@ -90,6 +91,9 @@ def lib() -> ctypes.CDLL:
     L.bfgt_hip_eval.restype = _i
     L.bfgt_hip_eval.argtypes = [_i, _i, _d, _d, _i, _i, _i, _i, _i, _i, _i, _ip, _ip, _dp, _dp, _dp, _i, _dp, _dp,
                                 _dp, _i, _i, _dp, _i, _dp, _dp, _dp, _dp, _i]
+    L.bfgt_hip_boxfgt.restype = _i
+    L.bfgt_hip_boxfgt.argtypes = [_i, _i, _d, _d, _i, _i, _i, _i, _i, _ip, _i, _ip, _ip, _dp, _dp, _dp, _i, _dp,
+                                  _dp, _dp, _i, _i, _dp, _i, _dp, _dp, _dp, _dp, _i]
     L.bfgt_plan_create.restype = _i
     L.bfgt_plan_create.argtypes = [ctypes.POINTER(_vp), _i, _i, _d, _d, _i, _i, _i, _i, _i, _i, _ip, _ip, _dp, _dp,
                                    _i, _i, _vp]

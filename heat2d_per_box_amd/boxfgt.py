@@ -4,9 +4,11 @@
 scalars and numpy arrays, with the same meaning, in-place side effects
 (``itree``'s colleague lists are recomputed, ``pot/grad/hess`` are
 accumulated on leaf boxes, ``timeinfo`` is zeroed and filled) and the same
-restrictions the GPU library states (2-D, ``ifnewtree=0``), extra
-targets included (``ifpghtarg``, ``targs``, ``pote/grade/hesse``), raising
-:class:`BoxFGTError` where the reference would silently misbehave.
+restrictions the GPU library states (2-D), extra targets included
+(``ifpghtarg``, ``targs``, ``pote/grade/hesse``), raising
+:class:`BoxFGTError` where the reference would silently misbehave.  With
+``ifnewtree=1`` the tree (``itree``, ``centers``) and pot/grad/hess are
+adapted to pot in place and the new ``nlevels`` is returned.
 
 ``BoxFGT`` is the resident form: the plan (lists, FP64 tables, schedules)
 lives on the GPU and each call runs on device tensors already in HBM.
@@ -54,8 +56,9 @@ def _i32(a, name):
 
 def boxfgt(nd, ndim, delta, eps, ipoly, iperiod, norder, npbox, nboxes, nlevels, ltree, itree, iptr, centers,
            boxsize, fvals, ifpgh, pot, grad=None, hess=None, ifnewtree=0, ntarg=0, targs=None, ifpghtarg=0,
-           pote=None, grade=None, hesse=None, timeinfo=None, precision: str = "fp64") -> None:
-    """Drop-in for the reference ``boxfgt`` call on host arrays (synchronous)."""
+           pote=None, grade=None, hesse=None, timeinfo=None, precision: str = "fp64") -> int:
+    """Drop-in for the reference ``boxfgt`` call on host arrays (synchronous).
+    Returns nlevels, which a call with ifnewtree=1 may change."""
     itree = _i32(itree, "itree")
     iptr = _i32(iptr, "iptr")
     centers = _f64(centers, "centers")
@@ -83,12 +86,16 @@ def boxfgt(nd, ndim, delta, eps, ipoly, iperiod, norder, npbox, nboxes, nlevels,
         if ifpghtarg >= 3 and (hesse is None or hesse.size < 3 * ntarg * nd):
             raise ValueError("hesse must hold (nd, 3, ntarg)")
     ti = np.zeros(20) if timeinfo is None else _f64(timeinfo, "timeinfo")
-    rc = lib().bfgt_hip_eval(
-        nd, ndim, delta, eps, ipoly, iperiod, norder, npbox, nboxes, nlevels, ltree, _ptr(itree, _ip),
+    if ifnewtree == 1 and centers.shape[0] * centers.shape[-1] < 2 * nboxes:
+        raise ValueError("centers must hold (2, nboxes)")
+    nl = ctypes.c_int(nlevels)
+    rc = lib().bfgt_hip_boxfgt(
+        nd, ndim, delta, eps, ipoly, iperiod, norder, npbox, nboxes, ctypes.byref(nl), ltree, _ptr(itree, _ip),
         _ptr(iptr, _ip), _ptr(centers), _ptr(boxsize), _ptr(fvals), ifpgh, _ptr(pot), _ptr(grad), _ptr(hess),
         ifnewtree, ntarg, _ptr(targs), ifpghtarg, _ptr(pote), _ptr(grade), _ptr(hesse), _ptr(ti),
         PRECISIONS[precision])
     check(rc)
+    return nl.value
 
 
 def release_plans() -> None:

@@ -37,7 +37,8 @@ HIP_FLAGS = [
     "-mllvm", "-amdgpu-mfma-vgpr-form",
 ]
 
-SOURCES = ["bfgt_tables.hip", "bfgt_pw.hip", "bfgt_near.hip", "bfgt_targets.hip", "bfgt_tree.hip", "bfgt_plan.cpp"]
+SOURCES = ["bfgt_tables.hip", "bfgt_pw.hip", "bfgt_near.hip", "bfgt_targets.hip", "bfgt_tree.hip", "bfgt_newtree.hip",
+           "bfgt_plan.cpp"]
 
 
 def _compile(src: str, verbose: bool) -> str:

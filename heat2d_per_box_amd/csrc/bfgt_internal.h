@@ -262,6 +262,40 @@ __device__ inline int drow32(int q, int r) { return 4 * q + r; }
 // status helpers
 int set_error(int code, const std::string& msg);
 
+// The reference's tree arrays seen through itree and iptr (ids 1-based
+// inside the arrays, -1 for none); first()/last() give 0-based box ids.
+struct TreeView {
+    int nlevels = 0, nboxes = 0;
+    int *laddr, *ilevel, *iparent, *nchild, *ichild, *ncoll, *coll;
+    const double *centers, *boxsize;
+    int first(int l) const { return laddr[2 * l] - 1; }
+    int last(int l) const { return laddr[2 * l + 1] - 1; }
+};
+
+// 1-D rules of a node family on the host (bfgt_tree.hip): nodes, values ->
+// coefficients, child/parent interpolation, tail masks, product weights
+struct TreeRules {
+    int n = 0, ipoly = 0, iptype = 2;
+    std::vector<double> x, w;  // ascending nodes on [-1, 1], quadrature weights
+    std::vector<double> U;     // values -> coefficients, [k*n + j]
+    std::vector<double> P2C;   // [s][i][k] = P_k(x_i/2 -+ 1/2), s = 0 (minus), 1 (plus)
+    std::vector<double> C2P;   // [s][i][k] = P_k(2 x_i + 1) (s = 0, i < n/2), P_k(2 x_{n/2+i} - 1) (s = 1)
+    std::vector<double> mask;  // tail mask [kx + n*ky] of the iptype norm
+    std::vector<double> mask2; // tail mask of the l2 norm (merging)
+    std::vector<double> wq;    // product weights [ix + n*iy]
+    double rsum = 1.0, rsum2 = 1.0;
+};
+TreeRules make_rules(int n, int ipoly, int iptype);
+
+// tree lists (bfgt_plan.cpp)
+double per_dist(double d, int iperiod, double bs0);
+int check_tree(int nboxes_cap, int nlevels, int ltree, int* itree, const int* iptr, const double* centers,
+               const double* boxsize, TreeView& t);
+void computecoll(TreeView& t, int iperiod);
+std::vector<char> pwexp_flags(const TreeView& t, int npwlevel);
+std::vector<std::vector<int>> modified_list1(const TreeView& t, int npwlevel, const std::vector<char>& pwexp,
+                                             int iperiod);
+
 // tables (bfgt_tables.hip): launched asynchronously, then waited for
 int build_tables(Plan& p);
 int finish_tables(Plan& p);
@@ -274,6 +308,35 @@ int launch_downward(Plan& p, double* pot, double* grad, double* hess);
 int launch_near(Plan& p, const double* fvals, double* pot, double* grad, double* hess);
 bool near_use_mfma(const Plan& p);
 int build_near_padded(Plan& p);
+
+// tree refinement inside the transform (ifnewtree = 1)
+size_t down_smem_bytes(int n, int npw, int ntab);
+int unfold_local(Plan& p, cplx* out);                                   // bfgt_pw.hip
+int launch_pw_eval_list(Plan& p, const PwLeaf* list, int count, const cplx* loc, double* pot, double* grad,
+                        double* hess);                                  // bfgt_pw.hip
+int launch_near_list(Plan& p, int ntgt, const int* tgt, const int* tlev, const int* ptr, const NearPair* pairs,
+                     const double* tloc, int half, const double* fvals, double* pot, double* grad,
+                     double* hess);                                     // bfgt_near.hip
+int build_loctab_refined(Plan& p, int mref, double* tloc, int* indloc, int* half);  // bfgt_tables.hip
+
+// The tree side of one host-array call with ifnewtree = 1 (bfgt_newtree.hip):
+// the caller's tree arrays (copied in, written back by the caller), the
+// plan that has just been executed, and its device value arrays, all of
+// capacity nbmax boxes.
+struct NewTreeIO {
+    int nbmax = 0, ltree = 0;
+    int* nlevels = nullptr;   // in/out
+    int* itree = nullptr;     // in/out, ltree entries
+    const int* iptr = nullptr;
+    double* centers = nullptr;  // in/out, 2 * nbmax
+    const double* boxsize = nullptr;
+    const double* fvals = nullptr;  // device
+    double *pot = nullptr, *grad = nullptr, *hess = nullptr;  // device, accumulated in place
+    double t_refine = 0, t_coarsen = 0, t_rebalance = 0;
+    int nnew = 0, ncapped = 0;
+    TreeNode* d_nodes = nullptr;  // device TreeNodes of the final tree (for targets); caller frees
+};
+int newtree_refine(Plan& p, double eps, NewTreeIO& io);
 
 // leaf-node coordinates (bfgt_targets.hip)
 int launch_leaf_points(Plan& p, double* xy);

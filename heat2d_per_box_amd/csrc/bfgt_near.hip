@@ -29,7 +29,7 @@ __global__ __launch_bounds__(kNearThreads) void k_near(int n, int nd, int ntab, 
                                                        const double* __restrict__ tloc,
                                                        const double* __restrict__ fvals, double* __restrict__ pot,
                                                        double* __restrict__ grad, double* __restrict__ hess,
-                                                       double oscale) {
+                                                       double oscale, int half) {
     extern __shared__ double smem[];
     const int nn = n * n;
     double* sf = smem;       // nn
@@ -38,7 +38,7 @@ __global__ __launch_bounds__(kNearThreads) void k_near(int n, int nd, int ntab, 
     const int lev = tlev[blockIdx.x];
     const int p0 = ptr[blockIdx.x], p1 = ptr[blockIdx.x + 1];
     const int tid = threadIdx.x;
-    const double* T = tloc + (size_t)lev * 13 * 3 * nn;
+    const double* T = tloc + (size_t)lev * (2 * half + 1) * 3 * nn;
     for (int d = 0; d < nd; d++) {
         double acc[kNearMaxQ][6];
 #pragma unroll
@@ -51,8 +51,8 @@ __global__ __launch_bounds__(kNearThreads) void k_near(int n, int nd, int ntab, 
             __syncthreads();
             for (int e = tid; e < nn; e += kNearThreads) sf[e] = fs[(size_t)e * nd + d];
             __syncthreads();
-            const double* Tx = T + (size_t)(pr.ix + 6) * 3 * nn;
-            const double* Ty = T + (size_t)(pr.iy + 6) * 3 * nn;
+            const double* Tx = T + (size_t)(pr.ix + half) * 3 * nn;
+            const double* Ty = T + (size_t)(pr.iy + half) * 3 * nn;
             // x pass: ff_t(k1, j2) = sum_j1 Tx_t(k1, j1) f(j1, j2)
             for (int e = tid; e < ntab * nn; e += kNearThreads) {
                 int t = e / nn, r = e % nn;
@@ -780,7 +780,21 @@ int launch_near(Plan& p, const double* fvals, double* pot, double* grad, double*
     int ntab = p.ifpgh;
     size_t sh = near_smem_bytes(p.n, ntab);
     k_near<<<p.nnear_t, kNearThreads, sh, p.stream>>>(p.n, p.nd, ntab, p.d_near_tgt, p.d_near_lev, p.d_near_ptr,
-                                                      p.d_near, p.d_tloc, fvals, pot, grad, hess, p.out_scale);
+                                                      p.d_near, p.d_tloc, fvals, pot, grad, hess, p.out_scale, 6);
+    BFGT_HIP_CHECK(hipGetLastError());
+    return BFGT_OK;
+}
+
+// Near field of an explicit list of targets (boxes refined below the
+// tree's leaves): ntgt targets tgt[i] at levels tlev[i], their (source,
+// offsets) pairs in CSR ptr/pairs, tables with offsets -half..half.
+int launch_near_list(Plan& p, int ntgt, const int* tgt, const int* tlev, const int* ptr, const NearPair* pairs,
+                     const double* tloc, int half, const double* fvals, double* pot, double* grad, double* hess) {
+    if (ntgt == 0) return BFGT_OK;
+    int ntab = p.ifpgh;
+    size_t sh = near_smem_bytes(p.n, ntab);
+    k_near<<<ntgt, kNearThreads, sh, p.stream>>>(p.n, p.nd, ntab, tgt, tlev, ptr, pairs, tloc, fvals, pot, grad,
+                                                 hess, 1.0, half);
     BFGT_HIP_CHECK(hipGetLastError());
     return BFGT_OK;
 }

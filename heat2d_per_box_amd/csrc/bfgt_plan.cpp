@@ -48,13 +48,7 @@ namespace {
 
 constexpr int kChunkLeaves = 8;
 
-struct TreeView {
-    int nlevels = 0, nboxes = 0;
-    int *laddr, *ilevel, *iparent, *nchild, *ichild, *ncoll, *coll;
-    const double *centers, *boxsize;
-    int first(int l) const { return laddr[2 * l] - 1; }
-    int last(int l) const { return laddr[2 * l + 1] - 1; }
-};
+}  // namespace
 
 double per_dist(double d, int iperiod, double bs0) {
     d = std::fabs(d);
@@ -192,6 +186,8 @@ std::vector<std::vector<int>> modified_list1(const TreeView& t, int npwlevel, co
         }
     return L;
 }
+
+namespace {
 
 // index of the mp->loc shift (gnd_find_pwshift_ind, bfgt/bfgt_pwrouts.f:1848-1892)
 int pwshift_index(int iperiod, const double* tc, const double* sc, double bs0, double xmin) {
@@ -1638,9 +1634,11 @@ static int hip_eval(int nd, int ndim, double delta, double eps, int ipoly, int i
                     int nboxes, int nlevels, int ltree, int* itree, const int* iptr, const double* centers,
                     const double* boxsize, const double* fvals, int ifpgh, double* pot, double* grad, double* hess,
                     int ifnewtree, int ntarg, const double* targs, int ifpghtarg, double* pote, double* grade,
-                    double* hesse, double* timeinfo, int precision) {
+                    double* hesse, double* timeinfo, int precision, int* nlevels_io, double* centers_io) {
     auto t0 = std::chrono::steady_clock::now();
-    if (ifnewtree != 0) return set_error(BFGT_ERR_IFNEWTREE, "ifnewtree=1 is not supported");
+    if (ifnewtree != 0 && ifnewtree != 1) return set_error(BFGT_ERR_IFNEWTREE, "ifnewtree must be 0 or 1");
+    if (ifnewtree == 1 && (!nlevels_io || !centers_io))
+        return set_error(BFGT_ERR_IFNEWTREE, "ifnewtree=1 rewrites the tree: call bfgt_hip_boxfgt");
     if (ifpghtarg < 0 || ifpghtarg > 3) return set_error(BFGT_ERR_TARGETS, "ifpghtarg must be 0..3");
     if (ifpghtarg > 0 && ntarg < 0) return set_error(BFGT_ERR_ARG, "ntarg must be >= 0");
     const bool with_targets = ifpghtarg > 0 && ntarg > 0;
@@ -1790,12 +1788,40 @@ static int hip_eval(int nd, int ndim, double delta, double eps, int ipoly, int i
     double ti[20];
     int rc = bfgt::plan_execute(p, dfv, dpot, dgrad, dhess, ti, 1.0);
     if (rc) return finish(rc);
+    // ifnewtree = 1: the tree is refined, coarsened and rebalanced to
+    // resolve pot (bfgt/boxfgt.f:1172-1483); itree, centers and nlevels are
+    // rewritten and the values follow the boxes
+    bfgt::NewTreeIO ntio;
+    if (ifnewtree == 1) {
+        ntio.nbmax = nboxes;
+        ntio.ltree = ltree;
+        ntio.nlevels = nlevels_io;
+        ntio.itree = itree;
+        ntio.iptr = iptr;
+        ntio.centers = centers_io;
+        ntio.boxsize = boxsize;
+        ntio.fvals = dfv;
+        ntio.pot = dpot;
+        ntio.grad = dgrad;
+        ntio.hess = dhess;
+        rc = bfgt::newtree_refine(p, eps, ntio);
+        if (rc) {
+            hipFree(ntio.d_nodes);
+            return finish(rc);
+        }
+    }
     double t_targets = 0;
     if (with_targets) {
         hipEvent_t e0 = p.ev[6], e1 = p.ev[7];
         BFGT_EVAL_CHECK(hipEventRecord(e0, p.stream));
+        bfgt::TreeNode* plan_nodes = p.d_tnodes;
+        if (ntio.d_nodes) p.d_tnodes = ntio.d_nodes;  // locate the targets in the new tree
         rc = bfgt::plan_targets(p, ntarg, dtg, ifpghtarg, dpot, dgrad, dhess, dpe, dge, dhe);
-        if (rc) return finish(rc);
+        p.d_tnodes = plan_nodes;
+        if (rc) {
+            hipFree(ntio.d_nodes);
+            return finish(rc);
+        }
         BFGT_EVAL_CHECK(hipEventRecord(e1, p.stream));
         BFGT_EVAL_CHECK(hipMemcpyAsync(pote, dpe, nt * nd * sizeof(double), hipMemcpyDeviceToHost, p.stream));
         if (ifpghtarg >= 2)
@@ -1807,6 +1833,7 @@ static int hip_eval(int nd, int ndim, double delta, double eps, int ipoly, int i
         BFGT_EVAL_CHECK(hipEventElapsedTime(&ms, e0, e1));
         t_targets = ms * 1e-3;
     }
+    hipFree(ntio.d_nodes);
     auto t2 = std::chrono::steady_clock::now();
     BFGT_EVAL_CHECK(bfgt::stage_download(p.stream, pot, dpot, nv * sizeof(double)));
     if (ifpgh >= 2) BFGT_EVAL_CHECK(bfgt::stage_download(p.stream, grad, dgrad, 2 * nv * sizeof(double)));
@@ -1825,6 +1852,13 @@ static int hip_eval(int nd, int ndim, double delta, double eps, int ipoly, int i
         timeinfo[15] = std::chrono::duration<double>(t1 - t0).count() - t_plan_host - t_tables;  // copies in, checks
         timeinfo[16] = std::chrono::duration<double>(t3 - t2).count();  // copies out
         timeinfo[17] = reused ? 1.0 : 0.0;  // the call ran on a kept plan
+        if (ifnewtree == 1) {
+            timeinfo[8] = ntio.t_refine;      // slots 9-11 of the reference: refine, coarsen, rebalance
+            timeinfo[9] = ntio.t_coarsen;
+            timeinfo[10] = ntio.t_rebalance;
+            timeinfo[18] = ntio.nnew;         // boxes the refinement created
+            timeinfo[19] = ntio.ncapped;      // leaves at the finest level that asked to be split
+        }
     }
     return finish(BFGT_OK);
 }
@@ -1837,8 +1871,23 @@ int bfgt_hip_eval(int nd, int ndim, double delta, double eps, int ipoly, int ipe
     try {
         return hip_eval(nd, ndim, delta, eps, ipoly, iperiod, norder, npbox, nboxes, nlevels, ltree, itree, iptr,
                         centers, boxsize, fvals, ifpgh, pot, grad, hess, ifnewtree, ntarg, targs, ifpghtarg, pote,
-                        grade, hesse, timeinfo, precision);
+                        grade, hesse, timeinfo, precision, nullptr, nullptr);
     } catch (const std::exception& e) {  // host allocation: no C++ exception crosses the C ABI
+        return set_error(BFGT_ERR_RESOURCE, std::string("boxfgt: ") + e.what());
+    }
+}
+
+int bfgt_hip_boxfgt(int nd, int ndim, double delta, double eps, int ipoly, int iperiod, int norder, int npbox,
+                    int nboxes, int* nlevels, int ltree, int* itree, const int* iptr, double* centers,
+                    const double* boxsize, const double* fvals, int ifpgh, double* pot, double* grad, double* hess,
+                    int ifnewtree, int ntarg, const double* targs, int ifpghtarg, double* pote, double* grade,
+                    double* hesse, double* timeinfo, int precision) {
+    if (!nlevels || !centers) return set_error(BFGT_ERR_ARG, "null nlevels or centers");
+    try {
+        return hip_eval(nd, ndim, delta, eps, ipoly, iperiod, norder, npbox, nboxes, *nlevels, ltree, itree, iptr,
+                        centers, boxsize, fvals, ifpgh, pot, grad, hess, ifnewtree, ntarg, targs, ifpghtarg, pote,
+                        grade, hesse, timeinfo, precision, nlevels, centers);
+    } catch (const std::exception& e) {
         return set_error(BFGT_ERR_RESOURCE, std::string("boxfgt: ") + e.what());
     }
 }

@@ -1497,7 +1497,66 @@ __global__ void k_m2l_fold(int nd, int P, int gy, int nanc, const int* __restric
     }
 }
 
+// The local expansions of the cutoff boxes back in the plain layout
+// [box][density][m1 * H + m2] from the mirror-folded one: L(m) = (S + D)/2,
+// L(npw-1-m) = (S - D)/2, L = S on the middle frequency of an odd expansion.
+__global__ void k_unfold(int nd, int P, int nanc, const Fold* __restrict__ fold, cplx* __restrict__ loc) {
+    const int H = (P + 1) / 2, nexp = P * H;
+    const int Mh4 = round_up(H, 4), H16 = round_up(H, 16);
+    const long tot = (long)nanc * nd * H * H;
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < tot; e += (long)gridDim.x * blockDim.x) {
+        const int r = (int)(e % (H * H));
+        const long ad = e / (H * H);
+        const int m = r / H, k2 = r % H, mm = P - 1 - m;
+        const Fold f = fold[ad * Mh4 * H16 + (long)m * H16 + k2];
+        cplx* L = loc + ad * nexp;
+        if (mm != m) {
+            L[(long)m * H + k2] = cplx{0.5 * (f.S.x + f.D.x), 0.5 * (f.S.y + f.D.y)};
+            L[(long)mm * H + k2] = cplx{0.5 * (f.S.x - f.D.x), 0.5 * (f.S.y - f.D.y)};
+        } else {
+            L[(long)m * H + k2] = f.S;
+        }
+    }
+}
+
 }  // namespace
+
+// The cutoff boxes' local expansions of the last execution, [box][density]
+// [nexp] (the vector kernels keep them so; the matrix-core path folds them).
+int unfold_local(Plan& p, cplx* out) {
+    if (!p.has_pw || p.nanc == 0) return BFGT_OK;
+    if (!use_mfma(p)) {
+        BFGT_HIP_CHECK(hipMemcpyAsync(out, p.d_loc, sizeof(cplx) * (size_t)p.nanc * p.nd * p.nexp,
+                                      hipMemcpyDeviceToDevice, p.stream));
+        return BFGT_OK;
+    }
+    const long tot = (long)p.nanc * p.nd * p.H * p.H;
+    const int blocks = (int)std::min<long>((tot + 255) / 256, 4096);
+    k_unfold<<<blocks, 256, 0, p.stream>>>(p.nd, p.npw, p.nanc, reinterpret_cast<const Fold*>(p.d_loc), out);
+    BFGT_HIP_CHECK(hipGetLastError());
+    return BFGT_OK;
+}
+
+// pot/grad/hess += the cutoff boxes' local expansions (plain layout, as
+// unfold_local leaves them) evaluated on an explicit list of boxes: box
+// slot, level, cutoff box and scaled offset from it per entry.
+int launch_pw_eval_list(Plan& p, const PwLeaf* list, int count, const cplx* loc, double* pot, double* grad,
+                        double* hess) {
+    if (count == 0 || !p.has_pw) return BFGT_OK;
+    size_t sh = down_smem_bytes(p.n, p.npw, p.ifpgh);
+    const int q = (p.n * p.n + kVecThreads - 1) / kVecThreads;
+    auto run = [&](auto kern) -> int {
+        if (sh > 65536)
+            BFGT_HIP_CHECK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
+        kern<<<count, kVecThreads, sh, p.stream>>>(p.n, p.npw, p.nd, p.ifpgh, list, p.d_tpw, p.d_ts, loc, pot, grad,
+                                                   hess, 1.0);
+        BFGT_HIP_CHECK(hipGetLastError());
+        return BFGT_OK;
+    };
+    if (q <= 1) return run(k_down_vec<1>);
+    if (q <= 2) return run(k_down_vec<2>);
+    return run(k_down_vec<4>);
+}
 
 bool use_mfma(const Plan& p) { return p.n <= 16 && (p.npw + 1) / 2 <= 16 * kUpMaxTM; }
 

@@ -404,16 +404,46 @@ __device__ void sparse_pattern(double eps, int n, const double* a, int* ind) {
         }
 }
 
-// Near-field tables.  grid (6, nlevels+1): blockIdx.x picks one of the six
-// computed offsets k in {-1,-3, 0,-4, -2,-6} (source twice, equal, half the
-// target size); the block also writes the mirror offset -k.  block 256.
+// The offsets mk_loctab_all computes for targets refined up to mref times
+// (bfgt/bfgt_volrouts.f:548-563): k(i0, j0) for source/target size ratios
+// 2^i0, i0 = mref+1 .. -1, each also giving its mirror -k; the tables of a
+// level occupy slots -half..half (half = 2^(mref+1) (mref+3),
+// bfgt/boxfgt.f:625), target centre k / 2^(mref+1) source half-widths away.
+struct LoctabSet {
+    int nblk, half;
+    double cdiv;
+    int k[64], i0[64];
+};
+
+LoctabSet loctab_set(int mref) {
+    LoctabSet S{};
+    const int m0 = mref + 1;
+    S.half = (1 << (mref + 1)) * (mref + 3);
+    S.cdiv = (double)(1 << m0);
+    for (int i0 = m0; i0 >= -1; i0--)
+        for (int j0 = 1; j0 <= (1 << m0); j0++) {
+            int k;
+            if (i0 > 0) k = -(1 << (m0 - i0)) - (1 << (m0 - i0 + 1)) * (j0 - 1);
+            else if (i0 == 0) k = -(1 << (m0 - i0 + 1)) * (j0 - 1);
+            else k = -(1 << m0) - (1 << (m0 + 1)) * (j0 - 1);
+            if (k < -S.half) break;
+            S.k[S.nblk] = k;
+            S.i0[S.nblk] = i0;
+            S.nblk++;
+        }
+    return S;
+}
+
+// Near-field tables.  grid (set.nblk, nlevels+1): blockIdx.x picks one of the
+// set's computed offsets k (for mref = 0 the six k in {-1,-3, 0,-4, -2,-6}:
+// source twice, equal, half the target size); the block also writes the
+// mirror offset -k.  block 256.
 __global__ void k_loctab(double eps, int ipoly, int n, double delta, const double* boxsize, const double* nodes,
-                         double* tloc, int* indloc) {
+                         LoctabSet set, double* tloc, int* indloc) {
     NodeLayout L{n};
-    const int kk[6] = {-1, -3, 0, -4, -2, -6};
-    const int i0s[6] = {1, 1, 0, 0, -1, -1};
-    int k = kk[blockIdx.x];
-    int i0 = i0s[blockIdx.x];
+    int k = set.k[blockIdx.x];
+    int i0 = set.i0[blockIdx.x];
+    const int nslot = 2 * set.half + 1;
     int lev = blockIdx.y;
     double boxdim = boxsize[lev];
     extern __shared__ double sh[];
@@ -427,7 +457,7 @@ __global__ void k_loctab(double eps, int ipoly, int n, double delta, const doubl
     __shared__ int indm[2][2 * (kMaxOrder + 1)];
 
     double scale = ldexp(1.0, i0);
-    double cen = k / 2.0;
+    double cen = k / set.cdiv;
     double bs = boxdim * scale;
     double sigma = 4 * delta / (bs * bs);
     double lambda = sqrt(sigma);
@@ -541,13 +571,13 @@ __global__ void k_loctab(double eps, int ipoly, int n, double delta, const doubl
     }
     // masked writes: tloc[lev][q][t][j][m]
     for (int s = 0; s < 2; s++) {
-        int q = ((s == 0) ? -k : k) + 6;
+        int q = ((s == 0) ? -k : k) + set.half;
         if (k == 0 && s == 1) break;
         const double* T = (s == 0 || k == 0) ? B : A;
         const int* im = indm[s];
-        int* out_ind = indloc + ((size_t)lev * 13 + q) * 2 * (n + 1);
+        int* out_ind = indloc + ((size_t)lev * nslot + q) * 2 * (n + 1);
         for (int e = tid; e < 2 * (n + 1); e += blockDim.x) out_ind[e] = im[e];
-        double* out = tloc + ((size_t)lev * 13 + q) * 3 * nn;
+        double* out = tloc + ((size_t)lev * nslot + q) * 3 * nn;
         int jr0 = im[2 * n], jr1 = im[2 * n + 1];
         for (int e = tid; e < 3 * nn; e += blockDim.x) {
             int t = e / nn, r = e % nn;
@@ -591,8 +621,9 @@ int build_tables(Plan& p) {
     // near-field tables for every level
     size_t sh = sizeof(double) * 9 * n * n;
     if (sh > 65536) BFGT_HIP_CHECK(hipFuncSetAttribute((const void*)k_loctab, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
-    k_loctab<<<dim3(6, p.nlevels + 1), 256, sh, p.stream>>>(p.eps, p.ipoly, n, p.delta, p.d_boxsize, p.d_nodes,
-                                                            p.d_tloc, p.d_indloc);
+    const LoctabSet set0 = loctab_set(0);
+    k_loctab<<<dim3(set0.nblk, p.nlevels + 1), 256, sh, p.stream>>>(p.eps, p.ipoly, n, p.delta, p.d_boxsize,
+                                                                    p.d_nodes, set0, p.d_tloc, p.d_indloc);
     BFGT_HIP_CHECK(hipGetLastError());
     if (p.has_pw) {
         int nlev_pw = p.nlevels - p.nlevstart + 1;
@@ -613,6 +644,24 @@ int build_tables(Plan& p) {
     BFGT_HIP_CHECK(hipMemcpyAsync(p.h_indloc.data(), p.d_indloc, nind * sizeof(int), hipMemcpyDeviceToHost,
                                   p.stream));
     BFGT_HIP_CHECK(hipEventRecord(e1, p.stream));
+    return BFGT_OK;
+}
+
+// Near-field tables for targets refined up to mref times below the tree's
+// leaves, every level: tloc [level][2 half + 1][3][target][source], masked
+// like the plan's own, slots the enumeration does not reach left zero.
+int build_loctab_refined(Plan& p, int mref, double* tloc, int* indloc, int* half) {
+    const int n = p.n;
+    const LoctabSet set = loctab_set(mref);
+    const size_t nslot = 2 * set.half + 1;
+    BFGT_HIP_CHECK(hipMemsetAsync(tloc, 0, sizeof(double) * (p.nlevels + 1) * nslot * 3 * n * n, p.stream));
+    BFGT_HIP_CHECK(hipMemsetAsync(indloc, 0, sizeof(int) * (p.nlevels + 1) * nslot * 2 * (n + 1), p.stream));
+    size_t sh = sizeof(double) * 9 * n * n;
+    if (sh > 65536) BFGT_HIP_CHECK(hipFuncSetAttribute((const void*)k_loctab, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
+    k_loctab<<<dim3(set.nblk, p.nlevels + 1), 256, sh, p.stream>>>(p.eps, p.ipoly, n, p.delta, p.d_boxsize,
+                                                                   p.d_nodes, set, tloc, indloc);
+    BFGT_HIP_CHECK(hipGetLastError());
+    *half = set.half;
     return BFGT_OK;
 }
 

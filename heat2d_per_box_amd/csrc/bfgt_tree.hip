@@ -39,11 +39,13 @@ namespace {
 constexpr int kTreeMaxLevel = 28;          // cell coordinates use 29 bits
 constexpr unsigned long long kEmptyKey = ~0ull;
 
+}  // namespace
+
 // --------------------------------------------------------------------------
 // 1-D rules on the host: nodes, values -> coefficients, child/parent
 // interpolation, tail mask and product weights.
 
-void poly_values(int ipoly, double x, int n, double* p) {
+static void poly_values(int ipoly, double x, int n, double* p) {
     p[0] = 1.0;
     if (n == 1) return;
     p[1] = x;
@@ -51,19 +53,7 @@ void poly_values(int ipoly, double x, int n, double* p) {
         p[k + 1] = ipoly == 0 ? ((2 * k + 1) * x * p[k] - k * p[k - 1]) / (k + 1) : 2 * x * p[k] - p[k - 1];
 }
 
-struct TreeRules {
-    int n = 0, ipoly = 0, iptype = 2;
-    std::vector<double> x, w;  // ascending nodes on [-1, 1], quadrature weights
-    std::vector<double> U;     // values -> coefficients, [k*n + j]
-    std::vector<double> P2C;   // [s][i][k] = P_k(x_i/2 -+ 1/2), s = 0 (minus), 1 (plus)
-    std::vector<double> C2P;   // [s][i][k] = P_k(2 x_i + 1) (s = 0, i < n/2), P_k(2 x_{n/2+i} - 1) (s = 1)
-    std::vector<double> mask;  // tail mask [kx + n*ky] of the iptype norm
-    std::vector<double> mask2; // tail mask of the l2 norm (merging)
-    std::vector<double> wq;    // product weights [ix + n*iy]
-    double rsum = 1.0, rsum2 = 1.0;
-};
-
-void tail_mask(int n, int iptype, std::vector<double>& m, double& rsum) {
+static void tail_mask(int n, int iptype, std::vector<double>& m, double& rsum) {
     // tens_prod_get_rmask (common/polytens.f:1579-1621): (p1+1)^q + (p2+1)^q >= n^q, q = max(iptype, 1)
     int q = std::max(iptype, 1);
     auto pw = [q](long v) { return q == 1 ? v : v * v; };
@@ -153,6 +143,8 @@ TreeRules make_rules(int n, int ipoly, int iptype) {
         for (int ix = 0; ix < n; ix++) r.wq[ix + n * iy] = r.w[ix] * r.w[iy];
     return r;
 }
+
+namespace {
 
 // --------------------------------------------------------------------------
 // device side

@@ -4,9 +4,11 @@
 ! interface (bfgt/boxfgt.f:5-8, declarations :91-109), so the heat,
 ! unsteady-Stokes and Navier-Stokes drivers link against this object instead
 ! of bfgt/boxfgt.o with no source change.  It forwards to the HIP library
-! through the C ABI `bfgt_hip_eval` (include/boxfgt_hip.h) via iso_c_binding,
-! and stops with a message where the reference would silently misbehave
-! (ifnewtree=1, extra targets, more than 200 plane waves, ...).
+! through the C ABI `bfgt_hip_boxfgt` (include/boxfgt_hip.h) via
+! iso_c_binding -- every argument by reference as in the reference, so that
+! ifnewtree=1 can rewrite itree, centers and nlevels -- and stops with a
+! message where the reference would silently misbehave (a tree outgrowing
+! nboxes, more than 200 plane waves, ...).
 !
 ! `boxfgt_set_precision(iprec)` selects the arithmetic of later calls:
 ! 0 = all-FP64 (default), 1 = split-FP32 with FP64 weights and accumulation.
@@ -16,17 +18,18 @@ module boxfgt_hip_mod
   use iso_c_binding
   implicit none
   private
-  public :: bfgt_hip_eval_c, bfgt_last_error_c, bfgt_hip_release_plans_c, boxfgt_precision, c_string
+  public :: bfgt_hip_boxfgt_c, bfgt_last_error_c, bfgt_hip_release_plans_c, boxfgt_precision, c_string
 
   integer, save :: boxfgt_precision = 0
 
   interface
-    integer(c_int) function bfgt_hip_eval_c(nd, ndim, delta, eps, ipoly, iperiod, norder, npbox, &
+    integer(c_int) function bfgt_hip_boxfgt_c(nd, ndim, delta, eps, ipoly, iperiod, norder, npbox, &
         nboxes, nlevels, ltree, itree, iptr, centers, boxsize, fvals, ifpgh, pot, grad, hess, &
         ifnewtree, ntarg, targs, ifpghtarg, pote, grade, hesse, timeinfo, precision) &
-        bind(C, name="bfgt_hip_eval")
+        bind(C, name="bfgt_hip_boxfgt")
       import :: c_int, c_double
-      integer(c_int), value :: nd, ndim, ipoly, iperiod, norder, npbox, nboxes, nlevels, ltree
+      integer(c_int), value :: nd, ndim, ipoly, iperiod, norder, npbox, nboxes, ltree
+      integer(c_int) :: nlevels
       real(c_double), value :: delta, eps
       integer(c_int) :: itree(*), iptr(*)
       real(c_double) :: centers(*), boxsize(*), fvals(*)
@@ -101,7 +104,7 @@ subroutine boxfgt(nd, ndim, delta, eps, ipoly, iperiod, norder, npbox, &
   real(8) :: targs(ndim, *), pote(*), grade(*), hesse(*), timeinfo(*)
   integer(c_int) :: ier
 
-  ier = bfgt_hip_eval_c(nd, ndim, delta, eps, ipoly, iperiod, norder, npbox, nboxes, nlevels, &
+  ier = bfgt_hip_boxfgt_c(nd, ndim, delta, eps, ipoly, iperiod, norder, npbox, nboxes, nlevels, &
       ltree, itree, iptr, centers, boxsize, fvals, ifpgh, pot, grad, hess, ifnewtree, ntarg, &
       targs, ifpghtarg, pote, grade, hesse, timeinfo, boxfgt_precision)
   if (ier /= 0) then

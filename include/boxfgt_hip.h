@@ -25,7 +25,7 @@ extern "C" {
 #define BFGT_ERR_NDIM 1      /* ndim != 2 */
 #define BFGT_ERR_IPOLY 2     /* ipoly not 0 (Legendre) or 1 (Chebyshev) */
 #define BFGT_ERR_IFPGH 3     /* ifpgh not in 1..3 */
-#define BFGT_ERR_IFNEWTREE 4 /* ifnewtree=1 (tree refinement inside the FGT) is not supported */
+#define BFGT_ERR_IFNEWTREE 4 /* ifnewtree not 0 or 1, or 1 through an entry that cannot write the tree */
 #define BFGT_ERR_TARGETS 5   /* ifpghtarg outside 0..3 */
 #define BFGT_ERR_NORDER 6    /* norder < 1, norder > 32, or npbox != norder**2 */
 #define BFGT_ERR_NPW 7       /* plane-wave length above 200 (the reference's ws/ts(200)) */
@@ -53,7 +53,8 @@ extern "C" {
  * leaves zero: 14 the host part of a new plan (lists, schedules), 15 its
  * table generation, 16 the rest of slot 1 (copies in, checks), 17 the copies
  * out, 18 = 1 if the call reused a kept plan).  ifnewtree must be
- * 0.  With ifpghtarg = 1..3 and ntarg > 0 the leaf pot (and grad, hess) are
+ * 0 here (BFGT_ERR_IFNEWTREE otherwise: the refining call writes nlevels and
+ * centers, which this signature passes read-only; see bfgt_hip_boxfgt).  With ifpghtarg = 1..3 and ntarg > 0 the leaf pot (and grad, hess) are
  * interpolated at targs(2,ntarg) into pote(nd,ntarg), grade(nd,2,ntarg),
  * hesse(nd,3,ntarg), which are overwritten (bfgtmain STEP 11,
  * bfgt/boxfgt.f:1494-1563); channels above ifpgh are computed into scratch
@@ -78,6 +79,41 @@ int bfgt_hip_eval(int nd, int ndim, double delta, double eps, int ipoly, int ipe
                   const double *boxsize, const double *fvals, int ifpgh, double *pot, double *grad, double *hess,
                   int ifnewtree, int ntarg, const double *targs, int ifpghtarg, double *pote, double *grade,
                   double *hesse, double *timeinfo, int precision);
+
+/*
+ * The complete boxfgt contract, nlevels and centers written back: the same
+ * call as bfgt_hip_eval (arguments, kept plans, staging, status codes) plus
+ * ifnewtree = 1.
+ * Replaces: subroutine boxfgt(...) -- bfgt/boxfgt.f:5-8, every argument by
+ * reference as there; with ifnewtree=1 the tree is adapted to pot after the
+ * transform (bfgtmain STEPS 8-10, bfgt/boxfgt.f:1172-1483):
+ *  - STEP 8: leaves above the finest level whose pot is not resolved
+ *    (l2 tail of its tensor coefficients over eps * the l2 norm of pot) are
+ *    split; the children's pot/grad/hess are evaluated from the cutoff-level
+ *    local expansions and the parent's near-field list with tables for
+ *    targets refined up to 3 times, and split in turn within the reference's
+ *    generation limits;
+ *  - STEP 9 (when the refinement added at least a tenth of the boxes):
+ *    boxes whose four leaf children's pot, interpolated onto the parent, is
+ *    resolved are merged, grad/hess interpolated too;
+ *  - STEP 10: colleague lists, then 2:1 level restriction with the values of
+ *    new boxes interpolated from their parents.
+ * itree (all lists), centers, *nlevels (it may fall when coarsening empties
+ * the finest level) and pot/grad/hess of every box of the new tree are
+ * written, the boxes numbered as the reference numbers them; extra targets
+ * are then located in the new tree.  nboxes is the capacity of the box
+ * arrays (iptr's layout of itree must hold as many); a tree that would
+ * outgrow it returns BFGT_ERR_RESOURCE with the caller's arrays partly
+ * rewritten.  timeinfo slots 9-11 time the three steps, slot 19 counts the
+ * boxes the refinement created and slot 20 the leaves at the finest level
+ * that asked to be split: they are not split (the reference would create a
+ * level that boxsize(0:nlevels) and laddr cannot hold).
+ */
+int bfgt_hip_boxfgt(int nd, int ndim, double delta, double eps, int ipoly, int iperiod, int norder, int npbox,
+                    int nboxes, int *nlevels, int ltree, int *itree, const int *iptr, double *centers,
+                    const double *boxsize, const double *fvals, int ifpgh, double *pot, double *grad, double *hess,
+                    int ifnewtree, int ntarg, const double *targs, int ifpghtarg, double *pote, double *grade,
+                    double *hesse, double *timeinfo, int precision);
 
 /* Free the plans, device buffers and pinned staging buffers kept by
  * bfgt_hip_eval (no counterpart in the reference, whose boxfgt keeps nothing
