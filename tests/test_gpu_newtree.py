@@ -70,6 +70,24 @@ def test_newtree_matches_oracle_and_reference(device, case):
             assert _rel(g[:nb][lv], getattr(rr, name)[:nb][lv]) < 1e-12, name
 
 
+def test_newtree_c3_workload(device):
+    """The nd = 3 spot workload on its 4,017-box tree at delta = 1e-3
+    (pot + grad + hess): 320 boxes added, tree and values as the oracle's."""
+    from heat2d_per_box_amd.densities import make_workload
+
+    w = make_workload("c3", "small", delta=1e-3, eps=1e-9)
+    tc = with_capacity(w.tree, 6 * w.tree.nboxes)
+    t, pot, grad, hess, ti, _ = _run_gpu(tc, w.fvals, w.nd, w.delta, w.eps, 0, 1, w.norder, w.ifpgh)
+    ro = O.oracle_boxfgt_newtree(tc, w.fvals, w.nd, w.delta, w.eps, 0, 1, w.norder, w.ifpgh)
+    assert ro.info["nnew"] > 0 and ti[18] == ro.info["nnew"]
+    msgs = same_tree_arrays(t, ro.tree)
+    assert not msgs, msgs
+    nb = t.nboxes
+    lv = t.nchild == 0
+    for name, g in (("pot", pot), ("grad", grad), ("hess", hess)):
+        assert _rel(g[:nb][lv], getattr(ro, name)[:nb][lv]) < 1e-12, name
+
+
 def test_newtree_targets_in_the_new_tree(device):
     """Extra targets after the refinement are interpolated in the new tree
     (bfgtmain STEP 11 runs after STEPS 8-10)."""
