@@ -928,15 +928,19 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_down_pair(
 // merge and translation stay FP64) and the pot/grad/hess accumulation.
 //
 
-// Forward upward pass, split-FP32 (one wave per (chunk, y tile), one leaf at
-// a time, the same two products).
+// Forward upward pass, split-FP32: one wave per (chunk, y tile), with the
+// column sharing of the FP64 pass -- the y contractions of a column's leaves
+// are summed in the FP32 accumulator G, then one x contraction per column
+// piece (cpc[chunk] = (first piece, count, slot), cols[piece] = (first
+// leaf, count); the leaves of a piece share the level and the x cell).
 template <int NKS, int NTM>
 __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_up_split(
     int n, int P, int ps, int nd, int nTN, int nitems, const double* __restrict__ fvals,
-    const PwLeaf* __restrict__ leaves, const UpChunk* __restrict__ chunks, const cplx* __restrict__ tab, int lstride,
-    const cplx* __restrict__ phase, const int* __restrict__ levbase, cplx* __restrict__ partial) {
+    const PwLeaf* __restrict__ leaves, const int4* __restrict__ cpc, const int2* __restrict__ cols,
+    const cplx* __restrict__ tab, int lstride, const cplx* __restrict__ phase, const int* __restrict__ levbase,
+    cplx* __restrict__ partial) {
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
-    const int item = blockIdx.x * kWavesPerBlock + wave;
+    const int item = __builtin_amdgcn_readfirstlane(blockIdx.x * kWavesPerBlock + wave);
     if (item >= nitems) return;  // no block-wide barriers in this kernel
     const int ci = item / nTN, tN = item % nTN;
     const int q = lane >> 4, cc = lane & 15;
@@ -946,35 +950,39 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_up_split(
     int pidx[NKS];
 #pragma unroll
     for (int s = 0; s < NKS; s++) pidx[s] = ccl + n * min(4 * s + q, n - 1);
-    const UpChunk ch = chunks[ci];
+    const int4 ch = cpc[ci];
     for (int d = 0; d < nd; d++) {
         v4f acc[4][NTM];
 #pragma unroll
         for (int p = 0; p < 4; p++)
 #pragma unroll
             for (int t = 0; t < NTM; t++) acc[p][t] = v4f{0, 0, 0, 0};
-        for (int li = 0; li < ch.count; li++) {
-            const PwLeaf lf = leaves[ch.first + li];
-            const cplx* T = tab + (size_t)lf.level * lstride;
-            const cplx* ph = phase + levbase[lf.level];
-            const size_t pb0 = (size_t)lf.box * npbox;
-            cplx py = ph[lf.jy * ps + k2];
-            py.y = -py.y;
-            // G = F (T diag py): K index m2 = 4s + q
+        for (int c = 0; c < ch.y; c++) {
+            const int2 col = cols[ch.x + c];
+            const PwLeaf l0 = leaves[col.x];
+            const cplx* T = tab + (size_t)l0.level * lstride;
+            const cplx* ph = phase + levbase[l0.level];
+            // G = sum over the column's leaves of F (T diag py): K index m2 = 4s + q
             v4f Gr = {0, 0, 0, 0}, Gi = {0, 0, 0, 0};
+            for (int li = 0; li < col.y; li++) {
+                const PwLeaf lf = leaves[col.x + li];
+                const size_t pb0 = (size_t)lf.box * npbox;
+                cplx py = ph[lf.jy * ps + k2];
+                py.y = -py.y;
 #pragma unroll
-            for (int s = 0; s < NKS; s++) {
-                const int m2 = 4 * s + q;
-                cplx ty = cmul(T[m2 * ps + k2], py);
-                Split a = split(fvals[(pb0 + pidx[s]) * nd + d]);
-                Gr = mfma3(a, split(ty.x), Gr);
-                Gi = mfma3(a, split(ty.y), Gi);
+                for (int s = 0; s < NKS; s++) {
+                    const int m2 = 4 * s + q;
+                    cplx ty = cmul(T[m2 * ps + k2], py);
+                    Split a = split(fvals[(pb0 + pidx[s]) * nd + d]);
+                    Gr = mfma3(a, split(ty.x), Gr);
+                    Gi = mfma3(a, split(ty.y), Gi);
+                }
             }
             // out += (T diag px)^T G: K-step r has K index m1 = 4q + r
 #pragma unroll
             for (int t = 0; t < NTM; t++) {
                 const int k1 = 16 * t + cc;
-                cplx px = ph[lf.jx * ps + k1];
+                cplx px = ph[l0.jx * ps + k1];
                 px.y = -px.y;
 #pragma unroll
                 for (int r = 0; r < 4; r++) {
@@ -988,7 +996,7 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_up_split(
                 }
             }
         }
-        cplx* out = partial + ((size_t)ch.slot * nd + d) * nexp;
+        cplx* out = partial + ((size_t)ch.z * nd + d) * nexp;
 #pragma unroll
         for (int t = 0; t < NTM; t++) {
 #pragma unroll
@@ -1005,99 +1013,119 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_up_split(
     }
 }
 
-// Forward downward pass, split-FP32 (one leaf per wave, the same two products).
-template <int NTAB>
+// Forward downward pass, split-FP32: one wave per column group (grp[item] =
+// (first leaf, count), the leaves sharing the cutoff box, the level and the
+// x cell), as in the FP64 pass -- the x contraction FF of every y tile is
+// formed once (FP32 accumulators, NT tiles), then each leaf's y contraction
+// and output update.
+template <int NTAB, int NT>
 __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_down_split(
-    int n, int P, int ps, int nd, int nleaf, const PwLeaf* __restrict__ leaves, const cplx* __restrict__ tab,
-    int lstride, const cplx* __restrict__ phase, const int* __restrict__ levbase, const Fold* __restrict__ fold,
-    double* __restrict__ pot, double* __restrict__ grad, double* __restrict__ hess, double oscale) {
+    int n, int P, int ps, int nd, int ngrp, const int2* __restrict__ grp, const PwLeaf* __restrict__ leaves,
+    const cplx* __restrict__ tab, int lstride, const cplx* __restrict__ phase, const int* __restrict__ levbase,
+    const Fold* __restrict__ fold, double* __restrict__ pot, double* __restrict__ grad, double* __restrict__ hess,
+    double oscale) {
     constexpr int NOUT = nchan(NTAB);
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
     // wave-uniform: schedule entries are then read with scalar loads
     const int item = __builtin_amdgcn_readfirstlane(blockIdx.x * kWavesPerBlock + wave);
-    if (item >= nleaf) return;  // no block-wide barriers in this kernel
+    if (item >= ngrp) return;  // no block-wide barriers in this kernel
     const int q = lane >> 4, cc = lane & 15;
     const int H = (P + 1) / 2, P2 = P / 2, npbox = n * n;
     const int Mh4 = round_up(H, 4), H16 = round_up(H, 16);
-    const int nks1 = Mh4 / 4, nT = H16 / 16;
+    const int nks1 = Mh4 / 4;
     const int tstride = 16 * ps;
-    const PwLeaf lf = leaves[item];
-    const cplx* E = tab + (size_t)lf.level * lstride;
-    const cplx* ph = phase + levbase[lf.level];
-    const cplx* phx = ph + lf.jx * ps;
-    const cplx* phy = ph + lf.jy * ps;
+    const int2 g = grp[item];
+    const PwLeaf l0 = leaves[g.x];
+    const cplx* E = tab + (size_t)l0.level * lstride;
+    const cplx* ph = phase + levbase[l0.level];
+    const cplx* phx = ph + l0.jx * ps;
+    constexpr int stride_o[6] = {1, 2, 2, 3, 3, 3}, comp_o[6] = {0, 0, 1, 0, 1, 2};
+    double* dst[NOUT];
+    dst[0] = pot;
+    if constexpr (NTAB >= 2) dst[1] = dst[2] = grad;
+    if constexpr (NTAB >= 3) dst[3] = dst[4] = dst[5] = hess;
+    const double w2 = 2.0 * oscale;
     for (int d = 0; d < nd; d++) {
-        const Fold* F = fold + ((size_t)lf.anc * nd + d) * Mh4 * H16;
-        v4f out[NOUT];
+        const Fold* F = fold + ((size_t)l0.anc * nd + d) * Mh4 * H16;
+        // x contraction of the group, every y tile
+        v4f FFr[NTAB][NT], FFi[NTAB][NT];
 #pragma unroll
-        for (int o = 0; o < NOUT; o++) out[o] = v4f{0, 0, 0, 0};
-        for (int tM = 0; tM < nT; tM++) {
-            const int m2 = 16 * tM + cc;
-            v4f FFr[NTAB], FFi[NTAB];
+        for (int t = 0; t < NTAB; t++)
 #pragma unroll
-            for (int t = 0; t < NTAB; t++) FFr[t] = FFi[t] = v4f{0, 0, 0, 0};
-            for (int s = 0; s < nks1; s++) {
-                const int m = 4 * s + q;
-                const Fold sd = F[m * H16 + m2];
-                const cplx px = phx[m];
+            for (int tM = 0; tM < NT; tM++) FFr[t][tM] = FFi[t][tM] = v4f{0, 0, 0, 0};
+        for (int s = 0; s < nks1; s++) {
+            const int m = 4 * s + q;
+            const cplx px = phx[m];
+            Split er[NTAB], ei[NTAB];
+#pragma unroll
+            for (int t = 0; t < NTAB; t++) {
+                cplx e = cmul(E[t * tstride + m * 16 + cc], px);
+                er[t] = split(e.x);
+                ei[t] = split(e.y);
+            }
+#pragma unroll
+            for (int tM = 0; tM < NT; tM++) {
+                const Fold sd = F[m * H16 + 16 * tM + cc];
                 Split sr = split(sd.S.x), si = split(sd.S.y), dr = split(sd.D.x), di = neg(split(sd.D.y));
 #pragma unroll
                 for (int t = 0; t < NTAB; t++) {
-                    cplx e = cmul(E[t * tstride + m * 16 + cc], px);
-                    Split er = split(e.x), ei = split(e.y);
-                    FFr[t] = mfma3(sr, er, FFr[t]);
-                    FFr[t] = mfma3(di, ei, FFr[t]);
-                    FFi[t] = mfma3(si, er, FFi[t]);
-                    FFi[t] = mfma3(dr, ei, FFi[t]);
-                }
-            }
-            // register r of FF holds FF^T[16 tM + 4q + r][cc]: K-step r, K index 4q + r
-#pragma unroll
-            for (int r = 0; r < 4; r++) {
-                const int m2s = 16 * tM + drow32(q, r);
-                cplx pyw = phy[m2s];
-                const double w = (m2s >= P2) ? 0.5 : 1.0;
-                pyw.x *= w;
-                pyw.y *= w;
-                Split eur[NTAB], eui[NTAB];
-#pragma unroll
-                for (int u = 0; u < NTAB; u++) {
-                    cplx eu = cmul(E[u * tstride + m2s * 16 + cc], pyw);
-                    eur[u] = split(eu.x);
-                    eui[u] = split(-eu.y);
-                }
-#pragma unroll
-                for (int o = 0; o < NOUT; o++) {
-                    out[o] = mfma2(eur[chan_y(o)], FFr[chan_x(o)][r], out[o]);
-                    out[o] = mfma2(eui[chan_y(o)], FFi[chan_x(o)][r], out[o]);
+                    FFr[t][tM] = mfma3(sr, er[t], FFr[t][tM]);
+                    FFr[t][tM] = mfma3(di, ei[t], FFr[t][tM]);
+                    FFi[t][tM] = mfma3(si, er[t], FFi[t][tM]);
+                    FFi[t][tM] = mfma3(dr, ei[t], FFi[t][tM]);
                 }
             }
         }
-        // register r of out^T holds [k2 = 4q + r][k1 = cc].  The outputs are
-        // read at clamped points for all r first (one memory round trip), then
-        // added and stored where valid.
-        const double w2 = 2.0 * oscale;
-        double* dst[NOUT];
-        dst[0] = pot;
-        if constexpr (NTAB >= 2) dst[1] = dst[2] = grad;
-        if constexpr (NTAB >= 3) dst[3] = dst[4] = dst[5] = hess;
-        constexpr int stride_o[6] = {1, 2, 2, 3, 3, 3}, comp_o[6] = {0, 0, 1, 0, 1, 2};
-        double old[4][NOUT];
+        for (int li = 0; li < g.y; li++) {
+            const PwLeaf lf = leaves[g.x + li];
+            const cplx* phy = ph + lf.jy * ps;
+            v4f out[NOUT];
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
-            const size_t pb = (size_t)lf.box * npbox + min(cc, n - 1) + (size_t)n * min(drow32(q, r), n - 1);
+            for (int o = 0; o < NOUT; o++) out[o] = v4f{0, 0, 0, 0};
 #pragma unroll
-            for (int o = 0; o < NOUT; o++) old[r][o] = dst[o][(pb * stride_o[o] + comp_o[o]) * nd + d];
-        }
-        if (cc < n) {
+            for (int tM = 0; tM < NT; tM++) {
+                // register r of FF holds FF^T[16 tM + 4q + r][cc]: K-step r, K index 4q + r
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const int m2s = 16 * tM + drow32(q, r);
+                    cplx pyw = phy[m2s];
+                    const double w = (m2s >= P2) ? 0.5 : 1.0;
+                    pyw.x *= w;
+                    pyw.y *= w;
+                    Split eur[NTAB], eui[NTAB];
+#pragma unroll
+                    for (int u = 0; u < NTAB; u++) {
+                        cplx eu = cmul(E[u * tstride + m2s * 16 + cc], pyw);
+                        eur[u] = split(eu.x);
+                        eui[u] = split(-eu.y);
+                    }
+#pragma unroll
+                    for (int o = 0; o < NOUT; o++) {
+                        out[o] = mfma2(eur[chan_y(o)], FFr[chan_x(o)][tM][r], out[o]);
+                        out[o] = mfma2(eui[chan_y(o)], FFi[chan_x(o)][tM][r], out[o]);
+                    }
+                }
+            }
+            // register r of out^T holds [k2 = 4q + r][k1 = cc].  The outputs
+            // are read at clamped points for all r first (one memory round
+            // trip), then added and stored where valid.
+            double old[4][NOUT];
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                const int k2 = drow32(q, r);
-                if (k2 < n) {
-                    const size_t pb = (size_t)lf.box * npbox + cc + (size_t)n * k2;
+                const size_t pb = (size_t)lf.box * npbox + min(cc, n - 1) + (size_t)n * min(drow32(q, r), n - 1);
 #pragma unroll
-                    for (int o = 0; o < NOUT; o++)
-                        dst[o][(pb * stride_o[o] + comp_o[o]) * nd + d] = old[r][o] + w2 * (double)out[o][r];
+                for (int o = 0; o < NOUT; o++) old[r][o] = dst[o][(pb * stride_o[o] + comp_o[o]) * nd + d];
+            }
+            if (cc < n) {
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const int k2 = drow32(q, r);
+                    if (k2 < n) {
+                        const size_t pb = (size_t)lf.box * npbox + cc + (size_t)n * k2;
+#pragma unroll
+                        for (int o = 0; o < NOUT; o++)
+                            dst[o][(pb * stride_o[o] + comp_o[o]) * nd + d] = old[r][o] + w2 * (double)out[o][r];
+                    }
                 }
             }
         }
@@ -1674,8 +1702,8 @@ static int launch_up_split2(Plan& p, const double* fvals, int nTN) {
     int nitems = p.nchunks * nTN;
     int blocks = (nitems + kWavesPerBlock - 1) / kWavesPerBlock;
     k_up_split<NKS, NTM><<<blocks, kWave * kWavesPerBlock, 0, p.stream>>>(
-        p.n, p.npw, p.pstride, p.nd, nTN, nitems, fvals, p.d_pwleaf, p.d_chunks, p.d_tup_f, 16 * p.pstride,
-        p.d_phase, p.d_levbase, p.d_mp);
+        p.n, p.npw, p.pstride, p.nd, nTN, nitems, fvals, p.d_pwleaf, p.d_upcc, p.d_upcols, p.d_tup_f,
+        16 * p.pstride, p.d_phase, p.d_levbase, p.d_mp);
     BFGT_HIP_CHECK(hipGetLastError());
     return BFGT_OK;
 }
@@ -1821,12 +1849,21 @@ static int launch_down_mfma(Plan& p, const cplx* tab, int lstride, int ymid_half
 
 template <int NTAB>
 static int launch_down_split(Plan& p, double* pot, double* grad, double* hess) {
-    int blocks = (p.npwleaf + kWavesPerBlock - 1) / kWavesPerBlock;
-    k_down_split<NTAB><<<blocks, kWave * kWavesPerBlock, 0, p.stream>>>(
-        p.n, p.npw, p.pstride, p.nd, p.npwleaf, p.d_pwleaf, p.d_tdn_f, 3 * 16 * p.pstride, p.d_phase, p.d_levbase,
-        reinterpret_cast<const Fold*>(p.d_loc), pot, grad, hess, p.out_scale);
-    BFGT_HIP_CHECK(hipGetLastError());
-    return BFGT_OK;
+    int blocks = (p.ncolgrp + kWavesPerBlock - 1) / kWavesPerBlock;
+    const int nT = round_up((p.npw + 1) / 2, 16) / 16;
+    auto run = [&](auto kern) -> int {
+        kern<<<blocks, kWave * kWavesPerBlock, 0, p.stream>>>(
+            p.n, p.npw, p.pstride, p.nd, p.ncolgrp, p.d_colgrp, p.d_pwleaf, p.d_tdn_f, 3 * 16 * p.pstride, p.d_phase,
+            p.d_levbase, reinterpret_cast<const Fold*>(p.d_loc), pot, grad, hess, p.out_scale);
+        BFGT_HIP_CHECK(hipGetLastError());
+        return BFGT_OK;
+    };
+    switch (nT) {
+        case 1: return run(k_down_split<NTAB, 1>);
+        case 2: return run(k_down_split<NTAB, 2>);
+        case 3: return run(k_down_split<NTAB, 3>);
+        default: return run(k_down_split<NTAB, 4>);
+    }
 }
 
 int launch_downward(Plan& p, double* pot, double* grad, double* hess) {
