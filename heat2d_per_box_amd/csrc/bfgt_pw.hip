@@ -382,6 +382,44 @@ __global__ __launch_bounds__(kWave * kUpReduceWaves) void k_up_col(
                                              comp_o[oo] * nd * (int)sizeof(double));
                 return o;
             };
+            if constexpr (NTAB == 1) {
+                // one table: its rows stay in registers for the column, and
+                // the leaf values of a whole pass are loaded one pass ahead
+                // (the loop's loads are then the leaf values alone, waited
+                // for in the order they were issued)
+                cplx tt[NKS];
+#pragma unroll
+                for (int s = 0; s < NKS; s++) tt[s] = bload_c(rT, oTy, 4 * s * rowb);
+                struct PV {
+                    cplx py[LPC];
+                    double a[NKS][LPC];
+                };
+                auto pv_at = [&](int li) {
+                    const Pass pa = pass_at(li);
+                    PV v;
+#pragma unroll
+                    for (int h = 0; h < LPC; h++) {
+                        v.py[h] = pa.py[h];
+#pragma unroll
+                        for (int s = 0; s < NKS; s++) v.a[s][h] = bload_d(pa.r[h][0], oF[s], 0);
+                    }
+                    return v;
+                };
+                PV vc = pv_at(0);
+                for (int li = 0; li < col.y; li += LPC) {
+                    const PV vn = pv_at(min(li + LPC, col.y - 1));
+                    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                    for (int s = 0; s < NKS; s++)
+#pragma unroll
+                        for (int h = 0; h < LPC; h++) {
+                            const cplx ty = cmul(tt[s], vc.py[h]);
+                            Gr[0] = mfma(vc.a[s][h], ty.x, Gr[0]);
+                            Gi[0] = mfma(vc.a[s][h], ty.y, Gi[0]);
+                        }
+                    vc = vn;
+                }
+            } else {
             Pass pc = pass_at(0);
             YOp yc = yload(pc, 0);
             for (int li = 0; li < col.y; li += LPC) {
@@ -406,6 +444,7 @@ __global__ __launch_bounds__(kWave * kUpReduceWaves) void k_up_col(
                     yc = yn;
                 }
                 pc = pn;
+            }
             }
 #pragma unroll
             for (int t = 0; t < NTM; t++) {
@@ -1654,7 +1693,7 @@ static int launch_up_mfma3(Plan& p, const UpArgs& u, int nTN) {
                                                             p.d_upquads, u.tab, u.lstride, u.ymid_half, p.d_phase,
                                                             p.d_levbase, p.d_mp);
     } else {
-        constexpr int LPC = NTAB == 1 ? 2 : 1;
+        constexpr int LPC = 1;
         constexpr bool BRC = NTM <= 2 && NTAB == 1;
         auto kern = br ? (ds ? k_up_col<NKS, NTM, NTAB, LPC, BRC, true> : k_up_col<NKS, NTM, NTAB, LPC, BRC, false>)
                        : (ds ? k_up_col<NKS, NTM, NTAB, LPC, false, true> : k_up_col<NKS, NTM, NTAB, LPC, false, false>);
