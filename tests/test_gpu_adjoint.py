@@ -207,6 +207,42 @@ def test_dense_transpose_matches_oracle(device, case):
     assert np.linalg.norm(got - want) / np.linalg.norm(want) < 1e-12
 
 
+PACKED = [
+    # (n, nd, ifpgh, delta, iperiod): the adjoint near field for n <= 8 with
+    # two or more densities holds two densities per product (k_near_adj_pack)
+    (4, 2, 3, 1e-4, 1),
+    (6, 3, 2, 1e-4, 0),
+    (8, 2, 1, 2e-4, 1),
+    (8, 3, 3, 1e-3, 1),
+]
+
+
+@pytest.mark.parametrize("case", PACKED, ids=lambda c: f"n{c[0]}-nd{c[1]}-pgh{c[2]}-d{c[3]:g}-ip{c[4]}")
+def test_dense_transpose_packed_near_field(device, case):
+    """Every density against the oracle's dense transpose (the densities are
+    independent: fbar_d = A^T g_d), near-field-dominated and mixed trees."""
+    n, nd, ifpgh, delta, iperiod = case
+    g = GaussianSum(np.array([[0.1, -0.2]]), np.array([0.03]), np.array([1.0]))
+    t = build_adaptive(lambda p: g(p)[..., None], 1, n, 0, 1e-4, 4, iperiod=iperiod, min_level=1, eta=0.0)
+    A, leaves = _oracle_dense(t, delta, 1e-9, 0, iperiod, n, ifpgh)
+    rng = np.random.default_rng(9)
+    nb, npb = t.centers.shape[0], n * n
+    gp = _rand_leaf(rng, t, (nb, npb, nd))
+    gg = _rand_leaf(rng, t, (nb, npb, 2, nd)) if ifpgh >= 2 else None
+    gh = _rand_leaf(rng, t, (nb, npb, 3, nd)) if ifpgh >= 3 else None
+    r = _gpu_forward_adjoint(t, nd, delta, 1e-9, 0, iperiod, n, ifpgh, np.zeros((nb, npb, nd)), gp, gg, gh)
+    assert r["plan"].info["n_near_pairs"] > 0
+    for d in range(nd):
+        gvec = [gp[leaves, :, d]]
+        if ifpgh >= 2:
+            gvec += [gg[leaves, :, 0, d], gg[leaves, :, 1, d]]
+        if ifpgh >= 3:
+            gvec += [gh[leaves, :, j, d] for j in range(3)]
+        want = A.T @ np.concatenate([v.reshape(-1) for v in gvec])
+        got = r["fbar"][leaves, :, d].reshape(-1)
+        assert np.linalg.norm(got - want) / np.linalg.norm(want) < 1e-12, d
+
+
 def test_autograd_matches_adjoint_and_fd(device):
     """torch.autograd through the Function: gradient in f and delta vs central FD."""
     import torch
