@@ -11,7 +11,7 @@ from heat2d_per_box_amd.boxfgt import BoxFGTError, boxfgt
 from heat2d_per_box_amd.densities import gaussian_sum
 from heat2d_per_box_amd.tree import build_adaptive, build_uniform, with_capacity
 from oracle import oracle as O
-from tests.test_oracle_newtree import CASES, _case
+from tests.test_oracle_newtree import CASES, RANDOM_SEEDS, _case, _random_case
 from tests.treecmp import same_tree_arrays
 
 pytestmark = pytest.mark.gpu
@@ -68,6 +68,22 @@ def test_newtree_matches_oracle_and_reference(device, case):
         assert not same_tree_arrays(t, rr.tree)
         for name, g in (("pot", pot), ("grad", grad), ("hess", hess))[:ifpgh]:
             assert _rel(g[:nb][lv], getattr(rr, name)[:nb][lv]) < 1e-12, name
+
+
+@pytest.mark.parametrize("seed", RANDOM_SEEDS)
+def test_newtree_random_trees(device, seed):
+    tc, fv, nd, delta, eps, ipoly, iperiod, n, ifpgh = _random_case(seed)
+    print(f"[newtree] random seed {seed}: n={n} nd={nd} ifpgh={ifpgh} delta={delta:.3g} eps={eps:g} "
+          f"iperiod={iperiod} ipoly={ipoly} boxes={tc.nboxes}", flush=True)
+    t, pot, grad, hess, ti, _ = _run_gpu(tc, fv, nd, delta, eps, ipoly, iperiod, n, ifpgh)
+    ro = O.oracle_boxfgt_newtree(tc, fv, nd, delta, eps, ipoly, iperiod, n, ifpgh)
+    assert ti[18] == ro.info["nnew"]
+    msgs = same_tree_arrays(t, ro.tree)
+    assert not msgs, msgs
+    nb = t.nboxes
+    lv = t.nchild == 0
+    for name, g in (("pot", pot), ("grad", grad), ("hess", hess))[:ifpgh]:
+        assert _rel(g[:nb][lv], getattr(ro, name)[:nb][lv]) < 1e-12, name
 
 
 def test_newtree_c3_workload(device):

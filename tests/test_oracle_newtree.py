@@ -40,6 +40,49 @@ def _case(seed, n, delta, iperiod, ifpgh, eps, lev, nd, ipoly):
     return with_capacity(t, 4 * t.nboxes + 400), fv
 
 
+# random trees and parameters whose refinement stays above the finest level
+RANDOM_SEEDS = [0, 3, 6, 7, 12, 14, 22, 23]
+
+
+def _random_case(seed):
+    """(capacity tree, fvals, nd, delta, eps, ipoly, iperiod, n, ifpgh) drawn
+    from seed: leaf order 4..16, nd 1..2, every ifpgh, delta 1e-5..0.3, eps
+    1e-6..1e-12, Legendre or Chebyshev, periodic or free space."""
+    from heat2d_per_box_amd.densities import GaussianSum
+
+    rng = np.random.default_rng(1000 + seed)
+    iperiod = int(rng.integers(0, 2))
+    ipoly = int(rng.integers(0, 2))
+    n = int(rng.choice([4, 6, 8, 10, 12, 16]))
+    nd = int(rng.integers(1, 3))
+    ifpgh = int(rng.integers(1, 4))
+    delta = float(10 ** rng.uniform(-5, -0.5))
+    eps = float(rng.choice([1e-6, 1e-9, 1e-12]))
+    m = int(rng.integers(1, 5))
+    g = GaussianSum(rng.uniform(-0.5, 0.5, (m, 2)), rng.uniform(0.01, 0.08, m), rng.uniform(-1, 1, m))
+    lev = int(rng.integers(4, 7))
+    fun = lambda p: np.stack([g(p), np.cos(2 * p[..., 0]) * g(p)][:nd], axis=-1)  # noqa: E731
+    t = build_adaptive(fun, nd, n, ipoly, 10 ** rng.uniform(-8, -4), lev, iperiod=iperiod,
+                       min_level=int(rng.integers(1, 3)))
+    fv = t.sample(fun, n, ipoly, nd)
+    return with_capacity(t, 8 * t.nboxes + 200), fv, nd, delta, eps, ipoly, iperiod, n, ifpgh
+
+
+@pytest.mark.parametrize("seed", RANDOM_SEEDS)
+def test_newtree_random_trees_match_reference(seed):
+    tc, fv, nd, delta, eps, ipoly, iperiod, n, ifpgh = _random_case(seed)
+    ro = O.oracle_boxfgt_newtree(tc, fv, nd, delta, eps, ipoly, iperiod, n, ifpgh)
+    rr = O.reference_boxfgt(tc, fv, nd, delta, eps, ipoly, iperiod, n, ifpgh, ifnewtree=1, nthreads=1)
+    assert ro.info["ncapped"] == 0
+    msgs = same_tree_arrays(ro.tree, rr.tree)
+    assert not msgs, msgs
+    nb = ro.tree.nboxes
+    lv = ro.tree.nchild == 0
+    for name in ("pot", "grad", "hess")[:ifpgh]:
+        a, b = getattr(ro, name)[:nb][lv], getattr(rr, name)[:nb][lv]
+        assert np.linalg.norm(a - b) <= 1e-12 * np.linalg.norm(b), name
+
+
 @pytest.mark.parametrize("case", CASES, ids=[f"s{c[0]}-n{c[1]}-d{c[2]:g}-p{c[3]}-f{c[4]}-nd{c[7]}-q{c[8]}"
                                              for c in CASES])
 def test_newtree_matches_reference(case):
