@@ -974,14 +974,17 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_down_pair(
 // leaf, count); the leaves of a piece share the level and the x cell).
 template <int NKS, int NTM>
 __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_up_split(
-    int n, int P, int ps, int nd, int nTN, int nitems, const double* __restrict__ fvals,
+    int n, int P, int ps, int nd, int nds, int nTN, int nitems, const double* __restrict__ fvals,
     const PwLeaf* __restrict__ leaves, const int4* __restrict__ cpc, const int2* __restrict__ cols,
     const cplx* __restrict__ tab, int lstride, const cplx* __restrict__ phase, const int* __restrict__ levbase,
     cplx* __restrict__ partial) {
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
-    const int item = __builtin_amdgcn_readfirstlane(blockIdx.x * kWavesPerBlock + wave);
+    const int item = __builtin_amdgcn_readfirstlane(xcd_block(blockIdx.x, gridDim.x) * kWavesPerBlock + wave);
     if (item >= nitems) return;  // no block-wide barriers in this kernel
-    const int ci = item / nTN, tN = item % nTN;
+    // item = (chunk, density if nds > 1, y tile): with nds = nd one density
+    // per wave, else the densities in turn
+    const int tN = item % nTN, ci = item / nTN / nds;
+    const int d0 = nds > 1 ? (item / nTN) % nds : 0, d1 = nds > 1 ? d0 + 1 : nd;
     const int q = lane >> 4, cc = lane & 15;
     const int H = (P + 1) / 2, Mlo = (P + 1) / 2, nexp = P * H, npbox = n * n;
     const int k2 = 16 * tN + cc;
@@ -990,7 +993,7 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_up_split(
 #pragma unroll
     for (int s = 0; s < NKS; s++) pidx[s] = ccl + n * min(4 * s + q, n - 1);
     const int4 ch = cpc[ci];
-    for (int d = 0; d < nd; d++) {
+    for (int d = d0; d < d1; d++) {
         v4f acc[4][NTM];
 #pragma unroll
         for (int p = 0; p < 4; p++)
@@ -1066,7 +1069,7 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_down_split(
     constexpr int NOUT = nchan(NTAB);
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
     // wave-uniform: schedule entries are then read with scalar loads
-    const int item = __builtin_amdgcn_readfirstlane(blockIdx.x * kWavesPerBlock + wave);
+    const int item = __builtin_amdgcn_readfirstlane(xcd_block(blockIdx.x, gridDim.x) * kWavesPerBlock + wave);
     if (item >= ngrp) return;  // no block-wide barriers in this kernel
     const int q = lane >> 4, cc = lane & 15;
     const int H = (P + 1) / 2, P2 = P / 2, npbox = n * n;
@@ -1165,6 +1168,219 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_down_split(
                         for (int o = 0; o < NOUT; o++)
                             dst[o][(pb * stride_o[o] + comp_o[o]) * nd + d] = old[r][o] + w2 * (double)out[o][r];
                     }
+                }
+            }
+        }
+    }
+}
+
+// Split-FP32 counterparts of k_up_quad and k_down_quad (n <= 8): the same
+// 2 x 2 leaf blocks over column pairs.  The FP32 accumulator holds D[4q +
+// r][c] (the FP64 one D[4r + q][c]), so a product that reads an accumulator
+// as its B operand takes K index 4q + r in K-step r, and the A operand is
+// loaded at that index.
+template <int NTM>
+__global__ __launch_bounds__(kWave * kWavesPerBlock) void k_up_split_quad(
+    int n, int P, int ps, int nd, int nds, int nTN, int nitems, const double* __restrict__ fvals,
+    const PwLeaf* __restrict__ leaves, const int4* __restrict__ chunks, const int4* __restrict__ segs,
+    const int4* __restrict__ quads, const cplx* __restrict__ tab, int lstride, const cplx* __restrict__ phase,
+    const int* __restrict__ levbase, cplx* __restrict__ partial) {
+    const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    const int item = __builtin_amdgcn_readfirstlane(xcd_block(blockIdx.x, gridDim.x) * kWavesPerBlock + wave);
+    if (item >= nitems) return;  // no block-wide barriers in this kernel
+    const int tN = item % nTN, ci = item / nTN / nds;  // as in k_up_split
+    const int d0 = nds > 1 ? (item / nTN) % nds : 0, d1 = nds > 1 ? d0 + 1 : nd;
+    const int q = lane >> 4, cc = lane & 15;
+    const int half = cc >> 3, xi = min(cc & 7, n - 1);
+    const int H = (P + 1) / 2, Mlo = (P + 1) / 2, nexp = P * H, npbox = n * n;
+    const int k2 = 16 * tN + cc;
+    const int4 ch = chunks[ci];
+    for (int d = d0; d < d1; d++) {
+        v4f acc[4][NTM];
+#pragma unroll
+        for (int p = 0; p < 4; p++)
+#pragma unroll
+            for (int t = 0; t < NTM; t++) acc[p][t] = v4f{0, 0, 0, 0};
+        for (int sg = ch.x; sg < ch.x + ch.y; sg++) {
+            const int4 seg = segs[sg];
+            const PwLeaf la = leaves[seg.z];
+            const int jxb = seg.w >= 0 ? leaves[seg.w].jx : la.jx;
+            const double wb = seg.w >= 0 ? 1.0 : 0.0;
+            const cplx* T = tab + (size_t)la.level * lstride;
+            const cplx* ph = phase + levbase[la.level];
+            // G = sum over the quads of F (T diag py): rows (column, x point),
+            // K (y cell, y point), one product for four leaves
+            v4f Gr = v4f{0, 0, 0, 0}, Gi = v4f{0, 0, 0, 0};
+            for (int k = seg.x; k < seg.x + seg.y; k++) {
+                const int4 qd = quads[k];
+                const int ra = qd.x >= 0 ? qd.x : qd.y, rb = qd.z >= 0 ? qd.z : qd.w;
+                const int jya = leaves[ra].jy, jyb = rb >= 0 ? leaves[rb].jy : jya;
+                const int lfa = half ? qd.y : qd.x, lfb = half ? qd.w : qd.z;
+                const size_t pba = (size_t)leaves[lfa >= 0 ? lfa : ra].box * npbox;
+                const size_t pbb = (size_t)leaves[lfb >= 0 ? lfb : ra].box * npbox;
+                const double wa = lfa >= 0 ? 1.0 : 0.0, wbr = lfb >= 0 ? 1.0 : 0.0;
+                cplx pya = ph[jya * ps + k2], pyb = ph[jyb * ps + k2];
+                pya.y = -pya.y;  // upward shift: conjugate of the table
+                pyb.y = -pyb.y;
+                if (rb < 0) pyb = cplx{0.0, 0.0};
+#pragma unroll
+                for (int s = 0; s < 4; s++) {
+                    const int yp = 4 * (s & 1) + q;  // y point of this K index
+                    const cplx py = (s >> 1) ? pyb : pya;
+                    const cplx ty = cmul(T[min(yp, 15) * ps + k2], py);
+                    const Split a = split(((s >> 1) ? wbr : wa) *
+                                          fvals[(((s >> 1) ? pbb : pba) + xi + (size_t)n * min(yp, n - 1)) * nd + d]);
+                    Gr = mfma3(a, split(ty.x), Gr);
+                    Gi = mfma3(a, split(ty.y), Gi);
+                }
+            }
+            // out += (T diag px)^T G: K-step r reads G's element r, K index
+            // 4q + r = (column q >> 1, x point 4 (q & 1) + r)
+#pragma unroll
+            for (int t = 0; t < NTM; t++) {
+                const int k1 = 16 * t + cc;
+                cplx pxa = ph[la.jx * ps + k1], pxb = ph[jxb * ps + k1];
+                pxa.y = -pxa.y;
+                pxb.x *= wb;
+                pxb.y *= -wb;
+                const cplx px = (q >> 1) ? pxb : pxa;
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const int xp = 4 * (q & 1) + r;
+                    const cplx tx = cmul(T[xp * ps + k1], px);
+                    const Split xr = split(tx.x), xi2 = split(tx.y);
+                    acc[0][t] = mfma2(xr, Gr[r], acc[0][t]);
+                    acc[1][t] = mfma2(xi2, Gi[r], acc[1][t]);
+                    acc[2][t] = mfma2(xr, Gi[r], acc[2][t]);
+                    acc[3][t] = mfma2(xi2, Gr[r], acc[3][t]);
+                }
+            }
+        }
+        cplx* out = partial + ((size_t)ch.z * nd + d) * nexp;
+#pragma unroll
+        for (int t = 0; t < NTM; t++) {
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                int k1 = 16 * t + drow32(q, r);
+                if (k1 < Mlo && k2 < H) {
+                    double AC = acc[0][t][r], BD = acc[1][t][r], AD = acc[2][t][r], BC = acc[3][t][r];
+                    out[k1 * H + k2] = cplx{AC - BD, AD + BC};
+                    int km = P - 1 - k1;
+                    if (km > k1) out[km * H + k2] = cplx{AC + BD, AD - BC};
+                }
+            }
+        }
+    }
+}
+
+template <int NTAB, int NT>
+__global__ __launch_bounds__(kWave * kWavesPerBlock) void k_down_split_quad(
+    int n, int P, int ps, int nd, int ngroups, const int4* __restrict__ qgroups, const int4* __restrict__ quads,
+    const PwLeaf* __restrict__ leaves, const cplx* __restrict__ tab, int lstride, const cplx* __restrict__ phase,
+    const int* __restrict__ levbase, const Fold* __restrict__ fold, double* __restrict__ pot,
+    double* __restrict__ grad, double* __restrict__ hess, double oscale) {
+    constexpr int NOUT = nchan(NTAB);
+    const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    const int item = __builtin_amdgcn_readfirstlane(xcd_block(blockIdx.x, gridDim.x) * kWavesPerBlock + wave);
+    if (item >= ngroups * nd) return;  // no block-wide barriers in this kernel
+    const int4 g = qgroups[item / nd];
+    const int d = item % nd;
+    const int q = lane >> 4, cc = lane & 15;
+    const int half = cc >> 3, ci = cc & 7;
+    const int H = (P + 1) / 2, P2 = P / 2, npbox = n * n;
+    const int Mh4 = round_up(H, 4), H16 = round_up(H, 16);
+    const int nks1 = Mh4 / 4;
+    const int tstride = 16 * ps;
+    const PwLeaf la = leaves[g.z];
+    const int jxb = g.w >= 0 ? leaves[g.w].jx : la.jx;
+    const cplx* E = tab + (size_t)la.level * lstride;
+    const cplx* ph = phase + levbase[la.level];
+    const cplx* phx = ph + (half ? jxb : la.jx) * ps;
+    const double wx = (half && g.w < 0) ? 0.0 : 1.0;
+    const double w2 = 2.0 * oscale;
+    constexpr int stride_o[6] = {1, 2, 2, 3, 3, 3}, comp_o[6] = {0, 0, 1, 0, 1, 2};
+    double* dst[NOUT];
+    dst[0] = pot;
+    if constexpr (NTAB >= 2) dst[1] = dst[2] = grad;
+    if constexpr (NTAB >= 3) dst[3] = dst[4] = dst[5] = hess;
+    const Fold* F = fold + ((size_t)la.anc * nd + d) * Mh4 * H16;
+    // x contraction of both columns, every y tile
+    v4f FFr[NTAB][NT], FFi[NTAB][NT];
+#pragma unroll
+    for (int t = 0; t < NTAB; t++)
+#pragma unroll
+        for (int tM = 0; tM < NT; tM++) FFr[t][tM] = FFi[t][tM] = v4f{0, 0, 0, 0};
+    for (int s = 0; s < nks1; s++) {
+        const int m = 4 * s + q;
+        cplx px = phx[m];
+        px.x *= wx;
+        px.y *= wx;
+        Split er[NTAB], ei[NTAB];
+#pragma unroll
+        for (int t = 0; t < NTAB; t++) {
+            const cplx e = cmul(E[t * tstride + m * 16 + ci], px);
+            er[t] = split(e.x);
+            ei[t] = split(e.y);
+        }
+#pragma unroll
+        for (int tM = 0; tM < NT; tM++) {
+            const Fold sd = F[m * H16 + 16 * tM + cc];
+            Split sr = split(sd.S.x), si = split(sd.S.y), dr = split(sd.D.x), di = neg(split(sd.D.y));
+#pragma unroll
+            for (int t = 0; t < NTAB; t++) {
+                FFr[t][tM] = mfma3(sr, er[t], FFr[t][tM]);
+                FFr[t][tM] = mfma3(di, ei[t], FFr[t][tM]);
+                FFi[t][tM] = mfma3(si, er[t], FFi[t][tM]);
+                FFi[t][tM] = mfma3(dr, ei[t], FFi[t][tM]);
+            }
+        }
+    }
+    for (int k = g.x; k < g.x + g.y; k++) {
+        const int4 qd = quads[k];
+        // this lane's row of the y contraction: y cell jyA (half 0) or jyB
+        const int ra = qd.x >= 0 ? qd.x : qd.y, rb = qd.z >= 0 ? qd.z : qd.w;
+        const int lrow = half ? (rb >= 0 ? rb : ra) : ra;
+        const double wy = (half && rb < 0) ? 0.0 : 1.0;
+        const cplx* phy = ph + leaves[lrow].jy * ps;
+        v4f out[NOUT];
+#pragma unroll
+        for (int o = 0; o < NOUT; o++) out[o] = v4f{0, 0, 0, 0};
+#pragma unroll
+        for (int tM = 0; tM < NT; tM++) {
+            // register r of FF holds FF^T[16 tM + 4q + r][cc]: K-step r, K index 4q + r
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int m2s = 16 * tM + drow32(q, r);
+                cplx pyw = phy[m2s];
+                const double w = (m2s >= P2 ? 0.5 : 1.0) * wy;
+                pyw.x *= w;
+                pyw.y *= w;
+                Split eur[NTAB], eui[NTAB];
+#pragma unroll
+                for (int u = 0; u < NTAB; u++) {
+                    const cplx eu = cmul(E[u * tstride + m2s * 16 + ci], pyw);
+                    eur[u] = split(eu.x);
+                    eui[u] = split(-eu.y);
+                }
+#pragma unroll
+                for (int o = 0; o < NOUT; o++) {
+                    out[o] = mfma2(eur[chan_y(o)], FFr[chan_x(o)][tM][r], out[o]);
+                    out[o] = mfma2(eui[chan_y(o)], FFi[chan_x(o)][tM][r], out[o]);
+                }
+            }
+        }
+        // element r of lane (q, cc) = D[row 4q + r][col cc]: leaf (column
+        // cc >> 3, row q >> 1), point (ci, 4 (q & 1) + r)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int lid = (q >> 1) ? (half ? qd.w : qd.z) : (half ? qd.y : qd.x);
+            const int yp = drow32(q, r) & 7;
+            if (lid >= 0 && ci < n && yp < n) {
+                const size_t pb = (size_t)leaves[lid].box * npbox + ci + (size_t)n * yp;
+#pragma unroll
+                for (int o = 0; o < NOUT; o++) {
+                    double& v = dst[o][(pb * stride_o[o] + comp_o[o]) * nd + d];
+                    v += w2 * (double)out[o][r];
                 }
             }
         }
@@ -1736,12 +1952,19 @@ static int launch_up_mfma(Plan& p, const UpArgs& u) {
     }
 }
 
+// densities on separate waves (nd) or in turn (1): separate where the
+// chunks alone give few waves, as in the FP64 pass
+static int split_density_waves(const Plan& p, int nTN) {
+    return ((long)nTN * p.nchunks < kUpSplitWaves) ? p.nd : 1;
+}
+
 template <int NKS, int NTM>
 static int launch_up_split2(Plan& p, const double* fvals, int nTN) {
-    int nitems = p.nchunks * nTN;
+    const int nds = split_density_waves(p, nTN);
+    int nitems = p.nchunks * nds * nTN;
     int blocks = (nitems + kWavesPerBlock - 1) / kWavesPerBlock;
     k_up_split<NKS, NTM><<<blocks, kWave * kWavesPerBlock, 0, p.stream>>>(
-        p.n, p.npw, p.pstride, p.nd, nTN, nitems, fvals, p.d_pwleaf, p.d_upcc, p.d_upcols, p.d_tup_f,
+        p.n, p.npw, p.pstride, p.nd, nds, nTN, nitems, fvals, p.d_pwleaf, p.d_upcc, p.d_upcols, p.d_tup_f,
         16 * p.pstride, p.d_phase, p.d_levbase, p.d_mp);
     BFGT_HIP_CHECK(hipGetLastError());
     return BFGT_OK;
@@ -1757,9 +1980,28 @@ static int launch_up_split1(Plan& p, const double* fvals, int nTM, int nTN) {
     }
 }
 
+template <int NTM>
+static int launch_up_split_quad(Plan& p, const double* fvals, int nTN) {
+    const int nds = split_density_waves(p, nTN);
+    const int nitems = p.nchunks * nds * nTN;
+    const int blocks = (nitems + kWavesPerBlock - 1) / kWavesPerBlock;
+    k_up_split_quad<NTM><<<blocks, kWave * kWavesPerBlock, 0, p.stream>>>(
+        p.n, p.npw, p.pstride, p.nd, nds, nTN, nitems, fvals, p.d_pwleaf, p.d_upqcc, p.d_upsegs, p.d_upquads, p.d_tup_f,
+        16 * p.pstride, p.d_phase, p.d_levbase, p.d_mp);
+    BFGT_HIP_CHECK(hipGetLastError());
+    return BFGT_OK;
+}
 static int launch_up_split(Plan& p, const double* fvals) {
     int H = (p.npw + 1) / 2;
     int nTM = (H + 15) / 16, nTN = (H + 15) / 16;
+    if (p.n <= 8) {  // 2 x 2 leaf blocks, as the FP64 pass
+        switch (nTM) {
+            case 1: return launch_up_split_quad<1>(p, fvals, nTN);
+            case 2: return launch_up_split_quad<2>(p, fvals, nTN);
+            case 3: return launch_up_split_quad<3>(p, fvals, nTN);
+            default: return launch_up_split_quad<4>(p, fvals, nTN);
+        }
+    }
     switch ((p.n + 3) / 4) {
         case 1: return launch_up_split1<1>(p, fvals, nTM, nTN);
         case 2: return launch_up_split1<2>(p, fvals, nTM, nTN);
@@ -1888,8 +2130,25 @@ static int launch_down_mfma(Plan& p, const cplx* tab, int lstride, int ymid_half
 
 template <int NTAB>
 static int launch_down_split(Plan& p, double* pot, double* grad, double* hess) {
-    int blocks = (p.ncolgrp + kWavesPerBlock - 1) / kWavesPerBlock;
     const int nT = round_up((p.npw + 1) / 2, 16) / 16;
+    if (p.n <= 8) {  // 2 x 2 leaf blocks over column pairs, one wave per (group, density)
+        const int qblocks = (p.nqgroup * p.nd + kWavesPerBlock - 1) / kWavesPerBlock;
+        auto runq = [&](auto kern) -> int {
+            kern<<<qblocks, kWave * kWavesPerBlock, 0, p.stream>>>(
+                p.n, p.npw, p.pstride, p.nd, p.nqgroup, p.d_qgroups, p.d_quads, p.d_pwleaf, p.d_tdn_f,
+                3 * 16 * p.pstride, p.d_phase, p.d_levbase, reinterpret_cast<const Fold*>(p.d_loc), pot, grad, hess,
+                p.out_scale);
+            BFGT_HIP_CHECK(hipGetLastError());
+            return BFGT_OK;
+        };
+        switch (nT) {
+            case 1: return runq(k_down_split_quad<NTAB, 1>);
+            case 2: return runq(k_down_split_quad<NTAB, 2>);
+            case 3: return runq(k_down_split_quad<NTAB, 3>);
+            default: return runq(k_down_split_quad<NTAB, 4>);
+        }
+    }
+    int blocks = (p.ncolgrp + kWavesPerBlock - 1) / kWavesPerBlock;
     auto run = [&](auto kern) -> int {
         kern<<<blocks, kWave * kWavesPerBlock, 0, p.stream>>>(
             p.n, p.npw, p.pstride, p.nd, p.ncolgrp, p.d_colgrp, p.d_pwleaf, p.d_tdn_f, 3 * 16 * p.pstride, p.d_phase,
