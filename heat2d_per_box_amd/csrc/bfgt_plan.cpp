@@ -40,6 +40,7 @@ int set_error(int code, const std::string& msg) {
 size_t node_buffer_doubles(int n);
 size_t up_smem_bytes(int n, int npw);
 size_t down_smem_bytes(int n, int npw, int ntab);
+size_t adj_eval_smem_bytes(int n, int npw, int ntab);
 size_t near_smem_bytes(int n, int ntab);
 size_t loc_buffer_cplx(const Plan& p);
 int build_pw_nodes(Plan& p);
@@ -977,7 +978,9 @@ int plan_create(Plan& p, int nd, int ndim, double delta, double eps, int ipoly, 
     if ((rc = upload(p, p.d_nearT_src, nsrc))) return rc;
     if ((rc = upload(p, p.d_nearT_ptr, nsptr))) return rc;
     if ((rc = upload(p, p.d_nearT, pairsT))) return rc;
-    p.has_adjoint = !p.has_pw || use_mfma(p);
+    // the vector kernels' adjoint of the evaluation stages the evaluation
+    // tables of every channel and one channel's cotangents at once
+    p.has_adjoint = !p.has_pw || use_mfma(p) || adj_eval_smem_bytes(p.n, p.npw, p.ifpgh) <= lds_max;
     // split-FP32 exists for the matrix-core leaf transforms only; a plan on the
     // vector kernels runs (and reports) FP64
     if (p.has_pw && !use_mfma(p)) p.precision = BFGT_PREC_FP64;
@@ -1074,7 +1077,7 @@ int plan_adjoint(Plan& p, const double* gpot, const double* ggrad, const double*
     if (!gpot || !fbar || (p.ifpgh >= 2 && !ggrad) || (p.ifpgh >= 3 && !ghess))
         return set_error(BFGT_ERR_ARG, "null device array");
     if (!p.has_adjoint)
-        return set_error(BFGT_ERR_RESOURCE, "the adjoint needs leaf order <= 16 and at most 128 plane waves");
+        return set_error(BFGT_ERR_RESOURCE, "the adjoint of this plan's evaluation does not fit the local data share");
     BFGT_HIP_CHECK(hipSetDevice(p.device));
     p.out_scale = 1.0;
     int rc;

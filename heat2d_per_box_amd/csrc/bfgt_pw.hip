@@ -1466,8 +1466,12 @@ __global__ __launch_bounds__(kVecThreads) void k_up_vec(int n, int npw, int nd, 
 
 constexpr int kDownChunk = 8;  // y-frequency columns staged in LDS at a time
 
+// Evaluation of the cutoff boxes' local expansions on the PW leaves, one
+// workgroup per leaf.  With adj = 1 the same pass is the adjoint of the leaf
+// transform: tpw is then the values-to-PW table tpoly ([level][m][k]), read
+// conjugated and transposed as one table, and every y frequency has weight 1.
 template <int NQ>
-__global__ __launch_bounds__(kVecThreads) void k_down_vec(int n, int npw, int nd, int ntab,
+__global__ __launch_bounds__(kVecThreads) void k_down_vec(int n, int npw, int nd, int ntab, int adj,
                                                           const PwLeaf* __restrict__ leaves,
                                                           const cplx* __restrict__ tpw, const double* __restrict__ ts,
                                                           const cplx* __restrict__ loc, double* __restrict__ pot,
@@ -1485,8 +1489,17 @@ __global__ __launch_bounds__(kVecThreads) void k_down_vec(int n, int npw, int nd
     cplx* spy = spx + npw;                     // H
     const PwLeaf lf = leaves[blockIdx.x];
     const int tid = threadIdx.x;
-    const cplx* E = tpw + (size_t)lf.level * 3 * npw * n;
-    for (int e = tid; e < ntab * npw * n; e += kVecThreads) sE[e] = E[e];
+    if (adj) {
+        const cplx* T = tpw + (size_t)lf.level * npw * n;  // [m][k]
+        for (int e = tid; e < npw * n; e += kVecThreads) {
+            const int k = e / n, i = e % n;
+            const cplx v = T[(size_t)i * npw + k];
+            sE[e] = cplx{v.x, -v.y};
+        }
+    } else {
+        const cplx* E = tpw + (size_t)lf.level * 3 * npw * n;
+        for (int e = tid; e < ntab * npw * n; e += kVecThreads) sE[e] = E[e];
+    }
     for (int k = tid; k < npw; k += kVecThreads) spx[k] = cexpi(ts[k] * lf.dx);
     for (int k = tid; k < H; k += kVecThreads) spy[k] = cexpi(ts[k] * lf.dy);
     for (int d = 0; d < nd; d++) {
@@ -1521,7 +1534,7 @@ __global__ __launch_bounds__(kVecThreads) void k_down_vec(int n, int npw, int nd
                     int k1 = q % n, k2 = q / n;
                     for (int mm = 0; mm < mc; mm++) {
                         int m2 = c0 + mm;
-                        double w = (m2 < npw2) ? 1.0 : 0.5;
+                        double w = (adj || m2 < npw2) ? 1.0 : 0.5;
                         cplx e0 = sE[(0 * npw + m2) * n + k2];
                         cplx f0 = sff[(0 * kDownChunk + mm) * n + k1];
                         acc[r][0] += w * (e0.x * f0.x - e0.y * f0.y);
@@ -1559,6 +1572,97 @@ __global__ __launch_bounds__(kVecThreads) void k_down_vec(int n, int npw, int nd
                     hess[(pb * 3 + 2) * nd + d] += w2 * acc[r][5];
                 }
             }
+        }
+    }
+}
+
+// Adjoint of the evaluation on the vector kernels, one workgroup per (chunk,
+// segment of the expansion) as k_up_vec: the cotangents g_o of every output
+// channel o of the chunk's leaves, contracted with the conjugated evaluation
+// tables of the channel (x table chan_x(o), y table chan_y(o)), the
+// conjugated leaf phases and the weight 1/2 on the middle y frequency of an
+// odd expansion, are summed into the chunk's partial expansion:
+//   out[k1][k2] += w(k2) px(k1) py(k2) sum_o sum_{m1,m2} g_o[m1][m2]
+//                  conj(E_x(o)[k1][m1]) conj(E_y(o)[k2][m2]).
+template <int kUpMaxE>
+__global__ __launch_bounds__(kVecThreads) void k_adj_eval_vec(int n, int npw, int nd, int ntab, LeafIn in,
+                                                              const PwLeaf* __restrict__ leaves,
+                                                              const UpChunk* __restrict__ chunks,
+                                                              const cplx* __restrict__ tpw,
+                                                              const double* __restrict__ ts,
+                                                              cplx* __restrict__ partial) {
+    extern __shared__ double smem[];
+    const int H = (npw + 1) / 2, npw2 = npw / 2;
+    const int nexp = npw * H;
+    const int npbox = n * n;
+    const int nout = ntab == 1 ? 1 : (ntab == 2 ? 3 : 6);
+    cplx* sE = reinterpret_cast<cplx*>(smem);         // ntab*npw*n : [t][k][i]
+    cplx* sff = sE + ntab * npw * n;                  // H*n : sff[k2*n + m1]
+    cplx* spx = sff + n * H;                          // npw
+    cplx* spy = spx + npw;                            // H (weights included)
+    double* sg = reinterpret_cast<double*>(spy + H);  // n*n : one channel's cotangent
+    const UpChunk ch = chunks[blockIdx.x];
+    const int tid = threadIdx.x;
+    const int e0 = blockIdx.y * kUpMaxE * kVecThreads;
+    for (int d = 0; d < nd; d++) {
+        cplx acc[kUpMaxE];
+#pragma unroll
+        for (int r = 0; r < kUpMaxE; r++) acc[r] = cplx{0.0, 0.0};
+        int cur_level = -1;
+        for (int li = 0; li < ch.count; li++) {
+            const PwLeaf lf = leaves[ch.first + li];
+            __syncthreads();
+            if (lf.level != cur_level) {
+                const cplx* E = tpw + (size_t)lf.level * 3 * npw * n;
+                for (int e = tid; e < ntab * npw * n; e += kVecThreads) sE[e] = E[e];
+                cur_level = lf.level;
+            }
+            for (int k = tid; k < npw; k += kVecThreads) spx[k] = cexpi(-ts[k] * lf.dx);
+            for (int k = tid; k < H; k += kVecThreads) {
+                const cplx v = cexpi(-ts[k] * lf.dy);
+                const double w = (k < npw2) ? 1.0 : 0.5;
+                spy[k] = cplx{w * v.x, w * v.y};
+            }
+            const size_t pb = (size_t)lf.box * npbox;
+            for (int o = 0; o < nout; o++) {
+                __syncthreads();  // sg and sff of the previous channel are consumed
+                for (int e = tid; e < npbox; e += kVecThreads) sg[e] = leaf_in(in, o, pb + e, nd, d);
+                __syncthreads();
+                const cplx* Ex = sE + chan_x(o) * npw * n;
+                const cplx* Ey = sE + chan_y(o) * npw * n;
+                // y contraction
+                for (int e = tid; e < n * H; e += kVecThreads) {
+                    const int m1 = e % n, k2 = e / n;
+                    double sr = 0.0, si = 0.0;
+                    for (int m2 = 0; m2 < n; m2++) {
+                        const double g = sg[m1 + n * m2];
+                        const cplx t = Ey[k2 * n + m2];
+                        sr = fma(t.x, g, sr);
+                        si = fma(-t.y, g, si);
+                    }
+                    sff[e] = cmul(cplx{sr, si}, spy[k2]);
+                }
+                __syncthreads();
+                // x contraction
+#pragma unroll
+                for (int r = 0; r < kUpMaxE; r++) {
+                    const int e = e0 + tid + r * kVecThreads;
+                    if (e < nexp) {
+                        const int k1 = e % npw, k2 = e / npw;
+                        const cplx* fcol = sff + n * k2;
+                        const cplx* ex = Ex + k1 * n;
+                        cplx s{0.0, 0.0};
+                        for (int m1 = 0; m1 < n; m1++) s = cfma(cplx{ex[m1].x, -ex[m1].y}, fcol[m1], s);
+                        acc[r] = cfma(s, spx[k1], acc[r]);
+                    }
+                }
+            }
+        }
+        cplx* out = partial + ((size_t)ch.slot * nd + d) * nexp;
+#pragma unroll
+        for (int r = 0; r < kUpMaxE; r++) {
+            const int e = e0 + tid + r * kVecThreads;
+            if (e < nexp) out[(size_t)(e % npw) * H + e / npw] = acc[r];
         }
     }
 }
@@ -1634,12 +1738,13 @@ __device__ inline cplx m2l_at(const cplx* __restrict__ wshift, const cplx* __res
 }
 
 __global__ void k_m2l(int nd, int nexp, const int* __restrict__ m2l_ptr, const int2* __restrict__ m2l,
-                      const cplx* __restrict__ wshift, const cplx* __restrict__ mp, cplx* __restrict__ loc) {
+                      const cplx* __restrict__ wshift, double csign, const cplx* __restrict__ mp,
+                      cplx* __restrict__ loc) {
     const int a = blockIdx.x;
     const int len = nd * nexp;
     const int j0 = m2l_ptr[a], j1 = m2l_ptr[a + 1];
     for (int e = blockIdx.y * blockDim.x + threadIdx.x; e < len; e += gridDim.y * blockDim.x)
-        loc[(size_t)a * len + e] = m2l_at(wshift, mp, nexp, len, a, e, j0, j1, m2l, e % nexp, 1.0);
+        loc[(size_t)a * len + e] = m2l_at(wshift, mp, nexp, len, a, e, j0, j1, m2l, e % nexp, csign);
 }
 
 // Same translation, written mirror-folded (struct Fold) for the MFMA
@@ -1831,8 +1936,8 @@ int launch_pw_eval_list(Plan& p, const PwLeaf* list, int count, const cplx* loc,
     auto run = [&](auto kern) -> int {
         if (sh > 65536)
             BFGT_HIP_CHECK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
-        kern<<<count, kVecThreads, sh, p.stream>>>(p.n, p.npw, p.nd, p.ifpgh, list, p.d_tpw, p.d_ts, loc, pot, grad,
-                                                   hess, 1.0);
+        kern<<<count, kVecThreads, sh, p.stream>>>(p.n, p.npw, p.nd, p.ifpgh, 0, list, p.d_tpw, p.d_ts, loc, pot,
+                                                   grad, hess, 1.0);
         BFGT_HIP_CHECK(hipGetLastError());
         return BFGT_OK;
     };
@@ -1857,6 +1962,11 @@ size_t loc_buffer_cplx(const Plan& p) {
 size_t up_smem_bytes(int n, int npw) {
     int H = (npw + 1) / 2;
     return sizeof(double) * n * n + sizeof(cplx) * ((size_t)npw * n + (size_t)n * H + npw + H);
+}
+
+size_t adj_eval_smem_bytes(int n, int npw, int ntab) {
+    int H = (npw + 1) / 2;
+    return sizeof(cplx) * ((size_t)ntab * npw * n + (size_t)n * H + npw + H) + sizeof(double) * n * n;
 }
 
 size_t down_smem_bytes(int n, int npw, int ntab) {
@@ -2056,17 +2166,20 @@ int launch_m2l(Plan& p) {
     if (use_mfma(p)) return launch_m2l_fold(p, p.d_m2lg_ptr, p.d_m2lg, p.d_m2lpatch, 1.0);
     int len = p.nd * p.nexp;
     int gy = (len + 255) / 256;
-    k_m2l<<<dim3(p.nanc, gy), 256, 0, p.stream>>>(p.nd, p.nexp, p.d_m2l_ptr, p.d_m2l, p.d_wshift, p.d_mp, p.d_loc);
+    k_m2l<<<dim3(p.nanc, gy), 256, 0, p.stream>>>(p.nd, p.nexp, p.d_m2l_ptr, p.d_m2l, p.d_wshift, 1.0, p.d_mp,
+                                                  p.d_loc);
     BFGT_HIP_CHECK(hipGetLastError());
     return BFGT_OK;
 }
 
+// adj = 1: the adjoint of the leaf transform (one table, tpoly), into pot
 template <int NQ>
-static int launch_down_vec(Plan& p, double* pot, double* grad, double* hess, size_t sh) {
+static int launch_down_vec(Plan& p, double* pot, double* grad, double* hess, size_t sh, int adj = 0) {
     if (sh > 65536)
         BFGT_HIP_CHECK(hipFuncSetAttribute((const void*)k_down_vec<NQ>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
-    k_down_vec<NQ><<<p.npwleaf, kVecThreads, sh, p.stream>>>(p.n, p.npw, p.nd, p.ifpgh, p.d_pwleaf, p.d_tpw, p.d_ts,
-                                                              p.d_loc, pot, grad, hess, p.out_scale);
+    k_down_vec<NQ><<<p.npwleaf, kVecThreads, sh, p.stream>>>(p.n, p.npw, p.nd, adj ? 1 : p.ifpgh, adj, p.d_pwleaf,
+                                                              adj ? p.d_tpoly : p.d_tpw, p.d_ts, p.d_loc, pot, grad,
+                                                              hess, p.out_scale);
     BFGT_HIP_CHECK(hipGetLastError());
     return BFGT_OK;
 }
@@ -2185,24 +2298,63 @@ int launch_downward(Plan& p, double* pot, double* grad, double* hess) {
 }
 
 // ---------------------------------------------------------------------------
-// adjoint (transposed operator) of the PW stages, MFMA path only
+// adjoint (transposed operator) of the PW stages
 // ---------------------------------------------------------------------------
 
 // cotangents of pot/grad/hess at PW leaves -> per-chunk partial sums of the
 // cutoff-level local-expansion cotangent
+template <int E>
+static int launch_adj_eval_vec(Plan& p, const LeafIn& in) {
+    const size_t sh = adj_eval_smem_bytes(p.n, p.npw, p.ifpgh);
+    if (sh > 65536)
+        BFGT_HIP_CHECK(
+            hipFuncSetAttribute((const void*)k_adj_eval_vec<E>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
+    const int nseg = (p.nexp + E * kVecThreads - 1) / (E * kVecThreads);
+    k_adj_eval_vec<E><<<dim3(p.nchunks, nseg), kVecThreads, sh, p.stream>>>(
+        p.n, p.npw, p.nd, p.ifpgh, in, p.d_pwleaf, p.d_chunks, p.d_tpw, p.d_ts, p.d_mp);
+    BFGT_HIP_CHECK(hipGetLastError());
+    return BFGT_OK;
+}
+
 int launch_adj_eval(Plan& p, const double* gpot, const double* ggrad, const double* ghess) {
+    p.merge_pieces = false;
     if (p.nchunks == 0) return BFGT_OK;
+    if (!use_mfma(p)) {
+        const LeafIn in{gpot, ggrad, ghess};
+        const int e = (p.nexp + kVecThreads - 1) / kVecThreads;
+        if (e <= 2) return launch_adj_eval_vec<2>(p, in);
+        if (e <= 4) return launch_adj_eval_vec<4>(p, in);
+        if (e <= 8) return launch_adj_eval_vec<8>(p, in);
+        return launch_adj_eval_vec<16>(p, in);  // segments of 4096 entries
+    }
     UpArgs u{LeafIn{gpot, ggrad, ghess}, p.d_tup_a, 3 * 16 * p.pstride, 1, p.ifpgh};
     return launch_up_mfma(p, u);
 }
 
-// local-expansion cotangent (in d_mp) -> multipole cotangent, mirror-folded in d_loc
-int launch_adj_m2l(Plan& p) { return launch_m2l_fold(p, p.d_m2lgT_ptr, p.d_m2lgT, p.d_m2lpatchT, -1.0); }
+// local-expansion cotangent (in d_mp) -> multipole cotangent in d_loc
+// (mirror-folded for the matrix-core kernels)
+int launch_adj_m2l(Plan& p) {
+    if (use_mfma(p)) return launch_m2l_fold(p, p.d_m2lgT_ptr, p.d_m2lgT, p.d_m2lpatchT, -1.0);
+    const int len = p.nd * p.nexp;
+    const int gy = (len + 255) / 256;
+    k_m2l<<<dim3(p.nanc, gy), 256, 0, p.stream>>>(p.nd, p.nexp, p.d_m2lT_ptr, p.d_m2lT, p.d_wshift, -1.0, p.d_mp,
+                                                  p.d_loc);
+    BFGT_HIP_CHECK(hipGetLastError());
+    return BFGT_OK;
+}
 
 // multipole cotangent -> leaf-value cotangent at PW leaves (accumulated)
 int launch_adj_transform(Plan& p, double* fbar) {
     if (p.npwleaf == 0) return BFGT_OK;
+    if (!use_mfma(p)) {
+        const size_t sh = down_smem_bytes(p.n, p.npw, 1);
+        const int q = (p.n * p.n + kVecThreads - 1) / kVecThreads;
+        if (q <= 1) return launch_down_vec<1>(p, fbar, nullptr, nullptr, sh, 1);
+        if (q <= 2) return launch_down_vec<2>(p, fbar, nullptr, nullptr, sh, 1);
+        return launch_down_vec<4>(p, fbar, nullptr, nullptr, sh, 1);
+    }
     return launch_down_mfma<1>(p, p.d_tdn_a, 16 * p.pstride, 0, fbar, nullptr, nullptr);
 }
+
 
 }  // namespace bfgt

@@ -180,8 +180,10 @@ int bfgt_plan_eval_targets(bfgt_plan_t plan, int ntarg, const double *d_targs, i
  * conjugated, transposed PW tables; the translation over the transposed
  * lists with conjugated shifts; near-field tables read transposed per
  * (target, source) pair.  Arithmetic is FP64 for every plan precision.
- * A plan with PW leaves needs norder <= 16 and npw <= 128 (the matrix-core
- * kernels; BFGT_ERR_RESOURCE otherwise); near-field-only plans have no limit.
+ * Every plan has an adjoint, on the matrix-core kernels (norder <= 16, npw
+ * <= 128) and on the vector kernels beyond them; BFGT_ERR_RESOURCE only where
+ * the vector evaluation's adjoint would not fit the local data share (its
+ * ifpgh evaluation tables and one leaf's cotangents of one channel at once).
  * timeinfo, when non-NULL, is filled like bfgt_plan_execute's, per
  * transposed stage. */
 int bfgt_plan_adjoint(bfgt_plan_t plan, const double *d_gpot, const double *d_ggrad, const double *d_ghess,

@@ -324,22 +324,60 @@ def test_dot_product_identity_vector_near_field(device, n, iperiod):
     assert err < 1e-13, (err, info)
 
 
-def test_adjoint_refused_on_vector_pw_kernels(device):
-    """A plan with PW leaves at leaf order 20 runs the vector kernels, which
-    have no transpose: the adjoint returns BFGT_ERR_RESOURCE and writes nothing."""
-    import torch
-    from heat2d_per_box_amd.boxfgt import BoxFGT, BoxFGTError
+VECTOR_DENSE = [
+    # (uniform level, n, delta, iperiod, ifpgh)
+    (1, 20, 2e-2, 0, 3),  # free space, translation at level 0
+    (2, 17, 1e-2, 0, 1),  # free space, PW lists among level-1 boxes
+    (1, 20, 5e-2, 1, 3),  # periodic: Fourier series regime
+]
+
+
+@pytest.mark.parametrize("case", VECTOR_DENSE, ids=lambda c: f"l{c[0]}-n{c[1]}-d{c[2]:g}-ip{c[3]}-pgh{c[4]}")
+def test_dense_transpose_vector_pw_kernels(device, case):
+    """Leaf orders above 16 run the vector kernels; their adjoint (evaluation
+    transposed with the conjugated tables, transposed translation lists,
+    leaf transform transposed) against the oracle's dense FP64 transpose."""
+    lev, n, delta, iperiod, ifpgh = case
+    t = build_uniform(lev, iperiod=iperiod)
+    A, leaves = _oracle_dense(t, delta, 1e-9, 0, iperiod, n, ifpgh)
+    rng = np.random.default_rng(11)
+    nb, npb = t.centers.shape[0], n * n
+    gp = _rand_leaf(rng, t, (nb, npb, 1))
+    gg = _rand_leaf(rng, t, (nb, npb, 2, 1)) if ifpgh >= 2 else None
+    gh = _rand_leaf(rng, t, (nb, npb, 3, 1)) if ifpgh >= 3 else None
+    r = _gpu_forward_adjoint(t, 1, delta, 1e-9, 0, iperiod, n, ifpgh, np.zeros((nb, npb, 1)), gp, gg, gh)
+    info = r["plan"].info
+    assert info["has_pw"] and not info["mfma"] and info["n_pw_leaves"] > 0
+    gvec = [gp[leaves, :, 0]]
+    if ifpgh >= 2:
+        gvec += [gg[leaves, :, 0, 0], gg[leaves, :, 1, 0]]
+    if ifpgh >= 3:
+        gvec += [gh[leaves, :, j, 0] for j in range(3)]
+    want = A.T @ np.concatenate([v.reshape(-1) for v in gvec])
+    got = r["fbar"][leaves, :, 0].reshape(-1)
+    assert np.linalg.norm(got - want) / np.linalg.norm(want) < 1e-12
+
+
+@pytest.mark.parametrize("n,iperiod", [(20, 1), (24, 0)])
+def test_dot_product_identity_vector_pw_kernels(device, n, iperiod):
+    """Leaf orders above 16 on a tree with PW leaves and near pairs
+    (delta = 5e-4: cutoff at level 3 of a level 2-4 tree), nd = 2, pot +
+    grad + hess: the vector kernels' adjoint of every stage."""
     from tests.test_gpu_parity import _vector_tree
 
-    t, fv = _vector_tree(20, 1, 1)
-    plan = BoxFGT(t.copy(), 1, 5e-4, 1e-9, 0, 20, 1)
-    assert plan.info["has_pw"] and not plan.info["mfma"]
-    g = torch.ones((t.centers.shape[0], 400, 1), dtype=torch.float64, device="cuda")
-    fbar = torch.zeros_like(g)
-    with pytest.raises(BoxFGTError) as ei:
-        plan.adjoint(g, fbar)
-    assert ei.value.code == 12
-    assert float(fbar.abs().max()) == 0.0
+    t, _ = _vector_tree(n, iperiod, 2)
+    err, info = _dot_identity(t, 2, 5e-4, 1e-9, 0, iperiod, n, 3, seed=n)
+    assert info["has_pw"] and not info["mfma"] and info["n_pw_leaves"] > 0 and info["n_near_pairs"] > 0
+    assert err < 1e-13, (err, info)
+
+
+def test_dot_product_identity_npw_above_128(device):
+    """An expansion longer than the matrix-core kernels take (npw = 142, n = 8,
+    free space) runs the vector kernels forward and adjoint."""
+    t = build_uniform(5, iperiod=0)
+    err, info = _dot_identity(t, 1, 9.399e-05, 1e-20, 0, 0, 8, 2, seed=3)
+    assert info["npw"] > 128 and not info["mfma"]
+    assert err < 1e-13, (err, info)
 
 
 def test_autograd_plan_cache_is_bounded(device):
