@@ -29,6 +29,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 FP64_PEAK_TFLOPS = 78.6  # MI355X dense FP64 (vector and matrix)
+SETTLE_S = 0.25  # seconds of untimed executions before the warm-up
 HBM_PEAK_GBS = 8000.0
 
 
@@ -310,6 +311,9 @@ def main():
     ap.add_argument("--no-tree", action="store_true", help="skip timing the adaptive tree build and re-mesh")
     ap.add_argument("--no-split", action="store_true", help="skip timing the other precision")
     ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--accumulate", action="store_true",
+                    help="time the accumulating execution (pot += A f) instead of the overwriting one "
+                         "(pot = A f on the leaves); both are reported")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="run a device copy of this size first (calibrates rocprof byte counters)")
     args = ap.parse_args()
@@ -360,17 +364,32 @@ def main():
 
     names = ["leaf_to_pw", "merge", "translate", "split", "pw_to_values", "near_field"]
     work = algorithmic_work(plan, w)
+    # the overwriting execution (pot = A f on the leaves, as the reference's
+    # callers use boxfgt: clear, call, add the scaled result) unless asked
+    ow = not args.accumulate
 
     # one checked call for the error columns (its first launches also load the
     # kernels), then the warm-up, which records events around every stage to
     # pick the dominant one (one such call even with --warmup 0)
-    plan(fv, pot, grad, hess)
+    plan(fv, pot, grad, hess, overwrite=ow)
     torch.cuda.synchronize()
     gpu_out = {"pot": pot.cpu().numpy(), "grad": grad.cpu().numpy() if grad is not None else None,
                "hess": hess.cpu().numpy() if hess is not None else None}
+    # settle: untimed executions for SETTLE_S seconds of wall time (at least
+    # 20), so that the GPU runs at its steady clocks when the warm-up starts
+    # (from idle, the first ~100 executions of the c2 step run up to 15 %
+    # slower)
+    t_settle = time.time()
+    n_settle = 0
+    while n_settle < 20 or time.time() - t_settle < SETTLE_S:
+        for _ in range(10):
+            plan(fv, pot, grad, hess, overwrite=ow)
+        n_settle += 10
+        torch.cuda.synchronize()
+    t_settle = time.time() - t_settle
     plan.profile_begin(max(args.warmup, 1))
     for _ in range(max(args.warmup, 1)):
-        plan(fv, pot, grad, hess)
+        plan(fv, pot, grad, hess, overwrite=ow)
     torch.cuda.synchronize()
     warm_sum, nwarm = plan.profile_end()
     warm_ms = {k: 1e3 * v / max(nwarm, 1) for k, v in zip(names, warm_sum)}
@@ -393,7 +412,7 @@ def main():
     for i in range(args.steps):
         if i == args.steps - n_dom:
             plan.profile_begin(n_dom, names.index(dom))
-        plan(fv, pot, grad, hess)
+        plan(fv, pot, grad, hess, overwrite=ow)
     e1.record(stream)
     torch.cuda.synchronize()
     dom_sum, ndom = plan.profile_end()
@@ -405,11 +424,23 @@ def main():
     # stages (outside the timed region)
     plan.profile_begin(args.steps)
     for _ in range(args.steps):
-        plan(fv, pot, grad, hess)
+        plan(fv, pot, grad, hess, overwrite=ow)
     torch.cuda.synchronize()
     stage_sum, nprof = plan.profile_end()
     stage_ms = {k: 1e3 * v / max(nprof, 1) for k, v in zip(names, stage_sum)}
     dom_ms = 1e3 * dom_sum[names.index(dom)] / max(ndom, 1)
+    # the other execution mode over K more calls (outside the timed region),
+    # after one untimed call in that mode
+    plan(fv, pot, grad, hess, overwrite=not ow)
+    torch.cuda.synchronize()
+    f0 = torch.cuda.Event(enable_timing=True)
+    f1 = torch.cuda.Event(enable_timing=True)
+    f0.record(stream)
+    for _ in range(args.steps):
+        plan(fv, pot, grad, hess, overwrite=not ow)
+    f1.record(stream)
+    torch.cuda.synchronize()
+    ms_other_mode = f0.elapsed_time(f1) / args.steps
     kind, amount = work[dom]
     dur = dom_ms * 1e-3
     if kind == "flop":
@@ -456,22 +487,22 @@ def main():
         pot2 = torch.zeros_like(pot)
         grad2 = torch.zeros_like(grad) if grad is not None else None
         hess2 = torch.zeros_like(hess) if hess is not None else None
-        plan2(fv, pot2, grad2, hess2)
+        plan2(fv, pot2, grad2, hess2, overwrite=ow)
         torch.cuda.synchronize()
         out2 = {"pot": pot2.cpu().numpy(), "grad": grad2.cpu().numpy() if grad2 is not None else None,
                 "hess": hess2.cpu().numpy() if hess2 is not None else None}
         for _ in range(args.warmup):
-            plan2(fv, pot2, grad2, hess2)
+            plan2(fv, pot2, grad2, hess2, overwrite=ow)
         torch.cuda.synchronize()
         e0.record(stream)
         for _ in range(args.steps):
-            plan2(fv, pot2, grad2, hess2)
+            plan2(fv, pot2, grad2, hess2, overwrite=ow)
         e1.record(stream)
         torch.cuda.synchronize()
         ms2 = e0.elapsed_time(e1) / args.steps
         plan2.profile_begin(args.steps)
         for _ in range(args.steps):
-            plan2(fv, pot2, grad2, hess2)
+            plan2(fv, pot2, grad2, hess2, overwrite=ow)
         torch.cuda.synchronize()
         ssum2, np2 = plan2.profile_end()
         lv = w.tree.nchild == 0
@@ -563,6 +594,13 @@ def main():
             "npw": plan.info["npw"],
             "near_pairs": plan.info["n_near_pairs"],
         },
+        "settle": {"calls": n_settle, "s": t_settle,
+                   "note": "untimed executions before the warm-up, until the GPU runs at its steady clocks"},
+        "execution": "overwrite" if ow else "accumulate",
+        "ms_per_step_" + ("accumulate" if ow else "overwrite"): ms_other_mode,
+        "execution_note": "overwrite: pot (grad, hess) = A fvals on the leaves (bfgt_plan_execute_overwrite); "
+                          "accumulate: pot += A fvals (bfgt_plan_execute, the reference's contract, which reads "
+                          "the old outputs); same results on zeroed outputs",
         "stage_ms": stage_ms,
         "stage_ms_note": "K more executions, untimed, with HIP events between every stage",
         "t_cold_s": t_remesh,

@@ -36,6 +36,7 @@ EXPORTS = [
     "bfgt_plan_create",
     "bfgt_plan_execute",
     "bfgt_plan_execute_scaled",
+    "bfgt_plan_execute_overwrite",
     "bfgt_plan_execute_noalloc",
     "bfgt_plan_leaf_points",
     "bfgt_plan_adjoint",
@@ -107,6 +108,8 @@ def lib() -> ctypes.CDLL:
     L.bfgt_plan_execute_noalloc.argtypes = [_vp, _vp, _vp, _vp, _vp, _dp]
     L.bfgt_plan_execute_scaled.restype = _i
     L.bfgt_plan_execute_scaled.argtypes = [_vp, _d, _vp, _vp, _vp, _vp, _dp]
+    L.bfgt_plan_execute_overwrite.restype = _i
+    L.bfgt_plan_execute_overwrite.argtypes = [_vp, _d, _vp, _vp, _vp, _vp, _dp]
     L.bfgt_plan_leaf_points.restype = _i
     L.bfgt_plan_leaf_points.argtypes = [_vp, _vp]
     L.bfgt_plan_destroy.restype = _i

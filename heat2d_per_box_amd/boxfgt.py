@@ -193,8 +193,10 @@ class BoxFGT:
             raise ValueError(f"{name} has shape {tuple(t.shape)}, expected {shape}")
         return ctypes.c_void_p(t.data_ptr())
 
-    def __call__(self, fvals, pot, grad=None, hess=None, timeinfo: bool = False, scale: float = 1.0):
-        """pot (+grad, hess) += scale * A fvals on device tensors."""
+    def __call__(self, fvals, pot, grad=None, hess=None, timeinfo: bool = False, scale: float = 1.0,
+                 overwrite: bool = False):
+        """pot (+grad, hess) += scale * A fvals on device tensors; with
+        ``overwrite`` the leaf entries are set to scale * A fvals instead."""
         nb, npb, nd = self.nboxes, self.npbox, self.nd
         f = self._check_tensor(fvals, (nb, npb, nd), "fvals")
         p = self._check_tensor(pot, (nb, npb, nd), "pot")
@@ -203,7 +205,9 @@ class BoxFGT:
         ti = np.zeros(20) if timeinfo else None
         tip = _ptr(ti) if ti is not None else None
         with self._ordered():
-            if scale == 1.0:
+            if overwrite:
+                check(lib().bfgt_plan_execute_overwrite(self._h, float(scale), f, p, g, h, tip))
+            elif scale == 1.0:
                 check(lib().bfgt_plan_execute(self._h, f, p, g, h, tip))
             else:
                 check(lib().bfgt_plan_execute_scaled(self._h, float(scale), f, p, g, h, tip))

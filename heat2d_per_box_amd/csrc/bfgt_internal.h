@@ -92,6 +92,11 @@ struct Plan {
     hipStream_t stream = nullptr;
     bool own_stream = false;
     double out_scale = 1.0;  // factor on the outputs of the current execution
+    // 1: the evaluation adds into pot/grad/hess; 0: it writes the PW leaves'
+    // outputs (an overwriting execution, which first clears the other leaves)
+    int accum = 1;
+    int nnonpw = 0;            // leaves without a PW evaluation
+    int* d_nonpw = nullptr;    // their boxes
     int device = 0;
 
     // host copies used for scheduling

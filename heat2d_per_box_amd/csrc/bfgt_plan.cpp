@@ -900,6 +900,20 @@ int plan_create(Plan& p, int nd, int ndim, double delta, double eps, int ipoly, 
     p.nnearT_s = (int)nsrc.size();
 
     if ((rc = upload(p, p.d_pwleaf, leaves))) return rc;
+    {
+        // leaves that no PW evaluation writes: above the cutoff level, or
+        // outside the PW lists, or every leaf of a plan without PW leaves
+        std::vector<char> pwl(t.nboxes, 0);
+        if (p.has_pw)
+            for (int l = p.nlevstart; l <= nlevels; l++)
+                for (int b = t.first(l); b <= t.last(l); b++)
+                    if (pwexp[b] == 1 && t.nchild[b] == 0) pwl[b] = 1;
+        std::vector<int> nonpw;
+        for (int b = 0; b < t.nboxes; b++)
+            if (t.nchild[b] == 0 && !pwl[b]) nonpw.push_back(b);
+        p.nnonpw = (int)nonpw.size();
+        if ((rc = upload(p, p.d_nonpw, nonpw))) return rc;
+    }
     if ((rc = upload(p, p.d_pwpair, pairs_pw))) return rc;
     if ((rc = upload(p, p.d_colgrp, colgrp))) return rc;
     if ((rc = upload(p, p.d_quads, quads))) return rc;
@@ -1007,13 +1021,35 @@ static void stage_marks(int s, int* lo, int* hi) {
     *hi = b[s][1];
 }
 
+// zero pot (grad, hess) of the leaves no PW evaluation writes
+__global__ void k_zero_leaves(int nleaf, const int* __restrict__ boxes, int per, int ifpgh, double* __restrict__ pot,
+                              double* __restrict__ grad, double* __restrict__ hess) {
+    const int nc = ifpgh == 1 ? 1 : (ifpgh == 2 ? 3 : 6);  // doubles per (point, density) over the three arrays
+    const long tot = (long)nleaf * per * nc;
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < tot; e += (long)gridDim.x * blockDim.x) {
+        const int i = (int)(e / ((long)per * nc)), r = (int)(e % ((long)per * nc));
+        const size_t b = (size_t)boxes[i];
+        if (r < per) pot[b * per + r] = 0.0;
+        else if (r < 3 * per) grad[b * 2 * per + (r - per)] = 0.0;
+        else hess[b * 3 * per + (r - 3 * per)] = 0.0;
+    }
+}
+
 int plan_execute(Plan& p, const double* fvals, double* pot, double* grad, double* hess, double* timeinfo,
-                 double scale) {
+                 double scale, bool overwrite) {
     if (!fvals || !pot || (p.ifpgh >= 2 && !grad) || (p.ifpgh >= 3 && !hess))
         return set_error(BFGT_ERR_ARG, "null device array");
     if (!std::isfinite(scale)) return set_error(BFGT_ERR_ARG, "scale must be finite");
     BFGT_HIP_CHECK(hipSetDevice(p.device));
     p.out_scale = scale;
+    p.accum = overwrite ? 0 : 1;
+    if (overwrite && p.nnonpw > 0) {
+        const int per = p.npbox * p.nd;
+        const long tot = (long)p.nnonpw * per * (p.ifpgh == 1 ? 1 : (p.ifpgh == 2 ? 3 : 6));
+        const int blocks = (int)std::min<long>((tot + 255) / 256, 4096);
+        k_zero_leaves<<<blocks, 256, 0, p.stream>>>(p.nnonpw, p.d_nonpw, per, p.ifpgh, pot, grad, hess);
+        BFGT_HIP_CHECK(hipGetLastError());
+    }
     int rc;
     // stage boundaries 0..5 (before the upward pass, after it, after the
     // merge, the translation, the downward pass and the near field); a plain
@@ -1080,6 +1116,7 @@ int plan_adjoint(Plan& p, const double* gpot, const double* ggrad, const double*
         return set_error(BFGT_ERR_RESOURCE, "the adjoint of this plan's evaluation does not fit the local data share");
     BFGT_HIP_CHECK(hipSetDevice(p.device));
     p.out_scale = 1.0;
+    p.accum = 1;  // fbar accumulates
     int rc;
     hipEvent_t* ev = timeinfo ? p.ev : nullptr;  // stage events only when timings are asked for
     auto mark = [&](int i) -> int {
@@ -1379,7 +1416,7 @@ int zero_cplx(hipStream_t s, cplx* a, size_t n) {
 int plan_execute_noalloc(Plan& p, const double* fvals, double* pot, double* grad, double* hess, double* timeinfo) {
     if (!fvals || !pot || (p.ifpgh >= 2 && !grad) || (p.ifpgh >= 3 && !hess))
         return set_error(BFGT_ERR_ARG, "null device array");
-    if (!p.has_pw) return plan_execute(p, fvals, pot, grad, hess, timeinfo, 1.0);  // no workspace used
+    if (!p.has_pw) return plan_execute(p, fvals, pot, grad, hess, timeinfo, 1.0, false);  // no workspace used
     const size_t mp_cplx = (size_t)p.nmpslot * p.nd * p.nexp;
     const size_t loc_cplx = (size_t)p.nanc * std::max(loc_buffer_cplx(p), 2 * fold_len(p));
     const size_t mp_bytes = (mp_cplx * sizeof(cplx) + 255) / 256 * 256;
@@ -1402,7 +1439,7 @@ int plan_execute_noalloc(Plan& p, const double* fvals, double* pot, double* grad
     cplx* own_loc = p.d_loc;
     p.d_mp = mp;
     p.d_loc = loc;
-    rc = plan_execute(p, fvals, pot, grad, hess, timeinfo, 1.0);
+    rc = plan_execute(p, fvals, pot, grad, hess, timeinfo, 1.0, false);
     p.d_mp = own_mp;
     p.d_loc = own_loc;
     return rc;
@@ -1447,7 +1484,7 @@ int bfgt_plan_create(bfgt_plan_t* out, int nd, int ndim, double delta, double ep
 int bfgt_plan_execute(bfgt_plan_t plan, const double* d_fvals, double* d_pot, double* d_grad, double* d_hess,
                       double* timeinfo) {
     if (!plan) return set_error(BFGT_ERR_ARG, "null plan");
-    return bfgt::plan_execute(plan->p, d_fvals, d_pot, d_grad, d_hess, timeinfo, 1.0);
+    return bfgt::plan_execute(plan->p, d_fvals, d_pot, d_grad, d_hess, timeinfo, 1.0, false);
 }
 
 int bfgt_plan_execute_noalloc(bfgt_plan_t plan, const double* d_fvals, double* d_pot, double* d_grad,
@@ -1459,7 +1496,13 @@ int bfgt_plan_execute_noalloc(bfgt_plan_t plan, const double* d_fvals, double* d
 int bfgt_plan_execute_scaled(bfgt_plan_t plan, double scale, const double* d_fvals, double* d_pot, double* d_grad,
                              double* d_hess, double* timeinfo) {
     if (!plan) return set_error(BFGT_ERR_ARG, "null plan");
-    return bfgt::plan_execute(plan->p, d_fvals, d_pot, d_grad, d_hess, timeinfo, scale);
+    return bfgt::plan_execute(plan->p, d_fvals, d_pot, d_grad, d_hess, timeinfo, scale, false);
+}
+
+int bfgt_plan_execute_overwrite(bfgt_plan_t plan, double scale, const double* d_fvals, double* d_pot,
+                                double* d_grad, double* d_hess, double* timeinfo) {
+    if (!plan) return set_error(BFGT_ERR_ARG, "null plan");
+    return bfgt::plan_execute(plan->p, d_fvals, d_pot, d_grad, d_hess, timeinfo, scale, true);
 }
 
 int bfgt_plan_leaf_points(bfgt_plan_t plan, double* d_xy) {
@@ -1789,7 +1832,7 @@ static int hip_eval(int nd, int ndim, double delta, double eps, int ipoly, int i
     if (timeinfo) BFGT_EVAL_CHECK(hipStreamSynchronize(p.stream));
     auto t1 = std::chrono::steady_clock::now();
     double ti[20];
-    int rc = bfgt::plan_execute(p, dfv, dpot, dgrad, dhess, ti, 1.0);
+    int rc = bfgt::plan_execute(p, dfv, dpot, dgrad, dhess, ti, 1.0, false);
     if (rc) return finish(rc);
     // ifnewtree = 1: the tree is refined, coarsened and rebalanced to
     // resolve pot (bfgt/boxfgt.f:1172-1483); itree, centers and nlevels are

@@ -545,7 +545,7 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_down_col(
     int n, int P, int ps, int nd, int ngroups, const int2* __restrict__ groups, const PwLeaf* __restrict__ leaves,
     const cplx* __restrict__ tab, int lstride, int ymid_half, const cplx* __restrict__ phase,
     const int* __restrict__ levbase, const Fold* __restrict__ fold, double* __restrict__ pot,
-    double* __restrict__ grad, double* __restrict__ hess, double oscale) {
+    double* __restrict__ grad, double* __restrict__ hess, double oscale, int accum) {
     constexpr int NOUT = nchan(NTAB);
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
     const int item = __builtin_amdgcn_readfirstlane(xcd_block(blockIdx.x, gridDim.x) * kWavesPerBlock + wave);
@@ -696,12 +696,21 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_down_col(
                 for (int r = 0; r < 4; r++)
                     orow[r] = (unsigned)(min(cc, n - 1) + n * min(drow(q, r), n - 1)) * (unsigned)(nd * sizeof(double));
                 double old[LG][4][NOUT];
+                if (accum) {
 #pragma unroll
-                for (int g = 0; g < LG; g++)
+                    for (int g = 0; g < LG; g++)
 #pragma unroll
-                    for (int r = 0; r < 4; r++)
+                        for (int r = 0; r < 4; r++)
 #pragma unroll
-                        for (int o = 0; o < NOUT; o++) old[g][r][o] = bload_d(rO[g][o], orow[r] * stride_o[o], 0);
+                            for (int o = 0; o < NOUT; o++) old[g][r][o] = bload_d(rO[g][o], orow[r] * stride_o[o], 0);
+                } else {
+#pragma unroll
+                    for (int g = 0; g < LG; g++)
+#pragma unroll
+                        for (int r = 0; r < 4; r++)
+#pragma unroll
+                            for (int o = 0; o < NOUT; o++) old[g][r][o] = 0.0;
+                }
                 if (cc < n) {
 #pragma unroll
                     for (int g = 0; g < LG; g++) {
@@ -734,7 +743,7 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_down_quad(
     int n, int P, int ps, int nd, int ngroups, const int4* __restrict__ qgroups, const int4* __restrict__ quads,
     const PwLeaf* __restrict__ leaves, const cplx* __restrict__ tab, int lstride, int ymid_half,
     const cplx* __restrict__ phase, const int* __restrict__ levbase, const Fold* __restrict__ fold,
-    double* __restrict__ pot, double* __restrict__ grad, double* __restrict__ hess, double oscale) {
+    double* __restrict__ pot, double* __restrict__ grad, double* __restrict__ hess, double oscale, int accum) {
     constexpr int NOUT = nchan(NTAB);
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
     const int item = __builtin_amdgcn_readfirstlane(xcd_block(blockIdx.x, gridDim.x) * kWavesPerBlock + wave);
@@ -836,7 +845,7 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_down_quad(
 #pragma unroll
                         for (int o = 0; o < NOUT; o++) {
                             double& v = dst[o][(pb * stride_o[o] + comp_o[o]) * nd + d];
-                            v += w2 * out[o][r];
+                            v = (accum ? v : 0.0) + w2 * out[o][r];
                         }
                     }
                 }
@@ -857,7 +866,7 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_down_pair(
     int n, int P, int ps, int nd, int npair, const int2* __restrict__ pairs, const PwLeaf* __restrict__ leaves,
     const cplx* __restrict__ tab, int lstride, int ymid_half, const cplx* __restrict__ phase,
     const int* __restrict__ levbase, const Fold* __restrict__ fold, double* __restrict__ pot,
-    double* __restrict__ grad, double* __restrict__ hess, double oscale) {
+    double* __restrict__ grad, double* __restrict__ hess, double oscale, int accum) {
     constexpr int NOUT = nchan(NTAB);
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
     const int item = __builtin_amdgcn_readfirstlane(xcd_block(blockIdx.x, gridDim.x) * kWavesPerBlock + wave);
@@ -938,15 +947,16 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_down_pair(
                 if ((k2p >> 3) == half && k2 < n) {
                     size_t pb = (size_t)lfo.box * npbox + ci + (size_t)n * k2;
                     const double w2 = 2.0 * oscale;
-                    pot[pb * nd + d] += w2 * out[0][r];
+                    auto put = [&](double& v, double x) { v = (accum ? v : 0.0) + w2 * x; };
+                    put(pot[pb * nd + d], out[0][r]);
                     if constexpr (NTAB >= 2) {
-                        grad[(pb * 2 + 0) * nd + d] += w2 * out[1][r];
-                        grad[(pb * 2 + 1) * nd + d] += w2 * out[2][r];
+                        put(grad[(pb * 2 + 0) * nd + d], out[1][r]);
+                        put(grad[(pb * 2 + 1) * nd + d], out[2][r]);
                     }
                     if constexpr (NTAB >= 3) {
-                        hess[(pb * 3 + 0) * nd + d] += w2 * out[3][r];
-                        hess[(pb * 3 + 1) * nd + d] += w2 * out[4][r];
-                        hess[(pb * 3 + 2) * nd + d] += w2 * out[5][r];
+                        put(hess[(pb * 3 + 0) * nd + d], out[3][r]);
+                        put(hess[(pb * 3 + 1) * nd + d], out[4][r]);
+                        put(hess[(pb * 3 + 2) * nd + d], out[5][r]);
                     }
                 }
             }
@@ -1065,7 +1075,7 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_down_split(
     int n, int P, int ps, int nd, int ngrp, const int2* __restrict__ grp, const PwLeaf* __restrict__ leaves,
     const cplx* __restrict__ tab, int lstride, const cplx* __restrict__ phase, const int* __restrict__ levbase,
     const Fold* __restrict__ fold, double* __restrict__ pot, double* __restrict__ grad, double* __restrict__ hess,
-    double oscale) {
+    double oscale, int accum) {
     constexpr int NOUT = nchan(NTAB);
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
     // wave-uniform: schedule entries are then read with scalar loads
@@ -1152,11 +1162,18 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_down_split(
             // are read at clamped points for all r first (one memory round
             // trip), then added and stored where valid.
             double old[4][NOUT];
+            if (accum) {
 #pragma unroll
-            for (int r = 0; r < 4; r++) {
-                const size_t pb = (size_t)lf.box * npbox + min(cc, n - 1) + (size_t)n * min(drow32(q, r), n - 1);
+                for (int r = 0; r < 4; r++) {
+                    const size_t pb = (size_t)lf.box * npbox + min(cc, n - 1) + (size_t)n * min(drow32(q, r), n - 1);
 #pragma unroll
-                for (int o = 0; o < NOUT; o++) old[r][o] = dst[o][(pb * stride_o[o] + comp_o[o]) * nd + d];
+                    for (int o = 0; o < NOUT; o++) old[r][o] = dst[o][(pb * stride_o[o] + comp_o[o]) * nd + d];
+                }
+            } else {
+#pragma unroll
+                for (int r = 0; r < 4; r++)
+#pragma unroll
+                    for (int o = 0; o < NOUT; o++) old[r][o] = 0.0;
             }
             if (cc < n) {
 #pragma unroll
@@ -1278,7 +1295,7 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_down_split_quad(
     int n, int P, int ps, int nd, int ngroups, const int4* __restrict__ qgroups, const int4* __restrict__ quads,
     const PwLeaf* __restrict__ leaves, const cplx* __restrict__ tab, int lstride, const cplx* __restrict__ phase,
     const int* __restrict__ levbase, const Fold* __restrict__ fold, double* __restrict__ pot,
-    double* __restrict__ grad, double* __restrict__ hess, double oscale) {
+    double* __restrict__ grad, double* __restrict__ hess, double oscale, int accum) {
     constexpr int NOUT = nchan(NTAB);
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
     const int item = __builtin_amdgcn_readfirstlane(xcd_block(blockIdx.x, gridDim.x) * kWavesPerBlock + wave);
@@ -1380,7 +1397,7 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_down_split_quad(
 #pragma unroll
                 for (int o = 0; o < NOUT; o++) {
                     double& v = dst[o][(pb * stride_o[o] + comp_o[o]) * nd + d];
-                    v += w2 * (double)out[o][r];
+                    v = (accum ? v : 0.0) + w2 * (double)out[o][r];
                 }
             }
         }
@@ -1476,7 +1493,7 @@ __global__ __launch_bounds__(kVecThreads) void k_down_vec(int n, int npw, int nd
                                                           const cplx* __restrict__ tpw, const double* __restrict__ ts,
                                                           const cplx* __restrict__ loc, double* __restrict__ pot,
                                                           double* __restrict__ grad, double* __restrict__ hess,
-                                                          double oscale) {
+                                                          double oscale, int accum) {
     extern __shared__ double smem[];
     const int H = (npw + 1) / 2;
     const int npw2 = npw / 2;
@@ -1561,15 +1578,16 @@ __global__ __launch_bounds__(kVecThreads) void k_down_vec(int n, int npw, int nd
             if (q < npbox) {
                 size_t pb = (size_t)lf.box * npbox + q;
                 const double w2 = 2.0 * oscale;
-                pot[pb * nd + d] += w2 * acc[r][0];
+                auto put = [&](double& v, double x) { v = (accum ? v : 0.0) + w2 * x; };
+                put(pot[pb * nd + d], acc[r][0]);
                 if (ntab >= 2) {
-                    grad[(pb * 2 + 0) * nd + d] += w2 * acc[r][1];
-                    grad[(pb * 2 + 1) * nd + d] += w2 * acc[r][2];
+                    put(grad[(pb * 2 + 0) * nd + d], acc[r][1]);
+                    put(grad[(pb * 2 + 1) * nd + d], acc[r][2]);
                 }
                 if (ntab >= 3) {
-                    hess[(pb * 3 + 0) * nd + d] += w2 * acc[r][3];
-                    hess[(pb * 3 + 1) * nd + d] += w2 * acc[r][4];
-                    hess[(pb * 3 + 2) * nd + d] += w2 * acc[r][5];
+                    put(hess[(pb * 3 + 0) * nd + d], acc[r][3]);
+                    put(hess[(pb * 3 + 1) * nd + d], acc[r][4]);
+                    put(hess[(pb * 3 + 2) * nd + d], acc[r][5]);
                 }
             }
         }
@@ -1937,7 +1955,7 @@ int launch_pw_eval_list(Plan& p, const PwLeaf* list, int count, const cplx* loc,
         if (sh > 65536)
             BFGT_HIP_CHECK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
         kern<<<count, kVecThreads, sh, p.stream>>>(p.n, p.npw, p.nd, p.ifpgh, 0, list, p.d_tpw, p.d_ts, loc, pot,
-                                                   grad, hess, 1.0);
+                                                   grad, hess, 1.0, 1);
         BFGT_HIP_CHECK(hipGetLastError());
         return BFGT_OK;
     };
@@ -2179,7 +2197,7 @@ static int launch_down_vec(Plan& p, double* pot, double* grad, double* hess, siz
         BFGT_HIP_CHECK(hipFuncSetAttribute((const void*)k_down_vec<NQ>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
     k_down_vec<NQ><<<p.npwleaf, kVecThreads, sh, p.stream>>>(p.n, p.npw, p.nd, adj ? 1 : p.ifpgh, adj, p.d_pwleaf,
                                                               adj ? p.d_tpoly : p.d_tpw, p.d_ts, p.d_loc, pot, grad,
-                                                              hess, p.out_scale);
+                                                              hess, p.out_scale, adj ? 1 : p.accum);
     BFGT_HIP_CHECK(hipGetLastError());
     return BFGT_OK;
 }
@@ -2195,7 +2213,7 @@ static int launch_down_col(Plan& p, const cplx* tab, int lstride, int ymid_half,
     constexpr int LG = NTAB == 1 ? 4 : 1;
     k_down_col<NTAB, NTP, LG><<<blocks, kWave * kWavesPerBlock, 0, p.stream>>>(
         p.n, p.npw, p.pstride, p.nd, p.ncolgrp, p.d_colgrp, p.d_pwleaf, tab, lstride, ymid_half, p.d_phase,
-        p.d_levbase, reinterpret_cast<const Fold*>(p.d_loc), pot, grad, hess, p.out_scale);
+        p.d_levbase, reinterpret_cast<const Fold*>(p.d_loc), pot, grad, hess, p.out_scale, p.accum);
     BFGT_HIP_CHECK(hipGetLastError());
     return BFGT_OK;
 }
@@ -2206,7 +2224,7 @@ static int launch_down_pair(Plan& p, const cplx* tab, int lstride, int ymid_half
     int blocks = (p.npair + kWavesPerBlock - 1) / kWavesPerBlock;
     k_down_pair<NTAB, NTP><<<blocks, kWave * kWavesPerBlock, 0, p.stream>>>(
         p.n, p.npw, p.pstride, p.nd, p.npair, p.d_pwpair, p.d_pwleaf, tab, lstride, ymid_half, p.d_phase,
-        p.d_levbase, reinterpret_cast<const Fold*>(p.d_loc), pot, grad, hess, p.out_scale);
+        p.d_levbase, reinterpret_cast<const Fold*>(p.d_loc), pot, grad, hess, p.out_scale, p.accum);
     BFGT_HIP_CHECK(hipGetLastError());
     return BFGT_OK;
 }
@@ -2217,7 +2235,7 @@ static int launch_down_quad(Plan& p, const cplx* tab, int lstride, int ymid_half
     int blocks = (p.nqgroup * p.nd + kWavesPerBlock - 1) / kWavesPerBlock;
     k_down_quad<NTAB, NTP><<<blocks, kWave * kWavesPerBlock, 0, p.stream>>>(
         p.n, p.npw, p.pstride, p.nd, p.nqgroup, p.d_qgroups, p.d_quads, p.d_pwleaf, tab, lstride, ymid_half,
-        p.d_phase, p.d_levbase, reinterpret_cast<const Fold*>(p.d_loc), pot, grad, hess, p.out_scale);
+        p.d_phase, p.d_levbase, reinterpret_cast<const Fold*>(p.d_loc), pot, grad, hess, p.out_scale, p.accum);
     BFGT_HIP_CHECK(hipGetLastError());
     return BFGT_OK;
 }
@@ -2250,7 +2268,7 @@ static int launch_down_split(Plan& p, double* pot, double* grad, double* hess) {
             kern<<<qblocks, kWave * kWavesPerBlock, 0, p.stream>>>(
                 p.n, p.npw, p.pstride, p.nd, p.nqgroup, p.d_qgroups, p.d_quads, p.d_pwleaf, p.d_tdn_f,
                 3 * 16 * p.pstride, p.d_phase, p.d_levbase, reinterpret_cast<const Fold*>(p.d_loc), pot, grad, hess,
-                p.out_scale);
+                p.out_scale, p.accum);
             BFGT_HIP_CHECK(hipGetLastError());
             return BFGT_OK;
         };
@@ -2265,7 +2283,7 @@ static int launch_down_split(Plan& p, double* pot, double* grad, double* hess) {
     auto run = [&](auto kern) -> int {
         kern<<<blocks, kWave * kWavesPerBlock, 0, p.stream>>>(
             p.n, p.npw, p.pstride, p.nd, p.ncolgrp, p.d_colgrp, p.d_pwleaf, p.d_tdn_f, 3 * 16 * p.pstride, p.d_phase,
-            p.d_levbase, reinterpret_cast<const Fold*>(p.d_loc), pot, grad, hess, p.out_scale);
+            p.d_levbase, reinterpret_cast<const Fold*>(p.d_loc), pot, grad, hess, p.out_scale, p.accum);
         BFGT_HIP_CHECK(hipGetLastError());
         return BFGT_OK;
     };

@@ -198,6 +198,20 @@ int bfgt_plan_adjoint(bfgt_plan_t plan, const double *d_gpot, const double *d_gg
 int bfgt_plan_execute_scaled(bfgt_plan_t plan, double scale, const double *d_fvals, double *d_pot, double *d_grad,
                              double *d_hess, double *timeinfo);
 
+/* pot (+grad, hess) = scale * A fvals on the leaf boxes: bfgt_plan_execute_scaled
+ * with the leaf outputs overwritten instead of accumulated (entries of
+ * non-leaf boxes are not touched).  No counterpart in the reference, whose
+ * boxfgt adds into pot/grad/hess; its callers clear the output before every
+ * call and add the scaled result into the volume potential afterwards
+ * (hpots/hpots2dadap.f:320-354: vpotinc = 0, boxfgt, vpot += w vpotinc).
+ * Here the evaluation writes every PW leaf's outputs once, without reading
+ * them, and the leaves it does not reach (above the cutoff level, or all of
+ * them in a plan without PW leaves) are cleared before the near field adds
+ * into them.  The results equal bfgt_plan_execute_scaled's on zeroed
+ * outputs.  Asynchronous on the plan's stream; allocates nothing. */
+int bfgt_plan_execute_overwrite(bfgt_plan_t plan, double scale, const double *d_fvals, double *d_pot,
+                                double *d_grad, double *d_hess, double *timeinfo);
+
 /* Device coordinates d_xy(2, npbox, nboxes) of the tensor-grid nodes of every
  * leaf (zeros for non-leaf boxes), point i + n*j at centre + (x_i, x_j) *
  * boxsize/2 -- where the callers evaluate their forcing term
