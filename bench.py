@@ -311,6 +311,8 @@ def main():
     ap.add_argument("--no-tree", action="store_true", help="skip timing the adaptive tree build and re-mesh")
     ap.add_argument("--no-split", action="store_true", help="skip timing the other precision")
     ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--settle-s", type=float, default=SETTLE_S,
+                    help="seconds of untimed executions before the warm-up (shorter under a profiler)")
     ap.add_argument("--accumulate", action="store_true",
                     help="time the accumulating execution (pot += A f) instead of the overwriting one "
                          "(pot = A f on the leaves); both are reported")
@@ -381,7 +383,7 @@ def main():
     # slower)
     t_settle = time.time()
     n_settle = 0
-    while n_settle < 20 or time.time() - t_settle < SETTLE_S:
+    while n_settle < 20 or time.time() - t_settle < args.settle_s:
         for _ in range(10):
             plan(fv, pot, grad, hess, overwrite=ow)
         n_settle += 10

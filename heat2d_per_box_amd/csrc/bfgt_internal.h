@@ -95,8 +95,9 @@ struct Plan {
     // 1: the evaluation adds into pot/grad/hess; 0: it writes the PW leaves'
     // outputs (an overwriting execution, which first clears the other leaves)
     int accum = 1;
-    int nnonpw = 0;            // leaves without a PW evaluation
+    int nnonpw = 0;            // leaves that neither a PW evaluation nor the near field writes
     int* d_nonpw = nullptr;    // their boxes
+    int* d_near_fresh = nullptr;  // per near-field target: 1 if no PW evaluation writes it
     int device = 0;
 
     // host copies used for scheduling

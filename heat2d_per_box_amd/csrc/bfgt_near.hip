@@ -29,13 +29,17 @@ __global__ __launch_bounds__(kNearThreads) void k_near(int n, int nd, int ntab, 
                                                        const double* __restrict__ tloc,
                                                        const double* __restrict__ fvals, double* __restrict__ pot,
                                                        double* __restrict__ grad, double* __restrict__ hess,
-                                                       double oscale, int half) {
+                                                       double oscale, int half, const int* __restrict__ tfresh,
+                                                       int accum) {
     extern __shared__ double smem[];
     const int nn = n * n;
     double* sf = smem;       // nn
     double* sff = sf + nn;   // ntab * nn : [t][k1 + n*j2]
     const int b = tgt[blockIdx.x];
     const int lev = tlev[blockIdx.x];
+    // an overwriting execution writes the outputs of a target that no PW
+    // evaluation wrote, instead of adding to them
+    const bool fresh = accum == 0 && tfresh[blockIdx.x] != 0;
     const int p0 = ptr[blockIdx.x], p1 = ptr[blockIdx.x + 1];
     const int tid = threadIdx.x;
     const double* T = tloc + (size_t)lev * (2 * half + 1) * 3 * nn;
@@ -103,15 +107,16 @@ __global__ __launch_bounds__(kNearThreads) void k_near(int n, int nd, int ntab, 
             int q = tid + r * kNearThreads;
             if (q < nn && p1 > p0) {
                 size_t pb = (size_t)b * nn + q;
-                pot[pb * nd + d] += oscale * acc[r][0];
+                auto put = [&](double& v, double x) { v = (fresh ? 0.0 : v) + oscale * x; };
+                put(pot[pb * nd + d], acc[r][0]);
                 if (ntab >= 2) {
-                    grad[(pb * 2 + 0) * nd + d] += oscale * acc[r][1];
-                    grad[(pb * 2 + 1) * nd + d] += oscale * acc[r][2];
+                    put(grad[(pb * 2 + 0) * nd + d], acc[r][1]);
+                    put(grad[(pb * 2 + 1) * nd + d], acc[r][2]);
                 }
                 if (ntab >= 3) {
-                    hess[(pb * 3 + 0) * nd + d] += oscale * acc[r][3];
-                    hess[(pb * 3 + 1) * nd + d] += oscale * acc[r][4];
-                    hess[(pb * 3 + 2) * nd + d] += oscale * acc[r][5];
+                    put(hess[(pb * 3 + 0) * nd + d], acc[r][3]);
+                    put(hess[(pb * 3 + 1) * nd + d], acc[r][4]);
+                    put(hess[(pb * 3 + 2) * nd + d], acc[r][5]);
                 }
             }
         }
@@ -241,7 +246,7 @@ __global__ __launch_bounds__(64 * kNearWaves) void k_near_mfma(
     int n, int nd, int nitems, const int* __restrict__ tgt, const int* __restrict__ tlev,
     const int* __restrict__ ptr, const NearPair* __restrict__ pairs, const double* __restrict__ tnearT,
     const double* __restrict__ fvals, double* __restrict__ pot, double* __restrict__ grad,
-    double* __restrict__ hess, double oscale) {
+    double* __restrict__ hess, double oscale, const int* __restrict__ tfresh, int accum) {
     constexpr int NOUT = near_chan(NTAB);
     const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
     // wave-uniform: schedule entries are then read with scalar loads
@@ -250,6 +255,7 @@ __global__ __launch_bounds__(64 * kNearWaves) void k_near_mfma(
     const int tw = item % NSPLIT, it = item / NSPLIT / nd, d = item / NSPLIT % nd;
     const int q = lane >> 4, c = lane & 15;
     const int b = tgt[it];
+    const bool fresh = accum == 0 && tfresh[it] != 0;  // as in k_near
     const int p0 = ptr[it], p1 = ptr[it + 1];
     const double* T = tnearT + (size_t)tlev[it] * 13 * 3 * 256;
     const int nn = n * n;
@@ -314,13 +320,20 @@ __global__ __launch_bounds__(64 * kNearWaves) void k_near_mfma(
             if constexpr (NTAB >= 3) dst[3] = dst[4] = dst[5] = hess;
             constexpr int stride_o[6] = {1, 2, 2, 3, 3, 3}, comp_o[6] = {0, 0, 1, 0, 1, 2};
             double old[4][NOUT];
+            if (!fresh) {
 #pragma unroll
-            for (int r = 0; r < 4; r++) {
-                const size_t pb = (size_t)b * nn + min(c, n - 1) + (size_t)n * min(4 * r + q, n - 1);
+                for (int r = 0; r < 4; r++) {
+                    const size_t pb = (size_t)b * nn + min(c, n - 1) + (size_t)n * min(4 * r + q, n - 1);
 #pragma unroll
-                for (int o = 0; o < NOUT; o++)
-                    if (near_part_has(TW, near_x(o)))
-                        old[r][o] = dst[o][(pb * stride_o[o] + comp_o[o]) * nd + d];
+                    for (int o = 0; o < NOUT; o++)
+                        if (near_part_has(TW, near_x(o)))
+                            old[r][o] = dst[o][(pb * stride_o[o] + comp_o[o]) * nd + d];
+                }
+            } else {
+#pragma unroll
+                for (int r = 0; r < 4; r++)
+#pragma unroll
+                    for (int o = 0; o < NOUT; o++) old[r][o] = 0.0;
             }
             if (c < n) {
 #pragma unroll
@@ -359,7 +372,7 @@ __global__ __launch_bounds__(64 * kNearWaves) void k_near_split(
     int n, int nd, int nitems, const int* __restrict__ tgt, const int* __restrict__ tlev,
     const int* __restrict__ ptr, const NearPair* __restrict__ pairs, const double* __restrict__ tnearT,
     const double* __restrict__ fvals, double* __restrict__ pot, double* __restrict__ grad,
-    double* __restrict__ hess, double oscale) {
+    double* __restrict__ hess, double oscale, const int* __restrict__ tfresh, int accum) {
     constexpr int NOUT = near_chan(NTAB);
     const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
     const int item = __builtin_amdgcn_readfirstlane(blockIdx.x * kNearWaves + wave);
@@ -367,6 +380,7 @@ __global__ __launch_bounds__(64 * kNearWaves) void k_near_split(
     const int it = item / nd, d = item % nd;
     const int q = lane >> 4, c = lane & 15;
     const int b = tgt[it];
+    const bool fresh = accum == 0 && tfresh[it] != 0;  // as in k_near
     const int p0 = ptr[it], p1 = ptr[it + 1];
     const double* T = tnearT + (size_t)tlev[it] * 13 * 3 * 256;
     const int nn = n * n;
@@ -416,7 +430,7 @@ __global__ __launch_bounds__(64 * kNearWaves) void k_near_split(
 #pragma unroll
                 for (int o = 0; o < NOUT; o++) {
                     double& v = dst[o][(pb * stride_o[o] + comp_o[o]) * nd + d];
-                    v += oscale * (double)out[o][r];
+                    v = (fresh ? 0.0 : v) + oscale * (double)out[o][r];
                 }
             }
         }
@@ -454,7 +468,7 @@ __global__ __launch_bounds__(64 * kNearWaves) void k_near_pack(
     int n, int nd, int nitems, const int* __restrict__ tgt, const int* __restrict__ tlev,
     const int* __restrict__ ptr, const NearPair* __restrict__ pairs, const double* __restrict__ tnearT,
     const double* __restrict__ fvals, double* __restrict__ pot, double* __restrict__ grad,
-    double* __restrict__ hess, double oscale) {
+    double* __restrict__ hess, double oscale, const int* __restrict__ tfresh, int accum) {
     static_assert(NKS <= 2, "two densities per tile need n <= 8");
     constexpr int NX = NTAB == 3 ? 2 : 1;  // tiles, one wave each
     const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
@@ -464,6 +478,7 @@ __global__ __launch_bounds__(64 * kNearWaves) void k_near_pack(
     const int X = item % NX, it = item / NX / ndp, d0 = 2 * (item / NX % ndp);
     const int q = lane >> 4, c = lane & 15, h = c >> 3, cc = c & 7;
     const int b = tgt[it];
+    const bool fresh = accum == 0 && tfresh[it] != 0;  // as in k_near
     const int p0 = ptr[it], p1 = ptr[it + 1];
     const double* T = tnearT + (size_t)tlev[it] * 13 * 3 * 256;
     const int nn = n * n;
@@ -522,7 +537,7 @@ __global__ __launch_bounds__(64 * kNearWaves) void k_near_pack(
                 const int str = o[r] <= 0 ? 1 : (o[r] <= 2 ? 2 : 3);
                 const int cmp = o[r] <= 0 ? 0 : (o[r] <= 2 ? o[r] - 1 : o[r] - 3);
                 const size_t pb = (size_t)b * nn + cl + (size_t)n * min(4 * (r % 2) + q, n - 1);
-                old[hh][r] = o[r] >= 0 ? dst[(pb * str + cmp) * nd + dd] : 0.0;
+                old[hh][r] = (o[r] >= 0 && !fresh) ? dst[(pb * str + cmp) * nd + dd] : 0.0;
             }
         }
 #pragma unroll
@@ -653,7 +668,7 @@ int launch_near_mfma3(Plan& p, const double* fvals, double* pot, double* grad, d
     int blocks = (nitems + kNearWaves - 1) / kNearWaves;
     k_near_mfma<NKS, NTAB, NSPLIT><<<blocks, 64 * kNearWaves, 0, p.stream>>>(
         p.n, p.nd, nitems, p.d_near_tgt, p.d_near_lev, p.d_near_ptr, p.d_near, p.d_tnearT, fvals, pot, grad, hess,
-        p.out_scale);
+        p.out_scale, p.d_near_fresh, p.accum);
     BFGT_HIP_CHECK(hipGetLastError());
     return BFGT_OK;
 }
@@ -684,7 +699,7 @@ int launch_near_pack2(Plan& p, const double* fvals, double* pot, double* grad, d
     int blocks = (nitems + kNearWaves - 1) / kNearWaves;
     k_near_pack<NKS, NTAB><<<blocks, 64 * kNearWaves, 0, p.stream>>>(p.n, p.nd, nitems, p.d_near_tgt, p.d_near_lev,
                                                                    p.d_near_ptr, p.d_near, p.d_tnearT, fvals, pot,
-                                                                   grad, hess, p.out_scale);
+                                                                   grad, hess, p.out_scale, p.d_near_fresh, p.accum);
     BFGT_HIP_CHECK(hipGetLastError());
     return BFGT_OK;
 }
@@ -695,7 +710,7 @@ int launch_near_split2(Plan& p, const double* fvals, double* pot, double* grad, 
     int blocks = (nitems + kNearWaves - 1) / kNearWaves;
     k_near_split<NKS, NTAB><<<blocks, 64 * kNearWaves, 0, p.stream>>>(p.n, p.nd, nitems, p.d_near_tgt, p.d_near_lev,
                                                                     p.d_near_ptr, p.d_near, p.d_tnearT, fvals, pot,
-                                                                    grad, hess, p.out_scale);
+                                                                    grad, hess, p.out_scale, p.d_near_fresh, p.accum);
     BFGT_HIP_CHECK(hipGetLastError());
     return BFGT_OK;
 }
@@ -780,7 +795,8 @@ int launch_near(Plan& p, const double* fvals, double* pot, double* grad, double*
     int ntab = p.ifpgh;
     size_t sh = near_smem_bytes(p.n, ntab);
     k_near<<<p.nnear_t, kNearThreads, sh, p.stream>>>(p.n, p.nd, ntab, p.d_near_tgt, p.d_near_lev, p.d_near_ptr,
-                                                      p.d_near, p.d_tloc, fvals, pot, grad, hess, p.out_scale, 6);
+                                                      p.d_near, p.d_tloc, fvals, pot, grad, hess, p.out_scale, 6,
+                                                      p.d_near_fresh, p.accum);
     BFGT_HIP_CHECK(hipGetLastError());
     return BFGT_OK;
 }
@@ -794,7 +810,7 @@ int launch_near_list(Plan& p, int ntgt, const int* tgt, const int* tlev, const i
     int ntab = p.ifpgh;
     size_t sh = near_smem_bytes(p.n, ntab);
     k_near<<<ntgt, kNearThreads, sh, p.stream>>>(p.n, p.nd, ntab, tgt, tlev, ptr, pairs, tloc, fvals, pot, grad,
-                                                 hess, 1.0, half);
+                                                 hess, 1.0, half, nullptr, 1);
     BFGT_HIP_CHECK(hipGetLastError());
     return BFGT_OK;
 }

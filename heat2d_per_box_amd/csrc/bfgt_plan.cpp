@@ -901,18 +901,26 @@ int plan_create(Plan& p, int nd, int ndim, double delta, double eps, int ipoly, 
 
     if ((rc = upload(p, p.d_pwleaf, leaves))) return rc;
     {
-        // leaves that no PW evaluation writes: above the cutoff level, or
-        // outside the PW lists, or every leaf of a plan without PW leaves
-        std::vector<char> pwl(t.nboxes, 0);
+        // for the overwriting execution: leaves that no PW evaluation writes
+        // (above the cutoff level, or every leaf of a plan without PW leaves)
+        // are written by the near field where they are its targets and
+        // cleared beforehand where they are not
+        std::vector<char> pwl(t.nboxes, 0), isnt(t.nboxes, 0);
         if (p.has_pw)
             for (int l = p.nlevstart; l <= nlevels; l++)
                 for (int b = t.first(l); b <= t.last(l); b++)
                     if (pwexp[b] == 1 && t.nchild[b] == 0) pwl[b] = 1;
+        std::vector<int> fresh(ntgt.size());
+        for (size_t i = 0; i < ntgt.size(); i++) {
+            fresh[i] = pwl[ntgt[i]] ? 0 : 1;
+            isnt[ntgt[i]] = 1;
+        }
         std::vector<int> nonpw;
         for (int b = 0; b < t.nboxes; b++)
-            if (t.nchild[b] == 0 && !pwl[b]) nonpw.push_back(b);
+            if (t.nchild[b] == 0 && !pwl[b] && !isnt[b]) nonpw.push_back(b);
         p.nnonpw = (int)nonpw.size();
         if ((rc = upload(p, p.d_nonpw, nonpw))) return rc;
+        if ((rc = upload(p, p.d_near_fresh, fresh))) return rc;
     }
     if ((rc = upload(p, p.d_pwpair, pairs_pw))) return rc;
     if ((rc = upload(p, p.d_colgrp, colgrp))) return rc;
