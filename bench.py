@@ -453,7 +453,8 @@ def main():
     kern = {"leaf_to_pw": "k_up", "merge": "k_merge", "translate": "k_m2l", "pw_to_values": "k_down",
             "near_field": "k_near"}
     roof["kernel"] = kern[dom]
-    tr = measured_traffic(f"{args.config}/{args.scale}", args.precision, kern[dom])
+    # the recorded counters are of the overwriting execution
+    tr = measured_traffic(f"{args.config}/{args.scale}", args.precision, kern[dom]) if ow else None
     roof["traffic"] = tr["bytes"] if tr else None
     if tr:
         roof["traffic_detail"] = tr
