@@ -5,11 +5,11 @@ pot (+grad, hess) = scale * A fvals on the leaf boxes, whatever the arrays
 held, must equal scale * A fvals accumulated into zeroed arrays bit for bit
 (both add the same FP64 terms in the same order onto 0.0), and entries of
 non-leaf boxes must keep what they held.  Covered: PW-only, mixed and
-near-field-dominated trees (leaves above the cutoff level, which the
-overwriting call clears before the near field adds into them), a plan with
-no PW leaves, periodic and free space, n <= 8 (2 x 2 leaf blocks, leaf
-pairs), n = 16 (column groups), the vector kernels (n = 20), split-FP32 and
-one to three densities.
+near-field-dominated trees (leaves above the cutoff level, which the near
+field writes instead of adding to), a plan with no PW leaves, periodic and
+free space, n <= 8 (2 x 2 leaf blocks, leaf pairs), n = 16 (column groups),
+the vector kernels (n = 20), split-FP32, one to three densities, and the
+timed c2 and c5 workloads at full size.
 """
 import numpy as np
 import pytest
@@ -74,19 +74,21 @@ def test_overwrite_equals_accumulate_on_zeros(device, name, scale_kind, kw, prec
     _check(w.tree, acc, ovw, before)
 
 
-@pytest.mark.parametrize("name", ["c2", "c5"])
-def test_overwrite_mixed_full_size(device, name):
-    """The headline shapes with leaves above the cutoff level (cleared, then
-    the near field adds into them)."""
-    w = make_workload(name, "mixed")
+@pytest.mark.parametrize("name,scale_kind", [("c2", "full"), ("c5", "full"), ("c2", "mixed"), ("c5", "mixed")])
+def test_overwrite_full_size(device, name, scale_kind):
+    """The timed workloads at full size (c2 is the bench's default), and the
+    headline shapes with leaves above the cutoff level, which the near field
+    writes instead of adding to."""
+    w = make_workload(name, scale_kind)
     acc, ovw, before, info = _both(w.tree, w.fvals, w.nd, w.delta, w.eps, w.ipoly, w.iperiod, w.norder, w.ifpgh)
-    assert info["n_near_pairs"] > 0 and info["n_pw_leaves"] < w.tree.nleaves
+    if scale_kind == "mixed":
+        assert info["n_near_pairs"] > 0 and info["n_pw_leaves"] < w.tree.nleaves
     _check(w.tree, acc, ovw, before)
 
 
 def test_overwrite_without_pw_leaves(device):
-    """delta small enough that every leaf is a near-field target: all leaves
-    are cleared, then the near field adds (nd = 2, pot + grad + hess)."""
+    """delta small enough that every leaf is a near-field target: the near
+    field writes every leaf (nd = 2, pot + grad + hess)."""
     t = build_uniform(3, iperiod=1)
     rng = np.random.default_rng(4)
     fv = rng.standard_normal((t.centers.shape[0], 64, 2))
