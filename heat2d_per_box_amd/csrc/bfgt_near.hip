@@ -563,6 +563,91 @@ __global__ __launch_bounds__(64 * kNearWaves) void k_near_pack(
 //   W_v   = sum_{o: x(o) = v} G_o Ty_{y(o)}   (K = target y index)
 //   fbar += sum_v Tx_v^T W_v                (K = target x index)
 // with element s of W_v's accumulator the B operand of K-step s.
+// Split-FP32 counterpart of k_near_pack (n <= 8, two or more densities):
+// the same tiles, x contraction FF_X[(h, j2)][(g, k1)] on the FP32 matrix
+// cores with hi/lo operands.  The FP32 accumulator holds row 4q + r of FF in
+// element r of lane (q, c) -- density half q / 2, source y index
+// 4 (q % 2) + r -- so the y contraction of density hh takes K-step r with K
+// index 4q + r and an A operand that is zero on the lanes of the other
+// density half (two products per density instead of the FP64 kernel's split
+// of the K-steps between the densities).  The output accumulator's element r
+// of lane (q, c) is row 4q + r: y-table half q / 2, target y index
+// 4 (q % 2) + r.
+template <int NKS, int NTAB>
+__global__ __launch_bounds__(64 * kNearWaves) void k_near_pack_split(
+    int n, int nd, int nitems, const int* __restrict__ tgt, const int* __restrict__ tlev,
+    const int* __restrict__ ptr, const NearPair* __restrict__ pairs, const double* __restrict__ tnearT,
+    const double* __restrict__ fvals, double* __restrict__ pot, double* __restrict__ grad,
+    double* __restrict__ hess, double oscale, const int* __restrict__ tfresh, int accum) {
+    static_assert(NKS <= 2, "two densities per tile need n <= 8");
+    constexpr int NX = NTAB == 3 ? 2 : 1;  // tiles, one wave each
+    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+    const int item = __builtin_amdgcn_readfirstlane(blockIdx.x * kNearWaves + wave);
+    if (item >= nitems) return;  // no block-wide barriers in this kernel
+    const int ndp = (nd + 1) / 2;
+    const int X = item % NX, it = item / NX / ndp, d0 = 2 * (item / NX % ndp);
+    const int q = lane >> 4, c = lane & 15, h = c >> 3, cc = c & 7;
+    const int b = tgt[it];
+    const bool fresh = accum == 0 && tfresh[it] != 0;  // as in k_near
+    const int p0 = ptr[it], p1 = ptr[it + 1];
+    const double* T = tnearT + (size_t)tlev[it] * 13 * 3 * 256;
+    const int nn = n * n;
+    const int cl = min(cc, n - 1);
+    const int dh = min(d0 + h, nd - 1);
+    const int tx = h ? pack_tx<NTAB>(X, 1) : pack_tx<NTAB>(X, 0);
+    const int ty = h ? pack_ty<NTAB>(X, 1) : pack_ty<NTAB>(X, 0);
+    const int qh = q >> 1;  // density half of the FF rows this lane holds
+    v4f out[2] = {v4f{0, 0, 0, 0}, v4f{0, 0, 0, 0}};
+    for (int p = p0; p < p1; p += 2) {
+        const bool two = p + 1 < p1;
+        const NearPair pr[2] = {pairs[p], pairs[two ? p + 1 : p]};
+        v4f FF[2];
+#pragma unroll
+        for (int s2 = 0; s2 < 2; s2++) {
+            const double* fs = fvals + (size_t)pr[s2].src * nn * nd + dh;
+            const double* Tx = T + (size_t)(pr[s2].ix + 6) * 3 * 256;
+            const double wgt = (s2 == 0 || two) ? 1.0 : 0.0;
+            FF[s2] = v4f{0, 0, 0, 0};
+#pragma unroll
+            for (int s = 0; s < NKS; s++) {
+                const int j1 = 4 * s + q;
+                const Split a = split(wgt * fs[(size_t)(min(j1, n - 1) + n * cl) * nd]);
+                FF[s2] = mfma3(a, split(Tx[tx * 256 + j1 * 16 + cc]), FF[s2]);
+            }
+        }
+#pragma unroll
+        for (int s2 = 0; s2 < 2; s2++) {
+            const double* Ty = T + (size_t)(pr[s2].iy + 6) * 3 * 256;
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int j2 = 4 * (q & 1) + r;
+                const Split ya = split(Ty[ty * 256 + j2 * 16 + cc]);  // A[(v, k2 = cc)][j2]
+                const Split z = Split{0.0f, 0.0f};
+#pragma unroll
+                for (int hh = 0; hh < 2; hh++) out[hh] = mfma2(qh == hh ? ya : z, FF[s2][r], out[hh]);
+            }
+        }
+    }
+    if (p1 > p0) {
+        const int o = pack_chan<NTAB>(X, h, qh);
+        double* dst = o <= 0 ? pot : (o <= 2 ? grad : hess);
+        const int str = o <= 0 ? 1 : (o <= 2 ? 2 : 3);
+        const int cmp = o <= 0 ? 0 : (o <= 2 ? o - 1 : o - 3);
+#pragma unroll
+        for (int hh = 0; hh < 2; hh++) {
+            const int dd = d0 + hh;
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int k2 = 4 * (q & 1) + r;
+                if (o >= 0 && dd < nd && k2 < n && cc < n) {
+                    double& v = dst[(((size_t)b * nn + cc + (size_t)n * k2) * str + cmp) * nd + dd];
+                    v = (fresh ? 0.0 : v) + oscale * (double)out[hh][r];
+                }
+            }
+        }
+    }
+}
+
 template <int NKS, int NTAB, int NPP>
 __global__ __launch_bounds__(64 * kNearWaves) void k_near_adj_mfma(
     int n, int nd, int nitems, const int* __restrict__ src, const int* __restrict__ ptr,
@@ -704,6 +789,18 @@ int launch_near_pack2(Plan& p, const double* fvals, double* pot, double* grad, d
     return BFGT_OK;
 }
 
+template <int NTAB>
+int launch_near_pack_split(Plan& p, const double* fvals, double* pot, double* grad, double* hess) {
+    int nitems = p.nnear_t * ((p.nd + 1) / 2) * (NTAB == 3 ? 2 : 1);
+    int blocks = (nitems + kNearWaves - 1) / kNearWaves;
+    auto kern = p.n <= 4 ? k_near_pack_split<1, NTAB> : k_near_pack_split<2, NTAB>;
+    kern<<<blocks, 64 * kNearWaves, 0, p.stream>>>(p.n, p.nd, nitems, p.d_near_tgt, p.d_near_lev, p.d_near_ptr,
+                                                   p.d_near, p.d_tnearT, fvals, pot, grad, hess, p.out_scale,
+                                                   p.d_near_fresh, p.accum);
+    BFGT_HIP_CHECK(hipGetLastError());
+    return BFGT_OK;
+}
+
 template <int NKS, int NTAB>
 int launch_near_split2(Plan& p, const double* fvals, double* pot, double* grad, double* hess) {
     int nitems = p.nnear_t * p.nd;
@@ -775,6 +872,11 @@ size_t near_smem_bytes(int n, int ntab) { return sizeof(double) * (size_t)n * n 
 int launch_near(Plan& p, const double* fvals, double* pot, double* grad, double* hess) {
     if (p.nnear_t == 0) return BFGT_OK;
     if (near_use_mfma(p) && p.precision == BFGT_PREC_SPLIT32) {
+        if (p.n <= 8 && p.nd >= 2) {  // two densities per tile, as the FP64 pass
+            if (p.ifpgh == 1) return launch_near_pack_split<1>(p, fvals, pot, grad, hess);
+            if (p.ifpgh == 2) return launch_near_pack_split<2>(p, fvals, pot, grad, hess);
+            return launch_near_pack_split<3>(p, fvals, pot, grad, hess);
+        }
         switch ((p.n + 3) / 4) {
             case 1: return launch_near_split1<1>(p, fvals, pot, grad, hess);
             case 2: return launch_near_split1<2>(p, fvals, pot, grad, hess);

@@ -49,12 +49,14 @@ def test_split32_error_vs_oracle(device, name, scale, kw):
             assert errs[k] < 1e-5, errs
 
 
-@pytest.mark.parametrize("n,nd,ifpgh", [(8, 1, 2), (8, 3, 3), (16, 2, 3), (12, 1, 1)])
+@pytest.mark.parametrize("n,nd,ifpgh", [(8, 1, 2), (8, 3, 3), (4, 2, 2), (6, 4, 1), (5, 3, 3), (16, 2, 3),
+                                         (12, 1, 1)])
 def test_split32_near_field_only(device, n, nd, ifpgh):
     """A tree whose leaves are all near-field targets (delta = 1e-6): the
     split path's near field alone, against the oracle, with the bars above
-    (n = 8 with three densities is the case the FP64 path packs two densities
-    per tile for; the split near field runs one wave per target and density)."""
+    (n <= 8 with two or more densities packs two densities per tile, as the
+    FP64 path does, odd nd leaving the last tile half used; otherwise one
+    wave per target and density)."""
     from heat2d_per_box_amd.tree import build_uniform
 
     t = build_uniform(2)
