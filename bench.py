@@ -302,14 +302,14 @@ def main():
     ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5"])
     ap.add_argument("--scale", default="full", choices=["full", "mixed", "small", "tiny"],
                     help="mixed: c2 or c5 at full size with near-field leaves (floor below the cutoff level)")
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "split32"])
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "split32", "bf16x3"])
     ap.add_argument("--delta", type=float, default=None, help="override delta (config c3 only)")
     ap.add_argument("--eps", type=float, default=None, help="override eps (config c3 only)")
     ap.add_argument("--densities", type=int, default=None,
                     help="keep the first K densities of the workload (e.g. c3 with one density)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-tree", action="store_true", help="skip timing the adaptive tree build and re-mesh")
-    ap.add_argument("--no-split", action="store_true", help="skip timing the other precision")
+    ap.add_argument("--no-split", action="store_true", help="skip timing the other precisions")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--settle-s", type=float, default=SETTLE_S,
                     help="seconds of untimed executions before the warm-up (shorter under a profiler)")
@@ -482,10 +482,10 @@ def main():
         else:
             stages_roof[k] = {"bound": "hbm", "achieved_gbs": amt / t / 1e9, "frac": amt / t / 1e9 / HBM_PEAK_GBS}
 
-    # the other arithmetic on the same tree (split-FP32 leaf transforms vs all-FP64)
-    other = None
-    if not args.no_split:
-        oprec = "split32" if args.precision == "fp64" else "fp64"
+    # the other arithmetics on the same tree (split-FP32 and three-slice bf16
+    # leaf transforms vs all-FP64)
+    others = []
+    for oprec in ([] if args.no_split else [x for x in ("fp64", "split32", "bf16x3") if x != args.precision]):
         plan2 = BoxFGT(w.tree.copy(), w.nd, w.delta, w.eps, w.ipoly, w.norder, w.ifpgh, precision=oprec)
         pot2 = torch.zeros_like(pot)
         grad2 = torch.zeros_like(grad) if grad is not None else None
@@ -514,10 +514,11 @@ def main():
             if gpu_out[k] is not None:
                 den = np.linalg.norm(gpu_out[k][lv])
                 diff[k] = float(np.linalg.norm((out2[k] - gpu_out[k])[lv]) / den) if den > 0 else 0.0
-        other = {"precision": oprec, "ms_per_step": ms2, "value": npts / (ms2 * 1e-3),
-                 "time_ratio_vs_" + args.precision: ms2 / ms,
-                 "stage_ms": {k: 1e3 * v / max(np2, 1) for k, v in zip(names, ssum2)},
-                 "rel_diff_vs_" + args.precision + "_gpu": diff, "_out": out2}
+        others.append({"precision": oprec, "ran_precision": ["fp64", "split32", "bf16x3"][plan2.info["precision"]],
+                       "ms_per_step": ms2, "value": npts / (ms2 * 1e-3),
+                       "time_ratio_vs_" + args.precision: ms2 / ms,
+                       "stage_ms": {k: 1e3 * v / max(np2, 1) for k, v in zip(names, ssum2)},
+                       "rel_diff_vs_" + args.precision + "_gpu": diff, "_out": out2})
         plan2.close()
 
     # adjoint (transposed operator) on the same plan: cotangents of every output
@@ -629,13 +630,14 @@ def main():
         out["tree_remesh"] = tree_leg()
         print("[bench] timing ifnewtree=1", file=sys.stderr, flush=True)
         out["newtree"] = newtree_leg()
-    if other is not None:
-        out2 = other.pop("_out")
+    outs2 = {}
+    for other in others:
+        outs2[other["precision"]] = other.pop("_out")
         out[other["precision"]] = other
     if not args.no_cpu_baseline:
         print("[bench] timing the CPU baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(w, gpu_out, args.cpu_threads,
-                                           {other["precision"]: out2} if other is not None else None)
+                                           outs2 or None)
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out))

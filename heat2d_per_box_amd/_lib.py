@@ -27,7 +27,7 @@ STATUS = {
     12: "BFGT_ERR_RESOURCE",
     13: "BFGT_ERR_CALLBACK",
 }
-PRECISIONS = {"fp64": 0, "split32": 1}
+PRECISIONS = {"fp64": 0, "split32": 1, "bf16x3": 2}
 
 EXPORTS = [
     "bfgt_hip_eval",

@@ -23,6 +23,8 @@ struct cplx {
 // One PW leaf: the leaf box, its level, its ancestor at the cutoff level,
 // its cell (jx, jy) inside that ancestor at the leaf's level, and the scaled
 // offset (c_leaf - c_anc)/sqrt(delta0).
+typedef short v4s __attribute__((ext_vector_type(4)));  // four bf16 bit patterns (MFMA operand)
+
 struct PwLeaf {
     int box;
     int level;
@@ -98,6 +100,9 @@ struct Plan {
     int nnonpw = 0;            // leaves that neither a PW evaluation nor the near field writes
     int* d_nonpw = nullptr;    // their boxes
     int* d_near_fresh = nullptr;  // per near-field target: 1 if no PW evaluation writes it
+    // three-slice bf16 passes: phased, sliced tables per (level, cell)
+    int b3nkb = 0;  // 16-long blocks of x frequencies of the downward x contraction
+    v4s *d_b3dx = nullptr, *d_b3dy = nullptr, *d_b3uy = nullptr, *d_b3ux = nullptr;
     int device = 0;
 
     // host copies used for scheduling
@@ -265,6 +270,46 @@ __device__ inline v4f mfma2(Split a, float b, v4f c) {
 }
 __device__ inline int drow32(int q, int r) { return 4 * q + r; }
 
+// Three-slice bf16 products (BFGT_PREC_BF16X3): a value x is cut into three
+// bf16 slices, x = h + m + l + O(2^-26 |x|) (each slice the bf16 nearest to
+// what the previous ones leave), and a*b is formed from the six slice
+// products of weight above 2^-24 -- hl, lh, mm, mh, hm, hh, smallest first --
+// each exact in FP32, on v_mfma_f32_16x16x16_bf16 with FP32 accumulation.
+// Lane layout on gfx950 (measured): A holds A[c][4q..4q+3], B holds
+// B[4q..4q+3][c], element r of the accumulator holds D[4q + r][c]: the four
+// elements of an FP32 accumulator are exactly the B operand of a following
+// product whose 16-long K dimension is that accumulator's rows.
+typedef __bf16 v4bf __attribute__((ext_vector_type(4)));
+struct Slices3 {
+    v4s h, m, l;
+};
+// slices of four FP32 values: each slice rounds (to nearest even, the
+// packed conversion v_cvt_pk_bf16_f32) what the previous ones leave, and
+// those residuals are exact in FP32
+__device__ inline Slices3 slice3(float x0, float x1, float x2, float x3) {
+    const v4bf h = {(__bf16)x0, (__bf16)x1, (__bf16)x2, (__bf16)x3};
+    const float r0 = x0 - (float)h[0], r1 = x1 - (float)h[1], r2 = x2 - (float)h[2], r3 = x3 - (float)h[3];
+    const v4bf m = {(__bf16)r0, (__bf16)r1, (__bf16)r2, (__bf16)r3};
+    const v4bf l = {(__bf16)(r0 - (float)m[0]), (__bf16)(r1 - (float)m[1]), (__bf16)(r2 - (float)m[2]),
+                    (__bf16)(r3 - (float)m[3])};
+    return Slices3{__builtin_bit_cast(v4s, h), __builtin_bit_cast(v4s, m), __builtin_bit_cast(v4s, l)};
+}
+__device__ inline Slices3 slice3(const float (&x)[4]) { return slice3(x[0], x[1], x[2], x[3]); }
+// of four FP64 values: their FP32 roundings carry the 2^-24 the slices keep
+__device__ inline Slices3 slice3(const double (&x)[4]) {
+    return slice3((float)x[0], (float)x[1], (float)x[2], (float)x[3]);
+}
+__device__ inline v4f mfmab(v4s a, v4s b, v4f c) { return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0); }
+// c += a*b over a 16-long K block, both factors sliced
+__device__ inline v4f mfma6(const Slices3& a, const Slices3& b, v4f c) {
+    c = mfmab(a.h, b.l, c);
+    c = mfmab(a.l, b.h, c);
+    c = mfmab(a.m, b.m, c);
+    c = mfmab(a.m, b.h, c);
+    c = mfmab(a.h, b.m, c);
+    return mfmab(a.h, b.h, c);
+}
+
 // status helpers
 int set_error(int code, const std::string& msg);
 
@@ -354,6 +399,7 @@ int launch_targets(Plan& p, int ntarg, const double* targs, int ifpghtarg, const
 // adjoint (bfgt_pw.hip, bfgt_near.hip)
 bool use_mfma(const Plan& p);
 int build_padded_tables(Plan& p);
+int build_b3_tables(Plan& p);
 size_t fold_len(const Plan& p);
 int launch_adj_eval(Plan& p, const double* gpot, const double* ggrad, const double* ghess);
 int launch_adj_m2l(Plan& p);

@@ -871,7 +871,7 @@ size_t near_smem_bytes(int n, int ntab) { return sizeof(double) * (size_t)n * n 
 
 int launch_near(Plan& p, const double* fvals, double* pot, double* grad, double* hess) {
     if (p.nnear_t == 0) return BFGT_OK;
-    if (near_use_mfma(p) && p.precision == BFGT_PREC_SPLIT32) {
+    if (near_use_mfma(p) && p.precision != BFGT_PREC_FP64) {  // split-FP32 for both reduced precisions
         if (p.n <= 8 && p.nd >= 2) {  // two densities per tile, as the FP64 pass
             if (p.ifpgh == 1) return launch_near_pack_split<1>(p, fvals, pot, grad, hess);
             if (p.ifpgh == 2) return launch_near_pack_split<2>(p, fvals, pot, grad, hess);

@@ -369,7 +369,7 @@ int plan_create(Plan& p, int nd, int ndim, double delta, double eps, int ipoly, 
     if (nd < 1 || !(delta > 0) || !(eps > 0) || eps >= 1 || !itree || !iptr || !centers || !boxsize)
         return set_error(BFGT_ERR_ARG, "invalid scalar or null array argument");
     if (iperiod != 0 && iperiod != 1) return set_error(BFGT_ERR_ARG, "iperiod must be 0 or 1");
-    if (precision != BFGT_PREC_FP64 && precision != BFGT_PREC_SPLIT32)
+    if (precision != BFGT_PREC_FP64 && precision != BFGT_PREC_SPLIT32 && precision != BFGT_PREC_BF16X3)
         return set_error(BFGT_ERR_ARG, "unknown precision");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return set_error(BFGT_ERR_NODEVICE, "no GPU visible");
@@ -494,6 +494,14 @@ int plan_create(Plan& p, int nd, int ndim, double delta, double eps, int ipoly, 
         if ((rc = dalloc(p, p.d_tdn_f, 3 * per))) return rc;
         if ((rc = dalloc(p, p.d_tdn_a, per))) return rc;
         if ((rc = build_padded_tables(p))) return rc;
+        if (p.precision == BFGT_PREC_BF16X3) {
+            // the phased, sliced tables per (level, cell); a tree too deep
+            // below the cutoff level for them runs split-FP32 instead
+            const long ncell = (1L << (nlevels - p.nlevstart + 1)) - 1;
+            const long bytes = ncell * 6 * 64 * 8 * (long)(2 * ifpgh + 2) * (((npw + 1) / 2 + 15) / 16 + 1);
+            if (bytes > (256L << 20)) p.precision = BFGT_PREC_SPLIT32;
+            else if ((rc = build_b3_tables(p))) return rc;
+        }
     }
 
     // PW schedule: leaves grouped by their cutoff-level ancestor
@@ -1601,7 +1609,7 @@ int bfgt_plan_work(bfgt_plan_t plan, long long* counts, int ncounts) {
     if (!plan || (!counts && ncounts > 0)) return set_error(BFGT_ERR_ARG, "null argument");
     const bfgt::Plan& p = plan->p;
     const long long L = p.npwleaf;
-    const bool fp64_mfma = p.has_pw && bfgt::use_mfma(p) && p.precision != BFGT_PREC_SPLIT32;
+    const bool fp64_mfma = p.has_pw && bfgt::use_mfma(p) && p.precision == BFGT_PREC_FP64;
     const bool big = p.n > 8;
     // x contractions of the downward pass: one per column group (n > 8), one
     // per column of a column pair evaluated as 2 x 2 blocks (n <= 8, when

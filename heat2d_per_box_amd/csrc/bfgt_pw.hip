@@ -1405,6 +1405,323 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_down_split_quad(
 }
 
 // ---------------------------------------------------------------------------
+// three-slice bf16 variants of the leaf transforms (v_mfma_f32_16x16x16_bf16)
+// ---------------------------------------------------------------------------
+//
+// BFGT_PREC_BF16X3: every operand is cut into three bf16 slices and each
+// product formed from six slice products accumulated in FP32 (mfma6).  The
+// point and frequency contractions are 16 long, one K block of the bf16
+// instruction; its accumulator's element r of lane (q, c) is row 4q + r, the
+// four K values lane (q, c) holds as the B operand of the next product, so
+// the two contractions of each pass chain through registers as in the FP64
+// kernels.  Schedules as the split-FP32 column kernels (n <= 16, one wave per
+// (chunk, y tile) upward, per column group downward); tables, phases and
+// their products in FP64, the merge, translation and output accumulation in
+// FP64.
+
+// The phased tables of both passes depend only on the level, the cell of
+// the leaf inside its cutoff box and the frequencies, so they are formed and
+// sliced once per plan (k_b3_tables), per cell: the cells of level l are
+// numbered (2^(l - L0) - 1) + j, j the cell's x or y index.  Each entry is
+// the six operand registers (slices h, m, l of the real part, then of the
+// second part) of one lane of one 16 x 16 block, stored lane-contiguous:
+//   kind 0, down x (B):  E_t[m][i] px_j[m]         (K m = 16 b + 4q + k, col i = c)
+//   kind 1, down y (A):  E_u[k2][i] py_j[k2] w(k2)  (row i = c, K k2 = 16 b + 4q + k; real, -imag)
+//   kind 2, up y (B):    T[m2][k2] conj(py_j[k2])   (K m2 = 4q + k, col k2 = 16 b + c)
+//   kind 3, up x (A):    T[m1][k1] conj(px_j[k1])   (row k1 = 16 b + c, K m1 = 4q + k)
+// zero past the retained frequencies (H) where those meet nonzero operands.
+__device__ inline int b3_cell(int level, int lev0, int j) { return (1 << (level - lev0)) - 1 + j; }
+__device__ inline void b3_load(const v4s* __restrict__ tab, size_t entry, int lane, Slices3& a, Slices3& b) {
+    const v4s* e = tab + entry * 6 * 64 + lane;
+    a.h = e[0];
+    a.m = e[64];
+    a.l = e[128];
+    b.h = e[192];
+    b.m = e[256];
+    b.l = e[320];
+}
+
+__global__ void k_b3_tables(int kind, int P, int ps, int lev0, int ncell, int ntab, int nblk,
+                            const cplx* __restrict__ tup_f, const cplx* __restrict__ tdn_f,
+                            const cplx* __restrict__ phase, const int* __restrict__ levbase, v4s* __restrict__ out) {
+    const int H = (P + 1) / 2, P2 = P / 2;
+    const long tot = (long)ncell * ntab * nblk * 64;
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < tot; e += (long)gridDim.x * blockDim.x) {
+        const int lane = (int)(e % 64);
+        long r = e / 64;
+        const int b = (int)(r % nblk);
+        r /= nblk;
+        const int t = (int)(r % ntab);
+        const int cell = (int)(r / ntab);
+        const int lo = 31 - __clz(cell + 1);
+        const int l = lev0 + lo, j = cell + 1 - (1 << lo);
+        const cplx* ph = phase + levbase[l] + (size_t)j * ps;
+        const cplx* E = tdn_f + (size_t)l * 3 * 16 * ps + (size_t)t * 16 * ps;  // [m][i]
+        const cplx* T = tup_f + (size_t)l * 16 * ps;                             // [m][k]
+        const int q = lane >> 4, c = lane & 15;
+        double re[4], im[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            cplx v{0.0, 0.0};
+            if (kind == 0) {
+                const int m = 16 * b + 4 * q + k;
+                if (m < H) v = cmul(E[m * 16 + c], ph[m]);
+            } else if (kind == 1) {
+                const int k2 = 16 * b + 4 * q + k;
+                if (k2 < H) {
+                    v = cmul(E[k2 * 16 + c], ph[k2]);
+                    const double w = k2 >= P2 ? 0.5 : 1.0;
+                    v = cplx{w * v.x, -w * v.y};
+                }
+            } else if (kind == 2) {
+                const int k2 = 16 * b + c;
+                const cplx p = ph[k2];
+                v = cmul(T[(4 * q + k) * ps + k2], cplx{p.x, -p.y});
+            } else {
+                const int k1 = 16 * b + c;
+                const cplx p = ph[k1];
+                v = cmul(T[(4 * q + k) * ps + k1], cplx{p.x, -p.y});
+            }
+            re[k] = v.x;
+            im[k] = v.y;
+        }
+        const Slices3 sr = slice3(re), si = slice3(im);
+        v4s* o = out + (size_t)(r * nblk + b) * 6 * 64 + lane;
+        o[0] = sr.h;
+        o[64] = sr.m;
+        o[128] = sr.l;
+        o[192] = si.h;
+        o[256] = si.m;
+        o[320] = si.l;
+    }
+}
+
+// Upward pass: per column piece G = sum over its leaves of F (T diag py)
+// (rows: x points, K: y points; the leaf values sliced per leaf, the phased
+// table slices read per leaf cell), then acc += (T diag px)^T G (K: x
+// points; G sliced once per column, the phased x table read per column).
+template <int NTM, bool BR>
+__global__ __launch_bounds__(kWave * kUpReduceWaves) void k_up_b3(
+    int n, int P, int nd, int nds, int nTN, int nchunks, const double* __restrict__ fvals,
+    const PwLeaf* __restrict__ leaves, const int4* __restrict__ cpc, const int2* __restrict__ cols, int lev0,
+    const v4s* __restrict__ uy, const v4s* __restrict__ ux, cplx* __restrict__ partial) {
+    constexpr int NACC = 16 * NTM;  // accumulator floats per lane
+    // with BR the sums of waves 1.. of a workgroup, as in k_up_col
+    __shared__ float red[BR ? (kUpReduceWaves - 1) * NACC * kWave : 1];
+    const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    // workgroup b: y tile b % nTN (and density (b / nTN) % nd with nds = nd)
+    // of kUpReduceWaves consecutive chunks, XCD-contiguous as in k_up_col
+    const int lb = xcd_block(blockIdx.x, gridDim.x);
+    const int tN = lb % nTN;
+    const int d0 = nds > 1 ? (lb / nTN) % nds : 0, d1 = nds > 1 ? d0 + 1 : nd;
+    const int ci = __builtin_amdgcn_readfirstlane((lb / nTN / nds) * kUpReduceWaves + wave);
+    const bool valid = ci < nchunks;  // waves past the end still meet the barriers
+    const int q = lane >> 4, cc = lane & 15;
+    const int H = (P + 1) / 2, Mlo = (P + 1) / 2, nexp = P * H, npbox = n * n;
+    const int k2 = 16 * tN + cc;
+    // this lane's leaf points: x = cc, y = 4q + k (clamped; rows y >= n and
+    // columns x >= n meet zero table rows)
+    int pidx[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) pidx[k] = min(cc, n - 1) + n * min(4 * q + k, n - 1);
+    const int4 ch = cpc[valid ? ci : 0];
+    for (int d = d0; d < d1; d++) {
+        v4f acc[4][NTM];
+#pragma unroll
+        for (int p = 0; p < 4; p++)
+#pragma unroll
+            for (int t = 0; t < NTM; t++) acc[p][t] = v4f{0, 0, 0, 0};
+        for (int c = 0; valid && c < ch.y; c++) {
+            const int2 col = cols[ch.x + c];
+            const PwLeaf l0 = leaves[col.x];
+            v4f Gr = {0, 0, 0, 0}, Gi = {0, 0, 0, 0};
+            // a leaf's operands (its phased y table slices and values) are
+            // loaded while the previous leaf's products are issued
+            struct UOp {
+                Slices3 br, bi;
+                double a[4];
+            };
+            auto uop_at = [&](int li) {
+                const PwLeaf lf = leaves[col.x + min(li, col.y - 1)];
+                UOp o;
+                b3_load(uy, (size_t)b3_cell(lf.level, lev0, lf.jy) * nTN + tN, lane, o.br, o.bi);
+                const size_t pb0 = (size_t)lf.box * npbox;
+#pragma unroll
+                for (int k = 0; k < 4; k++) o.a[k] = fvals[(pb0 + pidx[k]) * nd + d];
+                return o;
+            };
+            UOp uc = uop_at(0);
+            for (int li = 0; li < col.y; li++) {
+                const UOp un = uop_at(li + 1);
+                __builtin_amdgcn_sched_barrier(0);
+                const Slices3 sa = slice3(uc.a);
+                Gr = mfma6(sa, uc.br, Gr);
+                Gi = mfma6(sa, uc.bi, Gi);
+                uc = un;
+            }
+            // G's element k on lane (q, c) = G[x point 4q + k][k2]: the B operand as it stands
+            const Slices3 gr = slice3(Gr[0], Gr[1], Gr[2], Gr[3]), gi = slice3(Gi[0], Gi[1], Gi[2], Gi[3]);
+            const size_t ex = (size_t)b3_cell(l0.level, lev0, l0.jx) * NTM;
+#pragma unroll
+            for (int t = 0; t < NTM; t++) {
+                Slices3 sxr, sxi;
+                b3_load(ux, ex + t, lane, sxr, sxi);
+                acc[0][t] = mfma6(sxr, gr, acc[0][t]);  // AC
+                acc[1][t] = mfma6(sxi, gi, acc[1][t]);  // BD
+                acc[2][t] = mfma6(sxr, gi, acc[2][t]);  // AD
+                acc[3][t] = mfma6(sxi, gr, acc[3][t]);  // BC
+            }
+        }
+        if constexpr (BR) {
+            // the waves that continue a piece hand their sums to its first wave
+            if (valid && ch.w == 0) {
+                float* mine = red + (wave - 1) * NACC * kWave;
+#pragma unroll
+                for (int p = 0; p < 4; p++)
+#pragma unroll
+                    for (int t = 0; t < NTM; t++)
+#pragma unroll
+                        for (int r = 0; r < 4; r++) mine[((p * NTM + t) * 4 + r) * kWave + lane] = acc[p][t][r];
+            }
+            __syncthreads();
+            for (int w = wave + 1; valid && w < wave + ch.w; w++) {
+                const float* other = red + (w - 1) * NACC * kWave;
+#pragma unroll
+                for (int p = 0; p < 4; p++)
+#pragma unroll
+                    for (int t = 0; t < NTM; t++)
+#pragma unroll
+                        for (int r = 0; r < 4; r++) acc[p][t][r] += other[((p * NTM + t) * 4 + r) * kWave + lane];
+            }
+            __syncthreads();  // the slots are refilled for the next density
+        }
+        if (!valid || ch.w == 0) continue;
+        cplx* out = partial + ((size_t)ch.z * nd + d) * nexp;
+#pragma unroll
+        for (int t = 0; t < NTM; t++) {
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                int k1 = 16 * t + drow32(q, r);
+                if (k1 < Mlo && k2 < H) {
+                    double AC = acc[0][t][r], BD = acc[1][t][r], AD = acc[2][t][r], BC = acc[3][t][r];
+                    out[k1 * H + k2] = cplx{AC - BD, AD + BC};
+                    int km = P - 1 - k1;
+                    if (km > k1) out[km * H + k2] = cplx{AC + BD, AD - BC};
+                }
+            }
+        }
+    }
+}
+
+// Downward pass: per column group FF_t = S^T a_t + i D^T b_t (rows: y
+// frequencies of a tile, K: x frequencies in blocks of 16; the folded
+// expansion sliced per group, the phased x tables read per group), then per
+// leaf out_o += Re[(E_y(o) diag(py w))^T FF_x(o)] (K: the tile's y
+// frequencies; FF sliced once per group, the phased y tables read per leaf).
+template <int NTAB, int NT>
+__global__ __launch_bounds__(kWave * kWavesPerBlock) void k_down_b3(
+    int n, int P, int nd, int ngrp, const int2* __restrict__ grp, const PwLeaf* __restrict__ leaves, int lev0,
+    int nkb, const v4s* __restrict__ dx, const v4s* __restrict__ dy, const Fold* __restrict__ fold,
+    double* __restrict__ pot, double* __restrict__ grad, double* __restrict__ hess, double oscale, int accum) {
+    constexpr int NOUT = nchan(NTAB);
+    const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    const int item = __builtin_amdgcn_readfirstlane(xcd_block(blockIdx.x, gridDim.x) * kWavesPerBlock + wave);
+    if (item >= ngrp) return;  // no block-wide barriers in this kernel
+    const int q = lane >> 4, cc = lane & 15;
+    const int H = (P + 1) / 2, npbox = n * n;
+    const int Mh4 = round_up(H, 4), H16 = round_up(H, 16);
+    const int2 g = grp[item];
+    const PwLeaf l0 = leaves[g.x];
+    const size_t ex = (size_t)b3_cell(l0.level, lev0, l0.jx) * NTAB;
+    constexpr int stride_o[6] = {1, 2, 2, 3, 3, 3}, comp_o[6] = {0, 0, 1, 0, 1, 2};
+    double* dst[NOUT];
+    dst[0] = pot;
+    if constexpr (NTAB >= 2) dst[1] = dst[2] = grad;
+    if constexpr (NTAB >= 3) dst[3] = dst[4] = dst[5] = hess;
+    const double w2 = 2.0 * oscale;
+    for (int d = 0; d < nd; d++) {
+        const Fold* F = fold + ((size_t)l0.anc * nd + d) * Mh4 * H16;
+        v4f FFr[NTAB][NT], FFi[NTAB][NT];
+#pragma unroll
+        for (int t = 0; t < NTAB; t++)
+#pragma unroll
+            for (int tM = 0; tM < NT; tM++) FFr[t][tM] = FFi[t][tM] = v4f{0, 0, 0, 0};
+        for (int kb = 0; kb < nkb; kb++) {
+            // this lane's x frequencies m = 16 kb + 4q + k: rows past the
+            // folded expansion read its last row, against zero table rows
+            int mk[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++) mk[k] = min(16 * kb + 4 * q + k, Mh4 - 1);
+            Slices3 er[NTAB], ei[NTAB];
+#pragma unroll
+            for (int t = 0; t < NTAB; t++) b3_load(dx, (ex + t) * nkb + kb, lane, er[t], ei[t]);
+#pragma unroll
+            for (int tM = 0; tM < NT; tM++) {
+                double sr[4], si[4], dr[4], di[4];
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    const Fold f = F[mk[k] * H16 + 16 * tM + cc];
+                    sr[k] = f.S.x;
+                    si[k] = f.S.y;
+                    dr[k] = f.D.x;
+                    di[k] = -f.D.y;
+                }
+                const Slices3 ssr = slice3(sr), ssi = slice3(si), sdr = slice3(dr), sdi = slice3(di);
+#pragma unroll
+                for (int t = 0; t < NTAB; t++) {
+                    FFr[t][tM] = mfma6(ssr, er[t], FFr[t][tM]);
+                    FFr[t][tM] = mfma6(sdi, ei[t], FFr[t][tM]);
+                    FFi[t][tM] = mfma6(ssi, er[t], FFi[t][tM]);
+                    FFi[t][tM] = mfma6(sdr, ei[t], FFi[t][tM]);
+                }
+            }
+        }
+        // FF's element k on lane (q, c) = FF[y frequency 16 tM + 4q + k][x point c]
+        Slices3 bFr[NTAB][NT], bFi[NTAB][NT];
+#pragma unroll
+        for (int t = 0; t < NTAB; t++)
+#pragma unroll
+            for (int tM = 0; tM < NT; tM++) {
+                bFr[t][tM] = slice3(FFr[t][tM][0], FFr[t][tM][1], FFr[t][tM][2], FFr[t][tM][3]);
+                bFi[t][tM] = slice3(FFi[t][tM][0], FFi[t][tM][1], FFi[t][tM][2], FFi[t][tM][3]);
+            }
+        for (int li = 0; li < g.y; li++) {
+            const PwLeaf lf = leaves[g.x + li];
+            const size_t ey = (size_t)b3_cell(lf.level, lev0, lf.jy) * NTAB;
+            v4f out[NOUT];
+#pragma unroll
+            for (int o = 0; o < NOUT; o++) out[o] = v4f{0, 0, 0, 0};
+#pragma unroll
+            for (int tM = 0; tM < NT; tM++) {
+                Slices3 ar[NTAB], ai[NTAB];
+#pragma unroll
+                for (int u = 0; u < NTAB; u++) b3_load(dy, (ey + u) * NT + tM, lane, ar[u], ai[u]);
+#pragma unroll
+                for (int o = 0; o < NOUT; o++) {
+                    out[o] = mfma6(ar[chan_y(o)], bFr[chan_x(o)][tM], out[o]);
+                    out[o] = mfma6(ai[chan_y(o)], bFi[chan_x(o)][tM], out[o]);
+                }
+            }
+            // element r of out on lane (q, c): y point 4q + r, x point c
+            if (cc < n) {
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const int yp = drow32(q, r);
+                    if (yp < n) {
+                        const size_t pb = (size_t)lf.box * npbox + cc + (size_t)n * yp;
+#pragma unroll
+                        for (int o = 0; o < NOUT; o++) {
+                            double& v = dst[o][(pb * stride_o[o] + comp_o[o]) * nd + d];
+                            v = (accum ? v : 0.0) + w2 * (double)out[o][r];
+                        }
+                    }
+                }
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // vector-FMA kernels (leaf orders above 16 or expansions above 128 waves)
 // ---------------------------------------------------------------------------
 
@@ -2138,10 +2455,38 @@ static int launch_up_split(Plan& p, const double* fvals) {
     }
 }
 
+// chunks of one cutoff box summed per workgroup where there is a merge to
+// shrink and the FP32 accumulators fit LDS (at most two x tiles), as in the
+// FP64 pass
+template <int NTM>
+static int launch_up_b3_2(Plan& p, const double* fvals, int nTN) {
+    const int nds = split_density_waves(p, nTN);
+    const bool br = NTM <= 2 && p.nmerge > 0;
+    const int blocks = nTN * nds * ((p.nchunks + kUpReduceWaves - 1) / kUpReduceWaves);
+    auto kern = br ? k_up_b3<NTM, NTM <= 2> : k_up_b3<NTM, false>;
+    kern<<<blocks, kWave * kUpReduceWaves, 0, p.stream>>>(p.n, p.npw, p.nd, nds, nTN, p.nchunks, fvals, p.d_pwleaf,
+                                                          br ? p.d_upccb : p.d_upcc, p.d_upcols, p.nlevstart,
+                                                          p.d_b3uy, p.d_b3ux, p.d_mp);
+    p.merge_pieces = br;
+    BFGT_HIP_CHECK(hipGetLastError());
+    return BFGT_OK;
+}
+static int launch_up_b3(Plan& p, const double* fvals) {
+    const int H = (p.npw + 1) / 2;
+    const int nTM = (H + 15) / 16, nTN = (H + 15) / 16;
+    switch (nTM) {
+        case 1: return launch_up_b3_2<1>(p, fvals, nTN);
+        case 2: return launch_up_b3_2<2>(p, fvals, nTN);
+        case 3: return launch_up_b3_2<3>(p, fvals, nTN);
+        default: return launch_up_b3_2<4>(p, fvals, nTN);
+    }
+}
+
 int launch_upward(Plan& p, const double* fvals) {
     p.merge_pieces = false;
     if (p.nchunks == 0) return BFGT_OK;
     if (use_mfma(p) && p.precision == BFGT_PREC_SPLIT32) return launch_up_split(p, fvals);
+    if (use_mfma(p) && p.precision == BFGT_PREC_BF16X3) return launch_up_b3(p, fvals);
     if (use_mfma(p)) {
         UpArgs u{LeafIn{fvals, nullptr, nullptr}, p.d_tup_f, 16 * p.pstride, 0, 1};
         return launch_up_mfma(p, u);
@@ -2295,8 +2640,32 @@ static int launch_down_split(Plan& p, double* pot, double* grad, double* hess) {
     }
 }
 
+template <int NTAB>
+static int launch_down_b3(Plan& p, double* pot, double* grad, double* hess) {
+    const int nT = round_up((p.npw + 1) / 2, 16) / 16;
+    const int blocks = (p.ncolgrp + kWavesPerBlock - 1) / kWavesPerBlock;
+    auto run = [&](auto kern) -> int {
+        kern<<<blocks, kWave * kWavesPerBlock, 0, p.stream>>>(
+            p.n, p.npw, p.nd, p.ncolgrp, p.d_colgrp, p.d_pwleaf, p.nlevstart, p.b3nkb, p.d_b3dx, p.d_b3dy,
+            reinterpret_cast<const Fold*>(p.d_loc), pot, grad, hess, p.out_scale, p.accum);
+        BFGT_HIP_CHECK(hipGetLastError());
+        return BFGT_OK;
+    };
+    switch (nT) {
+        case 1: return run(k_down_b3<NTAB, 1>);
+        case 2: return run(k_down_b3<NTAB, 2>);
+        case 3: return run(k_down_b3<NTAB, 3>);
+        default: return run(k_down_b3<NTAB, 4>);
+    }
+}
+
 int launch_downward(Plan& p, double* pot, double* grad, double* hess) {
     if (p.npwleaf == 0) return BFGT_OK;
+    if (use_mfma(p) && p.precision == BFGT_PREC_BF16X3) {
+        if (p.ifpgh == 1) return launch_down_b3<1>(p, pot, grad, hess);
+        if (p.ifpgh == 2) return launch_down_b3<2>(p, pot, grad, hess);
+        return launch_down_b3<3>(p, pot, grad, hess);
+    }
     if (use_mfma(p) && p.precision == BFGT_PREC_SPLIT32) {
         if (p.ifpgh == 1) return launch_down_split<1>(p, pot, grad, hess);
         if (p.ifpgh == 2) return launch_down_split<2>(p, pot, grad, hess);
@@ -2358,6 +2727,39 @@ int launch_adj_m2l(Plan& p) {
     k_m2l<<<dim3(p.nanc, gy), 256, 0, p.stream>>>(p.nd, p.nexp, p.d_m2lT_ptr, p.d_m2lT, p.d_wshift, -1.0, p.d_mp,
                                                   p.d_loc);
     BFGT_HIP_CHECK(hipGetLastError());
+    return BFGT_OK;
+}
+
+// The phased, sliced tables of the three-slice bf16 passes (k_b3_tables):
+// every cell of every level from the cutoff level down.
+int build_b3_tables(Plan& p) {
+    if (!p.has_pw || !use_mfma(p)) return BFGT_OK;
+    const int H = (p.npw + 1) / 2;
+    const int nT = (H + 15) / 16;
+    p.b3nkb = (round_up(H, 4) + 15) / 16;
+    const int ncell = (1 << (p.nlevels - p.nlevstart + 1)) - 1;
+    const int ntab = p.ifpgh;
+    const size_t per = (size_t)6 * 64;  // v4s per entry
+    const size_t n_dx = (size_t)ncell * ntab * p.b3nkb * per, n_dy = (size_t)ncell * ntab * nT * per;
+    const size_t n_uy = (size_t)ncell * nT * per, n_ux = (size_t)ncell * nT * per;
+    auto alloc = [&](v4s*& dst, size_t count) -> int {
+        BFGT_HIP_CHECK(hipMalloc(&dst, sizeof(v4s) * count));
+        p.allocations.push_back(dst);
+        return BFGT_OK;
+    };
+    int rc;
+    if ((rc = alloc(p.d_b3dx, n_dx)) || (rc = alloc(p.d_b3dy, n_dy)) || (rc = alloc(p.d_b3uy, n_uy)) ||
+        (rc = alloc(p.d_b3ux, n_ux)))
+        return rc;
+    const int kinds[4][2] = {{ntab, p.b3nkb}, {ntab, nT}, {1, nT}, {1, nT}};
+    v4s* outs[4] = {p.d_b3dx, p.d_b3dy, p.d_b3uy, p.d_b3ux};
+    for (int k = 0; k < 4; k++) {
+        const long tot = (long)ncell * kinds[k][0] * kinds[k][1] * 64;
+        const int blocks = (int)std::min<long>((tot + 255) / 256, 4096);
+        k_b3_tables<<<blocks, 256, 0, p.stream>>>(k, p.npw, p.pstride, p.nlevstart, ncell, kinds[k][0], kinds[k][1],
+                                                   p.d_tup_f, p.d_tdn_f, p.d_phase, p.d_levbase, outs[k]);
+        BFGT_HIP_CHECK(hipGetLastError());
+    }
     return BFGT_OK;
 }
 

@@ -39,6 +39,7 @@ extern "C" {
 /* arithmetic of the leaf transforms and near-field contractions */
 #define BFGT_PREC_FP64 0    /* all-FP64 (the drop-in) */
 #define BFGT_PREC_SPLIT32 1 /* split-FP32 hi/lo products, FP64 weights/phases/accumulation */
+#define BFGT_PREC_BF16X3 2  /* leaf transforms as three-slice bf16 products (FP32 accumulation), near field split-FP32 */
 
 /*
  * One complete box FGT call on host arrays, synchronous.
