@@ -60,10 +60,12 @@ extern "C" {
  * hesse(nd,3,ntarg), which are overwritten (bfgtmain STEP 11,
  * bfgt/boxfgt.f:1494-1563); channels above ifpgh are computed into scratch
  * arrays and never written to grad/hess (bfgt/boxfgt.f:207-277).
- * `precision` selects BFGT_PREC_FP64 or BFGT_PREC_SPLIT32.  Split-FP32 applies
- * to the matrix-core leaf transforms and near field (norder <= 16, npw <=
- * 128); a plan outside those sizes runs FP64 and reports precision 0
- * (bfgt_plan_query).
+ * `precision` selects BFGT_PREC_FP64, BFGT_PREC_SPLIT32 or BFGT_PREC_BF16X3.
+ * The reduced precisions apply to the matrix-core leaf transforms and near
+ * field (norder <= 16, npw <= 128; BF16X3 runs the near field split-FP32); a
+ * plan outside those sizes runs FP64 and reports precision 0, and a BF16X3
+ * plan whose per-cell tables would exceed 256 MB runs split-FP32 and reports
+ * 1 (bfgt_plan_query).
  * The plan built for a call (schedules, tables, device buffers) is kept and
  * reused by later calls with the same tree arrays (laddr, levels, parents,
  * children, centres, box sizes), parameters and device -- the drivers call
