@@ -11,7 +11,8 @@
 ! nboxes, more than 200 plane waves, ...).
 !
 ! `boxfgt_set_precision(iprec)` selects the arithmetic of later calls:
-! 0 = all-FP64 (default), 1 = split-FP32 with FP64 weights and accumulation.
+! 0 = all-FP64 (default), 1 = split-FP32 with FP64 weights and accumulation,
+! 2 = three-slice bf16 leaf transforms (FP32 accumulation), near field split-FP32.
 ! The library keeps the plan of a call for later calls on the same tree with
 ! the same parameters; `boxfgt_release_plans()` frees the kept plans.
 module boxfgt_hip_mod
