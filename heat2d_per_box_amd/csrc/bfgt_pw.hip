@@ -1535,8 +1535,7 @@ __global__ __launch_bounds__(kWave * kUpReduceWaves) void k_up_b3(
             const int2 col = cols[ch.x + c];
             const PwLeaf l0 = leaves[col.x];
             v4f Gr = {0, 0, 0, 0}, Gi = {0, 0, 0, 0};
-            // a leaf's operands (its phased y table slices and values) are
-            // loaded while the previous leaf's products are issued
+            // a leaf's operands: its phased y table slices and values
             struct UOp {
                 Slices3 br, bi;
                 double a[4];
@@ -1550,14 +1549,16 @@ __global__ __launch_bounds__(kWave * kUpReduceWaves) void k_up_b3(
                 for (int k = 0; k < 4; k++) o.a[k] = fvals[(pb0 + pidx[k]) * nd + d];
                 return o;
             };
+            // the next leaf's operands are loaded into the same registers
+            // once this leaf's products are issued
             UOp uc = uop_at(0);
             for (int li = 0; li < col.y; li++) {
-                const UOp un = uop_at(li + 1);
-                __builtin_amdgcn_sched_barrier(0);
                 const Slices3 sa = slice3(uc.a);
                 Gr = mfma6(sa, uc.br, Gr);
                 Gi = mfma6(sa, uc.bi, Gi);
-                uc = un;
+                __builtin_amdgcn_sched_barrier(0);
+                if (li + 1 < col.y) uc = uop_at(li + 1);
+                __builtin_amdgcn_sched_barrier(0);
             }
             // G's element k on lane (q, c) = G[x point 4q + k][k2]: the B operand as it stands
             const Slices3 gr = slice3(Gr[0], Gr[1], Gr[2], Gr[3]), gi = slice3(Gi[0], Gi[1], Gi[2], Gi[3]);
