@@ -1609,18 +1609,24 @@ int bfgt_plan_work(bfgt_plan_t plan, long long* counts, int ncounts) {
     if (!plan || (!counts && ncounts > 0)) return set_error(BFGT_ERR_ARG, "null argument");
     const bfgt::Plan& p = plan->p;
     const long long L = p.npwleaf;
-    const bool fp64_mfma = p.has_pw && bfgt::use_mfma(p) && p.precision == BFGT_PREC_FP64;
+    const bool mfma = p.has_pw && bfgt::use_mfma(p);
+    const bool split = mfma && p.precision == BFGT_PREC_SPLIT32, b3 = mfma && p.precision == BFGT_PREC_BF16X3;
     const bool big = p.n > 8;
     // x contractions of the downward pass: one per column group (n > 8), one
     // per column of a column pair evaluated as 2 x 2 blocks (n <= 8, when
     // there are enough pairs), else one per leaf
     const long long xc_down = big ? p.ncolgrp : ((long long)p.nqgroup * p.nd >= 1024 ? p.nxcol_quad : L);
-    // the forward upward pass sums chunks per workgroup (at most two x
-    // tiles) and then merges workgroup pieces
-    const bool pieces = fp64_mfma && (p.npw + 1) / 2 <= 32 && p.nmerge > 0;
+    // forward passes: split-FP32 shares them as FP64 does, except that its
+    // n <= 8 downward pass always takes the column pairs; the three-slice
+    // bf16 passes run the column kernels at every order
+    const long long fwd_up = !mfma ? L : (b3 || big) ? p.nxcol_up : p.nxcol_upq;
+    const long long fwd_down = !mfma ? L : b3 ? p.ncolgrp : (split && !big) ? p.nxcol_quad : xc_down;
+    // the FP64 and three-slice bf16 upward passes sum chunks per workgroup
+    // (at most two x tiles) and then merge workgroup pieces
+    const bool pieces = mfma && !split && (p.npw + 1) / 2 <= 32 && p.nmerge > 0;
     long long v[7] = {
-        fp64_mfma ? (big ? p.nxcol_up : p.nxcol_upq) : L,        // forward upward pass
-        fp64_mfma ? xc_down : L,                                 // forward downward pass
+        fwd_up,                                                  // forward upward pass
+        fwd_down,                                                // forward downward pass
         L,                                                       // PW leaves
         p.has_pw ? (big || p.ifpgh > 1 ? p.nxcol_up : p.nxcol_upq) : L,  // adjoint evaluation
         p.has_pw ? xc_down : L,                                  // adjoint leaf transform
