@@ -25,12 +25,12 @@ def _noalloc_fn():
     return lib().bfgt_plan_execute_noalloc
 
 
-def _run(w, noalloc):
+def _run(w, noalloc, precision="fp64"):
     import torch
     from heat2d_per_box_amd._lib import check
     from heat2d_per_box_amd.boxfgt import BoxFGT
 
-    plan = BoxFGT(w.tree.copy(), w.nd, w.delta, w.eps, w.ipoly, w.norder, w.ifpgh)
+    plan = BoxFGT(w.tree.copy(), w.nd, w.delta, w.eps, w.ipoly, w.norder, w.ifpgh, precision=precision)
     nb, npb, nd = w.fvals.shape
     fv = torch.from_numpy(np.ascontiguousarray(w.fvals)).cuda()
     pot = torch.zeros((nb, npb, nd), dtype=torch.float64, device="cuda")
@@ -90,3 +90,21 @@ def test_noalloc_refuses_plans_above_the_limit(device):
     torch.cuda.synchronize()
     assert all(float(t.abs().max()) == 0.0 for t in outs)
     plan.close()
+
+
+@pytest.mark.parametrize("precision", ["split32", "bf16x3"])
+def test_noalloc_other_arithmetics(device, precision):
+    """Split-FP32 and three-slice bf16 plans (their own partial-sum and
+    folded-expansion use of the workspace): the same two calls in a row, bit
+    for bit against bfgt_plan_execute, and within those arithmetics' bar of
+    the oracle."""
+    for name, scale, kw in CALLS:
+        w = make_workload(name, scale, **kw)
+        print(f"noalloc {precision} case {name}/{scale}/{kw}: leaves={w.tree.nleaves}", flush=True)
+        a = _run(w, noalloc=False, precision=precision)
+        b = _run(w, noalloc=True, precision=precision)
+        for k in ("pot", "grad", "hess"):
+            if a[k] is not None:
+                assert np.array_equal(a[k], b[k]), (name, k)
+        ref = O.oracle_boxfgt(w.tree, w.fvals, w.nd, w.delta, w.eps, w.ipoly, w.iperiod, w.norder, w.ifpgh)
+        assert leaf_rel_err(w.tree, b["pot"], ref.pot) < 1e-6
