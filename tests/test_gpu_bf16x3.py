@@ -48,7 +48,13 @@ def test_bf16x3_error_vs_oracle(device, name, scale, kw):
     ref = O.oracle_boxfgt(w.tree, w.fvals, w.nd, w.delta, w.eps, w.ipoly, w.iperiod, w.norder, w.ifpgh)
     got = run_gpu(w.tree, w.fvals, w.nd, w.delta, w.eps, w.ipoly, w.iperiod, w.norder, w.ifpgh,
                   precision="bf16x3")
-    assert got["plan"].info["precision"] == 2
+    info = got["plan"].info
+    assert info["precision"] == 2
+    if info["n_pw_leaves"] > 0:
+        # the column kernels share each x contraction over a column (group)
+        assert info["xc_up"] <= info["n_pw_leaves"] and info["xc_down"] <= info["n_pw_leaves"]
+        if scale == "full":
+            assert info["xc_down"] < info["n_pw_leaves"]
     errs = _errs(w.tree, got, ref, w.ifpgh)
     print(f"bf16x3 {name}/{scale}/{kw}: npw={got['plan'].info['npw']} errors {errs}", flush=True)
     assert 0 < errs["pot"] < 2e-6, errs
