@@ -70,11 +70,10 @@ struct TreeNode {
     int pad;
 };
 
-// leaves per column group of the downward pass (leaf order > 8); the
-// three-slice bf16 pass, whose registers do not grow with the group, takes
-// longer groups
-constexpr int kColLeaves = 4;
-constexpr int kColLeavesB3 = 8;
+// leaves per column group of the downward passes (leaf order > 8), and in
+// the short groups (fewer than 2048 long ones; the split-FP32 pass)
+constexpr int kColLeaves = 8;
+constexpr int kColLeavesShort = 4;
 // waves (chunks) of one upward workgroup that sum the chunks of a cutoff box
 constexpr int kUpReduceWaves = 4;
 // below this many (chunk, y tile) waves the upward pass puts each density on
@@ -156,8 +155,8 @@ struct Plan {
     bool merge_pieces = false;     // the last upward pass summed chunks per workgroup
     long long merge_reads = 0;     // expansions the merge sums
     int2* d_colgrp = nullptr;      // (first leaf, count)
-    int ncolgrp_b3 = 0;            // the same runs cut for the three-slice bf16 downward pass
-    int2* d_colgrp_b3 = nullptr;
+    int ncolgrp_short = 0;         // the same runs in groups of at most kColLeavesShort
+    int2* d_colgrp_short = nullptr;
     int nqgroup = 0;               // n <= 8: column pairs evaluated as 2 x 2 leaf blocks
     int4* d_qgroups = nullptr;     // (first quad, count, a leaf of column A, a leaf of column B or -1)
     int4* d_quads = nullptr;       // leaves (A, jyA), (B, jyA), (A, jyB), (B, jyB) or -1

@@ -706,9 +706,12 @@ int plan_create(Plan& p, int nd, int ndim, double delta, double eps, int ipoly, 
         i += two ? 2 : 1;
     }
     p.npair = (int)pairs_pw.size();
-    // column groups for the downward pass: runs of leaves with the same box,
-    // level and x cell, cut to at most kColLeaves leaves (one wave each)
-    std::vector<int2> colgrp, colgrp_b3;
+    // column groups for the downward passes: runs of leaves with the same
+    // box, level and x cell, cut to at most kColLeaves leaves where that
+    // leaves 2048 groups (one wave each), else to kColLeavesShort; the group's
+    // x contraction serves all its leaves and the kernels' registers do not
+    // grow with the group.  The split-FP32 pass keeps the short groups.
+    std::vector<int2> colgrp, colgrp_short;
     {
         std::vector<int2> cols;
         for (int i = 0; i < p.npwleaf;) {
@@ -719,23 +722,17 @@ int plan_create(Plan& p, int nd, int ndim, double delta, double eps, int ipoly, 
             cols.push_back(make_int2(i, j - i));
             i = j;
         }
-        auto cut = [&](int cap) {
-            colgrp.clear();
+        auto cut = [&](int cap, std::vector<int2>& out) {
+            out.clear();
             for (const int2& c : cols)
-                for (int k = 0; k < c.y; k += cap) colgrp.push_back(make_int2(c.x + k, std::min(cap, c.y - k)));
+                for (int k = 0; k < c.y; k += cap) out.push_back(make_int2(c.x + k, std::min(cap, c.y - k)));
         };
-        if (p.precision == BFGT_PREC_BF16X3) {
-            // the three-slice pass slices a group's x contraction once for
-            // all its leaves: longer groups while two waves per SIMD remain
-            // (16-leaf groups measured slower: c2 downward 0.051 -> 0.071 ms)
-            cut(kColLeavesB3);
-            if (colgrp.size() >= 2048) colgrp_b3 = colgrp;
-        }
-        cut(kColLeaves);
+        cut(kColLeavesShort, colgrp_short);
+        cut(kColLeaves, colgrp);
+        if (colgrp.size() < 2048) colgrp = colgrp_short;
     }
     p.ncolgrp = (int)colgrp.size();
-    if (colgrp_b3.empty()) colgrp_b3 = colgrp;
-    p.ncolgrp_b3 = (int)colgrp_b3.size();
+    p.ncolgrp_short = (int)colgrp_short.size();
     // for n <= 8: pairs of columns (same box and level) whose leaves are
     // evaluated as 2 x 2 blocks -- quads (A, jyA), (B, jyA), (A, jyB), (B,
     // jyB) of leaf indices (-1 where no leaf) over consecutive y cells of
@@ -941,7 +938,7 @@ int plan_create(Plan& p, int nd, int ndim, double delta, double eps, int ipoly, 
     }
     if ((rc = upload(p, p.d_pwpair, pairs_pw))) return rc;
     if ((rc = upload(p, p.d_colgrp, colgrp))) return rc;
-    if (p.precision == BFGT_PREC_BF16X3 && (rc = upload(p, p.d_colgrp_b3, colgrp_b3))) return rc;
+    if (p.precision == BFGT_PREC_SPLIT32 && (rc = upload(p, p.d_colgrp_short, colgrp_short))) return rc;
     if ((rc = upload(p, p.d_quads, quads))) return rc;
     if ((rc = upload(p, p.d_qgroups, qgroups))) return rc;
     {
@@ -1627,10 +1624,11 @@ int bfgt_plan_work(bfgt_plan_t plan, long long* counts, int ncounts) {
     // there are enough pairs), else one per leaf
     const long long xc_down = big ? p.ncolgrp : ((long long)p.nqgroup * p.nd >= 1024 ? p.nxcol_quad : L);
     // forward passes: split-FP32 shares them as FP64 does, except that its
-    // n <= 8 downward pass always takes the column pairs; the three-slice
-    // bf16 passes run the column kernels at every order
+    // downward pass takes the short column groups (n > 8) and always the
+    // column pairs (n <= 8); the three-slice bf16 passes run the column
+    // kernels at every order
     const long long fwd_up = !mfma ? L : (b3 || big) ? p.nxcol_up : p.nxcol_upq;
-    const long long fwd_down = !mfma ? L : b3 ? p.ncolgrp_b3 : (split && !big) ? p.nxcol_quad : xc_down;
+    const long long fwd_down = !mfma ? L : b3 ? p.ncolgrp : split ? (big ? p.ncolgrp_short : p.nxcol_quad) : xc_down;
     // the FP64 and three-slice bf16 upward passes sum chunks per workgroup
     // (at most two x tiles) and then merge workgroup pieces
     const bool pieces = mfma && !split && (p.npw + 1) / 2 <= 32 && p.nmerge > 0;

@@ -2625,11 +2625,12 @@ static int launch_down_split(Plan& p, double* pot, double* grad, double* hess) {
             default: return runq(k_down_split_quad<NTAB, 4>);
         }
     }
-    int blocks = (p.ncolgrp + kWavesPerBlock - 1) / kWavesPerBlock;
+    int blocks = (p.ncolgrp_short + kWavesPerBlock - 1) / kWavesPerBlock;
     auto run = [&](auto kern) -> int {
         kern<<<blocks, kWave * kWavesPerBlock, 0, p.stream>>>(
-            p.n, p.npw, p.pstride, p.nd, p.ncolgrp, p.d_colgrp, p.d_pwleaf, p.d_tdn_f, 3 * 16 * p.pstride, p.d_phase,
-            p.d_levbase, reinterpret_cast<const Fold*>(p.d_loc), pot, grad, hess, p.out_scale, p.accum);
+            p.n, p.npw, p.pstride, p.nd, p.ncolgrp_short, p.d_colgrp_short, p.d_pwleaf, p.d_tdn_f,
+            3 * 16 * p.pstride, p.d_phase, p.d_levbase, reinterpret_cast<const Fold*>(p.d_loc), pot, grad, hess,
+            p.out_scale, p.accum);
         BFGT_HIP_CHECK(hipGetLastError());
         return BFGT_OK;
     };
@@ -2644,10 +2645,10 @@ static int launch_down_split(Plan& p, double* pot, double* grad, double* hess) {
 template <int NTAB>
 static int launch_down_b3(Plan& p, double* pot, double* grad, double* hess) {
     const int nT = round_up((p.npw + 1) / 2, 16) / 16;
-    const int blocks = (p.ncolgrp_b3 + kWavesPerBlock - 1) / kWavesPerBlock;
+    const int blocks = (p.ncolgrp + kWavesPerBlock - 1) / kWavesPerBlock;
     auto run = [&](auto kern) -> int {
         kern<<<blocks, kWave * kWavesPerBlock, 0, p.stream>>>(
-            p.n, p.npw, p.nd, p.ncolgrp_b3, p.d_colgrp_b3, p.d_pwleaf, p.nlevstart, p.b3nkb, p.d_b3dx, p.d_b3dy,
+            p.n, p.npw, p.nd, p.ncolgrp, p.d_colgrp, p.d_pwleaf, p.nlevstart, p.b3nkb, p.d_b3dx, p.d_b3dy,
             reinterpret_cast<const Fold*>(p.d_loc), pot, grad, hess, p.out_scale, p.accum);
         BFGT_HIP_CHECK(hipGetLastError());
         return BFGT_OK;
